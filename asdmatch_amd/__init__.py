@@ -1,0 +1,348 @@
+"""asdmatch_amd -- MI355X (gfx950) ASDMatch matching engine, Python side.
+
+A thin ctypes layer over the C ABI of ``libasdm.so`` (declared in ``include/asdm.h``).  It mirrors
+the reference's call sequence in main.cpp:11-147:
+
+    Graph.load(path)                 Graph::loadGraphFromTextFile   relation/graph.cpp:68-148
+    ctx.filter(q, g) -> CandSpace    CandidateSpace::buildCandCFL   relation/candidate_space.cpp:550-721
+    cs.plan(q) -> Plan               simplePlan / optCostPlan       plan/optimizer.cpp
+    ctx.count(plan, cs)              adaptiveShareJoin (count)      matching/adaptive.cpp:1179-2279
+    ctx.local_count(plan, cs)        ALL_LEVEL + LOCAL_COUNT build  matching/join.cpp:1016-1041
+
+The compute path is the HIP library: if ``libasdm.so`` is missing this module raises on import,
+and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libasdm.so")
+
+ASDM_OK = 0
+ASDM_E_EMPTY_CANDIDATES = 1
+ASDM_PLAN_LOCAL = 1
+
+_ERRNAMES = {
+    -1: "ASDM_E_INVALID", -2: "ASDM_E_IO", -3: "ASDM_E_OOM", -4: "ASDM_E_HIP",
+    -5: "ASDM_E_NODEVICE", -6: "ASDM_E_BUDGET", -7: "ASDM_E_UNSUPPORTED",
+}
+
+
+class AsdmError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libasdm.so not built at {LIB_PATH}; run __graft_entry__.build()")
+_lib = C.CDLL(LIB_PATH)
+
+_P = C.c_void_p
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class Stats(C.Structure):
+    _fields_ = [("filter_ms", C.c_double), ("plan_ms", C.c_double), ("exec_ms", C.c_double),
+                ("kernel_ms", C.c_double), ("alg_bytes", C.c_uint64), ("rows_total", C.c_uint64),
+                ("launches", C.c_uint64)]
+
+
+def _sig(name, res, *args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_sig("asdm_version", C.c_char_p)
+_sig("asdm_ctx_create", C.c_int, C.c_int, C.POINTER(_P))
+_sig("asdm_ctx_destroy", None, _P)
+_sig("asdm_last_error", C.c_char_p, _P)
+_sig("asdm_ctx_set_profiling", C.c_int, _P, C.c_int)
+_sig("asdm_kernel_stat", C.c_int, _P, C.c_uint32, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), _u64p, _u64p)
+_sig("asdm_last_stats", C.c_int, _P, C.POINTER(Stats))
+_sig("asdm_graph_load_text", C.c_int, C.c_char_p, C.POINTER(_P))
+_sig("asdm_graph_from_edges", C.c_int, C.c_uint32, _u32p, C.c_uint64, _u32p, C.POINTER(_P))
+_sig("asdm_graph_write_text", C.c_int, _P, C.c_char_p)
+_sig("asdm_graph_free", None, _P)
+_sig("asdm_graph_info", C.c_int, _P, _u32p, _u64p, _u32p)
+_sig("asdm_graph_csr", C.c_int, _P, _u32p, _u32p, _u32p)
+_sig("asdm_graph_edges", C.c_int, _P, _u32p, C.c_uint64, _u64p)
+_sig("asdm_graph_upload", C.c_int, _P, _P)
+_sig("asdm_gen_chung_lu", C.c_int, C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.c_double, C.c_uint64,
+     C.POINTER(_P))
+_sig("asdm_gen_rmat", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32,
+     C.c_double, C.c_uint64, C.POINTER(_P))
+_sig("asdm_gen_query", C.c_int, _P, C.c_uint32, C.c_double, C.c_uint32, C.c_uint64, C.POINTER(_P))
+_sig("asdm_filter_cfl", C.c_int, _P, _P, _P, C.POINTER(_P))
+_sig("asdm_cs_free", None, _P)
+_sig("asdm_cs_root", C.c_uint32, _P)
+_sig("asdm_cs_sizes", C.c_int, _P, _u32p)
+_sig("asdm_cs_candidates", C.c_int, _P, C.c_uint32, _u32p, C.c_uint32, _u32p)
+_sig("asdm_cs_arc_weight", C.c_int, _P, C.c_uint32, _u64p)
+_sig("asdm_plan_create", C.c_int, _P, _P, _u32p, C.c_uint32, C.POINTER(_P))
+_sig("asdm_plan_create_with_cards", C.c_int, _P, _P, _u64p, C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
+     C.POINTER(_P))
+_sig("asdm_plan_free", None, _P)
+_sig("asdm_plan_order", C.c_int, _P, _u32p, _u32p)
+_sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
+_sig("asdm_local_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p, _u64p)
+_sig("asdm_intersect_k", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
+
+
+def version() -> str:
+    return _lib.asdm_version().decode()
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _ptr(a: np.ndarray, t=C.c_uint32):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Graph:
+    """Label-sorted CSR graph owned by libasdm (host memory, optionally mirrored in HBM)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.asdm_graph_free(h)
+            self._h = None
+
+    @staticmethod
+    def _check(rc, what):
+        if rc != ASDM_OK:
+            raise AsdmError(rc, what)
+
+    @classmethod
+    def load(cls, path: str) -> "Graph":
+        h = _P()
+        cls._check(_lib.asdm_graph_load_text(os.fsencode(path), C.byref(h)), f"load {path}")
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, labels: Sequence[int], edges) -> "Graph":
+        lab = _u32(labels)
+        e = _u32(edges).reshape(-1)
+        h = _P()
+        cls._check(_lib.asdm_graph_from_edges(len(lab), _ptr(lab), len(e) // 2, _ptr(e), C.byref(h)), "from_edges")
+        return cls(h)
+
+    @classmethod
+    def chung_lu(cls, n, ne, beta, nlabels, zipf_s, seed) -> "Graph":
+        h = _P()
+        cls._check(_lib.asdm_gen_chung_lu(n, ne, beta, nlabels, zipf_s, seed, C.byref(h)), "gen_chung_lu")
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a, b, c, nlabels, zipf_s, seed) -> "Graph":
+        h = _P()
+        cls._check(_lib.asdm_gen_rmat(scale, edge_factor, a, b, c, nlabels, zipf_s, seed, C.byref(h)), "gen_rmat")
+        return cls(h)
+
+    def random_walk_query(self, k: int, density: float, max_extra: int, seed: int) -> "Graph":
+        h = _P()
+        self._check(_lib.asdm_gen_query(self._h, k, density, max_extra, seed, C.byref(h)), "gen_query")
+        return Graph(h)
+
+    def info(self):
+        n, m, nl = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _lib.asdm_graph_info(self._h, C.byref(n), C.byref(m), C.byref(nl))
+        return n.value, m.value, nl.value
+
+    @property
+    def n(self) -> int:
+        return self.info()[0]
+
+    def csr(self):
+        n, m, _ = self.info()
+        off = np.zeros(n + 1, np.uint32)
+        nbr = np.zeros(max(m, 1), np.uint32)
+        lab = np.zeros(max(n, 1), np.uint32)
+        _lib.asdm_graph_csr(self._h, _ptr(off), _ptr(nbr), _ptr(lab))
+        return off, nbr[:m], lab[:n]
+
+    def edges(self) -> np.ndarray:
+        ne = C.c_uint64()
+        _lib.asdm_graph_edges(self._h, None, 0, C.byref(ne))
+        out = np.zeros(2 * max(ne.value, 1), np.uint32)
+        self._check(_lib.asdm_graph_edges(self._h, _ptr(out), ne.value, C.byref(ne)), "edges")
+        return out[: 2 * ne.value].reshape(-1, 2)
+
+    def write(self, path: str):
+        self._check(_lib.asdm_graph_write_text(self._h, os.fsencode(path)), f"write {path}")
+
+
+class Context:
+    """One HIP device + stream.  Raises AsdmError(ASDM_E_NODEVICE) without a gfx950 GPU."""
+
+    def __init__(self, device: int = 0):
+        h = _P()
+        rc = _lib.asdm_ctx_create(device, C.byref(h))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "asdm_ctx_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.asdm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc not in (ASDM_OK,):
+            raise AsdmError(rc, f"{what}: {_lib.asdm_last_error(self._h).decode()}")
+
+    def upload(self, g: Graph):
+        self._check(_lib.asdm_graph_upload(self._h, g._h), "upload")
+
+    def filter(self, q: Graph, g: Graph) -> Optional["CandSpace"]:
+        """CFL filter + candidate edges; None when some candidate set is empty."""
+        self.upload(g)
+        h = _P()
+        rc = _lib.asdm_filter_cfl(self._h, q._h, g._h, C.byref(h))
+        if rc == ASDM_E_EMPTY_CANDIDATES:
+            return None
+        self._check(rc, "filter_cfl")
+        return CandSpace(h, q.n)
+
+    def count(self, plan: "Plan", cs: "CandSpace", budget_bytes: int = 0) -> int:
+        out = C.c_uint64()
+        self._check(_lib.asdm_count(self._h, plan._h, cs._h, budget_bytes, C.byref(out)), "count")
+        return out.value
+
+    def local_count(self, plan: "Plan", cs: "CandSpace", nv: int, budget_bytes: int = 0):
+        mat = np.zeros((nv, cs.nq), np.uint64)
+        out = C.c_uint64()
+        self._check(_lib.asdm_local_count(self._h, plan._h, cs._h, budget_bytes, _ptr(mat, C.c_uint64),
+                                          C.byref(out)), "local_count")
+        return mat, out.value
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _lib.asdm_last_stats(self._h, C.byref(s))
+        return s
+
+    def set_profiling(self, on: bool):
+        _lib.asdm_ctx_set_profiling(self._h, 1 if on else 0)
+
+    def kernel_stats(self):
+        out = []
+        i = 0
+        name = C.create_string_buffer(64)
+        while True:
+            ms, la, by = C.c_double(), C.c_uint64(), C.c_uint64()
+            if _lib.asdm_kernel_stat(self._h, i, name, 64, C.byref(ms), C.byref(la), C.byref(by)) != ASDM_OK:
+                break
+            out.append({"name": name.value.decode(), "ms": ms.value, "launches": la.value, "bytes": by.value})
+            i += 1
+        return out
+
+    def intersect(self, lists) -> np.ndarray:
+        arrs = [_u32(a) for a in lists]
+        k = len(arrs)
+        ptrs = (_u32p * k)(*[_ptr(a) for a in arrs])
+        sizes = _u32([len(a) for a in arrs])
+        cap = int(min(len(a) for a in arrs)) if arrs else 0
+        out = np.zeros(max(cap, 1), np.uint32)
+        n = C.c_uint32()
+        self._check(_lib.asdm_intersect_k(self._h, ptrs, _ptr(sizes), k, _ptr(out), cap, C.byref(n)), "intersect")
+        return out[: n.value]
+
+
+class CandSpace:
+    def __init__(self, handle, nq):
+        self._h = handle
+        self.nq = nq
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.asdm_cs_free(self._h)
+            self._h = None
+
+    @property
+    def root(self) -> int:
+        return int(_lib.asdm_cs_root(self._h))
+
+    def sizes(self) -> np.ndarray:
+        s = np.zeros(self.nq, np.uint32)
+        _lib.asdm_cs_sizes(self._h, _ptr(s))
+        return s
+
+    def candidates(self, u: int) -> np.ndarray:
+        n = C.c_uint32()
+        _lib.asdm_cs_candidates(self._h, u, None, 0, C.byref(n))
+        out = np.zeros(max(n.value, 1), np.uint32)
+        rc = _lib.asdm_cs_candidates(self._h, u, _ptr(out), n.value, C.byref(n))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "cs_candidates")
+        return out[: n.value]
+
+    def arc_weight(self, a: int) -> int:
+        w = C.c_uint64()
+        _lib.asdm_cs_arc_weight(self._h, a, C.byref(w))
+        return w.value
+
+    def plan(self, q: Graph, order=None, local: bool = False, cards=None) -> "Plan":
+        h = _P()
+        flags = ASDM_PLAN_LOCAL if local else 0
+        if cards:
+            masks = np.array(list(cards.keys()), np.uint64)
+            vals = np.array(list(cards.values()), np.float64)
+            rc = _lib.asdm_plan_create_with_cards(q._h, self._h, _ptr(masks, C.c_uint64),
+                                                  vals.ctypes.data_as(C.POINTER(C.c_double)), len(masks), flags,
+                                                  C.byref(h))
+        else:
+            o = None if order is None else _ptr(_u32(order))
+            rc = _lib.asdm_plan_create(q._h, self._h, o, flags, C.byref(h))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "plan_create")
+        return Plan(h, self.nq)
+
+
+class Plan:
+    def __init__(self, handle, nq):
+        self._h = handle
+        self.nq = nq
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.asdm_plan_free(self._h)
+            self._h = None
+
+    def order(self):
+        o = np.zeros(self.nq, np.uint32)
+        d = C.c_uint32()
+        _lib.asdm_plan_order(self._h, _ptr(o), C.byref(d))
+        return o.tolist(), d.value
+
+
+def count_embeddings(ctx: Context, q: Graph, g: Graph, budget_bytes: int = 0, order=None) -> int:
+    """Filter, plan and count: the reference's `Number of matches` (0 when a candidate set is empty)."""
+    cs = ctx.filter(q, g)
+    if cs is None:
+        return 0
+    plan = cs.plan(q, order=order)
+    return ctx.count(plan, cs, budget_bytes)
+
+
+def local_count(ctx: Context, q: Graph, g: Graph, budget_bytes: int = 0):
+    cs = ctx.filter(q, g)
+    if cs is None:
+        return np.zeros((g.n, q.n), np.uint64), 0
+    plan = cs.plan(q, local=True)
+    return ctx.local_count(plan, cs, g.n, budget_bytes)

@@ -1,0 +1,351 @@
+// Context, device memory, prefix scan, fills, graph upload and the k-way intersection primitive.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+
+#include "engine.h"
+#include "kernels_common.h"
+
+namespace asdm {
+
+int check_hip(asdm_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return ASDM_OK;
+    if (ctx) ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? ASDM_E_OOM : ASDM_E_HIP;
+}
+
+int dev_alloc(asdm_ctx *ctx, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMallocAsync(p, bytes, ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->err = "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
+        return ASDM_E_OOM;
+    }
+    return ASDM_OK;
+}
+
+void dev_free(asdm_ctx *ctx, void *p) {
+    if (p) (void)hipFreeAsync(p, ctx->stream);
+}
+
+// ---------------------------------------------------------------- scan
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanTile = (uint64_t)kScanThreads * kScanItems;
+
+__global__ void k_scan_tile_sums(const uint32_t *__restrict__ in, uint64_t n, uint64_t *__restrict__ tile_sums) {
+    __shared__ uint64_t lds[kScanThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += in[base + k];
+    uint64_t total;
+    (void)block_exclusive_scan<kScanThreads>(s, lds, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the tile sums in place, total into *total_out.
+__global__ void k_scan_tiles(uint64_t *__restrict__ tile_sums, uint64_t ntiles, uint64_t *__restrict__ total_out) {
+    __shared__ uint64_t lds[kScanThreads / 64 + 1];
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < ntiles; b += kScanThreads) {
+        uint64_t i = b + threadIdx.x;
+        uint64_t v = i < ntiles ? tile_sums[i] : 0;
+        uint64_t tot;
+        uint64_t ex = block_exclusive_scan<kScanThreads>(v, lds, tot);
+        if (i < ntiles) tile_sums[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_apply(const uint32_t *__restrict__ in, uint64_t n, const uint64_t *__restrict__ tile_pref,
+                             uint64_t *__restrict__ out) {
+    __shared__ uint64_t lds[kScanThreads / 64 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint64_t tot;
+    uint64_t ex = block_exclusive_scan<kScanThreads>(s, lds, tot) + tile_pref[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+}
+
+int scan_u32_to_u64(asdm_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n) {
+    if (n == 0) return fill_u64(ctx, out, 1, 0);
+    uint64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    uint64_t *tiles = nullptr;
+    int rc = dev_alloc(ctx, (void **)&tiles, sizeof(uint64_t) * ntiles);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)ntiles), dim3(kScanThreads), 0, ctx->stream, in, n, tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, ctx->stream, tiles, ntiles, out + n);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(kScanThreads), 0, ctx->stream, in, n, tiles, out);
+    dev_free(ctx, tiles);
+    return check_hip(ctx, hipGetLastError(), "scan");
+}
+
+// ---------------------------------------------------------------- fills
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+static unsigned grid_for(uint64_t n, unsigned block) {
+    uint64_t g = (n + block - 1) / block;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, 65536));
+}
+
+int fill_u64(asdm_ctx *ctx, uint64_t *p, uint64_t n, uint64_t v) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, v);
+    return check_hip(ctx, hipGetLastError(), "fill");
+}
+int fill_u32(asdm_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, p, n, v);
+    return check_hip(ctx, hipGetLastError(), "fill");
+}
+
+// ---------------------------------------------------------------- timing
+KTimer::KTimer(asdm_ctx *c, const char *name) : ctx(c), slot(-1) {
+    if (!ctx->profiling) return;
+    for (size_t i = 0; i < ctx->kstats.size(); ++i)
+        if (ctx->kstats[i].name == name) slot = (int)i;
+    if (slot < 0) {
+        ctx->kstats.push_back({name, 0, 0, 0});
+        slot = (int)ctx->kstats.size() - 1;
+    }
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, ctx->stream);
+}
+
+void KTimer::stop(uint64_t bytes) {
+    if (!ctx->profiling || slot < 0) return;
+    (void)hipEventRecord(b, ctx->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    auto &k = ctx->kstats[slot];
+    k.ms += ms;
+    k.launches += 1;
+    k.bytes += bytes;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    slot = -1;
+}
+
+void release_device_graph(asdm_graph &g) {
+    auto &d = g.dev;
+    if (d.device < 0) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(d.device);
+    (void)hipFree(d.off);
+    (void)hipFree(d.nbr);
+    (void)hipFree(d.lab);
+    (void)hipFree(d.loff);
+    (void)hipFree(d.vbl_off);
+    (void)hipFree(d.vbl);
+    (void)hipSetDevice(cur);
+    d = DeviceGraph{};
+}
+
+// ---------------------------------------------------------------- intersection primitive
+// One wave per call: the shortest list is the probe; each lane tests probe elements by binary
+// search in the other lists; survivors are compacted in order with a ballot prefix.
+__global__ void k_intersect_k(const uint32_t *__restrict__ flat, const uint64_t *__restrict__ offs, uint32_t k,
+                              uint32_t *__restrict__ out, uint32_t *__restrict__ nout) {
+    const int lane = threadIdx.x & 63;
+    uint32_t pj = 0;
+    uint64_t plen = offs[1] - offs[0];
+    for (uint32_t j = 1; j < k; ++j) {
+        uint64_t len = offs[j + 1] - offs[j];
+        if (len < plen) { plen = len; pj = j; }
+    }
+    const uint32_t *probe = flat + offs[pj];
+    uint32_t run = 0;
+    for (uint64_t t0 = 0; t0 < plen; t0 += 64) {
+        uint64_t t = t0 + lane;
+        bool ok = t < plen;
+        uint32_t x = ok ? probe[t] : 0;
+        for (uint32_t j = 0; j < k && ok; ++j) {
+            if (j == pj) continue;
+            const uint32_t *l = flat + offs[j];
+            uint32_t n = (uint32_t)(offs[j + 1] - offs[j]);
+            uint32_t p = lower_bound_u32(l, n, x);
+            ok = p < n && l[p] == x;
+        }
+        uint64_t bal = __ballot(ok);
+        if (ok) out[run + __popcll(bal & ((1ull << lane) - 1))] = x;
+        run += __popcll(bal);
+    }
+    if (lane == 0) *nout = run;
+}
+
+}  // namespace asdm
+
+using namespace asdm;
+
+extern "C" {
+
+int asdm_ctx_create(int device, asdm_ctx **out) {
+    if (!out) return ASDM_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return ASDM_E_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) return ASDM_E_INVALID;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ASDM_E_NODEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ASDM_E_NODEVICE;
+    auto ctx = std::make_unique<asdm_ctx>();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) return ASDM_E_HIP;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return ASDM_E_HIP;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    if (hipMalloc((void **)&ctx->d_counters, sizeof(uint64_t) * 8) != hipSuccess) return ASDM_E_OOM;
+    *out = ctx.release();
+    return ASDM_OK;
+}
+
+void asdm_ctx_destroy(asdm_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->d_counters);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *asdm_last_error(const asdm_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out) {
+    if (!ctx || !out) return ASDM_E_INVALID;
+    *out = ctx->stats;
+    return ASDM_OK;
+}
+
+// Per-kernel device timing, collected while profiling is on (HIP events on the context stream).
+int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) {
+    if (!ctx) return ASDM_E_INVALID;
+    ctx->profiling = on != 0;
+    ctx->kstats.clear();
+    return ASDM_OK;
+}
+
+int asdm_kernel_stat(const asdm_ctx *ctx, uint32_t i, char *name, uint32_t name_cap, double *ms, uint64_t *launches,
+                     uint64_t *bytes) {
+    if (!ctx) return ASDM_E_INVALID;
+    if (i >= ctx->kstats.size()) return ASDM_E_INVALID;
+    const auto &k = ctx->kstats[i];
+    if (name && name_cap) {
+        std::strncpy(name, k.name.c_str(), name_cap - 1);
+        name[name_cap - 1] = 0;
+    }
+    if (ms) *ms = k.ms;
+    if (launches) *launches = k.launches;
+    if (bytes) *bytes = k.bytes;
+    return ASDM_OK;
+}
+
+int asdm_graph_upload(asdm_ctx *ctx, asdm_graph *g) {
+    if (!ctx || !g) return ASDM_E_INVALID;
+    if (g->dev.device == ctx->device) return ASDM_OK;
+    if (g->dev.device >= 0) release_device_graph(*g);
+    (void)hipSetDevice(ctx->device);
+    const uint64_t L1 = (uint64_t)g->nlab + 1;
+    if ((uint64_t)g->n * L1 >= (1ull << 33))
+        return ctx->fail(ASDM_E_UNSUPPORTED, "per-vertex label offset table too large (n * (labels + 1) >= 2^33)");
+    // loff[v * (L + 1) + l] = first arc of label >= l in v's adjacency
+    std::vector<uint32_t> loff((size_t)(g->n * L1));
+    for (uint32_t v = 0; v < g->n; ++v) {
+        uint32_t i = g->off[v], e = g->off[v + 1];
+        uint32_t *row = loff.data() + (size_t)v * L1;
+        for (uint32_t l = 0; l <= g->nlab; ++l) {
+            while (i < e && g->lab[g->nbr[i]] < l) ++i;
+            row[l] = i;
+        }
+    }
+    DeviceGraph d;
+    d.device = ctx->device;
+    auto up = [&](uint32_t **dst, const uint32_t *src, size_t n) -> int {
+        if (hipMalloc((void **)dst, sizeof(uint32_t) * std::max<size_t>(n, 1)) != hipSuccess) return ASDM_E_OOM;
+        if (n && hipMemcpy(*dst, src, sizeof(uint32_t) * n, hipMemcpyHostToDevice) != hipSuccess) return ASDM_E_HIP;
+        return ASDM_OK;
+    };
+    int rc = up(&d.off, g->off.data(), (size_t)g->n + 1);
+    if (!rc) rc = up(&d.nbr, g->nbr.data(), g->m);
+    if (!rc) rc = up(&d.lab, g->lab.data(), g->n);
+    if (!rc) rc = up(&d.loff, loff.data(), loff.size());
+    if (!rc) rc = up(&d.vbl_off, g->vbl_off.data(), g->vbl_off.size());
+    if (!rc) rc = up(&d.vbl, g->vbl.data(), g->vbl.size());
+    g->dev = d;
+    if (rc) {
+        release_device_graph(*g);
+        return ctx->fail(rc, "graph upload failed");
+    }
+    return ASDM_OK;
+}
+
+int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k, uint32_t *out,
+                     uint32_t cap, uint32_t *n) {
+    if (!ctx || !lists || !sizes || !n || k == 0) return ASDM_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    std::vector<uint64_t> offs(k + 1, 0);
+    for (uint32_t j = 0; j < k; ++j) offs[j + 1] = offs[j] + sizes[j];
+    uint64_t minsz = UINT64_MAX;
+    for (uint32_t j = 0; j < k; ++j) minsz = std::min<uint64_t>(minsz, sizes[j]);
+    uint32_t *dflat = nullptr, *dout = nullptr, *dn = nullptr;
+    uint64_t *doffs = nullptr;
+    int rc = dev_alloc(ctx, (void **)&dflat, sizeof(uint32_t) * std::max<uint64_t>(offs[k], 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&doffs, sizeof(uint64_t) * (k + 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&dout, sizeof(uint32_t) * std::max<uint64_t>(minsz, 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&dn, sizeof(uint32_t));
+    for (uint32_t j = 0; j < k && !rc; ++j)
+        if (sizes[j])
+            rc = check_hip(ctx, hipMemcpyAsync(dflat + offs[j], lists[j], sizeof(uint32_t) * sizes[j], hipMemcpyHostToDevice,
+                                               ctx->stream), "copy");
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(doffs, offs.data(), sizeof(uint64_t) * (k + 1), hipMemcpyHostToDevice, ctx->stream), "copy");
+    if (!rc) {
+        hipLaunchKernelGGL(k_intersect_k, dim3(1), dim3(64), 0, ctx->stream, dflat, doffs, k, dout, dn);
+        rc = check_hip(ctx, hipGetLastError(), "intersect launch");
+    }
+    uint32_t hn = 0;
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&hn, dn, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!rc) {
+        *n = hn;
+        if (out && hn > cap) rc = ctx->fail(ASDM_E_INVALID, "output capacity too small");
+        else if (out && hn)
+            rc = check_hip(ctx, hipMemcpy(out, dout, sizeof(uint32_t) * hn, hipMemcpyDeviceToHost), "copy");
+    }
+    dev_free(ctx, dflat);
+    dev_free(ctx, doffs);
+    dev_free(ctx, dout);
+    dev_free(ctx, dn);
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Internal structures shared by the HIP translation units and the host planner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "host.h"
+
+namespace asdm {
+
+constexpr uint32_t kMaxQ = 32;      // MAX_PATTERN_SIZE (config/config.h:31)
+constexpr uint32_t kMaxGroup = 4;   // largest same-label leaf group counted by inclusion-exclusion
+constexpr uint32_t kMaxParts = 15;  // Bell(4)
+
+// Device plan: everything the enumeration kernels need about the matching order.  Lives in
+// device memory; kernels read it with wave-uniform indices.
+struct DPlan {
+    uint32_t nq;                     // query vertices
+    uint32_t D;                      // enumerated depths; depths D..nq-1 are leaves
+    uint32_t u[kMaxQ];               // query vertex at each depth
+    uint64_t coff[kMaxQ];            // offset of C(u_d) in the concatenated candidate array
+    uint32_t csz[kMaxQ];             // |C(u_d)|
+    uint32_t nb[kMaxQ];              // backward neighbours of depth d (all at depths < min(d, D))
+    uint32_t bdep[kMaxQ][kMaxQ];     // their depths
+    uint64_t bbase[kMaxQ][kMaxQ];    // candidate-edge offset base of arc (u_bdep -> u_d)
+    uint32_t ns[kMaxQ];              // earlier enumerated depths with the same label as depth d
+    uint32_t sdep[kMaxQ][kMaxQ];
+    // leaf groups: leaves with equal labels, counted jointly by inclusion-exclusion
+    uint32_t ngroups;
+    uint32_t gsize[kMaxQ];
+    uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
+    // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient
+    uint32_t nparts[kMaxGroup + 1];
+    uint32_t pnblk[kMaxGroup + 1][kMaxParts];
+    uint32_t pblk[kMaxGroup + 1][kMaxParts][kMaxGroup];
+    int64_t pcoef[kMaxGroup + 1][kMaxParts];
+};
+
+}  // namespace asdm
+
+struct asdm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    asdm_stats stats{};
+    bool profiling = false;
+    // per-kernel device time (ms), launches and algorithmic bytes, accumulated while profiling
+    struct KStat {
+        std::string name;
+        double ms = 0;
+        uint64_t launches = 0;
+        uint64_t bytes = 0;
+    };
+    std::vector<KStat> kstats;
+    uint64_t *d_counters = nullptr;  // [8] device scratch counters
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+};
+
+struct asdm_cs {
+    asdm_ctx *ctx = nullptr;
+    uint32_t nq = 0, nv = 0, root = 0;
+    std::vector<uint32_t> qlab, qoff, qnbr;  // query CSR copy (label-sorted)
+    std::vector<uint32_t> csize;             // |C(u)|
+    std::vector<uint64_t> cand_off;          // [nq + 1] offsets into d_cand
+    uint32_t *d_cand = nullptr;              // concatenated ascending candidate lists
+    uint64_t *d_bits = nullptr;              // [nq][W] membership bitmaps of C(u)
+    uint32_t *d_rank = nullptr;              // [nq][W] exclusive popcount prefix per 64-bit word
+    uint32_t W = 0;                          // 64-bit words per bitmap
+    uint32_t narcs = 0;
+    std::vector<uint32_t> arc_u, arc_u2;     // query arcs in query CSR order
+    std::vector<uint64_t> ce_base;           // [narcs + 1] first index of arc a in d_ce_off
+    std::vector<uint64_t> arc_weight;        // total candidate-edge list length per arc
+    uint64_t *d_ce_off = nullptr;            // [ce_base[narcs] + 1] list offsets into d_ce_val
+    uint32_t *d_ce_val = nullptr;            // candidate-edge lists: positions in C(u2), ascending
+    uint64_t ce_total = 0;
+    double filter_ms = 0;
+    uint32_t arc_index(uint32_t u, uint32_t u2) const {
+        for (uint32_t a = qoff[u]; a < qoff[u + 1]; ++a)
+            if (qnbr[a] == u2) return a;
+        return UINT32_MAX;
+    }
+};
+
+struct asdm_plan {
+    asdm::DPlan h{};                 // host copy
+    std::vector<uint32_t> order;
+    uint32_t D = 0;
+    bool local = false;
+    double plan_ms = 0;
+};
+
+namespace asdm {
+
+// Device helpers implemented in ctx.hip
+int check_hip(asdm_ctx *ctx, hipError_t e, const char *what);
+int dev_alloc(asdm_ctx *ctx, void **p, size_t bytes);
+void dev_free(asdm_ctx *ctx, void *p);
+// out[0..n] = exclusive prefix of in[0..n) with out[n] = total.
+int scan_u32_to_u64(asdm_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);
+int fill_u64(asdm_ctx *ctx, uint64_t *p, uint64_t n, uint64_t v);
+int fill_u32(asdm_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
+
+// Planner (plan.cpp)
+int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, const uint64_t *masks,
+               const double *cards, uint32_t ncards, asdm_plan *out, std::string &err);
+// CFL root choice and BFS order (plan.cpp): base_count[u] = |{v : label, degree, NLF ok}|.
+uint32_t cfl_root(const asdm_graph *q, const asdm_graph *g, const std::vector<uint32_t> &base_count);
+std::vector<uint32_t> bfs_order(const asdm_graph *q, uint32_t root);
+
+// Kernel timing helpers (ctx.hip)
+struct KTimer {
+    asdm_ctx *ctx;
+    int slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(asdm_ctx *c, const char *name);
+    void stop(uint64_t bytes = 0);
+};
+
+}  // namespace asdm

@@ -1,0 +1,557 @@
+// Enumeration engine: exact uint64 embedding counts and LOCAL_COUNT matrices.
+//
+// The count is the number of injective, label- and edge-preserving maps q -> g (the number the
+// reference's adaptiveShareJoin produces, matching/adaptive.cpp:1179-2279, independent of plan).
+// Execution model (DFS over chunks, BFS inside a chunk):
+//   * depths 0..D-1 of the plan are enumerated level by level into frontier tables of partial
+//     embeddings (row = positions in C(u_0..u_{d-1})), in chunks that fit a per-level budget;
+//   * depths D..nq-1 are leaves: an independent set whose neighbours are all matched, so their
+//     completions are counted in closed form per row: a product over label groups, with
+//     inclusion-exclusion over set partitions inside a group of equal-label leaves (the
+//     quantity numUniqueCombine computes, utility/utils.cpp:145-201; leaves of different labels
+//     can never collide).  This replaces traverse(count) (matching/join.cpp:910-1014).
+// Candidates of depth d are the intersection of the candidate-edge lists of its matched
+// neighbours (LeapfrogJoin's role, utility/compute_set_intersection.cpp:193-233): the shortest
+// list is the probe, an aligned group of G lanes walks it and binary-searches the others, and
+// survivors are compacted in order with a ballot prefix.  Injectivity is tested against the
+// data ids of earlier depths with the same label (other labels cannot collide).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "engine.h"
+#include "kernels_common.h"
+
+namespace asdm {
+
+struct ListRef {
+    uint64_t lo;
+    uint32_t len;
+};
+
+__device__ inline ListRef list_of(const DPlan *__restrict__ P, const uint64_t *__restrict__ ce_off,
+                                  const uint32_t *__restrict__ row, uint32_t d, uint32_t j) {
+    const uint64_t base = P->bbase[d][j] + row[P->bdep[d][j]];
+    const uint64_t lo = ce_off[base];
+    return {lo, (uint32_t)(ce_off[base + 1] - lo)};
+}
+
+// Data ids of the matched vertices that share depth d's label (at most 8 held in registers).
+struct SameLabel {
+    uint32_t y[8];
+    uint32_t ns;
+    const uint32_t *row;
+    const uint32_t *sdep;
+    const uint64_t *coff;
+};
+
+__device__ inline void load_same(SameLabel &S, const DPlan *__restrict__ P, const uint32_t *__restrict__ cand,
+                                 const uint32_t *__restrict__ row, uint32_t d) {
+    S.ns = P->ns[d];
+    S.row = row;
+    S.sdep = P->sdep[d];
+    S.coff = P->coff;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        uint32_t dep = P->sdep[d][s];
+        S.y[s] = (uint32_t)s < S.ns ? cand[P->coff[dep] + row[dep]] : 0xffffffffu;
+    }
+}
+
+__device__ inline bool injective(const SameLabel &S, const uint32_t *__restrict__ cand, uint32_t xid) {
+    bool ok = true;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) ok &= xid != S.y[s];
+    for (uint32_t s = 8; s < S.ns && ok; ++s) {
+        uint32_t dep = S.sdep[s];
+        ok = xid != cand[S.coff[dep] + S.row[dep]];
+    }
+    return ok;
+}
+
+// Number of list elements (positions in C(u_d)) whose data id is one of the same-label matched
+// vertices: each such vertex is located in C(u_d) and then in the list by binary search.
+__device__ inline uint32_t same_hits(const SameLabel &S, const uint32_t *__restrict__ cand, uint64_t cand_d,
+                                     uint32_t csz, const uint32_t *__restrict__ list, uint32_t len) {
+    uint32_t h = 0;
+    for (uint32_t s = 0; s < S.ns; ++s) {
+        uint32_t y;
+        if (s < 8) {
+            y = S.y[0];
+#pragma unroll
+            for (int t = 1; t < 8; ++t)
+                if ((uint32_t)t == s) y = S.y[t];
+        } else {
+            uint32_t dep = S.sdep[s];
+            y = cand[S.coff[dep] + S.row[dep]];
+        }
+        const uint32_t x = lower_bound_u32(cand + cand_d, csz, y);
+        if (x < csz && cand[cand_d + x] == y && contains_u32(list, len, x)) ++h;
+    }
+    return h;
+}
+
+// -------------------------------------------------------------------------- expand
+// Rows of width d -> candidates of depth d.  Count mode writes counts[row]; fill mode writes rows
+// of width d + 1 at out[(outoff[row] - out_base) ...] in (row, candidate) order.
+template <int G, bool FILL>
+__global__ void __launch_bounds__(256) k_expand(const DPlan *__restrict__ P, uint32_t d,
+                                                const uint32_t *__restrict__ rows, uint64_t nrows,
+                                                const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                                const uint32_t *__restrict__ cand, uint32_t *__restrict__ counts,
+                                                const uint64_t *__restrict__ outoff, uint64_t out_base,
+                                                uint32_t *__restrict__ out, unsigned long long *__restrict__ bytes_ctr) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint32_t nb = P->nb[d];
+    const uint64_t cand_d = P->coff[d];
+    const uint32_t csz = P->csz[d];
+    uint64_t bytes = 0;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+        const uint32_t *row = rows + it * d;
+        uint32_t pj = 0;
+        ListRef pr = list_of(P, ce_off, row, d, 0);
+        for (uint32_t j = 1; j < nb; ++j) {
+            ListRef r = list_of(P, ce_off, row, d, j);
+            if (r.len < pr.len) {
+                pr = r;
+                pj = j;
+            }
+        }
+        SameLabel S;
+        load_same(S, P, cand, row, d);
+        const uint32_t *probe = ce_val + pr.lo;
+        if constexpr (!FILL) {
+            uint32_t c;
+            if (nb == 1) {
+                c = lg == 0 ? pr.len - same_hits(S, cand, cand_d, csz, probe, pr.len) : 0;
+                bytes += lg == 0 ? 4ull * d + 16 : 0;
+            } else {
+                c = 0;
+                for (uint32_t t = lg; t < pr.len; t += G) {
+                    const uint32_t x = probe[t];
+                    bool ok = true;
+                    for (uint32_t j = 0; j < nb && ok; ++j) {
+                        if (j == pj) continue;
+                        ListRef r = list_of(P, ce_off, row, d, j);
+                        ok = contains_u32(ce_val + r.lo, r.len, x);
+                    }
+                    if (ok) ok = injective(S, cand, cand[cand_d + x]);
+                    c += ok;
+                }
+                bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+                if (lg == 0) bytes += 4ull * d + 16ull * nb;
+            }
+            c = (uint32_t)group_sum<G>(c);
+            if (lg == 0) counts[it] = c;
+        } else {
+            uint64_t o = outoff[it] - out_base;
+            for (uint32_t t0 = 0; t0 < pr.len; t0 += G) {
+                const uint32_t t = t0 + lg;
+                bool ok = t < pr.len;
+                uint32_t x = 0;
+                if (ok) {
+                    x = probe[t];
+                    for (uint32_t j = 0; j < nb && ok; ++j) {
+                        if (j == pj) continue;
+                        ListRef r = list_of(P, ce_off, row, d, j);
+                        ok = contains_u32(ce_val + r.lo, r.len, x);
+                    }
+                    if (ok) ok = injective(S, cand, cand[cand_d + x]);
+                }
+                const uint64_t gm = group_bits<G>(__ballot(ok));
+                if (ok) {
+                    uint32_t *dst = out + (o + __popcll(gm & ((1ull << lg) - 1))) * (d + 1);
+                    for (uint32_t k = 0; k < d; ++k) dst[k] = row[k];
+                    dst[d] = x;
+                }
+                o += __popcll(gm);
+            }
+            const uint64_t nout = outoff[it + 1] - outoff[it];
+            bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+            if (lg == 0) bytes += 4ull * d + 16ull * nb + 4ull * (d + 1) * nout;
+        }
+    }
+    bytes = group_sum<64>(bytes);
+    if ((threadIdx.x & 63) == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+}
+
+// -------------------------------------------------------------------------- leaves
+// |∩ of the candidate-edge lists of every leaf in `mask` (leaf indices of group g) minus matched
+// same-label vertices|, reduced over the lane group.
+template <int G>
+__device__ inline uint64_t leaf_set_count(const DPlan *__restrict__ P, uint32_t g, uint32_t mask,
+                                          const uint32_t *__restrict__ row, const uint64_t *__restrict__ ce_off,
+                                          const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                          const SameLabel &S, uint64_t &bytes) {
+    const int lg = threadIdx.x & (G - 1);
+    // pick the probe: shortest list among all lists of the leaves in mask
+    uint32_t pe = 0, pj = 0, nlists = 0;
+    ListRef pr{0, 0xffffffffu};
+    for (uint32_t i = 0; i < P->gsize[g]; ++i) {
+        if (!(mask >> i & 1)) continue;
+        const uint32_t e = P->gleaf[g][i];
+        for (uint32_t j = 0; j < P->nb[e]; ++j) {
+            ListRef r = list_of(P, ce_off, row, e, j);
+            ++nlists;
+            if (r.len < pr.len) {
+                pr = r;
+                pe = e;
+                pj = j;
+            }
+        }
+    }
+    const uint32_t e0 = P->gleaf[g][0];
+    const uint64_t cand_e = P->coff[e0];
+    const uint32_t *probe = ce_val + pr.lo;
+    if (nlists == 1) {
+        if (lg == 0) {
+            bytes += 16 + 8ull * S.ns;
+            return pr.len - same_hits(S, cand, cand_e, P->csz[e0], probe, pr.len);
+        }
+        return 0;
+    }
+    uint64_t c = 0;
+    for (uint32_t t = lg; t < pr.len; t += G) {
+        const uint32_t x = probe[t];
+        bool ok = true;
+        for (uint32_t i = 0; i < P->gsize[g] && ok; ++i) {
+            if (!(mask >> i & 1)) continue;
+            const uint32_t e = P->gleaf[g][i];
+            for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
+                if (e == pe && j == pj) continue;
+                ListRef r = list_of(P, ce_off, row, e, j);
+                ok = contains_u32(ce_val + r.lo, r.len, x);
+            }
+        }
+        if (ok) ok = injective(S, cand, cand[cand_e + x]);
+        c += ok;
+    }
+    bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+    return group_sum<G>(c);
+}
+
+template <int G>
+__device__ inline uint64_t group_count(const DPlan *__restrict__ P, uint32_t g, const uint32_t *__restrict__ row,
+                                       const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                       const uint32_t *__restrict__ cand, uint64_t &bytes) {
+    SameLabel S;
+    load_same(S, P, cand, row, P->gleaf[g][0]);
+    const uint32_t k = P->gsize[g];
+    if (k == 1) return leaf_set_count<G>(P, g, 1u, row, ce_off, ce_val, cand, S, bytes);
+    // intersection sizes of every nonempty subset of the group, then the partition sum
+    uint64_t cs[1u << kMaxGroup];
+    for (uint32_t m = 1; m < (1u << k); ++m) cs[m] = leaf_set_count<G>(P, g, m, row, ce_off, ce_val, cand, S, bytes);
+    uint64_t tot = 0;
+    for (uint32_t p = 0; p < P->nparts[k]; ++p) {
+        uint64_t term = (uint64_t)P->pcoef[k][p];
+        for (uint32_t b = 0; b < P->pnblk[k][p]; ++b) term *= cs[P->pblk[k][p][b]];
+        tot += term;
+    }
+    return tot;
+}
+
+// Rows of width D -> sum over rows of the product of the leaf-group counts (mod 2^64).
+template <int G>
+__global__ void __launch_bounds__(256) k_leaf(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
+                                              uint64_t nrows, const uint64_t *__restrict__ ce_off,
+                                              const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                              unsigned long long *__restrict__ total,
+                                              unsigned long long *__restrict__ bytes_ctr) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint32_t D = P->D, ng = P->ngroups;
+    uint64_t acc = 0, bytes = 0;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+        const uint32_t *row = rows + it * D;
+        uint64_t prod = 1;
+        for (uint32_t g = 0; g < ng; ++g) {
+            uint64_t c = group_count<G>(P, g, row, ce_off, ce_val, cand, bytes);
+            c = __shfl(c, (int)((threadIdx.x & 63) & ~(G - 1)), 64);  // broadcast the group leader's value
+            prod *= c;
+            if (prod == 0) break;
+        }
+        if (lg == 0) {
+            acc += prod;
+            bytes += 4ull * D;
+        }
+    }
+    acc = group_sum<64>(acc);
+    bytes = group_sum<64>(bytes);
+    if ((threadIdx.x & 63) == 0) {
+        if (acc) atomicAdd(total, (unsigned long long)acc);
+        if (bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+    }
+}
+
+// LOCAL_COUNT leaves (every leaf group has one leaf): per row, each matched vertex receives the
+// number of completions, and each leaf candidate x receives the product of the other leaves'
+// counts.  mat is |Vg| x |Vq| row-major; additions are integer atomics (order-independent).
+template <int G>
+__global__ void __launch_bounds__(256) k_leaf_local(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
+                                                    uint64_t nrows, const uint64_t *__restrict__ ce_off,
+                                                    const uint32_t *__restrict__ ce_val,
+                                                    const uint32_t *__restrict__ cand,
+                                                    unsigned long long *__restrict__ mat,
+                                                    unsigned long long *__restrict__ total) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint32_t D = P->D, ng = P->ngroups, nq = P->nq;
+    uint64_t acc = 0, dummy = 0;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+        const uint32_t *row = rows + it * D;
+        uint64_t cnt[kMaxQ];
+        uint64_t prod = 1;
+        for (uint32_t g = 0; g < ng; ++g) {
+            uint64_t c = group_count<G>(P, g, row, ce_off, ce_val, cand, dummy);
+            c = __shfl(c, (int)((threadIdx.x & 63) & ~(G - 1)), 64);
+            cnt[g] = c;
+            prod *= c;
+        }
+        if (prod == 0) continue;
+        if (lg == 0) acc += prod;
+        for (uint32_t j = lg; j < D; j += G) {
+            const uint32_t v = cand[P->coff[j] + row[j]];
+            atomicAdd(&mat[(uint64_t)v * nq + P->u[j]], (unsigned long long)prod);
+        }
+        for (uint32_t g = 0; g < ng; ++g) {
+            uint64_t others = 1;
+            for (uint32_t h = 0; h < ng; ++h)
+                if (h != g) others *= cnt[h];
+            const uint32_t e = P->gleaf[g][0];
+            SameLabel S;
+            load_same(S, P, cand, row, e);
+            uint32_t pj = 0;
+            ListRef pr = list_of(P, ce_off, row, e, 0);
+            for (uint32_t j = 1; j < P->nb[e]; ++j) {
+                ListRef r = list_of(P, ce_off, row, e, j);
+                if (r.len < pr.len) {
+                    pr = r;
+                    pj = j;
+                }
+            }
+            for (uint32_t t = lg; t < pr.len; t += G) {
+                const uint32_t x = ce_val[pr.lo + t];
+                bool ok = true;
+                for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
+                    if (j == pj) continue;
+                    ListRef r = list_of(P, ce_off, row, e, j);
+                    ok = contains_u32(ce_val + r.lo, r.len, x);
+                }
+                const uint32_t xid = cand[P->coff[e] + x];
+                if (ok) ok = injective(S, cand, xid);
+                if (ok) atomicAdd(&mat[(uint64_t)xid * nq + P->u[e]], (unsigned long long)others);
+            }
+        }
+    }
+    acc = group_sum<64>(acc);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(total, (unsigned long long)acc);
+}
+
+__global__ void k_iota(uint32_t *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+// -------------------------------------------------------------------------- host driver
+namespace {
+
+constexpr int kG = 16;  // lanes per row
+
+unsigned groups_grid(uint64_t items, int G) {
+    uint64_t threads = items * (uint64_t)G;
+    uint64_t blocks = (threads + 255) / 256;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 16));
+}
+
+struct Engine {
+    asdm_ctx *ctx;
+    const asdm_plan *plan;
+    const asdm_cs *cs;
+    DPlan *dplan = nullptr;
+    uint32_t D;
+    uint64_t budget;                  // bytes of row storage per level
+    std::vector<uint32_t *> rows;     // [D + 1] level buffers (rows of width d)
+    std::vector<uint64_t> cap;        // rows capacity per level
+    std::vector<uint32_t *> counts;   // [D] per-row candidate counts
+    std::vector<uint64_t *> offs;     // [D] exclusive scan of counts (+ total)
+    unsigned long long *d_total = nullptr;
+    unsigned long long *d_bytes = nullptr;
+    unsigned long long *d_mat = nullptr;  // local mode
+    uint64_t rows_total = 0, launches = 0;
+    std::vector<uint64_t> hoff;       // host copy of offsets when splitting
+
+    int init() {
+        D = plan->D;
+        int rc = dev_alloc(ctx, (void **)&dplan, sizeof(DPlan));
+        if (!rc) rc = check_hip(ctx, hipMemcpyAsync(dplan, &plan->h, sizeof(DPlan), hipMemcpyHostToDevice, ctx->stream), "plan upload");
+        if (rc) return rc;
+        rows.assign(D + 1, nullptr);
+        cap.assign(D + 1, 0);
+        counts.assign(D + 1, nullptr);
+        offs.assign(D + 1, nullptr);
+        for (uint32_t d = 1; d <= D; ++d) {
+            uint64_t c = std::max<uint64_t>(1, budget / (4ull * d));
+            if (d == 1) c = std::max<uint64_t>(c, cs->csize[plan->order[0]]);
+            cap[d] = c;
+            rc = dev_alloc(ctx, (void **)&rows[d], 4ull * d * c);
+            if (!rc && d < D) rc = dev_alloc(ctx, (void **)&counts[d], 4ull * c);
+            if (!rc && d < D) rc = dev_alloc(ctx, (void **)&offs[d], 8ull * (c + 1));
+            if (rc) return rc;
+        }
+        rc = fill_u64(ctx, ctx->d_counters, 8, 0);
+        d_total = (unsigned long long *)ctx->d_counters;
+        d_bytes = (unsigned long long *)ctx->d_counters + 1;
+        return rc;
+    }
+
+    void release() {
+        dev_free(ctx, dplan);
+        for (auto p : rows) dev_free(ctx, p);
+        for (auto p : counts) dev_free(ctx, p);
+        for (auto p : offs) dev_free(ctx, p);
+    }
+
+    int leaf(uint32_t *r, uint64_t n) {
+        ++launches;
+        KTimer kt(ctx, d_mat ? "leaf_local" : "leaf_count");
+        if (d_mat)
+            hipLaunchKernelGGL((k_leaf_local<kG>), dim3(groups_grid(n, kG)), dim3(256), 0, ctx->stream, dplan, r, n,
+                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_mat, d_total);
+        else
+            hipLaunchKernelGGL((k_leaf<kG>), dim3(groups_grid(n, kG)), dim3(256), 0, ctx->stream, dplan, r, n,
+                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes);
+        kt.stop();
+        return check_hip(ctx, hipGetLastError(), "leaf kernel");
+    }
+
+    // rows r[0, n) at depth d (width d)
+    int process(uint32_t d, uint32_t *r, uint64_t n) {
+        if (n == 0) return ASDM_OK;
+        rows_total += n;
+        if (d == D) return leaf(r, n);
+        ++launches;
+        {
+            KTimer kt(ctx, "expand_count");
+            hipLaunchKernelGGL((k_expand<kG, false>), dim3(groups_grid(n, kG)), dim3(256), 0, ctx->stream, dplan, d, r, n,
+                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, counts[d], nullptr, 0, nullptr, d_bytes);
+            kt.stop();
+        }
+        int rc = check_hip(ctx, hipGetLastError(), "expand count");
+        if (!rc) rc = scan_u32_to_u64(ctx, counts[d], offs[d], n);
+        uint64_t total = 0;
+        if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&total, offs[d] + n, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (rc || total == 0) return rc;
+        const uint64_t c = cap[d + 1];
+        if (total <= c) return fill_and_recurse(d, r, 0, n, 0, total);
+        // split the rows so that each chunk's output fits the next level
+        std::vector<uint64_t> h(n + 1);
+        if ((rc = check_hip(ctx, hipMemcpy(h.data(), offs[d], 8 * (n + 1), hipMemcpyDeviceToHost), "D2H offsets"))) return rc;
+        uint64_t i0 = 0;
+        while (i0 < n) {
+            // largest i1 with h[i1] - h[i0] <= c
+            uint64_t i1 = (uint64_t)(std::upper_bound(h.begin() + i0, h.begin() + n + 1, h[i0] + c) - h.begin()) - 1;
+            if (i1 == i0) return ctx->fail(ASDM_E_BUDGET, "one partial embedding expands beyond the frontier budget");
+            if ((rc = fill_and_recurse(d, r, i0, i1, h[i0], h[i1] - h[i0]))) return rc;
+            i0 = i1;
+        }
+        return ASDM_OK;
+    }
+
+    int fill_and_recurse(uint32_t d, uint32_t *r, uint64_t i0, uint64_t i1, uint64_t out_base, uint64_t nout) {
+        ++launches;
+        {
+            KTimer kt(ctx, "expand_fill");
+            hipLaunchKernelGGL((k_expand<kG, true>), dim3(groups_grid(i1 - i0, kG)), dim3(256), 0, ctx->stream, dplan, d,
+                               r + i0 * d, i1 - i0, cs->d_ce_off, cs->d_ce_val, cs->d_cand, nullptr, offs[d] + i0,
+                               out_base, rows[d + 1], d_bytes);
+            kt.stop();
+        }
+        int rc = check_hip(ctx, hipGetLastError(), "expand fill");
+        if (rc) return rc;
+        return process(d + 1, rows[d + 1], nout);
+    }
+
+    int run() {
+        const uint32_t c0 = cs->csize[plan->order[0]];
+        hipLaunchKernelGGL(k_iota, dim3(groups_grid(c0, 1)), dim3(256), 0, ctx->stream, rows[1], (uint64_t)c0);
+        int rc = check_hip(ctx, hipGetLastError(), "iota");
+        if (rc) return rc;
+        return process(1, rows[1], c0);
+    }
+};
+
+uint64_t default_budget(asdm_ctx *ctx) {
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    uint64_t b = freeb / 16;
+    return std::max<uint64_t>(64ull << 20, std::min<uint64_t>(b, 4ull << 30));
+}
+
+int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget, uint64_t *mat_host,
+               uint64_t *count) {
+    if (!ctx || !plan || !cs || !count) return ASDM_E_INVALID;
+    if (plan->h.nq != cs->nq) return ctx->fail(ASDM_E_INVALID, "plan and candidate space disagree");
+    if (mat_host && !plan->local) return ctx->fail(ASDM_E_INVALID, "local counting needs a plan made with ASDM_PLAN_LOCAL");
+    (void)hipSetDevice(ctx->device);
+    auto t0 = std::chrono::steady_clock::now();
+    Engine E{ctx, plan, cs};
+    E.budget = budget ? std::max<uint64_t>(budget, 1ull << 12) : default_budget(ctx);
+    int rc = E.init();
+    uint64_t matn = (uint64_t)cs->nv * cs->nq;
+    if (!rc && mat_host) {
+        rc = dev_alloc(ctx, (void **)&E.d_mat, 8ull * std::max<uint64_t>(matn, 1));
+        if (!rc) rc = fill_u64(ctx, (uint64_t *)E.d_mat, matn, 0);
+    }
+    hipEvent_t ea, eb;
+    (void)hipEventCreate(&ea);
+    (void)hipEventCreate(&eb);
+    (void)hipEventRecord(ea, ctx->stream);
+    if (!rc) rc = E.run();
+    (void)hipEventRecord(eb, ctx->stream);
+    uint64_t res[2] = {0, 0};
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(res, ctx->d_counters, 16, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc && mat_host) rc = check_hip(ctx, hipMemcpyAsync(mat_host, E.d_mat, 8 * matn, hipMemcpyDeviceToHost, ctx->stream), "D2H matrix");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    float kms = 0;
+    (void)hipEventElapsedTime(&kms, ea, eb);
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (E.d_mat) dev_free(ctx, E.d_mat);
+    E.release();
+    (void)hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    *count = res[0];
+    ctx->stats.exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats.kernel_ms = kms;
+    ctx->stats.alg_bytes = res[1];
+    ctx->stats.rows_total = E.rows_total;
+    ctx->stats.launches = E.launches;
+    ctx->stats.plan_ms = plan->plan_ms;
+    ctx->stats.filter_ms = cs->filter_ms;
+    return ASDM_OK;
+}
+
+}  // namespace
+}  // namespace asdm
+
+using namespace asdm;
+
+extern "C" {
+
+int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint64_t *count) {
+    return run_engine(ctx, plan, cs, budget_bytes, nullptr, count);
+}
+
+int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint64_t *mat,
+                     uint64_t *count) {
+    if (!mat) return ASDM_E_INVALID;
+    return run_engine(ctx, plan, cs, budget_bytes, mat, count);
+}
+
+}  // extern "C"

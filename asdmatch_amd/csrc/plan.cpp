@@ -1,0 +1,332 @@
+// Host planner: CFL root selection and BFS order for the filter, and the matching plan (order,
+// leaf set, device plan tables) for the enumeration engine.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <queue>
+
+#include "engine.h"
+
+namespace asdm {
+
+// selectCFLRoot (relation/candidate_space.cpp:723-763): keep the three query vertices of smallest
+// rank |V_L(u)| / deg(u) in a std::priority_queue (so equal ranks leave in the same order as in the
+// reference), then pick the smallest |B(u)| / deg(u), first one on ties.
+uint32_t cfl_root(const asdm_graph *q, const asdm_graph *g, const std::vector<uint32_t> &base_count) {
+    auto cmp = [](const std::pair<uint32_t, double> &l, const std::pair<uint32_t, double> &r) { return l.second < r.second; };
+    std::priority_queue<std::pair<uint32_t, double>, std::vector<std::pair<uint32_t, double>>, decltype(cmp)> pq(cmp);
+    for (uint32_t u = 0; u < q->n; ++u)
+        pq.emplace(u, (double)g->label_count(q->lab[u]) / (double)q->deg(u));
+    while (pq.size() > 3) pq.pop();
+    uint32_t root = 0;
+    double best = (double)g->n;
+    while (!pq.empty()) {
+        uint32_t u = pq.top().first;
+        double score = (double)base_count[u] / (double)q->deg(u);
+        if (score < best) {
+            root = u;
+            best = score;
+        }
+        pq.pop();
+    }
+    return root;
+}
+
+// buildCFLBFSTree's vertex order (relation/candidate_space.cpp:765-816): BFS from the root with
+// neighbours in query adjacency order.  Vertices unreachable from the root are not included.
+std::vector<uint32_t> bfs_order(const asdm_graph *q, uint32_t root) {
+    std::vector<uint32_t> order{root};
+    std::vector<char> seen(q->n, 0);
+    seen[root] = 1;
+    for (size_t h = 0; h < order.size(); ++h) {
+        uint32_t u = order[h];
+        for (uint32_t a = q->off[u]; a < q->off[u + 1]; ++a) {
+            uint32_t w = q->nbr[a];
+            if (!seen[w]) {
+                seen[w] = 1;
+                order.push_back(w);
+            }
+        }
+    }
+    return order;
+}
+
+namespace {
+
+bool adjacent(const asdm_cs *cs, uint32_t u, uint32_t w) { return cs->arc_index(u, w) != UINT32_MAX; }
+
+// Is the subgraph induced by `keep` (bitmask) connected?
+bool connected(const asdm_cs *cs, uint64_t keep) {
+    if (!keep) return false;
+    uint32_t start = (uint32_t)__builtin_ctzll(keep);
+    uint64_t seen = 1ull << start, frontier = seen;
+    while (frontier) {
+        uint64_t next = 0;
+        for (uint64_t f = frontier; f; f &= f - 1) {
+            uint32_t u = (uint32_t)__builtin_ctzll(f);
+            for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) {
+                uint32_t w = cs->qnbr[a];
+                if ((keep >> w & 1) && !(seen >> w & 1)) next |= 1ull << w;
+            }
+        }
+        seen |= next;
+        frontier = next;
+    }
+    return seen == keep;
+}
+
+// Expected list length when extending to u from matched neighbour p: W(p -> u) / |C(p)|.
+double fanout(const asdm_cs *cs, uint32_t p, uint32_t u) {
+    uint32_t a = cs->arc_index(p, u);
+    return (double)cs->arc_weight[a] / std::max<double>(1.0, cs->csize[p]);
+}
+
+// Estimated number of partial embeddings when u joins the matched set S (independence model):
+// |C(u)| * prod over matched neighbours p of W(p->u) / (|C(p)| |C(u)|).
+double growth(const asdm_cs *cs, uint64_t S, uint32_t u) {
+    double g = cs->csize[u];
+    for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) {
+        uint32_t p = cs->qnbr[a];
+        if (S >> p & 1) g *= (double)cs->arc_weight[cs->arc_index(p, u)] / std::max(1.0, (double)cs->csize[p] * cs->csize[u]);
+    }
+    return g;
+}
+
+void partitions(uint32_t k, std::vector<std::vector<uint32_t>> &out, std::vector<int64_t> &coef) {
+    // restricted growth strings over k elements
+    std::vector<uint32_t> a(k, 0);
+    std::function<void(uint32_t, uint32_t)> rec = [&](uint32_t i, uint32_t m) {
+        if (i == k) {
+            std::vector<uint32_t> blocks(m, 0);
+            for (uint32_t j = 0; j < k; ++j) blocks[a[j]] |= 1u << j;
+            int64_t c = 1;
+            for (uint32_t b : blocks) {
+                int s = __builtin_popcount(b);
+                int64_t f = 1;
+                for (int t = 2; t < s; ++t) f *= t;  // (s-1)!
+                c *= (s % 2 == 1 ? 1 : -1) * f;      // (-1)^(s-1) (s-1)!
+            }
+            out.push_back(blocks);
+            coef.push_back(c);
+            return;
+        }
+        for (uint32_t v = 0; v <= m; ++v) {
+            a[i] = v;
+            rec(i + 1, std::max(m, v + 1));
+        }
+    };
+    rec(0, 0);
+}
+
+}  // namespace
+
+int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_order, uint32_t flags,
+               const uint64_t *masks, const double *cards, uint32_t ncards, asdm_plan *out, std::string &err) {
+    const uint32_t nq = cs->nq;
+    const bool local = flags & ASDM_PLAN_LOCAL;
+    const uint32_t max_group = local ? 1 : kMaxGroup;
+    std::vector<uint32_t> order;
+    uint32_t D = 0;
+
+    // optional cardinalities keyed by subset mask (subsetToCard analogue)
+    auto card_of = [&](uint64_t mask, double fallback) {
+        for (uint32_t i = 0; i < ncards; ++i)
+            if (masks[i] == mask) return cards[i];
+        return fallback;
+    };
+
+    if (given_order) {
+        order.assign(given_order, given_order + nq);
+        std::vector<char> seen(nq, 0);
+        for (uint32_t u : order) {
+            if (u >= nq || seen[u]) {
+                err = "order is not a permutation of the query vertices";
+                return ASDM_E_INVALID;
+            }
+            seen[u] = 1;
+        }
+        // leaves: the longest suffix that is an independent set with all neighbours before it,
+        // within the label-group limits
+        D = nq;
+        while (D > 1) {
+            uint32_t u = order[D - 1];
+            bool ok = true;
+            for (uint32_t j = D; j < nq && ok; ++j) ok = !adjacent(cs, u, order[j]);
+            uint32_t same = 1;
+            for (uint32_t j = D; j < nq; ++j) same += cs->qlab[order[j]] == cs->qlab[u];
+            if (!ok || same > max_group) break;
+            --D;
+        }
+        if (D == nq) D = nq - 1 > 0 ? nq - 1 : 1;
+        if (nq == 1) D = 1;
+    } else {
+        // 1) leaves: an independent set whose removal keeps the rest connected, preferring large
+        //    expected fan-out (cheap to count, expensive to enumerate)
+        uint64_t all = nq == 64 ? ~0ull : ((1ull << nq) - 1);
+        uint64_t leaves = 0;
+        std::vector<std::pair<double, uint32_t>> cand;
+        for (uint32_t u = 0; u < nq; ++u) {
+            double f = 1e300;
+            for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) f = std::min(f, fanout(cs, cs->qnbr[a], u));
+            cand.emplace_back(-f, u);
+        }
+        std::sort(cand.begin(), cand.end());
+        if (nq > 1)
+            for (auto &c : cand) {
+                uint32_t u = c.second;
+                bool indep = true;
+                for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) indep &= !(leaves >> cs->qnbr[a] & 1);
+                if (!indep) continue;
+                uint32_t same = 1;
+                for (uint64_t l = leaves; l; l &= l - 1) same += cs->qlab[__builtin_ctzll(l)] == cs->qlab[u];
+                if (same > max_group) continue;
+                uint64_t rest = all & ~(leaves | (1ull << u));
+                if (!connected(cs, rest)) continue;
+                leaves |= 1ull << u;
+            }
+        // 2) prefix order over the rest: smallest candidate set first, then greedy smallest
+        //    estimated growth among vertices adjacent to the matched set
+        uint64_t rest = all & ~leaves;
+        uint32_t first = UINT32_MAX;
+        for (uint64_t r = rest; r; r &= r - 1) {
+            uint32_t u = (uint32_t)__builtin_ctzll(r);
+            if (first == UINT32_MAX || cs->csize[u] < cs->csize[first]) first = u;
+        }
+        order.push_back(first);
+        uint64_t S = 1ull << first;
+        double est = cs->csize[first];
+        while (S != rest) {
+            uint32_t best = UINT32_MAX;
+            double bestg = 0;
+            for (uint64_t r = rest & ~S; r; r &= r - 1) {
+                uint32_t u = (uint32_t)__builtin_ctzll(r);
+                bool conn = false;
+                for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) conn |= (S >> cs->qnbr[a] & 1) != 0;
+                if (!conn) continue;
+                double gr = card_of(S | (1ull << u), est * growth(cs, S, u));
+                if (best == UINT32_MAX || gr < bestg) {
+                    best = u;
+                    bestg = gr;
+                }
+            }
+            order.push_back(best);
+            S |= 1ull << best;
+            est = bestg;
+        }
+        D = (uint32_t)order.size();
+        for (uint64_t l = leaves; l; l &= l - 1) order.push_back((uint32_t)__builtin_ctzll(l));
+    }
+
+    // check connectivity of the enumerated prefix (every depth >= 1 needs a backward neighbour)
+    for (uint32_t d = 1; d < nq; ++d) {
+        bool has = false;
+        for (uint32_t j = 0; j < std::min(d, D); ++j) has |= adjacent(cs, order[j], order[d]);
+        if (!has) {
+            err = "matching order is not connected (depth " + std::to_string(d) + " has no earlier neighbour)";
+            return ASDM_E_UNSUPPORTED;
+        }
+    }
+
+    DPlan &P = out->h;
+    std::memset(&P, 0, sizeof(P));
+    P.nq = nq;
+    P.D = D;
+    for (uint32_t d = 0; d < nq; ++d) {
+        uint32_t u = order[d];
+        P.u[d] = u;
+        P.coff[d] = cs->cand_off[u];
+        P.csz[d] = cs->csize[u];
+        for (uint32_t j = 0; j < std::min(d, D); ++j) {
+            uint32_t a = cs->arc_index(order[j], u);
+            if (a != UINT32_MAX) {
+                P.bdep[d][P.nb[d]] = j;
+                P.bbase[d][P.nb[d]] = cs->ce_base[a];
+                P.nb[d]++;
+            }
+            if (cs->qlab[order[j]] == cs->qlab[u]) P.sdep[d][P.ns[d]++] = j;
+        }
+    }
+    // leaf groups by label
+    std::vector<char> done(nq, 0);
+    for (uint32_t d = D; d < nq; ++d) {
+        if (done[d]) continue;
+        uint32_t g = P.ngroups++;
+        for (uint32_t e = d; e < nq; ++e)
+            if (!done[e] && cs->qlab[order[e]] == cs->qlab[order[d]]) {
+                if (P.gsize[g] >= kMaxGroup) {
+                    err = "leaf label group larger than supported";
+                    return ASDM_E_UNSUPPORTED;
+                }
+                P.gleaf[g][P.gsize[g]++] = e;
+                done[e] = 1;
+            }
+    }
+    for (uint32_t k = 2; k <= kMaxGroup; ++k) {
+        std::vector<std::vector<uint32_t>> parts;
+        std::vector<int64_t> coef;
+        partitions(k, parts, coef);
+        P.nparts[k] = (uint32_t)parts.size();
+        for (size_t i = 0; i < parts.size(); ++i) {
+            P.pnblk[k][i] = (uint32_t)parts[i].size();
+            for (size_t b = 0; b < parts[i].size(); ++b) P.pblk[k][i][b] = parts[i][b];
+            P.pcoef[k][i] = coef[i];
+        }
+    }
+    out->order = order;
+    out->D = D;
+    out->local = local;
+    return ASDM_OK;
+}
+
+}  // namespace asdm
+
+using namespace asdm;
+
+extern "C" {
+
+int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const uint64_t *masks, const double *cards,
+                                uint32_t ncards, uint32_t flags, asdm_plan **out) {
+    if (!q || !cs || !out || q->n != cs->nq) return ASDM_E_INVALID;
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto p = std::make_unique<asdm_plan>();
+    std::string err;
+    int rc = build_plan(q, cs, nullptr, flags, masks, cards, ncards, p.get(), err);
+    if (rc) {
+        cs->ctx->err = err;
+        return rc;
+    }
+    p->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = p.release();
+    return ASDM_OK;
+}
+
+int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, asdm_plan **out) {
+    if (!q || !cs || !out || q->n != cs->nq) return ASDM_E_INVALID;
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    auto p = std::make_unique<asdm_plan>();
+    std::string err;
+    int rc = build_plan(q, cs, order, flags, nullptr, nullptr, 0, p.get(), err);
+    if (rc) {
+        cs->ctx->err = err;
+        return rc;
+    }
+    p->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = p.release();
+    return ASDM_OK;
+}
+
+void asdm_plan_free(asdm_plan *p) { delete p; }
+
+int asdm_plan_order(const asdm_plan *p, uint32_t *order, uint32_t *num_enumerated) {
+    if (!p) return ASDM_E_INVALID;
+    if (order) std::memcpy(order, p->order.data(), sizeof(uint32_t) * p->order.size());
+    if (num_enumerated) *num_enumerated = p->D;
+    return ASDM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+/*
+ * asdm.h -- C ABI of the MI355X (gfx950) ASDMatch matching engine (libasdm.so).
+ *
+ * Plain C types only: opaque handles, pointers and sizes.  Every entry point returns an int
+ * status (ASDM_OK = 0, negative on error); the message of the last error on a context is
+ * available from asdm_last_error().  A context owns one HIP stream and is used by one host thread
+ * at a time; distinct contexts may be used concurrently.
+ *
+ * Reference interfaces replaced (file:line in the reference repository):
+ *   asdm_graph_load_text        Graph::loadGraphFromTextFile        relation/graph.cpp:68-148
+ *   asdm_graph_from_edges       same layout, built from in-memory v/e records
+ *   asdm_graph_write_text       Graph::writeToTextFile              relation/graph.cpp:451-471
+ *   asdm_filter_cfl             CandidateSpace::buildCandCFL        relation/candidate_space.cpp:550-721
+ *                               (+ buildCandidateEdge(false)        relation/candidate_space.cpp:458-487)
+ *   asdm_cs_candidates          CandidateSpace::candidateSet        relation/candidate_space.h:62
+ *   asdm_cs_arc_weight          setQueryGraphWeights' edge weight   relation/candidate_space.cpp:507-524
+ *   asdm_plan_create            simplePlan / optCostPlan            plan/optimizer.cpp:7-200, :380-548
+ *   asdm_plan_set_cardinalities subsetToCard                        plan/estimator.cpp:7
+ *   asdm_count                  adaptiveShareJoin (count build)     matching/adaptive.cpp:1179-2279
+ *   asdm_local_count            adaptiveShareJoin + produceResult   matching/join.cpp:1016-1041,
+ *                               (ALL_LEVEL + LOCAL_COUNT build)     main.cpp:105-107,135-146
+ *   asdm_intersect_k            ComputeSetIntersection::LeapfrogJoin utility/compute_set_intersection.cpp:193-233
+ *   asdm_batch_count            (no counterpart: many queries per launch; see asdm_batch_count)
+ */
+#ifndef ASDM_H
+#define ASDM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define ASDM_OK 0
+#define ASDM_E_EMPTY_CANDIDATES 1 /* some C(u) is empty: count is 0; the CLI prints nothing */
+#define ASDM_E_INVALID (-1)       /* bad argument / shape */
+#define ASDM_E_IO (-2)            /* file cannot be opened or is malformed */
+#define ASDM_E_OOM (-3)           /* device or host allocation failed */
+#define ASDM_E_HIP (-4)           /* HIP runtime error (message has the HIP error string) */
+#define ASDM_E_NODEVICE (-5)      /* no usable gfx950 device */
+#define ASDM_E_BUDGET (-6)        /* frontier budget too small for one expansion */
+#define ASDM_E_UNSUPPORTED (-7)   /* query outside the supported envelope (e.g. > 32 vertices) */
+
+typedef struct asdm_ctx asdm_ctx;     /* device, stream, scratch arena */
+typedef struct asdm_graph asdm_graph; /* host graph (label-sorted CSR), optionally on device */
+typedef struct asdm_cs asdm_cs;       /* device candidate space of (query, data) */
+typedef struct asdm_plan asdm_plan;   /* host matching plan consumed by the engine */
+
+/* ---- context ---- */
+int asdm_ctx_create(int device, asdm_ctx **out);
+void asdm_ctx_destroy(asdm_ctx *ctx);
+const char *asdm_last_error(const asdm_ctx *ctx);
+/* Library version string, no device needed. */
+const char *asdm_version(void);
+
+/* ---- graphs (host side; no device needed) ---- */
+/* Reproduces the reference loader: header 't N M' (M undirected), 'v id label deg' records
+ * fixing offsets from declared degrees, 'e a b [extra tokens]' inserted in both directions,
+ * adjacency sorted by (label(nbr), nbr).  Declared degrees must equal the edge records. */
+int asdm_graph_load_text(const char *path, asdm_graph **out);
+/* Graph from n labels and ne undirected edges (edges[2*i], edges[2*i+1]), in record order. */
+int asdm_graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges,
+                          asdm_graph **out);
+int asdm_graph_write_text(const asdm_graph *g, const char *path);
+void asdm_graph_free(asdm_graph *g);
+/* n vertices, m arcs (= 2 x undirected edges), nlabels = max label + 1. */
+int asdm_graph_info(const asdm_graph *g, uint32_t *n, uint64_t *m, uint32_t *nlabels);
+/* Copies the label-sorted CSR out: off[n+1], nbr[m], lab[n] (any may be NULL). */
+int asdm_graph_csr(const asdm_graph *g, uint32_t *off, uint32_t *nbr, uint32_t *lab);
+/* Undirected edge list (a < b per edge, ascending (a, b)): edges[2*ne]. */
+int asdm_graph_edges(const asdm_graph *g, uint32_t *edges, uint64_t cap, uint64_t *ne);
+/* Puts CSR, labels, per-vertex label offsets and the vertices-by-label index into HBM. */
+int asdm_graph_upload(asdm_ctx *ctx, asdm_graph *g);
+
+/* ---- seeded synthetic inputs (portable xoshiro256** streams; no device needed) ---- */
+/* Chung-Lu power law: weights w_i ~ (i+1)^(-1/(beta-1)), exactly ne distinct undirected edges,
+ * no self-loops, vertex ids randomly permuted; labels iid Zipf(zipf_s) over nlabels (s = 0 gives
+ * uniform labels). */
+int asdm_gen_chung_lu(uint32_t n, uint64_t ne, double beta, uint32_t nlabels, double zipf_s,
+                      uint64_t seed, asdm_graph **out);
+/* R-MAT (a, b, c, d = 1-a-b-c), 2^scale vertices, edge_factor * 2^scale draws, symmetrized,
+ * deduplicated, self-loops dropped, ids permuted; labels as above. */
+int asdm_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint32_t nlabels,
+                  double zipf_s, uint64_t seed, asdm_graph **out);
+/* Query of k distinct vertices taken by a random walk on g (labels copied).  Kept edges: the
+ * walk's discovery tree plus a fraction `density` in [0,1] of the remaining induced edges
+ * (density 1 = all induced edges), but at most max_extra extra edges (UINT32_MAX = no cap).
+ * Returns ASDM_E_INVALID if no walk reaches k vertices. */
+int asdm_gen_query(const asdm_graph *g, uint32_t k, double density, uint32_t max_extra, uint64_t seed,
+                   asdm_graph **out);
+
+/* ---- filter (CFL) and candidate space ---- */
+/* Builds C(u) for every query vertex with the CFL rules of the reference (same root, same
+ * candidate sets) and the candidate-edge index, all in HBM.  Returns ASDM_E_EMPTY_CANDIDATES
+ * (and *out = NULL) when some C(u) is empty.  g must have been uploaded. */
+int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asdm_cs **out);
+void asdm_cs_free(asdm_cs *cs);
+uint32_t asdm_cs_root(const asdm_cs *cs);
+/* |C(u)| for every u (sizes[nq]). */
+int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes);
+/* Copies C(u) (ascending data ids) into out[cap]; *n = |C(u)|. */
+int asdm_cs_candidates(const asdm_cs *cs, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *n);
+/* Sum over v in C(u) of the candidate-edge list length for query arc a (query CSR order). */
+int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
+
+/* ---- plan ---- */
+/* order: NULL to let the planner choose, else a connected permutation of the query vertices.
+ * flags: ASDM_PLAN_LOCAL restricts leaf factorization to what local counting supports. */
+#define ASDM_PLAN_LOCAL 1u
+int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
+                     asdm_plan **out);
+void asdm_plan_free(asdm_plan *p);
+/* order[nq] and the number of enumerated (non-leaf) depths. */
+int asdm_plan_order(const asdm_plan *p, uint32_t *order, uint32_t *num_enumerated);
+/* Planner cardinalities keyed by query-vertex subset bitmask (bit u = query vertex u), the
+ * subsetToCard analogue.  Used to pick the order when asdm_plan_create is given order == NULL
+ * after this call on the same (q, cs) pair via asdm_plan_create_with_cards. */
+int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const uint64_t *masks,
+                                const double *cards, uint32_t ncards, uint32_t flags, asdm_plan **out);
+
+/* ---- execution ---- */
+/* Exact embedding count mod 2^64.  budget_bytes bounds the frontier memory per level (0 = a
+ * default sized for the device); any budget gives the same count. */
+int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes,
+               uint64_t *count);
+/* LOCAL_COUNT: mat[v * nq + u] = number of embeddings f with f(u) = v, row-major |Vg| x |Vq|,
+ * caller-owned host memory; *count = total.  The plan must be created with ASDM_PLAN_LOCAL. */
+int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes,
+                     uint64_t *mat, uint64_t *count);
+
+/* Per-kernel device timing (HIP events on the context stream), collected while profiling is on.
+ * Turning profiling on or off clears the table.  No reference counterpart (measurement only). */
+int asdm_ctx_set_profiling(asdm_ctx *ctx, int on);
+/* Entry i of the kernel table: name, summed device ms, launches, algorithmic bytes. */
+int asdm_kernel_stat(const asdm_ctx *ctx, uint32_t i, char *name, uint32_t name_cap, double *ms,
+                     uint64_t *launches, uint64_t *bytes);
+
+/* Statistics of the last asdm_count / asdm_local_count on this context. */
+typedef struct {
+    double filter_ms, plan_ms, exec_ms;   /* host wall clock of each phase */
+    double kernel_ms;                     /* sum of device time of the enumeration kernels */
+    uint64_t alg_bytes;                   /* algorithmic bytes: list reads + frontier traffic */
+    uint64_t rows_total;                  /* partial embeddings materialized */
+    uint64_t launches;                    /* kernel launches in the enumeration */
+} asdm_stats;
+int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out);
+
+/* ---- batched multi-query engine ---- */
+/* Counts nq queries against one uploaded data graph.  counts[nq] receive the exact counts;
+ * lat_ms[nq] (optional) the completion time of each query measured from the call start. */
+int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
+                     uint64_t budget_bytes, uint64_t *counts, float *lat_ms);
+
+/* ---- primitives (tests, autotuner) ---- */
+/* k-way intersection of ascending unique device-independent host lists, computed on the GPU:
+ * out[cap] receives the ascending intersection, *n its size. */
+int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k,
+                     uint32_t *out, uint32_t cap, uint32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASDM_H */
