@@ -395,17 +395,26 @@ struct Engine {
         counts.assign(D + 1, nullptr);
         offs.assign(D + 1, nullptr);
         for (uint32_t d = 1; d <= D; ++d) {
-            uint64_t c = std::max<uint64_t>(1, budget / (4ull * d));
+            uint64_t c = std::max<uint64_t>(1, std::min<uint64_t>(budget / (4ull * d), 1ull << 28));
             if (d == 1) c = std::max<uint64_t>(c, cs->csize[plan->order[0]]);
             cap[d] = c;
-            rc = dev_alloc(ctx, (void **)&rows[d], 4ull * d * c);
-            if (!rc && d < D) rc = dev_alloc(ctx, (void **)&counts[d], 4ull * c);
-            if (!rc && d < D) rc = dev_alloc(ctx, (void **)&offs[d], 8ull * (c + 1));
-            if (rc) return rc;
         }
+        if ((rc = ensure_rows(1))) return rc;
         rc = fill_u64(ctx, ctx->d_counters, 8, 0);
         d_total = (unsigned long long *)ctx->d_counters;
         d_bytes = (unsigned long long *)ctx->d_counters + 1;
+        return rc;
+    }
+
+    // Level buffers are allocated on first use, at the level's full capacity.
+    int ensure_rows(uint32_t d) {
+        if (rows[d]) return ASDM_OK;
+        return dev_alloc(ctx, (void **)&rows[d], 4ull * d * cap[d]);
+    }
+    int ensure_scan(uint32_t d) {
+        if (counts[d]) return ASDM_OK;
+        int rc = dev_alloc(ctx, (void **)&counts[d], 4ull * cap[d]);
+        if (!rc) rc = dev_alloc(ctx, (void **)&offs[d], 8ull * (cap[d] + 1));
         return rc;
     }
 
@@ -434,6 +443,8 @@ struct Engine {
         if (n == 0) return ASDM_OK;
         rows_total += n;
         if (d == D) return leaf(r, n);
+        int rc = ensure_scan(d);
+        if (rc) return rc;
         ++launches;
         {
             KTimer kt(ctx, "expand_count");
@@ -441,7 +452,7 @@ struct Engine {
                                cs->d_ce_off, cs->d_ce_val, cs->d_cand, counts[d], nullptr, 0, nullptr, d_bytes);
             kt.stop();
         }
-        int rc = check_hip(ctx, hipGetLastError(), "expand count");
+        rc = check_hip(ctx, hipGetLastError(), "expand count");
         if (!rc) rc = scan_u32_to_u64(ctx, counts[d], offs[d], n);
         uint64_t total = 0;
         if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&total, offs[d] + n, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
@@ -456,7 +467,20 @@ struct Engine {
         while (i0 < n) {
             // largest i1 with h[i1] - h[i0] <= c
             uint64_t i1 = (uint64_t)(std::upper_bound(h.begin() + i0, h.begin() + n + 1, h[i0] + c) - h.begin()) - 1;
-            if (i1 == i0) return ctx->fail(ASDM_E_BUDGET, "one partial embedding expands beyond the frontier budget");
+            if (i1 == i0) {
+                // a single row has more children than the level holds: grow that level
+                i1 = i0 + 1;
+                const uint64_t need = h[i1] - h[i0];
+                dev_free(ctx, rows[d + 1]);
+                rows[d + 1] = nullptr;
+                cap[d + 1] = need;
+                if (counts[d + 1]) {
+                    dev_free(ctx, counts[d + 1]);
+                    dev_free(ctx, offs[d + 1]);
+                    counts[d + 1] = nullptr;
+                    offs[d + 1] = nullptr;
+                }
+            }
             if ((rc = fill_and_recurse(d, r, i0, i1, h[i0], h[i1] - h[i0]))) return rc;
             i0 = i1;
         }
@@ -464,6 +488,8 @@ struct Engine {
     }
 
     int fill_and_recurse(uint32_t d, uint32_t *r, uint64_t i0, uint64_t i1, uint64_t out_base, uint64_t nout) {
+        int rc = ensure_rows(d + 1);
+        if (rc) return rc;
         ++launches;
         {
             KTimer kt(ctx, "expand_fill");
@@ -472,7 +498,7 @@ struct Engine {
                                out_base, rows[d + 1], d_bytes);
             kt.stop();
         }
-        int rc = check_hip(ctx, hipGetLastError(), "expand fill");
+        rc = check_hip(ctx, hipGetLastError(), "expand fill");
         if (rc) return rc;
         return process(d + 1, rows[d + 1], nout);
     }
@@ -489,8 +515,8 @@ struct Engine {
 uint64_t default_budget(asdm_ctx *ctx) {
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
-    uint64_t b = freeb / 16;
-    return std::max<uint64_t>(64ull << 20, std::min<uint64_t>(b, 4ull << 30));
+    uint64_t b = freeb / 32;
+    return std::max<uint64_t>(64ull << 20, std::min<uint64_t>(b, 1ull << 30));
 }
 
 int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget, uint64_t *mat_host,

@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ by running the reference built from its own
+sources (oracle/ref.mk -> oracle/_ref/).  Run in the build container, where /root/reference exists:
+
+    make -f oracle/ref.mk -j8 && python3 tests/golden/make_golden.py
+
+Fixtures (data only: inputs and the reference's outputs):
+  yeast/yeast.graph            the data graph shipped in the reference's dataset/yeast.tar.gz
+  yeast/queries/*.graph        a subset of its shipped query graphs
+  yeast/golden.json            per query: the reference's "Number of matches" from its
+                               ALL_LEVEL + LOCAL_COUNT build (oracle/_ref/ref_driver_local), the
+                               count-build output (oracle/_ref/ref_driver) where it finishes, and
+                               the reference's CFL root-independent candidate sets
+                               (oracle/_ref/ref_cs_dump) for a subset
+  synth.json                   seeded synthetic cases (generator parameters + sha256 of the graph
+                               text the generator writes) with the reference's counts and candidate
+                               set sizes; local-count matrices as .npz
+Why ALL_LEVEL: the count build's factorized path returns counts that disagree with exhaustive
+enumeration on multi-bag queries and vary between runs of the same input; the
+ALL_LEVEL build enumerates every embedding (matching/join.cpp:819-907,1016-1097) and is the
+reference's own exact path.
+"""
+import hashlib
+import io
+import json
+import os
+import subprocess
+import sys
+import tarfile
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+sys.path.insert(0, ROOT)
+import asdmatch_amd as A  # noqa: E402  (host-side generators and writer only)
+
+REF = "/root/reference"
+REFBIN = os.path.join(ROOT, "oracle", "_ref")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_ref(binary, q, d, extra=(), timeout=60):
+    try:
+        r = subprocess.run([os.path.join(REFBIN, binary), "-q", q, "-d", d, *extra], capture_output=True, text=True,
+                           timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return None, "timeout"
+    if r.returncode != 0:
+        return None, f"rc={r.returncode}"
+    for line in r.stdout.splitlines():
+        if line.startswith("Number of matches:"):
+            return int(line.split(":")[1]), "ok"
+    return None, "empty"  # the reference prints nothing when a candidate set is empty
+
+
+def cs_dump(q, d):
+    r = subprocess.run([os.path.join(REFBIN, "ref_cs_dump"), q, d], capture_output=True, text=True, timeout=120,
+                       check=True)
+    lines = r.stdout.splitlines()
+    if lines and lines[0] == "EMPTY":
+        return None
+    cands, weights = {}, []
+    for ln in lines:
+        p = ln.split()
+        if p[0] == "C":
+            cands[int(p[1])] = [int(x) for x in p[3:]]
+        elif p[0] == "W":
+            weights.append([int(p[1]), int(p[2]), int(p[3])])
+    return {"cand": [cands[u] for u in sorted(cands)], "arc_weight": weights}
+
+
+def sha256_file(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()
+
+
+def yeast():
+    ydir = os.path.join(OUT, "yeast")
+    qdir = os.path.join(ydir, "queries")
+    os.makedirs(qdir, exist_ok=True)
+    with tarfile.open(os.path.join(REF, "dataset", "yeast.tar.gz")) as tf:
+        data = tf.extractfile("yeast/data_graph/yeast.graph").read()
+        names = sorted(m.name for m in tf.getmembers() if m.name.startswith("yeast/query_graph/") and
+                       m.name.endswith(".graph"))
+        with open(os.path.join(ydir, "yeast.graph"), "wb") as f:
+            f.write(data)
+        qtext = {os.path.basename(n)[:-6]: tf.extractfile(n).read() for n in names}
+    dpath = os.path.join(ydir, "yeast.graph")
+    # 30 queries of each size: the first ones in name order whose reference run is quick
+    chosen, golden = [], {}
+    per = {}
+    for nm in sorted(qtext, key=lambda s: (int(s.split("_")[1]), int(s.split("_")[2]))):
+        size = int(nm.split("_")[1])
+        if per.get(size, 0) >= 30:
+            continue
+        tmp = os.path.join("/tmp", nm + ".graph")
+        with open(tmp, "wb") as f:
+            f.write(qtext[nm])
+        cnt, st = run_ref("ref_driver_local", tmp, dpath, timeout=10)
+        if cnt is None and st != "empty":
+            continue
+        per[size] = per.get(size, 0) + 1
+        with open(os.path.join(qdir, nm + ".graph"), "wb") as f:
+            f.write(qtext[nm])
+        cbuild, cst = run_ref("ref_driver", tmp, dpath, timeout=10)
+        rec = {"count": cnt if cnt is not None else 0, "empty": st == "empty", "count_build": cbuild,
+               "count_build_status": cst}
+        if per[size] <= 8:
+            rec["cs"] = cs_dump(tmp, dpath)
+        golden[nm] = rec
+        chosen.append(nm)
+        os.remove(tmp)
+    with open(os.path.join(ydir, "golden.json"), "w") as f:
+        json.dump({"data": "yeast.graph", "data_sha256": sha256_file(dpath), "queries": golden}, f, indent=0)
+    print("yeast:", len(chosen), "queries")
+
+
+# Seeded synthetic cases (same generator calls as the tests and bench).
+SYNTH = [
+    # config 1: Chung-Lu 3,000 V / 12,500 E, beta 2.5, 70 labels Zipf(1.0), 8-vertex walk query
+    dict(name="c1", kind="chung_lu", n=3000, ne=12500, beta=2.5, nl=70, zipf=1.0, seed=0xA5D00001,
+         queries=[(8, 1.0, 0xFFFFFFFF, 0xA5D00001 + 100 + i) for i in range(8)], local=True),
+    # denser, fewer labels: exercises same-label leaf groups and larger counts
+    dict(name="pl_small_l4", kind="chung_lu", n=400, ne=2400, beta=2.5, nl=4, zipf=0.0, seed=0xA5D00011,
+         queries=[(k, d, 0xFFFFFFFF, 0xA5D00011 + 7 * k + int(10 * d)) for k in (4, 5, 6) for d in (0.0, 1.0)],
+         local=True),
+    # reduced config 3: power-law, 16 uniform labels, local counts
+    dict(name="c3_mini", kind="chung_lu", n=2000, ne=16000, beta=2.5, nl=16, zipf=0.0, seed=0xA5D00003,
+         queries=[(6, 1.0, 0xFFFFFFFF, 0xA5D00003 + 50 + i) for i in range(3)], local=True),
+    # reduced config 4: R-MAT scale 11, 16 uniform labels
+    dict(name="rmat_small", kind="rmat", scale=11, ef=10, a=0.57, b=0.19, c=0.19, nl=16, zipf=0.0, seed=0xA5D00004,
+         queries=[(k, d, 0xFFFFFFFF, 0xA5D00004 + 13 * k + int(10 * d)) for k in (4, 6, 8) for d in (0.0, 0.5, 1.0)],
+         local=False),
+]
+
+
+def make_graph(s):
+    if s["kind"] == "chung_lu":
+        return A.Graph.chung_lu(s["n"], s["ne"], s["beta"], s["nl"], s["zipf"], s["seed"])
+    return A.Graph.rmat(s["scale"], s["ef"], s["a"], s["b"], s["c"], s["nl"], s["zipf"], s["seed"])
+
+
+def synth():
+    cases = []
+    for s in SYNTH:
+        g = make_graph(s)
+        dpath = f"/tmp/golden_{s['name']}.graph"
+        g.write(dpath)
+        case = {k: v for k, v in s.items() if k != "queries"}
+        case["data_sha256"] = sha256_file(dpath)
+        case["queries"] = []
+        for qi, (k, dens, mx, qseed) in enumerate(s["queries"]):
+            q = g.random_walk_query(k, dens, mx, qseed)
+            qpath = f"/tmp/golden_{s['name']}_q{qi}.graph"
+            q.write(qpath)
+            rec = {"k": k, "density": dens, "max_extra": mx, "seed": qseed, "query_sha256": sha256_file(qpath)}
+            if s["local"]:
+                csv = f"/tmp/golden_{s['name']}_q{qi}.csv"
+                cnt, st = run_ref("ref_driver_local", qpath, dpath, extra=("-c", csv), timeout=300)
+                rec["count"], rec["status"] = cnt, st
+                if cnt is not None:
+                    mat = np.loadtxt(csv, delimiter=",", dtype=np.uint64, ndmin=2)
+                    npz = f"{s['name']}_q{qi}_local.npz"
+                    np.savez_compressed(os.path.join(OUT, npz), local=mat)
+                    rec["local_npz"] = npz
+                    os.remove(csv)
+            else:
+                cnt, st = run_ref("ref_driver_local", qpath, dpath, timeout=300)
+                rec["count"], rec["status"] = cnt, st
+            cs = cs_dump(qpath, dpath)
+            rec["cand_sizes"] = None if cs is None else [len(c) for c in cs["cand"]]
+            rec["cand_sha256"] = None if cs is None else hashlib.sha256(
+                json.dumps(cs["cand"]).encode()).hexdigest()
+            rec["arc_weight"] = None if cs is None else cs["arc_weight"]
+            case["queries"].append(rec)
+            os.remove(qpath)
+            print(s["name"], qi, rec["count"], rec["status"])
+        os.remove(dpath)
+        cases.append(case)
+    with open(os.path.join(OUT, "synth.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+
+
+if __name__ == "__main__":
+    yeast()
+    synth()

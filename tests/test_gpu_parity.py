@@ -1,0 +1,166 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden outputs and the
+oracle, bit-exact.  Every test here runs kernels on the MI355X."""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import asdmatch_amd as A
+from oracle import oracle as O
+from tests import common as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+@pytest.fixture(scope="module")
+def yeast():
+    return A.Graph.load(T.yeast_data_path())
+
+
+def test_intersect_k_matches_numpy(ctx):
+    rng = np.random.default_rng(1)
+    for ratio in (1, 3, 10, 100, 1000, 10000):
+        for k in (1, 2, 3, 5):
+            small = 50
+            lists = []
+            for j in range(k):
+                n = small if j == 0 else small * ratio
+                lists.append(np.unique(rng.integers(0, 4 * small * ratio, size=n, dtype=np.uint64)).astype(np.uint32))
+            want = lists[0]
+            for l in lists[1:]:
+                want = np.intersect1d(want, l)
+            got = ctx.intersect(lists)
+            assert np.array_equal(got, want), (ratio, k)
+
+
+def test_filter_matches_reference_on_yeast(ctx, yeast):
+    gold = T.yeast_golden()["queries"]
+    n = 0
+    for name, rec in gold.items():
+        if "cs" not in rec:
+            continue
+        q = A.Graph.load(T.yeast_query_path(name))
+        cs = ctx.filter(q, yeast)
+        if rec["cs"] is None:
+            assert cs is None, name
+            continue
+        assert cs is not None, name
+        for u, ref in enumerate(rec["cs"]["cand"]):
+            assert np.array_equal(cs.candidates(u), np.array(ref, np.uint32)), (name, u)
+        for a, (_, _, w) in enumerate(rec["cs"]["arc_weight"]):
+            assert cs.arc_weight(a) == w, (name, a)
+        n += 1
+    assert n >= 30
+
+
+def test_counts_match_reference_on_yeast(ctx, yeast):
+    gold = T.yeast_golden()["queries"]
+    for name, rec in gold.items():
+        q = A.Graph.load(T.yeast_query_path(name))
+        got = A.count_embeddings(ctx, q, yeast)
+        assert got == rec["count"], name
+
+
+def test_root_matches_oracle(ctx, yeast):
+    og = O.OGraph.load(T.yeast_data_path())
+    for name in list(T.yeast_golden()["queries"])[::5]:
+        q = A.Graph.load(T.yeast_query_path(name))
+        cs = ctx.filter(q, yeast)
+        ocs = O.OCandSpace(O.OGraph.load(T.yeast_query_path(name)), og)
+        if cs is None:
+            assert ocs.empty
+            continue
+        assert cs.root == ocs.root, name
+
+
+def test_synthetic_counts_match_reference(ctx):
+    for case in T.synth_golden():
+        g = T.make_graph(case)
+        for rec in case["queries"]:
+            if rec["count"] is None:
+                continue
+            q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+            cs = ctx.filter(q, g)
+            assert list(cs.sizes()) == rec["cand_sizes"], (case["name"], rec["seed"])
+            plan = cs.plan(q)
+            say(case["name"], rec["seed"], plan.order())
+            assert ctx.count(plan, cs) == rec["count"], (case["name"], rec["seed"])
+
+
+def test_budget_does_not_change_counts(ctx):
+    case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
+    g = T.make_graph(case)
+    for rec in case["queries"]:
+        q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+        cs = ctx.filter(q, g)
+        plan = cs.plan(q)
+        for budget in (1 << 12, 1 << 16, 1 << 20, 0):
+            assert ctx.count(plan, cs, budget) == rec["count"], budget
+
+
+def test_order_does_not_change_counts(ctx):
+    case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
+    g = T.make_graph(case)
+    rng = random.Random(4)
+    for rec in case["queries"]:
+        q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+        e = q.edges().tolist()
+        adj = {i: set() for i in range(q.n)}
+        for a, b in e:
+            adj[a].add(b)
+            adj[b].add(a)
+        cs = ctx.filter(q, g)
+        for _ in range(4):
+            start = rng.randrange(q.n)
+            order, seen = [start], {start}
+            while len(order) < q.n:
+                fr = sorted({w for u in order for w in adj[u]} - seen)
+                w = rng.choice(fr)
+                order.append(w)
+                seen.add(w)
+            plan = cs.plan(q, order=order)
+            assert ctx.count(plan, cs) == rec["count"], order
+
+
+def _rand_graph(rng, n, p, nl):
+    labels = [rng.randrange(nl) for _ in range(n)]
+    edges = [(a, b) for a in range(n) for b in range(a + 1, n) if rng.random() < p]
+    return labels, edges
+
+
+def test_same_label_leaf_groups_against_bruteforce(ctx):
+    """Stars and trees whose leaves share labels exercise the inclusion-exclusion leaf counting."""
+    rng = random.Random(9)
+    for trial in range(12):
+        gl, ge = _rand_graph(rng, 60, 0.15, 2)
+        g, og = T.graph_from_lists(gl, ge)
+        for ql, qe in (
+            ([0, 1, 1, 1], [(0, 1), (0, 2), (0, 3)]),
+            ([0, 1, 1, 1, 1], [(0, 1), (0, 2), (0, 3), (0, 4)]),
+            ([0, 1, 0, 1, 1], [(0, 1), (1, 2), (2, 3), (2, 4), (0, 4)]),
+            ([1, 0, 1, 0, 1, 1], [(0, 1), (1, 2), (2, 3), (1, 4), (3, 5)]),
+        ):
+            q, oq = T.graph_from_lists(ql, qe)
+            want = O.bruteforce_count(oq, og)
+            assert A.count_embeddings(ctx, q, g) == want, (trial, ql, qe)
+
+
+def test_local_counts_match_reference(ctx):
+    for case in T.synth_golden():
+        if not case["local"]:
+            continue
+        g = T.make_graph(case)
+        for rec in case["queries"]:
+            if not rec.get("local_npz"):
+                continue
+            q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+            mat, c = A.local_count(ctx, q, g)
+            ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
+            assert c == rec["count"]
+            assert np.array_equal(mat, ref), (case["name"], rec["seed"])
+            assert all(int(s) == c for s in mat.sum(axis=0))

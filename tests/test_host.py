@@ -1,0 +1,123 @@
+"""Host-side parts of libasdm that need no GPU: the C ABI surface, the t/v/e loader against the
+oracle's restatement of relation/graph.cpp:68-148, the seeded generators, and loud failure when
+there is no gfx950 device."""
+import ctypes
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import asdmatch_amd as A
+from oracle import oracle as O
+from tests import common as T
+
+HEADER = os.path.join(T.GOLDEN, "..", "..", "include", "asdm.h")
+
+
+def test_every_declared_symbol_is_exported():
+    text = open(HEADER).read()
+    names = set(re.findall(r"\b(asdm_[a-z0-9_]+)\s*\(", text))
+    assert len(names) > 30
+    lib = ctypes.CDLL(A.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_loader_matches_oracle_on_yeast():
+    paths = [T.yeast_data_path()] + [T.yeast_query_path(n) for n in list(T.yeast_golden()["queries"])[:40]]
+    for p in paths:
+        a = A.Graph.load(p).csr()
+        b = O.OGraph.load(p).csr()
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), p
+
+
+def test_loader_rejects_bad_input():
+    with pytest.raises(A.AsdmError) as e:
+        A.Graph.load("/nonexistent/x.graph")
+    assert e.value.code == -2
+    with tempfile.NamedTemporaryFile("w", suffix=".graph", delete=False) as f:
+        f.write("t 2 1\nv 0 0 2\nv 1 0 1\ne 0 1\n")  # declared degree 2 but one edge
+        bad = f.name
+    try:
+        with pytest.raises(A.AsdmError):
+            A.Graph.load(bad)
+    finally:
+        os.remove(bad)
+
+
+def test_extra_tokens_on_edge_lines_are_ignored():
+    with tempfile.NamedTemporaryFile("w", suffix=".graph", delete=False) as f:
+        f.write("t 3 2\nv 0 1 1\nv 1 0 2\nv 2 1 1\ne 0 1 0\ne 1 2 0\n")
+        p = f.name
+    try:
+        off, nbr, lab = A.Graph.load(p).csr()
+        assert off.tolist() == [0, 1, 3, 4]
+        assert nbr.tolist() == [1, 0, 2, 1]
+        assert lab.tolist() == [1, 0, 1]
+    finally:
+        os.remove(p)
+
+
+def test_adjacency_sorted_by_label_then_id():
+    g = A.Graph.chung_lu(500, 3000, 2.5, 5, 1.0, 1)
+    off, nbr, lab = g.csr()
+    for v in range(g.n):
+        s = nbr[off[v]:off[v + 1]]
+        keys = list(zip(lab[s].tolist(), s.tolist()))
+        assert keys == sorted(keys)
+
+
+def test_generators_are_deterministic_and_well_formed():
+    a = A.Graph.chung_lu(3000, 12500, 2.5, 70, 1.0, 0xA5D00001)
+    b = A.Graph.chung_lu(3000, 12500, 2.5, 70, 1.0, 0xA5D00001)
+    assert np.array_equal(a.edges(), b.edges())
+    e = a.edges()
+    assert len(e) == 12500
+    assert np.all(e[:, 0] < e[:, 1])  # no self-loops, canonical order
+    assert len(np.unique(e[:, 0].astype(np.uint64) << 32 | e[:, 1])) == len(e)  # no duplicates
+    c = A.Graph.chung_lu(3000, 12500, 2.5, 70, 1.0, 0xA5D00002)
+    assert not np.array_equal(a.edges(), c.edges())
+    r = A.Graph.rmat(10, 8, 0.57, 0.19, 0.19, 16, 0.0, 9)
+    er = r.edges()
+    assert np.all(er[:, 0] < er[:, 1]) and len(er) <= 8 * 1024
+    assert r.info()[2] <= 16
+
+
+def test_random_walk_query_is_connected_induced_subgraph():
+    g = A.Graph.chung_lu(2000, 10000, 2.5, 8, 1.0, 4)
+    _, _, glab = g.csr()
+    ge = set(map(tuple, g.edges().tolist()))
+    for dens, cap in ((1.0, 0xFFFFFFFF), (0.0, 0xFFFFFFFF), (1.0, 2), (0.5, 0xFFFFFFFF)):
+        q = g.random_walk_query(10, dens, cap, 17)
+        qe = q.edges()
+        assert q.n == 10
+        assert len(qe) >= 9  # spanning tree at least
+        if dens == 0.0:
+            assert len(qe) == 9
+        if cap == 2:
+            assert len(qe) <= 11
+        # connected
+        adj = {i: set() for i in range(10)}
+        for a, b in qe.tolist():
+            adj[a].add(b)
+            adj[b].add(a)
+        seen, st = {0}, [0]
+        while st:
+            for w in adj[st.pop()]:
+                if w not in seen:
+                    seen.add(w)
+                    st.append(w)
+        assert len(seen) == 10
+
+
+def test_context_fails_loudly_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(A.AsdmError) as e:
+        A.Context(0)
+    assert e.value.code == -5
