@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libasdm.so")
 ASDM_OK = 0
 ASDM_E_EMPTY_CANDIDATES = 1
 ASDM_PLAN_LOCAL = 1
+ASDM_PLAN_NO_FACTOR = 2
 
 _ERRNAMES = {
     -1: "ASDM_E_INVALID", -2: "ASDM_E_IO", -3: "ASDM_E_OOM", -4: "ASDM_E_HIP",
@@ -196,6 +197,7 @@ class Context:
         self._h = h
 
     def close(self):
+        """Destroys the context now; candidate spaces made from it must not be used afterwards."""
         if getattr(self, "_h", None):
             _lib.asdm_ctx_destroy(self._h)
             self._h = None
@@ -218,7 +220,7 @@ class Context:
         if rc == ASDM_E_EMPTY_CANDIDATES:
             return None
         self._check(rc, "filter_cfl")
-        return CandSpace(h, q.n)
+        return CandSpace(h, q.n, self)
 
     def count(self, plan: "Plan", cs: "CandSpace", budget_bytes: int = 0) -> int:
         out = C.c_uint64()
@@ -265,14 +267,18 @@ class Context:
 
 
 class CandSpace:
-    def __init__(self, handle, nq):
+    """Device candidate space; keeps its Context alive (the context owns the stream it frees on)."""
+
+    def __init__(self, handle, nq, ctx):
         self._h = handle
         self.nq = nq
+        self._ctx = ctx
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # a candidate space is released on its context's stream, so only while that context lives
+        if getattr(self, "_h", None) and getattr(self._ctx, "_h", None):
             _lib.asdm_cs_free(self._h)
-            self._h = None
+        self._h = None
 
     @property
     def root(self) -> int:
@@ -297,9 +303,9 @@ class CandSpace:
         _lib.asdm_cs_arc_weight(self._h, a, C.byref(w))
         return w.value
 
-    def plan(self, q: Graph, order=None, local: bool = False, cards=None) -> "Plan":
+    def plan(self, q: Graph, order=None, local: bool = False, cards=None, factor: bool = True) -> "Plan":
         h = _P()
-        flags = ASDM_PLAN_LOCAL if local else 0
+        flags = (ASDM_PLAN_LOCAL if local else 0) | (0 if factor else ASDM_PLAN_NO_FACTOR)
         if cards:
             masks = np.array(list(cards.keys()), np.uint64)
             vals = np.array(list(cards.values()), np.float64)

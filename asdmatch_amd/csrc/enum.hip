@@ -181,15 +181,16 @@ __global__ void __launch_bounds__(256) k_expand(const DPlan *__restrict__ P, uin
 }
 
 // -------------------------------------------------------------------------- leaves
-// |∩ of the candidate-edge lists of every leaf in `mask` (leaf indices of group g) minus matched
-// same-label vertices|, reduced over the lane group.
+// |∩_{leaf e in mask} S_e| where S_e = ∩ of e's candidate-edge lists minus matched same-label
+// vertices, reduced over the lane group.  Lists of different leaves index different candidate
+// sets, so the probe element is carried as a data id and re-located in each other leaf's C(u_e).
 template <int G>
 __device__ inline uint64_t leaf_set_count(const DPlan *__restrict__ P, uint32_t g, uint32_t mask,
                                           const uint32_t *__restrict__ row, const uint64_t *__restrict__ ce_off,
                                           const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
                                           const SameLabel &S, uint64_t &bytes) {
     const int lg = threadIdx.x & (G - 1);
-    // pick the probe: shortest list among all lists of the leaves in mask
+    // probe: the shortest list among all lists of the leaves in mask
     uint32_t pe = 0, pj = 0, nlists = 0;
     ListRef pr{0, 0xffffffffu};
     for (uint32_t i = 0; i < P->gsize[g]; ++i) {
@@ -205,30 +206,34 @@ __device__ inline uint64_t leaf_set_count(const DPlan *__restrict__ P, uint32_t 
             }
         }
     }
-    const uint32_t e0 = P->gleaf[g][0];
-    const uint64_t cand_e = P->coff[e0];
+    const uint64_t cand_p = P->coff[pe];
     const uint32_t *probe = ce_val + pr.lo;
     if (nlists == 1) {
         if (lg == 0) {
             bytes += 16 + 8ull * S.ns;
-            return pr.len - same_hits(S, cand, cand_e, P->csz[e0], probe, pr.len);
+            return pr.len - same_hits(S, cand, cand_p, P->csz[pe], probe, pr.len);
         }
         return 0;
     }
     uint64_t c = 0;
     for (uint32_t t = lg; t < pr.len; t += G) {
         const uint32_t x = probe[t];
-        bool ok = true;
+        const uint32_t xid = cand[cand_p + x];
+        bool ok = injective(S, cand, xid);
         for (uint32_t i = 0; i < P->gsize[g] && ok; ++i) {
             if (!(mask >> i & 1)) continue;
             const uint32_t e = P->gleaf[g][i];
+            uint32_t pos = x;
+            if (e != pe) {
+                pos = lower_bound_u32(cand + P->coff[e], P->csz[e], xid);
+                ok = pos < P->csz[e] && cand[P->coff[e] + pos] == xid;
+            }
             for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
                 if (e == pe && j == pj) continue;
                 ListRef r = list_of(P, ce_off, row, e, j);
-                ok = contains_u32(ce_val + r.lo, r.len, x);
+                ok = contains_u32(ce_val + r.lo, r.len, pos);
             }
         }
-        if (ok) ok = injective(S, cand, cand[cand_e + x]);
         c += ok;
     }
     bytes += 4ull * ((pr.len + G - 1 - lg) / G);

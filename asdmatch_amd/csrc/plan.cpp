@@ -127,6 +127,7 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
                const uint64_t *masks, const double *cards, uint32_t ncards, asdm_plan *out, std::string &err) {
     const uint32_t nq = cs->nq;
     const bool local = flags & ASDM_PLAN_LOCAL;
+    const bool nofactor = flags & ASDM_PLAN_NO_FACTOR;
     const uint32_t max_group = local ? 1 : kMaxGroup;
     std::vector<uint32_t> order;
     uint32_t D = 0;
@@ -151,7 +152,7 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         // leaves: the longest suffix that is an independent set with all neighbours before it,
         // within the label-group limits
         D = nq;
-        while (D > 1) {
+        while (D > 1 && !(nofactor && D < nq)) {
             uint32_t u = order[D - 1];
             bool ok = true;
             for (uint32_t j = D; j < nq && ok; ++j) ok = !adjacent(cs, u, order[j]);
@@ -176,6 +177,7 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         std::sort(cand.begin(), cand.end());
         if (nq > 1)
             for (auto &c : cand) {
+                if (nofactor && leaves) break;
                 uint32_t u = c.second;
                 bool indep = true;
                 for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) indep &= !(leaves >> cs->qnbr[a] & 1);

@@ -96,6 +96,7 @@ int asdm_gen_query(const asdm_graph *g, uint32_t k, double density, uint32_t max
  * candidate sets) and the candidate-edge index, all in HBM.  Returns ASDM_E_EMPTY_CANDIDATES
  * (and *out = NULL) when some C(u) is empty.  g must have been uploaded. */
 int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asdm_cs **out);
+/* Must be called before the context that built cs is destroyed. */
 void asdm_cs_free(asdm_cs *cs);
 uint32_t asdm_cs_root(const asdm_cs *cs);
 /* |C(u)| for every u (sizes[nq]). */
@@ -109,6 +110,9 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
 /* order: NULL to let the planner choose, else a connected permutation of the query vertices.
  * flags: ASDM_PLAN_LOCAL restricts leaf factorization to what local counting supports. */
 #define ASDM_PLAN_LOCAL 1u
+/* Count only the last vertex in closed form (no multi-leaf factorization): a plain enumeration
+ * used to cross-check the leaf factorization. */
+#define ASDM_PLAN_NO_FACTOR 2u
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
 void asdm_plan_free(asdm_plan *p);
