@@ -30,7 +30,7 @@ ASDM_PLAN_NO_FACTOR = 2
 
 _ERRNAMES = {
     -1: "ASDM_E_INVALID", -2: "ASDM_E_IO", -3: "ASDM_E_OOM", -4: "ASDM_E_HIP",
-    -5: "ASDM_E_NODEVICE", -6: "ASDM_E_BUDGET", -7: "ASDM_E_UNSUPPORTED",
+    -5: "ASDM_E_NODEVICE", -6: "ASDM_E_BUDGET", -7: "ASDM_E_UNSUPPORTED", -8: "ASDM_E_TIMEOUT",
 }
 
 
@@ -67,6 +67,8 @@ _sig("asdm_ctx_create", C.c_int, C.c_int, C.POINTER(_P))
 _sig("asdm_ctx_destroy", None, _P)
 _sig("asdm_last_error", C.c_char_p, _P)
 _sig("asdm_ctx_set_profiling", C.c_int, _P, C.c_int)
+_sig("asdm_ctx_set_time_limit", C.c_int, _P, C.c_double)
+_sig("asdm_ctx_set_row_limit", C.c_int, _P, C.c_uint64)
 _sig("asdm_kernel_stat", C.c_int, _P, C.c_uint32, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), _u64p, _u64p)
 _sig("asdm_last_stats", C.c_int, _P, C.POINTER(Stats))
 _sig("asdm_graph_load_text", C.c_int, C.c_char_p, C.POINTER(_P))
@@ -93,6 +95,7 @@ _sig("asdm_plan_create_with_cards", C.c_int, _P, _P, _u64p, C.POINTER(C.c_double
      C.POINTER(_P))
 _sig("asdm_plan_free", None, _P)
 _sig("asdm_plan_order", C.c_int, _P, _u32p, _u32p)
+_sig("asdm_plan_estimate", C.c_int, _P, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
 _sig("asdm_local_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p, _u64p)
 _sig("asdm_intersect_k", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
@@ -239,6 +242,14 @@ class Context:
         _lib.asdm_last_stats(self._h, C.byref(s))
         return s
 
+    def set_time_limit(self, ms: float):
+        """Per-query limit for count/local_count (0 = none); exceeding it raises ASDM_E_TIMEOUT."""
+        self._check(_lib.asdm_ctx_set_time_limit(self._h, float(ms)), "set_time_limit")
+
+    def set_row_limit(self, rows: int):
+        """Limit on partial embeddings materialized per count (0 = none)."""
+        self._check(_lib.asdm_ctx_set_row_limit(self._h, int(rows)), "set_row_limit")
+
     def set_profiling(self, on: bool):
         _lib.asdm_ctx_set_profiling(self._h, 1 if on else 0)
 
@@ -329,6 +340,12 @@ class Plan:
         if getattr(self, "_h", None):
             _lib.asdm_plan_free(self._h)
             self._h = None
+
+    def estimate(self):
+        """(estimated rows at the leaf depth, estimated rows over all enumerated depths)."""
+        a, b = C.c_double(), C.c_double()
+        _lib.asdm_plan_estimate(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def order(self):
         o = np.zeros(self.nq, np.uint32)

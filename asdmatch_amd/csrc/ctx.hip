@@ -16,20 +16,58 @@ int check_hip(asdm_ctx *ctx, hipError_t e, const char *what) {
     return e == hipErrorOutOfMemory ? ASDM_E_OOM : ASDM_E_HIP;
 }
 
+static void trim_cache(asdm_ctx *ctx) {
+    if (ctx->cache.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->cache) (void)hipFree(b.p);
+    ctx->cache.clear();
+    ctx->cached_bytes = 0;
+}
+
 int dev_alloc(asdm_ctx *ctx, void **p, size_t bytes) {
     *p = nullptr;
-    if (bytes == 0) bytes = 16;
-    hipError_t e = hipMallocAsync(p, bytes, ctx->stream);
+    bytes = (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+    // best fit among cached blocks no more than 2x (+1 MiB) larger
+    int best = -1;
+    for (size_t i = 0; i < ctx->cache.size(); ++i) {
+        const size_t b = ctx->cache[i].bytes;
+        if (b >= bytes && b <= 2 * bytes + (1u << 20) && (best < 0 || b < ctx->cache[best].bytes)) best = (int)i;
+    }
+    if (best >= 0) {
+        asdm_ctx::Block blk{ctx->cache[best].p, ctx->cache[best].bytes};
+        ctx->cache.erase(ctx->cache.begin() + best);
+        ctx->cached_bytes -= blk.bytes;
+        ctx->live.push_back({blk.p, blk.bytes});
+        *p = blk.p;
+        return ASDM_OK;
+    }
+    hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        trim_cache(ctx);
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
         ctx->err = "device allocation of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
         return ASDM_E_OOM;
     }
+    ctx->live.push_back({*p, bytes});
     return ASDM_OK;
 }
 
 void dev_free(asdm_ctx *ctx, void *p) {
-    if (p) (void)hipFreeAsync(p, ctx->stream);
+    if (!p) return;
+    for (size_t i = 0; i < ctx->live.size(); ++i)
+        if (ctx->live[i].p == p) {
+            ctx->cache.push_back({p, ctx->live[i].bytes});
+            ctx->cached_bytes += ctx->live[i].bytes;
+            ctx->live[i] = ctx->live.back();
+            ctx->live.pop_back();
+            break;
+        }
+    if (ctx->cached_bytes > (64ull << 30)) trim_cache(ctx);
 }
 
 // ---------------------------------------------------------------- scan
@@ -219,11 +257,6 @@ int asdm_ctx_create(int device, asdm_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) return ASDM_E_HIP;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return ASDM_E_HIP;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-        uint64_t thr = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-    }
     if (hipMalloc((void **)&ctx->d_counters, sizeof(uint64_t) * 8) != hipSuccess) return ASDM_E_OOM;
     *out = ctx.release();
     return ASDM_OK;
@@ -233,6 +266,8 @@ void asdm_ctx_destroy(asdm_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto &b : ctx->cache) (void)hipFree(b.p);
+    for (auto &b : ctx->live) (void)hipFree(b.p);
     (void)hipFree(ctx->d_counters);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -243,6 +278,18 @@ const char *asdm_last_error(const asdm_ctx *ctx) { return ctx ? ctx->err.c_str()
 int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out) {
     if (!ctx || !out) return ASDM_E_INVALID;
     *out = ctx->stats;
+    return ASDM_OK;
+}
+
+int asdm_ctx_set_time_limit(asdm_ctx *ctx, double ms) {
+    if (!ctx || ms < 0) return ASDM_E_INVALID;
+    ctx->time_limit_ms = ms;
+    return ASDM_OK;
+}
+
+int asdm_ctx_set_row_limit(asdm_ctx *ctx, uint64_t rows) {
+    if (!ctx) return ASDM_E_INVALID;
+    ctx->row_limit = rows;
     return ASDM_OK;
 }
 

@@ -47,6 +47,8 @@ struct asdm_ctx {
     std::string err;
     asdm_stats stats{};
     bool profiling = false;
+    double time_limit_ms = 0;  // 0 = no limit
+    uint64_t row_limit = 0;    // 0 = no limit
     // per-kernel device time (ms), launches and algorithmic bytes, accumulated while profiling
     struct KStat {
         std::string name;
@@ -56,6 +58,15 @@ struct asdm_ctx {
     };
     std::vector<KStat> kstats;
     uint64_t *d_counters = nullptr;  // [8] device scratch counters
+    // caching allocator: freed blocks are kept for reuse (all work is ordered on `stream`, so a
+    // block freed by one launch sequence may be handed to the next one immediately)
+    struct Block {
+        void *p;
+        size_t bytes;
+    };
+    std::vector<Block> cache;
+    std::vector<Block> live;
+    size_t cached_bytes = 0;
     int fail(int code, const std::string &msg) {
         err = msg;
         return code;
@@ -93,6 +104,7 @@ struct asdm_plan {
     uint32_t D = 0;
     bool local = false;
     double plan_ms = 0;
+    double est_leaf = 0, est_total = 0;  // estimated rows at depth D and summed over depths 1..D
 };
 
 namespace asdm {

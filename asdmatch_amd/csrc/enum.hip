@@ -357,6 +357,160 @@ __global__ void __launch_bounds__(256) k_leaf_local(const DPlan *__restrict__ P,
     if ((threadIdx.x & 63) == 0 && acc) atomicAdd(total, (unsigned long long)acc);
 }
 
+// -------------------------------------------------------------------------- fused last level
+// Per-lane leaf evaluation for a row of depth D-1 extended by candidate x at depth D-1: the value
+// matched at depth j is (j == dx ? x : row[j]).
+struct RowView {
+    const uint32_t *row;
+    uint32_t dx, x;
+    __device__ inline uint32_t at(uint32_t j) const { return j == dx ? x : row[j]; }
+};
+
+__device__ inline ListRef list_of_v(const DPlan *__restrict__ P, const uint64_t *__restrict__ ce_off, const RowView &R,
+                                    uint32_t d, uint32_t j) {
+    const uint64_t base = P->bbase[d][j] + R.at(P->bdep[d][j]);
+    const uint64_t lo = ce_off[base];
+    return {lo, (uint32_t)(ce_off[base + 1] - lo)};
+}
+
+// Data id matched at depth dep.
+__device__ inline uint32_t matched_id(const DPlan *__restrict__ P, const uint32_t *__restrict__ cand, const RowView &R,
+                                      uint32_t dep) {
+    return cand[P->coff[dep] + R.at(dep)];
+}
+
+// One lane: |∩_{leaf e in mask of group g} S_e|.
+__device__ inline uint64_t leaf_set_count_lane(const DPlan *__restrict__ P, uint32_t g, uint32_t mask, const RowView &R,
+                                               const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                               const uint32_t *__restrict__ cand, const uint32_t *y, uint32_t ns) {
+    uint32_t pe = 0, pj = 0, nlists = 0;
+    ListRef pr{0, 0xffffffffu};
+    for (uint32_t i = 0; i < P->gsize[g]; ++i) {
+        if (!(mask >> i & 1)) continue;
+        const uint32_t e = P->gleaf[g][i];
+        for (uint32_t j = 0; j < P->nb[e]; ++j) {
+            ListRef r = list_of_v(P, ce_off, R, e, j);
+            ++nlists;
+            if (r.len < pr.len) {
+                pr = r;
+                pe = e;
+                pj = j;
+            }
+        }
+    }
+    const uint64_t cand_p = P->coff[pe];
+    const uint32_t *probe = ce_val + pr.lo;
+    if (nlists == 1) {
+        uint32_t h = 0;
+        for (uint32_t s = 0; s < ns; ++s) {
+            const uint32_t x = lower_bound_u32(cand + cand_p, P->csz[pe], y[s]);
+            if (x < P->csz[pe] && cand[cand_p + x] == y[s] && contains_u32(probe, pr.len, x)) ++h;
+        }
+        return pr.len - h;
+    }
+    uint64_t c = 0;
+    for (uint32_t t = 0; t < pr.len; ++t) {
+        const uint32_t x = probe[t];
+        const uint32_t xid = cand[cand_p + x];
+        bool ok = true;
+        for (uint32_t s = 0; s < ns && ok; ++s) ok = xid != y[s];
+        for (uint32_t i = 0; i < P->gsize[g] && ok; ++i) {
+            if (!(mask >> i & 1)) continue;
+            const uint32_t e = P->gleaf[g][i];
+            uint32_t pos = x;
+            if (e != pe) {
+                pos = lower_bound_u32(cand + P->coff[e], P->csz[e], xid);
+                ok = pos < P->csz[e] && cand[P->coff[e] + pos] == xid;
+            }
+            for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
+                if (e == pe && j == pj) continue;
+                ListRef r = list_of_v(P, ce_off, R, e, j);
+                ok = contains_u32(ce_val + r.lo, r.len, pos);
+            }
+        }
+        c += ok;
+    }
+    return c;
+}
+
+// One lane: completions of the leaf group g (inclusion-exclusion over set partitions for k >= 2).
+__device__ inline uint64_t group_count_lane(const DPlan *__restrict__ P, uint32_t g, const RowView &R,
+                                            const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                            const uint32_t *__restrict__ cand) {
+    const uint32_t e0 = P->gleaf[g][0];
+    const uint32_t ns = P->ns[e0];
+    uint32_t y[kMaxQ];
+    for (uint32_t s = 0; s < ns; ++s) y[s] = matched_id(P, cand, R, P->sdep[e0][s]);
+    const uint32_t k = P->gsize[g];
+    if (k == 1) return leaf_set_count_lane(P, g, 1u, R, ce_off, ce_val, cand, y, ns);
+    uint64_t cs[1u << kMaxGroup];
+    for (uint32_t m = 1; m < (1u << k); ++m) cs[m] = leaf_set_count_lane(P, g, m, R, ce_off, ce_val, cand, y, ns);
+    uint64_t tot = 0;
+    for (uint32_t p = 0; p < P->nparts[k]; ++p) {
+        uint64_t term = (uint64_t)P->pcoef[k][p];
+        for (uint32_t b = 0; b < P->pnblk[k][p]; ++b) term *= cs[P->pblk[k][p][b]];
+        tot += term;
+    }
+    return tot;
+}
+
+// Rows of width D-1 -> sum over their valid depth-(D-1) candidates x of the leaf product of
+// (row, x).  Lanes of a G-group split the row's candidate list; each lane evaluates its own
+// candidates' leaf products, so the depth-D frontier is never written.
+template <int G>
+__global__ void __launch_bounds__(256) k_leaf_fused(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
+                                                    uint64_t nrows, const uint64_t *__restrict__ ce_off,
+                                                    const uint32_t *__restrict__ ce_val,
+                                                    const uint32_t *__restrict__ cand,
+                                                    unsigned long long *__restrict__ total,
+                                                    unsigned long long *__restrict__ bytes_ctr) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint32_t d = P->D - 1, ng = P->ngroups;
+    const uint32_t nb = P->nb[d];
+    const uint64_t cand_d = P->coff[d];
+    uint64_t acc = 0, bytes = 0;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+        const uint32_t *row = rows + it * d;
+        uint32_t pj = 0;
+        ListRef pr = list_of(P, ce_off, row, d, 0);
+        for (uint32_t j = 1; j < nb; ++j) {
+            ListRef r = list_of(P, ce_off, row, d, j);
+            if (r.len < pr.len) {
+                pr = r;
+                pj = j;
+            }
+        }
+        SameLabel S;
+        load_same(S, P, cand, row, d);
+        const uint32_t *probe = ce_val + pr.lo;
+        for (uint32_t t = lg; t < pr.len; t += G) {
+            const uint32_t x = probe[t];
+            bool ok = true;
+            for (uint32_t j = 0; j < nb && ok; ++j) {
+                if (j == pj) continue;
+                ListRef r = list_of(P, ce_off, row, d, j);
+                ok = contains_u32(ce_val + r.lo, r.len, x);
+            }
+            if (ok) ok = injective(S, cand, cand[cand_d + x]);
+            if (!ok) continue;
+            const RowView R{row, d, x};
+            uint64_t prod = 1;
+            for (uint32_t g = 0; g < ng && prod; ++g) prod *= group_count_lane(P, g, R, ce_off, ce_val, cand);
+            acc += prod;
+            bytes += 8;
+        }
+        bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+        if (lg == 0) bytes += 4ull * d + 16ull * nb;
+    }
+    acc = group_sum<64>(acc);
+    bytes = group_sum<64>(bytes);
+    if ((threadIdx.x & 63) == 0) {
+        if (acc) atomicAdd(total, (unsigned long long)acc);
+        if (bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+    }
+}
+
 __global__ void k_iota(uint32_t *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = (uint32_t)i;
@@ -365,7 +519,8 @@ __global__ void k_iota(uint32_t *p, uint64_t n) {
 // -------------------------------------------------------------------------- host driver
 namespace {
 
-constexpr int kG = 16;  // lanes per row
+constexpr int kG = 16;                      // lanes per row
+constexpr uint64_t kLeafChunk = 1ull << 22;  // rows per leaf-count launch
 
 unsigned groups_grid(uint64_t items, int G) {
     uint64_t threads = items * (uint64_t)G;
@@ -389,6 +544,16 @@ struct Engine {
     unsigned long long *d_mat = nullptr;  // local mode
     uint64_t rows_total = 0, launches = 0;
     std::vector<uint64_t> hoff;       // host copy of offsets when splitting
+    std::chrono::steady_clock::time_point t_start;
+    bool fused = false;           // depth D-1 evaluated by k_leaf_fused
+    uint32_t g_fused = 16;        // lanes per row in k_leaf_fused
+    uint64_t fanout_last = 1;     // expected candidate-list length at depth D-1
+
+    bool timed_out() const {
+        if (ctx->time_limit_ms <= 0) return false;
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count() >
+               ctx->time_limit_ms;
+    }
 
     int init() {
         D = plan->D;
@@ -430,23 +595,63 @@ struct Engine {
         for (auto p : offs) dev_free(ctx, p);
     }
 
+    // Fused last level: rows at depth D-1, leaf products summed over their depth-(D-1) candidates.
+    int leaf_fused(uint32_t *r, uint64_t n) {
+        const uint64_t chunk = std::max<uint64_t>(1, kLeafChunk / std::max<uint64_t>(1, fanout_last));
+        for (uint64_t i0 = 0; i0 < n; i0 += chunk) {
+            const uint64_t m = std::min<uint64_t>(chunk, n - i0);
+            if (i0 && timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
+            ++launches;
+            KTimer kt(ctx, "leaf_fused");
+            const uint32_t *rr = r + i0 * (D - 1);
+            switch (g_fused) {
+            case 1: hipLaunchKernelGGL((k_leaf_fused<1>), dim3(groups_grid(m, 1)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
+            case 4: hipLaunchKernelGGL((k_leaf_fused<4>), dim3(groups_grid(m, 4)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
+            case 16: hipLaunchKernelGGL((k_leaf_fused<16>), dim3(groups_grid(m, 16)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
+            default: hipLaunchKernelGGL((k_leaf_fused<64>), dim3(groups_grid(m, 64)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
+            }
+            kt.stop();
+            int rc = check_hip(ctx, hipGetLastError(), "fused leaf kernel");
+            if (rc) return rc;
+            if (i0 + m < n) {
+                rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+                if (rc) return rc;
+            }
+        }
+        return ASDM_OK;
+    }
+
+    // Leaf counting in launches of at most kLeafChunk rows, so that the limits are checked often.
     int leaf(uint32_t *r, uint64_t n) {
-        ++launches;
-        KTimer kt(ctx, d_mat ? "leaf_local" : "leaf_count");
-        if (d_mat)
-            hipLaunchKernelGGL((k_leaf_local<kG>), dim3(groups_grid(n, kG)), dim3(256), 0, ctx->stream, dplan, r, n,
-                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_mat, d_total);
-        else
-            hipLaunchKernelGGL((k_leaf<kG>), dim3(groups_grid(n, kG)), dim3(256), 0, ctx->stream, dplan, r, n,
-                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes);
-        kt.stop();
-        return check_hip(ctx, hipGetLastError(), "leaf kernel");
+        for (uint64_t i0 = 0; i0 < n; i0 += kLeafChunk) {
+            const uint64_t m = std::min<uint64_t>(kLeafChunk, n - i0);
+            if (i0 && timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
+            ++launches;
+            KTimer kt(ctx, d_mat ? "leaf_local" : "leaf_count");
+            if (d_mat)
+                hipLaunchKernelGGL((k_leaf_local<kG>), dim3(groups_grid(m, kG)), dim3(256), 0, ctx->stream, dplan,
+                                   r + i0 * D, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_mat, d_total);
+            else
+                hipLaunchKernelGGL((k_leaf<kG>), dim3(groups_grid(m, kG)), dim3(256), 0, ctx->stream, dplan, r + i0 * D,
+                                   m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes);
+            kt.stop();
+            int rc = check_hip(ctx, hipGetLastError(), "leaf kernel");
+            if (rc) return rc;
+            if (i0 + m < n) {  // let the device catch up so the wall-clock check means something
+                rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+                if (rc) return rc;
+            }
+        }
+        return ASDM_OK;
     }
 
     // rows r[0, n) at depth d (width d)
     int process(uint32_t d, uint32_t *r, uint64_t n) {
         if (n == 0) return ASDM_OK;
+        if (timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
         rows_total += n;
+        if (ctx->row_limit && rows_total > ctx->row_limit)
+            return ctx->fail(ASDM_E_TIMEOUT, "per-query row limit exceeded");
         if (d == D) return leaf(r, n);
         int rc = ensure_scan(d);
         if (rc) return rc;
@@ -463,6 +668,15 @@ struct Engine {
         if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&total, offs[d] + n, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
         if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (rc || total == 0) return rc;
+        if (fused && d + 1 == D) {
+            // the leaf-depth rows are evaluated inside the fused kernel, never stored
+            rows_total += total;
+            if (ctx->row_limit && rows_total > ctx->row_limit)
+                return ctx->fail(ASDM_E_TIMEOUT, "per-query row limit exceeded");
+            fanout_last = std::max<uint64_t>(1, total / n);
+            g_fused = fanout_last < 3 ? 1 : fanout_last < 12 ? 4 : fanout_last < 48 ? 16 : 64;
+            return leaf_fused(r, n);
+        }
         const uint64_t c = cap[d + 1];
         if (total <= c) return fill_and_recurse(d, r, 0, n, 0, total);
         // split the rows so that each chunk's output fits the next level
@@ -532,6 +746,26 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     (void)hipSetDevice(ctx->device);
     auto t0 = std::chrono::steady_clock::now();
     Engine E{ctx, plan, cs};
+    E.t_start = t0;
+    // Fuse the last enumerated level into leaf counting when there is one (count mode only); the
+    // lane-group width follows the expected list length of that level (smallest arc fan-out).
+    // Fusion pays when each leaf count is a list length minus a few injectivity hits: every leaf
+    // group must be a single leaf with a single candidate-edge list.
+    bool cheap_leaves = true;
+    for (uint32_t g = 0; g < plan->h.ngroups; ++g)
+        cheap_leaves &= plan->h.gsize[g] == 1 && plan->h.nb[plan->h.gleaf[g][0]] == 1;
+    if (!mat_host && plan->D >= 2 && cheap_leaves) {
+        E.fused = true;
+        const uint32_t d = plan->D - 1;
+        double best = 1e300;
+        for (uint32_t j = 0; j < plan->h.nb[d]; ++j) {
+            const uint32_t p = plan->order[plan->h.bdep[d][j]];
+            const uint32_t a = cs->arc_index(p, plan->order[d]);
+            best = std::min(best, (double)cs->arc_weight[a] / std::max(1.0, (double)cs->csize[p]));
+        }
+        E.fanout_last = (uint64_t)std::max(1.0, best);
+        E.g_fused = best < 3 ? 1 : best < 12 ? 4 : best < 48 ? 16 : 64;
+    }
     E.budget = budget ? std::max<uint64_t>(budget, 1ull << 12) : default_budget(ctx);
     int rc = E.init();
     uint64_t matn = (uint64_t)cs->nv * cs->nq;

@@ -232,6 +232,19 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         }
     }
 
+    // estimated frontier sizes along the final order
+    {
+        double est = cs->csize[order[0]], tot = est;
+        uint64_t S = 1ull << order[0];
+        for (uint32_t d = 1; d < D; ++d) {
+            est = card_of(S | (1ull << order[d]), est * growth(cs, S, order[d]));
+            S |= 1ull << order[d];
+            tot += est;
+        }
+        out->est_leaf = est;
+        out->est_total = tot;
+    }
+
     DPlan &P = out->h;
     std::memset(&P, 0, sizeof(P));
     P.nq = nq;
@@ -323,6 +336,13 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
 }
 
 void asdm_plan_free(asdm_plan *p) { delete p; }
+
+int asdm_plan_estimate(const asdm_plan *p, double *rows_leaf, double *rows_total) {
+    if (!p) return ASDM_E_INVALID;
+    if (rows_leaf) *rows_leaf = p->est_leaf;
+    if (rows_total) *rows_total = p->est_total;
+    return ASDM_OK;
+}
 
 int asdm_plan_order(const asdm_plan *p, uint32_t *order, uint32_t *num_enumerated) {
     if (!p) return ASDM_E_INVALID;

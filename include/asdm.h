@@ -42,6 +42,7 @@ extern "C" {
 #define ASDM_E_NODEVICE (-5)      /* no usable gfx950 device */
 #define ASDM_E_BUDGET (-6)        /* frontier budget too small for one expansion */
 #define ASDM_E_UNSUPPORTED (-7)   /* query outside the supported envelope (e.g. > 32 vertices) */
+#define ASDM_E_TIMEOUT (-8)       /* the per-query time limit elapsed (count unknown) */
 
 typedef struct asdm_ctx asdm_ctx;     /* device, stream, scratch arena */
 typedef struct asdm_graph asdm_graph; /* host graph (label-sorted CSR), optionally on device */
@@ -52,6 +53,13 @@ typedef struct asdm_plan asdm_plan;   /* host matching plan consumed by the engi
 int asdm_ctx_create(int device, asdm_ctx **out);
 void asdm_ctx_destroy(asdm_ctx *ctx);
 const char *asdm_last_error(const asdm_ctx *ctx);
+/* Per-query wall-clock limit for asdm_count / asdm_local_count (0 = none), checked between
+ * kernel launches; an exceeded limit returns ASDM_E_TIMEOUT.  The paper's experiments use a
+ * one-hour limit per query in the same way. */
+int asdm_ctx_set_time_limit(asdm_ctx *ctx, double ms);
+/* Limit on partial embeddings materialized per count (0 = none), checked before each launch;
+ * exceeding it returns ASDM_E_TIMEOUT.  Independent of device speed. */
+int asdm_ctx_set_row_limit(asdm_ctx *ctx, uint64_t rows);
 /* Library version string, no device needed. */
 const char *asdm_version(void);
 
@@ -116,6 +124,9 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
 void asdm_plan_free(asdm_plan *p);
+/* Planner's estimate (independence model over candidate-edge list sizes, or the supplied
+ * cardinalities) of the partial embeddings materialized at the leaf depth and over all depths. */
+int asdm_plan_estimate(const asdm_plan *p, double *rows_leaf, double *rows_total);
 /* order[nq] and the number of enumerated (non-leaf) depths. */
 int asdm_plan_order(const asdm_plan *p, uint32_t *order, uint32_t *num_enumerated);
 /* Planner cardinalities keyed by query-vertex subset bitmask (bit u = query vertex u), the
