@@ -159,32 +159,47 @@ int fill_u32(asdm_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------- timing
+static int kstat_slot(asdm_ctx *ctx, const char *name) {
+    for (size_t i = 0; i < ctx->kstats.size(); ++i)
+        if (ctx->kstats[i].name == name) return (int)i;
+    ctx->kstats.push_back({name, 0, 0, 0});
+    return (int)ctx->kstats.size() - 1;
+}
+
 KTimer::KTimer(asdm_ctx *c, const char *name) : ctx(c), slot(-1) {
     if (!ctx->profiling) return;
-    for (size_t i = 0; i < ctx->kstats.size(); ++i)
-        if (ctx->kstats[i].name == name) slot = (int)i;
-    if (slot < 0) {
-        ctx->kstats.push_back({name, 0, 0, 0});
-        slot = (int)ctx->kstats.size() - 1;
-    }
+    slot = kstat_slot(ctx, name);
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     (void)hipEventRecord(a, ctx->stream);
 }
 
+// Records the end event only; the elapsed time is read later by resolve_timers, so timing does
+// not serialize the launch sequence.
 void KTimer::stop(uint64_t bytes) {
     if (!ctx->profiling || slot < 0) return;
     (void)hipEventRecord(b, ctx->stream);
-    (void)hipEventSynchronize(b);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
-    auto &k = ctx->kstats[slot];
-    k.ms += ms;
-    k.launches += 1;
-    k.bytes += bytes;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    ctx->kstats[slot].bytes += bytes;
+    ctx->pending.push_back({slot, a, b});
     slot = -1;
+}
+
+void resolve_timers(asdm_ctx *ctx) {
+    for (auto &t : ctx->pending) {
+        (void)hipEventSynchronize(t.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
+        ctx->kstats[t.slot].ms += ms;
+        ctx->kstats[t.slot].launches += 1;
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    ctx->pending.clear();
+}
+
+void add_kernel_bytes(asdm_ctx *ctx, const char *name, uint64_t bytes) {
+    if (!ctx->profiling) return;
+    ctx->kstats[kstat_slot(ctx, name)].bytes += bytes;
 }
 
 void release_device_graph(asdm_graph &g) {
@@ -296,6 +311,7 @@ int asdm_ctx_set_row_limit(asdm_ctx *ctx, uint64_t rows) {
 // Per-kernel device timing, collected while profiling is on (HIP events on the context stream).
 int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) {
     if (!ctx) return ASDM_E_INVALID;
+    resolve_timers(ctx);
     ctx->profiling = on != 0;
     ctx->kstats.clear();
     return ASDM_OK;
@@ -304,6 +320,7 @@ int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) {
 int asdm_kernel_stat(const asdm_ctx *ctx, uint32_t i, char *name, uint32_t name_cap, double *ms, uint64_t *launches,
                      uint64_t *bytes) {
     if (!ctx) return ASDM_E_INVALID;
+    resolve_timers(const_cast<asdm_ctx *>(ctx));
     if (i >= ctx->kstats.size()) return ASDM_E_INVALID;
     const auto &k = ctx->kstats[i];
     if (name && name_cap) {

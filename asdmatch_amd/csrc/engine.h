@@ -30,6 +30,13 @@ struct DPlan {
     uint32_t sdep[kMaxQ][kMaxQ];
     // leaf groups: leaves with equal labels, counted jointly by inclusion-exclusion
     uint32_t ngroups;
+    uint32_t nadj_last;              // leaves with a matched neighbour at depth D-1
+    // Triangle leaves: leaf e with exactly two matched neighbours (list indices 0 and 1) that are
+    // adjacent in the query.  tri_tab[e][k] = |list0 ∩ list1| for the k-th candidate edge of the
+    // arc (u_bdep[e][0] -> u_bdep[e][1]), whose offsets start at tri_base[e].
+    uint32_t tri[kMaxQ];
+    uint64_t tri_base[kMaxQ];
+    const uint32_t *tri_tab[kMaxQ];
     uint32_t gsize[kMaxQ];
     uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
     // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient
@@ -57,7 +64,13 @@ struct asdm_ctx {
         uint64_t bytes = 0;
     };
     std::vector<KStat> kstats;
-    uint64_t *d_counters = nullptr;  // [8] device scratch counters
+    struct PendingTimer {
+        int slot;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingTimer> pending;  // recorded, resolved after the stream has drained
+    uint64_t *d_counters = nullptr;  // [8] device counters: 0 count, 1..3 algorithmic bytes of
+                                     // expand_count, expand_fill, leaf kernels
     // caching allocator: freed blocks are kept for reuse (all work is ordered on `stream`, so a
     // block freed by one launch sequence may be handed to the next one immediately)
     struct Block {
@@ -133,5 +146,9 @@ struct KTimer {
     KTimer(asdm_ctx *c, const char *name);
     void stop(uint64_t bytes = 0);
 };
+// Resolves the recorded kernel timers (call after the stream has been synchronized).
+void resolve_timers(asdm_ctx *ctx);
+// Adds algorithmic bytes to the named kernel's entry (created if missing) while profiling.
+void add_kernel_bytes(asdm_ctx *ctx, const char *name, uint64_t bytes);
 
 }  // namespace asdm

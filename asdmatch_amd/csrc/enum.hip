@@ -180,6 +180,66 @@ __global__ void __launch_bounds__(256) k_expand(const DPlan *__restrict__ P, uin
     if ((threadIdx.x & 63) == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
 }
 
+// -------------------------------------------------------------------------- triangle leaves
+// For a triangle leaf e (matched neighbours p0 = u_bdep[e][0], p1 = u_bdep[e][1], p0 ~ p1 in the
+// query), the intersection of its two candidate-edge lists depends only on the candidate edge
+// (f(p0), f(p1)).  It is computed once per candidate edge of arc (p0 -> p1) -- the
+// materialization of the bag {p0, p1, e} -- instead of once per partial embedding.
+// One lane per candidate edge k (global index base.. of the arc's lists).
+__global__ void k_tri_table(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                            uint64_t arc_item_base, uint32_t nitems /* |C(p0)| */, uint64_t e_first,
+                            uint64_t nedges, uint64_t l0_base, uint64_t l1_base, uint32_t *__restrict__ tab) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nedges; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t ek = e_first + k;
+        // i = position of f(p0): last item whose list starts at or before ek
+        uint32_t lo = 0, hi = nitems;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (ce_off[arc_item_base + mid] <= ek) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t i0 = lo, i1 = ce_val[ek];
+        const uint64_t a0 = ce_off[l0_base + i0], a1 = ce_off[l0_base + i0 + 1];
+        const uint64_t b0 = ce_off[l1_base + i1], b1 = ce_off[l1_base + i1 + 1];
+        const uint32_t *A = ce_val + a0, *B = ce_val + b0;
+        uint32_t na = (uint32_t)(a1 - a0), nb = (uint32_t)(b1 - b0);
+        if (na > nb) {
+            const uint32_t *t = A; A = B; B = t;
+            uint32_t tn = na; na = nb; nb = tn;
+        }
+        uint32_t c = 0, p = 0;
+        for (uint32_t t = 0; t < na; ++t) {  // probe the shorter list, galloping in the longer one
+            const uint32_t x = A[t];
+            uint32_t step = 1, q = p;
+            while (q + step < nb && B[q + step] < x) { q += step; step <<= 1; }
+            uint32_t hi2 = q + step < nb ? q + step + 1 : nb;
+            p = q + lower_bound_u32(B + q, hi2 - q, x);
+            if (p < nb && B[p] == x) ++c;
+        }
+        tab[k] = c;
+    }
+}
+
+// Candidates of triangle leaf e given the two matched positions: |list0 ∩ list1| from the table,
+// minus matched same-label vertices lying in both lists.
+__device__ inline uint64_t tri_count(const DPlan *__restrict__ P, uint32_t e, uint32_t i0, uint32_t i1,
+                                     ListRef l0, ListRef l1, const uint64_t *__restrict__ ce_off,
+                                     const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                     const uint32_t *y, uint32_t ns) {
+    const uint64_t lo = ce_off[P->tri_base[e] + i0];
+    const uint32_t len = (uint32_t)(ce_off[P->tri_base[e] + i0 + 1] - lo);
+    const uint32_t k = lower_bound_u32(ce_val + lo, len, i1);  // (f(p0), f(p1)) is a candidate edge
+    uint64_t c = P->tri_tab[e][lo + k - ce_off[P->tri_base[e]]];
+    const uint64_t ce = P->coff[e];
+    for (uint32_t s = 0; s < ns; ++s) {
+        const uint32_t x = lower_bound_u32(cand + ce, P->csz[e], y[s]);
+        if (x < P->csz[e] && cand[ce + x] == y[s] && contains_u32(ce_val + l0.lo, l0.len, x) &&
+            contains_u32(ce_val + l1.lo, l1.len, x))
+            --c;
+    }
+    return c;
+}
+
 // -------------------------------------------------------------------------- leaves
 // |∩_{leaf e in mask} S_e| where S_e = ∩ of e's candidate-edge lists minus matched same-label
 // vertices, reduced over the lane group.  Lists of different leaves index different candidate
@@ -215,6 +275,7 @@ __device__ inline uint64_t leaf_set_count(const DPlan *__restrict__ P, uint32_t 
         }
         return 0;
     }
+
     uint64_t c = 0;
     for (uint32_t t = lg; t < pr.len; t += G) {
         const uint32_t x = probe[t];
@@ -400,6 +461,11 @@ __device__ inline uint64_t leaf_set_count_lane(const DPlan *__restrict__ P, uint
     }
     const uint64_t cand_p = P->coff[pe];
     const uint32_t *probe = ce_val + pr.lo;
+    if (mask == 1u && P->tri[P->gleaf[g][0]]) {
+        const uint32_t e = P->gleaf[g][0];
+        return tri_count(P, e, R.at(P->bdep[e][0]), R.at(P->bdep[e][1]), list_of_v(P, ce_off, R, e, 0),
+                         list_of_v(P, ce_off, R, e, 1), ce_off, ce_val, cand, y, ns);
+    }
     if (nlists == 1) {
         uint32_t h = 0;
         for (uint32_t s = 0; s < ns; ++s) {
@@ -498,10 +564,10 @@ __global__ void __launch_bounds__(256) k_leaf_fused(const DPlan *__restrict__ P,
             uint64_t prod = 1;
             for (uint32_t g = 0; g < ng && prod; ++g) prod *= group_count_lane(P, g, R, ce_off, ce_val, cand);
             acc += prod;
-            bytes += 8;
+            bytes += 4ull + 16ull * P->nadj_last;
         }
         bytes += 4ull * ((pr.len + G - 1 - lg) / G);
-        if (lg == 0) bytes += 4ull * d + 16ull * nb;
+        if (lg == 0) bytes += 4ull * d + 16ull * nb + 16ull * (ng - P->nadj_last);
     }
     acc = group_sum<64>(acc);
     bytes = group_sum<64>(bytes);
@@ -555,10 +621,43 @@ struct Engine {
                ctx->time_limit_ms;
     }
 
+    DPlan hplan;                       // plan copy with the triangle-table pointers filled in
+    std::vector<uint32_t *> tri_tabs;
+
+    int build_tri_tables() {
+        const DPlan &P = plan->h;
+        for (uint32_t e = P.D; e < P.nq; ++e) {
+            if (!P.tri[e]) continue;
+            const uint32_t p0 = plan->order[P.bdep[e][0]], p1 = plan->order[P.bdep[e][1]];
+            const uint32_t a = cs->arc_index(p0, p1);
+            const uint64_t nedges = cs->arc_weight[a];
+            uint32_t *tab = nullptr;
+            int rc = dev_alloc(ctx, (void **)&tab, 4 * std::max<uint64_t>(nedges, 1));
+            if (rc) return rc;
+            tri_tabs.push_back(tab);
+            hplan.tri_tab[e] = tab;
+            uint64_t first = 0;
+            rc = check_hip(ctx, hipMemcpyAsync(&first, cs->d_ce_off + cs->ce_base[a], 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+            if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+            if (rc) return rc;
+            if (nedges) {
+                KTimer kt(ctx, "tri_table");
+                const unsigned grid = (unsigned)std::min<uint64_t>((nedges + 255) / 256, 1u << 16);
+                hipLaunchKernelGGL(k_tri_table, dim3(grid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
+                                   cs->ce_base[a], cs->csize[p0], first, nedges, P.bbase[e][0], P.bbase[e][1], tab);
+                kt.stop(8ull * nedges);
+                if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;
+            }
+        }
+        return ASDM_OK;
+    }
+
     int init() {
         D = plan->D;
-        int rc = dev_alloc(ctx, (void **)&dplan, sizeof(DPlan));
-        if (!rc) rc = check_hip(ctx, hipMemcpyAsync(dplan, &plan->h, sizeof(DPlan), hipMemcpyHostToDevice, ctx->stream), "plan upload");
+        hplan = plan->h;
+        int rc = build_tri_tables();
+        if (!rc) rc = dev_alloc(ctx, (void **)&dplan, sizeof(DPlan));
+        if (!rc) rc = check_hip(ctx, hipMemcpyAsync(dplan, &hplan, sizeof(DPlan), hipMemcpyHostToDevice, ctx->stream), "plan upload");
         if (rc) return rc;
         rows.assign(D + 1, nullptr);
         cap.assign(D + 1, 0);
@@ -589,6 +688,7 @@ struct Engine {
     }
 
     void release() {
+        for (auto t : tri_tabs) dev_free(ctx, t);
         dev_free(ctx, dplan);
         for (auto p : rows) dev_free(ctx, p);
         for (auto p : counts) dev_free(ctx, p);
@@ -605,10 +705,10 @@ struct Engine {
             KTimer kt(ctx, "leaf_fused");
             const uint32_t *rr = r + i0 * (D - 1);
             switch (g_fused) {
-            case 1: hipLaunchKernelGGL((k_leaf_fused<1>), dim3(groups_grid(m, 1)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
-            case 4: hipLaunchKernelGGL((k_leaf_fused<4>), dim3(groups_grid(m, 4)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
-            case 16: hipLaunchKernelGGL((k_leaf_fused<16>), dim3(groups_grid(m, 16)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
-            default: hipLaunchKernelGGL((k_leaf_fused<64>), dim3(groups_grid(m, 64)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes); break;
+            case 1: hipLaunchKernelGGL((k_leaf_fused<1>), dim3(groups_grid(m, 1)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
+            case 4: hipLaunchKernelGGL((k_leaf_fused<4>), dim3(groups_grid(m, 4)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
+            case 16: hipLaunchKernelGGL((k_leaf_fused<16>), dim3(groups_grid(m, 16)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
+            default: hipLaunchKernelGGL((k_leaf_fused<64>), dim3(groups_grid(m, 64)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
             }
             kt.stop();
             int rc = check_hip(ctx, hipGetLastError(), "fused leaf kernel");
@@ -633,7 +733,7 @@ struct Engine {
                                    r + i0 * D, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_mat, d_total);
             else
                 hipLaunchKernelGGL((k_leaf<kG>), dim3(groups_grid(m, kG)), dim3(256), 0, ctx->stream, dplan, r + i0 * D,
-                                   m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes);
+                                   m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2);
             kt.stop();
             int rc = check_hip(ctx, hipGetLastError(), "leaf kernel");
             if (rc) return rc;
@@ -714,7 +814,7 @@ struct Engine {
             KTimer kt(ctx, "expand_fill");
             hipLaunchKernelGGL((k_expand<kG, true>), dim3(groups_grid(i1 - i0, kG)), dim3(256), 0, ctx->stream, dplan, d,
                                r + i0 * d, i1 - i0, cs->d_ce_off, cs->d_ce_val, cs->d_cand, nullptr, offs[d] + i0,
-                               out_base, rows[d + 1], d_bytes);
+                               out_base, rows[d + 1], d_bytes + 1);
             kt.stop();
         }
         rc = check_hip(ctx, hipGetLastError(), "expand fill");
@@ -753,7 +853,8 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     // group must be a single leaf with a single candidate-edge list.
     bool cheap_leaves = true;
     for (uint32_t g = 0; g < plan->h.ngroups; ++g)
-        cheap_leaves &= plan->h.gsize[g] == 1 && plan->h.nb[plan->h.gleaf[g][0]] == 1;
+        cheap_leaves &= plan->h.gsize[g] == 1 &&
+                        (plan->h.nb[plan->h.gleaf[g][0]] == 1 || plan->h.tri[plan->h.gleaf[g][0]]);
     if (!mat_host && plan->D >= 2 && cheap_leaves) {
         E.fused = true;
         const uint32_t d = plan->D - 1;
@@ -779,8 +880,8 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     (void)hipEventRecord(ea, ctx->stream);
     if (!rc) rc = E.run();
     (void)hipEventRecord(eb, ctx->stream);
-    uint64_t res[2] = {0, 0};
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(res, ctx->d_counters, 16, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    uint64_t res[4] = {0, 0, 0, 0};
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(res, ctx->d_counters, 32, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc && mat_host) rc = check_hip(ctx, hipMemcpyAsync(mat_host, E.d_mat, 8 * matn, hipMemcpyDeviceToHost, ctx->stream), "D2H matrix");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     float kms = 0;
@@ -792,9 +893,13 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     (void)hipStreamSynchronize(ctx->stream);
     if (rc) return rc;
     *count = res[0];
+    resolve_timers(ctx);
+    add_kernel_bytes(ctx, "expand_count", res[1]);
+    add_kernel_bytes(ctx, "expand_fill", res[2]);
+    add_kernel_bytes(ctx, E.fused ? "leaf_fused" : (mat_host ? "leaf_local" : "leaf_count"), res[3]);
     ctx->stats.exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats.kernel_ms = kms;
-    ctx->stats.alg_bytes = res[1];
+    ctx->stats.alg_bytes = res[1] + res[2] + res[3];
     ctx->stats.rows_total = E.rows_total;
     ctx->stats.launches = E.launches;
     ctx->stats.plan_ms = plan->plan_ms;

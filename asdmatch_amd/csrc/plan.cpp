@@ -175,20 +175,33 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
             cand.emplace_back(-f, u);
         }
         std::sort(cand.begin(), cand.end());
-        if (nq > 1)
+        // Pass 0 keeps only leaves that the fused last-level kernel counts with a few lookups --
+        // one query neighbour, or two adjacent ones (a triangle, counted from a per-candidate-edge
+        // table) -- with pairwise distinct labels, so no inclusion-exclusion pair intersections
+        // are needed per partial embedding.  Pass 1 (taken when pass 0 admits only the last
+        // vertex) is the plain rule: any independent set within the label-group limit.
+        auto cheap_leaf = [&](uint32_t u) {
+            const uint32_t d = cs->qoff[u + 1] - cs->qoff[u];
+            return d == 1 || (d == 2 && adjacent(cs, cs->qnbr[cs->qoff[u]], cs->qnbr[cs->qoff[u] + 1]));
+        };
+        for (int pass = 0; pass < 2 && nq > 1; ++pass) {
+            leaves = 0;
             for (auto &c : cand) {
                 if (nofactor && leaves) break;
                 uint32_t u = c.second;
+                if (pass == 0 && !cheap_leaf(u)) continue;
                 bool indep = true;
                 for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) indep &= !(leaves >> cs->qnbr[a] & 1);
                 if (!indep) continue;
                 uint32_t same = 1;
                 for (uint64_t l = leaves; l; l &= l - 1) same += cs->qlab[__builtin_ctzll(l)] == cs->qlab[u];
-                if (same > max_group) continue;
+                if (same > (pass == 0 ? 1u : max_group)) continue;
                 uint64_t rest = all & ~(leaves | (1ull << u));
                 if (!connected(cs, rest)) continue;
                 leaves |= 1ull << u;
             }
+            if (__builtin_popcountll(leaves) >= 2 || nofactor) break;
+        }
         // 2) prefix order over the rest: smallest candidate set first, then greedy smallest
         //    estimated growth among vertices adjacent to the matched set
         uint64_t rest = all & ~leaves;
@@ -279,6 +292,20 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
                 done[e] = 1;
             }
     }
+    for (uint32_t e = D; e < nq; ++e)
+        for (uint32_t j = 0; j < P.nb[e]; ++j)
+            if (P.bdep[e][j] + 1 == D) {
+                P.nadj_last++;
+                break;
+            }
+    for (uint32_t e = D; e < nq; ++e)
+        if (P.nb[e] == 2) {
+            const uint32_t a = cs->arc_index(order[P.bdep[e][0]], order[P.bdep[e][1]]);
+            if (a != UINT32_MAX) {
+                P.tri[e] = 1;
+                P.tri_base[e] = cs->ce_base[a];
+            }
+        }
     for (uint32_t k = 2; k <= kMaxGroup; ++k) {
         std::vector<std::vector<uint32_t>> parts;
         std::vector<int64_t> coef;

@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the matching hot path on one MI355X.
+
+Workload (config 2): seeded Chung-Lu graph, 1,000,000 vertices, 15,000,000 undirected edges,
+beta 2.5, 32 labels Zipf(1.2), resident in HBM; the admitted random-walk queries of the config-2
+query stream (asdmatch_amd/workload_manifest.json; admission rule in asdmatch_amd/workloads.py).
+One step = filter (CFL + candidate edges) + plan + exact count of every admitted query, through
+the C ABI.  Every count is checked against the manifest's exact count.
+
+Prints one JSON line: queries/s (value), embeddings/s, the roofline of the dominant kernel (its
+algorithmic bytes over its HIP-event-timed launches in the timed region) and the reference CPU
+path timed on this host on a bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+REF_LOCAL = os.path.join(ROOT, "oracle", "_ref", "ref_driver_local")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leaf_fused_c2_traffic.txt")
+
+
+def start_cpu_baseline(g, q, tmpdir):
+    """Reference ALL_LEVEL build (its exact path), one thread, on one admitted config-2 query."""
+    if not os.path.exists(REF_LOCAL):
+        return None
+    dpath = os.path.join(tmpdir, "c2.graph")
+    qpath = os.path.join(tmpdir, "c2_query.graph")
+    g.write(dpath)
+    q.write(qpath)
+    return subprocess.Popen([REF_LOCAL, "-q", qpath, "-d", dpath], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                            text=True)
+
+
+def finish_cpu_baseline(proc, expect, timeout):
+    if proc is None:
+        return None
+    try:
+        out, _ = proc.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        proc.kill()
+        proc.communicate()
+        return {"error": f"reference did not finish within {timeout} s"}
+    vals = {}
+    for key, pat in (("filter", r"Filtering time \(s\): ([0-9.eE+-]+)"), ("plan", r"Planning Time: ([0-9.eE+-]+)"),
+                     ("exec", r"Execution Time: ([0-9.eE+-]+)"), ("count", r"Number of matches: (\d+)")):
+        m = re.search(pat, out)
+        if m:
+            vals[key] = float(m.group(1)) if key != "count" else int(m.group(1))
+    if "exec" not in vals:
+        return {"error": "reference output not parsed", "tail": out[-300:]}
+    secs = vals.get("filter", 0) + vals.get("plan", 0) + vals["exec"]
+    return {"value": 1.0 / secs, "unit": "queries/s", "cores": 1, "kind": "reference",
+            "seconds": secs, "count": vals.get("count"), "count_matches_gpu": vals.get("count") == expect,
+            "embeddings_per_s": (vals.get("count") or 0) / secs}
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (PMC calibration) and exit")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the matching path runs on a single MI355X here "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe << 20
+        x = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 5}))
+        return
+
+    import asdmatch_amd as A
+    from asdmatch_amd import workloads as W
+
+    t0 = time.time()
+    g = W.graph("c2")
+    adm = W.admitted("c2")
+    queries = [(W.query(g, s, seed), seed, cnt) for s, seed, cnt in adm]
+    gen_s = time.time() - t0
+
+    tmpdir = tempfile.mkdtemp(prefix="asdm_bench_")
+    proc = None
+    if not args.no_cpu_baseline and queries:
+        sample = min(queries, key=lambda t: t[2])  # the smallest admitted query
+        proc = start_cpu_baseline(g, sample[0], tmpdir)
+
+    ctx = A.Context(0)
+    ctx.upload(g)
+
+    def step():
+        total = 0
+        for q, seed, expect in queries:
+            cs = ctx.filter(q, g)
+            plan = cs.plan(q)
+            c = ctx.count(plan, cs)
+            if c != expect:
+                raise SystemExit(f"count mismatch for query seed {seed}: {c} != {expect}")
+            total += c
+        return total
+
+    for _ in range(args.warmup):
+        step()
+    ctx.set_profiling(True)  # HIP events around every launch, resolved after the timed region
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    t1 = time.perf_counter()
+    emb = 0
+    for _ in range(args.steps):
+        emb += step()
+    hip.hipDeviceSynchronize()
+    t2 = time.perf_counter()
+    kstats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+
+    elapsed = t2 - t1
+    nq = len(queries) * args.steps
+    dom = max(kstats, key=lambda k: k["ms"]) if kstats else None
+    roofline = None
+    if dom and dom["ms"] > 0 and dom["launches"]:
+        bytes_per_launch = dom["bytes"] / dom["launches"]
+        avg_s = dom["ms"] / 1e3 / dom["launches"]
+        achieved = bytes_per_launch / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(), "kernel": dom["name"],
+                    "alg_bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4),
+                    "launches": dom["launches"]}
+
+    cpu = finish_cpu_baseline(proc, min(queries, key=lambda t: t[2])[2] if queries else None, timeout=600)
+    if cpu and "value" in cpu:
+        cpu["sample"] = ("reference ALL_LEVEL build (exact enumeration path), single thread, on the smallest "
+                         "admitted config-2 query; time = its Filtering + Planning + Execution lines "
+                         "(graph load excluded, as the GPU graph is resident)")
+
+    out = {
+        "metric": "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts",
+        "value": round(nq / elapsed, 4),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 ids / u64 counts",
+        "data": "synthetic",
+        "config": {"workload": "config 2: Chung-Lu 1M V / 15M E, beta 2.5, 32 labels Zipf(1.2); admitted "
+                               "random-walk queries, plain GPU engine (filter + plan + exact count)",
+                   "queries_per_step": len(queries), "query_sizes": [q.n for q, _, _ in queries],
+                   "admission_row_limit": W.load_manifest()["c2"]["row_limit"]},
+        "embeddings_per_s": round(emb / elapsed, 1),
+        "embeddings_per_step": emb // max(1, args.steps),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats],
+        "setup_s": round(gen_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

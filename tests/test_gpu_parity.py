@@ -134,7 +134,8 @@ def _rand_graph(rng, n, p, nl):
 
 
 def test_same_label_leaf_groups_against_bruteforce(ctx):
-    """Stars and trees whose leaves share labels exercise the inclusion-exclusion leaf counting."""
+    """Stars and trees whose leaves share labels exercise the inclusion-exclusion leaf counting;
+    triangles and diamonds exercise the per-candidate-edge triangle tables."""
     rng = random.Random(9)
     for trial in range(12):
         gl, ge = _rand_graph(rng, 60, 0.15, 2)
@@ -144,6 +145,10 @@ def test_same_label_leaf_groups_against_bruteforce(ctx):
             ([0, 1, 1, 1, 1], [(0, 1), (0, 2), (0, 3), (0, 4)]),
             ([0, 1, 0, 1, 1], [(0, 1), (1, 2), (2, 3), (2, 4), (0, 4)]),
             ([1, 0, 1, 0, 1, 1], [(0, 1), (1, 2), (2, 3), (1, 4), (3, 5)]),
+            # triangle and diamond leaves (edge-table path)
+            ([0, 1, 1], [(0, 1), (1, 2), (0, 2)]),
+            ([0, 1, 0, 1], [(0, 1), (1, 2), (2, 3), (3, 0), (0, 2)]),
+            ([0, 0, 1, 1, 0], [(0, 1), (1, 2), (0, 2), (2, 3), (3, 4), (2, 4)]),
         ):
             q, oq = T.graph_from_lists(ql, qe)
             want = O.bruteforce_count(oq, og)
