@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 REF_LOCAL = os.path.join(ROOT, "oracle", "_ref", "ref_driver_local")
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leaf_fused_c2_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_traffic.txt")
 
 
 def start_cpu_baseline(g, q, tmpdir):
@@ -65,14 +65,19 @@ def finish_cpu_baseline(proc, expect, timeout):
             "embeddings_per_s": (vals.get("count") or 0) / secs}
 
 
-def read_traffic():
+def read_traffic(kernel):
+    """Calibrated HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
+    vals = {}
     with open(TRAFFIC_FILE) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
-    return None
+            if ":" in line:
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    if vals.get("kernel") != kernel or "traffic_bytes_per_launch" not in vals:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
 
 
 def main():
@@ -156,7 +161,8 @@ def main():
         avg_s = dom["ms"] / 1e3 / dom["launches"]
         achieved = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(), "kernel": dom["name"],
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(dom["name"]),
+                    "kernel": dom["name"],
                     "alg_bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4),
                     "launches": dom["launches"]}
 
