@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "host.h"
@@ -118,6 +119,7 @@ struct asdm_plan {
     bool local = false;
     double plan_ms = 0;
     double est_leaf = 0, est_total = 0;  // estimated rows at depth D and summed over depths 1..D
+    std::vector<std::pair<uint64_t, double>> prefix_cards;  // (prefix subset mask, estimate)
 };
 
 namespace asdm {

@@ -256,6 +256,16 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         }
         out->est_leaf = est;
         out->est_total = tot;
+        // prefix cardinalities along the order (the subsetToCard entries this plan relies on)
+        out->prefix_cards.clear();
+        double e2 = cs->csize[order[0]];
+        uint64_t S2 = 1ull << order[0];
+        out->prefix_cards.emplace_back(S2, e2);
+        for (uint32_t d = 1; d < D; ++d) {
+            e2 = card_of(S2 | (1ull << order[d]), e2 * growth(cs, S2, order[d]));
+            S2 |= 1ull << order[d];
+            out->prefix_cards.emplace_back(S2, e2);
+        }
     }
 
     DPlan &P = out->h;
@@ -363,6 +373,18 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
 }
 
 void asdm_plan_free(asdm_plan *p) { delete p; }
+
+int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n) {
+    if (!p || !n) return ASDM_E_INVALID;
+    *n = (uint32_t)p->prefix_cards.size();
+    if (!masks || !cards) return ASDM_OK;
+    if (cap < *n) return ASDM_E_INVALID;
+    for (uint32_t i = 0; i < *n; ++i) {
+        masks[i] = p->prefix_cards[i].first;
+        cards[i] = p->prefix_cards[i].second;
+    }
+    return ASDM_OK;
+}
 
 int asdm_plan_estimate(const asdm_plan *p, double *rows_leaf, double *rows_total) {
     if (!p) return ASDM_E_INVALID;

@@ -168,6 +168,19 @@ int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out);
 int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
                      uint64_t budget_bytes, uint64_t *counts, float *lat_ms);
 
+/* ---- .card planner files ---- */
+/* Reads the subsetToCard map (and the stored planning time) of a .card file written by the
+ * reference's prepare.out (executable/prepare.cpp:65-71, plan/estimator.cpp:981-1008) or by
+ * asdm_card_write.  With masks == NULL only *n is returned. */
+int asdm_card_read(const char *path, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n,
+                   double *dp_time);
+/* Writes a .card file holding only the map and the time (no per-bag DP blobs, which this
+ * planner does not use); values are truncated to two decimals like saveSubsetToCard. */
+int asdm_card_write(const char *path, const uint64_t *masks, const double *cards, uint32_t n,
+                    double dp_time);
+/* Planner subset cardinalities of a plan: masks/cards of every prefix of its order. */
+int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n);
+
 /* ---- primitives (tests, autotuner) ---- */
 /* k-way intersection of ascending unique device-independent host lists, computed on the GPU:
  * out[cap] receives the ascending intersection, *n its size. */

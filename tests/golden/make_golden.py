@@ -183,6 +183,49 @@ def synth():
         json.dump(cases, f, indent=1)
 
 
+CARD_QUERIES = ["query_8_1", "query_10_7", "query_12_26", "query_16_52"]
+
+
+def cards():
+    """A few shipped yeast queries with the .card files the reference dataset ships for them, and
+    the reference's exact counts (for the CLI's -s path)."""
+    cdir = os.path.join(OUT, "yeast", "cards")
+    os.makedirs(cdir, exist_ok=True)
+    with tarfile.open(os.path.join(REF, "dataset", "yeast.tar.gz")) as tf:
+        for q in CARD_QUERIES:
+            for ext in (".graph", ".card"):
+                with open(os.path.join(cdir, q + ext), "wb") as f:
+                    f.write(tf.extractfile(f"yeast/query_graph/{q}{ext}").read())
+    dpath = os.path.join(OUT, "yeast", "yeast.graph")
+    counts = {q: run_ref("ref_driver_local", os.path.join(cdir, q + ".graph"), dpath, timeout=120)[0]
+              for q in CARD_QUERIES}
+    with open(os.path.join(cdir, "golden.json"), "w") as f:
+        json.dump({"source": "oracle/_ref/ref_driver_local (reference ALL_LEVEL build) on "
+                             "tests/golden/yeast/yeast.graph; .card files are the reference dataset's own",
+                   "counts": counts}, f, indent=1)
+
+
+def c2_sample():
+    """Config-2 graph and its admitted slot-5 query (the smallest admitted count): the reference's
+    exact build takes about a minute on it (tests/golden/c2_reference.json)."""
+    from asdmatch_amd import workloads as W
+
+    g = W.graph("c2")
+    slot, seed = 5, 2787872138
+    q = W.query(g, W.slots("c2")[slot], seed)
+    dpath, qpath = "/tmp/golden_c2.graph", "/tmp/golden_c2_q.graph"
+    g.write(dpath)
+    q.write(qpath)
+    cnt, _ = run_ref("ref_driver_local", qpath, dpath, timeout=1200)
+    wrong, _ = run_ref("ref_driver", qpath, dpath, timeout=1200)
+    with open(os.path.join(OUT, "c2_reference.json"), "w") as f:
+        json.dump({"source": "oracle/_ref/ref_driver_local (reference ALL_LEVEL build, exact enumeration)",
+                   "graph": "workloads.graph('c2')", "data_sha256": sha256_file(dpath), "slot": slot, "seed": seed,
+                   "count": cnt, "count_build_output": wrong}, f, indent=1)
+
+
 if __name__ == "__main__":
     yeast()
     synth()
+    cards()
+    c2_sample()

@@ -169,3 +169,17 @@ def test_local_counts_match_reference(ctx):
             assert c == rec["count"]
             assert np.array_equal(mat, ref), (case["name"], rec["seed"])
             assert all(int(s) == c for s in mat.sum(axis=0))
+
+
+def test_config2_count_matches_reference(ctx):
+    """Config-2 scale (1M vertices, 15M edges): the reference's exact build counted this admitted
+    query (2,265,571,276 embeddings, tests/golden/c2_reference.json)."""
+    import json
+
+    from asdmatch_amd import workloads as W
+
+    ref = json.load(open(f"{T.GOLDEN}/c2_reference.json"))
+    g = W.graph("c2")
+    s = W.slots("c2")[ref["slot"]]
+    q = W.query(g, s, ref["seed"])
+    assert A.count_embeddings(ctx, q, g) == ref["count"]
