@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import json
 import os
+import time
 from dataclasses import dataclass
 from typing import List, Optional
 
@@ -103,13 +104,16 @@ def admit(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: int = 40,
                     continue
                 plan = cs.plan(q, local=local)
                 try:
+                    t0 = time.perf_counter()
                     c = (ctx.local_count(plan, cs, g.n)[1] if local else ctx.count(plan, cs))
+                    dt = time.perf_counter() - t0
                 except AsdmError as e:
                     if e.code != -8:
                         raise
                     log(f"  {cfg} slot {i} draw {j}: rejected ({'guard' if 'time' in str(e) else 'rows'})")
                     continue
                 got = (i, seed, c)
+                log(f"  {cfg} slot {i} draw {j}: admitted, count {c}, {dt * 1e3:.2f} ms")
                 break
             log(f"{cfg} slot {i} k={s.k} -> {got}")
             if got:
