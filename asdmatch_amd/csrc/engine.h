@@ -88,6 +88,17 @@ struct asdm_ctx {
 };
 
 struct asdm_cs {
+    // Device buffers shared by the candidate spaces of one filter call (a batch of queries); the
+    // last view freed releases them.
+    struct Store {
+        uint32_t refs = 0;
+        uint32_t *d_cand = nullptr;
+        uint64_t *d_bits = nullptr;
+        uint32_t *d_rank = nullptr;
+        uint64_t *d_ce_off = nullptr;
+        uint32_t *d_ce_val = nullptr;
+    };
+    Store *store = nullptr;
     asdm_ctx *ctx = nullptr;
     uint32_t nq = 0, nv = 0, root = 0;
     std::vector<uint32_t> qlab, qoff, qnbr;  // query CSR copy (label-sorted)
@@ -101,7 +112,8 @@ struct asdm_cs {
     std::vector<uint32_t> arc_u, arc_u2;     // query arcs in query CSR order
     std::vector<uint64_t> ce_base;           // [narcs + 1] first index of arc a in d_ce_off
     std::vector<uint64_t> arc_weight;        // total candidate-edge list length per arc
-    uint64_t *d_ce_off = nullptr;            // [ce_base[narcs] + 1] list offsets into d_ce_val
+    uint64_t *d_ce_off = nullptr;            // [ce_base[narcs] + 1] list offsets into d_ce_val (global
+                                             // offsets of the shared store)
     uint32_t *d_ce_val = nullptr;            // candidate-edge lists: positions in C(u2), ascending
     uint64_t ce_total = 0;
     double filter_ms = 0;
@@ -132,6 +144,10 @@ void dev_free(asdm_ctx *ctx, void *p);
 int scan_u32_to_u64(asdm_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);
 int fill_u64(asdm_ctx *ctx, uint64_t *p, uint64_t n, uint64_t v);
 int fill_u32(asdm_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
+
+// Batched CFL filter + candidate edges (filter.hip)
+int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nb, asdm_cs **out,
+                int *status);
 
 // Planner (plan.cpp)
 int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, const uint64_t *masks,

@@ -33,7 +33,7 @@ struct QV {
 };
 
 struct Par {  // one parent test of a prune pass
-    uint32_t p;        // parent query vertex
+    uint32_t p;        // parent slot (query vertex of one query in the batch)
     uint32_t lab;      // L(p)
     uint32_t labcount; // |V_L(p)|
 };
@@ -104,9 +104,10 @@ __global__ void k_prune(const uint32_t *__restrict__ nbr, const uint32_t *__rest
 
 // Per query vertex: rank[w] = set bits before word w; cand[cand_off[u] + rank] = ascending ids.
 __global__ void k_compact(const uint64_t *__restrict__ bits, uint32_t W, uint32_t *__restrict__ rank,
-                          const uint64_t *__restrict__ cand_off, uint32_t *__restrict__ cand) {
+                          const uint64_t *__restrict__ cand_off, uint32_t *__restrict__ cand,
+                          const uint32_t *__restrict__ slots) {
     __shared__ uint64_t lds[1024 / 64 + 1];
-    const uint32_t u = blockIdx.x;
+    const uint32_t u = slots[blockIdx.x];
     const uint64_t *b = bits + (size_t)u * W;
     uint32_t *out = cand + cand_off[u];
     uint64_t carry = 0;
@@ -194,7 +195,11 @@ __global__ void k_cand_edges(const uint32_t *__restrict__ nbr, const uint32_t *_
     }
 }
 
-uint32_t cfl_dummy_anchor();
+// out[a] = offs[item_base[a]] for a in [0, n): the first candidate-edge offset of each arc.
+__global__ void k_gather_u64(const uint64_t *__restrict__ offs, const uint64_t *__restrict__ idx, uint32_t n,
+                             uint64_t *__restrict__ out) {
+    for (uint32_t a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) out[a] = offs[idx[a]];
+}
 
 }  // namespace asdm
 
@@ -228,92 +233,124 @@ void asdm_cs_free(asdm_cs *cs) {
     if (!cs) return;
     asdm_ctx *ctx = cs->ctx;
     (void)hipSetDevice(ctx->device);
-    dev_free(ctx, cs->d_cand);
-    dev_free(ctx, cs->d_bits);
-    dev_free(ctx, cs->d_rank);
-    dev_free(ctx, cs->d_ce_off);
-    dev_free(ctx, cs->d_ce_val);
+    if (cs->store) {
+        if (--cs->store->refs == 0) {
+            dev_free(ctx, cs->store->d_cand);
+            dev_free(ctx, cs->store->d_bits);
+            dev_free(ctx, cs->store->d_rank);
+            dev_free(ctx, cs->store->d_ce_off);
+            dev_free(ctx, cs->store->d_ce_val);
+            delete cs->store;
+        }
+    }
     (void)hipStreamSynchronize(ctx->stream);
     delete cs;
 }
 
-int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asdm_cs **out) {
-    if (!ctx || !q || !g || !out) return ASDM_E_INVALID;
-    *out = nullptr;
+}  // extern "C"
+
+namespace asdm {
+
+// CFL filter + candidate edges for nb queries against one uploaded data graph, with every phase
+// batched across the queries: one base-filter launch over all query vertices ("slots"), one
+// Gen launch per BFS position, one final prune, one compaction, one candidate-edge count/scan/
+// fill.  out[i] receives a view into the shared buffers, or nullptr with status[i] =
+// ASDM_E_EMPTY_CANDIDATES when some candidate set of query i is empty.
+int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nb, asdm_cs **out,
+                int *status) {
+    for (uint32_t i = 0; i < nb; ++i) {
+        out[i] = nullptr;
+        status[i] = ASDM_E_INVALID;
+    }
     if (g->dev.device != ctx->device) return ctx->fail(ASDM_E_INVALID, "data graph not uploaded to this device");
-    if (q->n == 0 || q->n > kMaxQ) return ctx->fail(ASDM_E_UNSUPPORTED, "query must have 1..32 vertices");
+    for (uint32_t i = 0; i < nb; ++i)
+        if (!qs[i] || qs[i]->n == 0 || qs[i]->n > kMaxQ)
+            return ctx->fail(ASDM_E_UNSUPPORTED, "query must have 1..32 vertices");
     (void)hipSetDevice(ctx->device);
     auto t0 = std::chrono::steady_clock::now();
-    const uint32_t nq = q->n, n = g->n, L = g->nlab;
+    const uint32_t n = g->n, L = g->nlab;
     const uint32_t W = (n + 63) / 64;
-    auto cs = std::make_unique<asdm_cs>();
-    cs->ctx = ctx;
-    cs->nq = nq;
-    cs->nv = n;
-    cs->W = W;
-    cs->qlab = q->lab;
-    cs->qoff = q->off;
-    cs->qnbr = q->nbr;
+    std::vector<uint32_t> sb(nb + 1, 0);  // first slot of each query
+    for (uint32_t i = 0; i < nb; ++i) sb[i + 1] = sb[i] + qs[i]->n;
+    const uint32_t S = sb[nb];
 
-    // query vertex descriptors
-    std::vector<QV> qv(nq);
-    for (uint32_t u = 0; u < nq; ++u) {
-        QV &d = qv[u];
-        std::memset(&d, 0, sizeof(d));
-        d.lab = q->lab[u];
-        d.deg = q->deg(u);
-        for (uint32_t i = q->off[u]; i < q->off[u + 1];) {
-            uint32_t l = q->lab[q->nbr[i]], j = i;
-            while (j < q->off[u + 1] && q->lab[q->nbr[j]] == l) ++j;
-            d.nlc_lab[d.nnlc] = l;
-            d.nlc_cnt[d.nnlc] = j - i;
-            d.nnlc++;
-            i = j;
+    std::vector<QV> qv(S);
+    for (uint32_t i = 0; i < nb; ++i) {
+        const asdm_graph *q = qs[i];
+        for (uint32_t u = 0; u < q->n; ++u) {
+            QV &d = qv[sb[i] + u];
+            std::memset(&d, 0, sizeof(d));
+            d.lab = q->lab[u];
+            d.deg = q->deg(u);
+            for (uint32_t k = q->off[u]; k < q->off[u + 1];) {
+                uint32_t l = q->lab[q->nbr[k]], j = k;
+                while (j < q->off[u + 1] && q->lab[q->nbr[j]] == l) ++j;
+                d.nlc_lab[d.nnlc] = l;
+                d.nlc_cnt[d.nnlc] = j - k;
+                d.nnlc++;
+                k = j;
+            }
         }
     }
     DevBuf dqv(ctx), dbase(ctx), dgen(ctx), dfin(ctx), dcnt(ctx);
-    int rc = dev_alloc(ctx, &dqv.p, sizeof(QV) * nq);
-    if (!rc) rc = dev_alloc(ctx, &dbase.p, sizeof(uint64_t) * (size_t)W * nq);
-    if (!rc) rc = dev_alloc(ctx, &dgen.p, sizeof(uint64_t) * (size_t)W * nq);
-    if (!rc) rc = dev_alloc(ctx, &dfin.p, sizeof(uint64_t) * (size_t)W * nq);
-    if (!rc) rc = dev_alloc(ctx, &dcnt.p, sizeof(uint32_t) * 3 * nq);  // base, gen, final counts
+    int rc = dev_alloc(ctx, &dqv.p, sizeof(QV) * S);
+    if (!rc) rc = dev_alloc(ctx, &dbase.p, sizeof(uint64_t) * (size_t)W * S);
+    if (!rc) rc = dev_alloc(ctx, &dgen.p, sizeof(uint64_t) * (size_t)W * S);
+    if (!rc) rc = dev_alloc(ctx, &dfin.p, sizeof(uint64_t) * (size_t)W * S);
+    if (!rc) rc = dev_alloc(ctx, &dcnt.p, sizeof(uint32_t) * 3 * S);
     if (rc) return rc;
-    uint32_t *cnt_base = dcnt.as<uint32_t>(), *cnt_gen = cnt_base + nq, *cnt_fin = cnt_gen + nq;
-    if ((rc = upload(ctx, dqv.p, qv.data(), sizeof(QV) * nq))) return rc;
-    if ((rc = fill_u32(ctx, cnt_base, 3 * nq, 0))) return rc;
+    uint32_t *cnt_base = dcnt.as<uint32_t>(), *cnt_gen = cnt_base + S, *cnt_fin = cnt_gen + S;
+    if ((rc = upload(ctx, dqv.p, qv.data(), sizeof(QV) * S))) return rc;
+    if ((rc = fill_u32(ctx, cnt_base, 3 * (uint64_t)S, 0))) return rc;
+    if ((rc = fill_u64(ctx, dgen.as<uint64_t>(), (uint64_t)W * S, 0))) return rc;
+    if ((rc = fill_u64(ctx, dfin.as<uint64_t>(), (uint64_t)W * S, 0))) return rc;
     {
-        dim3 grid((W * 64 + 255) / 256, nq);
         KTimer kt(ctx, "filter_base");
-        hipLaunchKernelGGL(k_base, grid, dim3(256), 0, ctx->stream, g->dev.lab, g->dev.off, g->dev.loff, n, L,
-                           dqv.as<QV>(), dbase.as<uint64_t>(), W, cnt_base);
-        kt.stop((uint64_t)n * 4 * 2 * nq);
+        hipLaunchKernelGGL(k_base, dim3((W * 64 + 255) / 256, S), dim3(256), 0, ctx->stream, g->dev.lab, g->dev.off,
+                           g->dev.loff, n, L, dqv.as<QV>(), dbase.as<uint64_t>(), W, cnt_base);
+        kt.stop((uint64_t)n * 8 * S);
         if ((rc = check_hip(ctx, hipGetLastError(), "k_base"))) return rc;
     }
-    std::vector<uint32_t> base_cnt(nq);
-    if ((rc = download(ctx, base_cnt.data(), cnt_base, sizeof(uint32_t) * nq))) return rc;
+    std::vector<uint32_t> base_cnt(S);
+    if ((rc = download(ctx, base_cnt.data(), cnt_base, sizeof(uint32_t) * S))) return rc;
 
-    // CFL root and BFS order (host, tiny)
-    const uint32_t root = cfl_root(q, g, base_cnt);
-    cs->root = root;
-    std::vector<uint32_t> order = bfs_order(q, root);
-    std::vector<uint32_t> pos(nq, UINT32_MAX);
-    for (uint32_t i = 0; i < order.size(); ++i) pos[order[i]] = i;
-
-    // parent lists: before (Gen passes, one launch per BFS position) and after (one launch)
-    std::vector<uint32_t> us, par_off{0};
+    // CFL roots and BFS orders (host); parent lists by BFS position for the Gen launches, and one
+    // list per reached slot for the final prune
+    std::vector<std::vector<uint32_t>> order(nb);
+    std::vector<uint32_t> roots(nb);
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; i < nb; ++i) {
+        std::vector<uint32_t> bc(base_cnt.begin() + sb[i], base_cnt.begin() + sb[i + 1]);
+        roots[i] = cfl_root(qs[i], g, bc);
+        order[i] = bfs_order(qs[i], roots[i]);
+        maxlen = std::max<uint32_t>(maxlen, (uint32_t)order[i].size());
+    }
+    std::vector<uint32_t> us, par_off{0}, launch_start;
     std::vector<Par> pars;
-    auto add_list = [&](uint32_t u, bool before) {
-        us.push_back(u);
+    auto add_list = [&](uint32_t i, uint32_t posi, bool before) {
+        const asdm_graph *q = qs[i];
+        const uint32_t u = order[i][posi];
+        std::vector<uint32_t> pos(q->n, UINT32_MAX);
+        for (uint32_t k = 0; k < order[i].size(); ++k) pos[order[i][k]] = k;
+        us.push_back(sb[i] + u);
         for (uint32_t a = q->off[u]; a < q->off[u + 1]; ++a) {
             uint32_t p = q->nbr[a];
             if (pos[p] == UINT32_MAX) continue;
-            if (before ? pos[p] < pos[u] : pos[p] > pos[u]) pars.push_back({p, q->lab[p], g->label_count(q->lab[p])});
+            if (before ? pos[p] < posi : pos[p] > posi)
+                pars.push_back({sb[i] + p, q->lab[p], g->label_count(q->lab[p])});
         }
         par_off.push_back((uint32_t)pars.size());
     };
-    for (uint32_t i = 0; i < order.size(); ++i) add_list(order[i], true);
-    const uint32_t nfinal_start = (uint32_t)us.size();
-    for (uint32_t i = 0; i < order.size(); ++i) add_list(order[i], false);
+    for (uint32_t k = 0; k < maxlen; ++k) {
+        launch_start.push_back((uint32_t)us.size());
+        for (uint32_t i = 0; i < nb; ++i)
+            if (k < order[i].size()) add_list(i, k, true);
+    }
+    const uint32_t final_start = (uint32_t)us.size();
+    launch_start.push_back(final_start);
+    for (uint32_t i = 0; i < nb; ++i)
+        for (uint32_t k = 0; k < order[i].size(); ++k) add_list(i, k, false);
+    const uint32_t nfinal = (uint32_t)us.size() - final_start;
     DevBuf dus(ctx), dpo(ctx), dpars(ctx);
     rc = dev_alloc(ctx, &dus.p, sizeof(uint32_t) * us.size());
     if (!rc) rc = dev_alloc(ctx, &dpo.p, sizeof(uint32_t) * par_off.size());
@@ -321,116 +358,218 @@ int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asd
     if (!rc) rc = upload(ctx, dus.p, us.data(), sizeof(uint32_t) * us.size());
     if (!rc) rc = upload(ctx, dpo.p, par_off.data(), sizeof(uint32_t) * par_off.size());
     if (!rc && !pars.empty()) rc = upload(ctx, dpars.p, pars.data(), sizeof(Par) * pars.size());
-    // unreached query vertices (disconnected query) keep empty Gen/final bitmaps
-    if (!rc) rc = fill_u64(ctx, dgen.as<uint64_t>(), (uint64_t)W * nq, 0);
-    if (!rc) rc = fill_u64(ctx, dfin.as<uint64_t>(), (uint64_t)W * nq, 0);
     if (rc) return rc;
     const uint32_t lab0 = n ? g->lab[0] : 0;
     {
         KTimer kt(ctx, "filter_gen");
-        for (uint32_t i = 0; i < order.size(); ++i) {
-            hipLaunchKernelGGL(k_prune, dim3((W * 64 + 255) / 256, 1), dim3(256), 0, ctx->stream, g->dev.nbr, g->dev.loff, n,
-                               L, lab0, dbase.as<uint64_t>(), dgen.as<uint64_t>(), dgen.as<uint64_t>(), cnt_gen, W,
-                               dus.as<uint32_t>() + i, dpo.as<uint32_t>() + i, dpars.as<Par>(), cnt_gen);
+        for (uint32_t k = 0; k < maxlen; ++k) {
+            const uint32_t a = launch_start[k], cnt = launch_start[k + 1] - a;
+            hipLaunchKernelGGL(k_prune, dim3((W * 64 + 255) / 256, cnt), dim3(256), 0, ctx->stream, g->dev.nbr,
+                               g->dev.loff, n, L, lab0, dbase.as<uint64_t>(), dgen.as<uint64_t>(), dgen.as<uint64_t>(),
+                               cnt_gen, W, dus.as<uint32_t>() + a, dpo.as<uint32_t>() + a, dpars.as<Par>(), cnt_gen);
         }
         kt.stop();
         if ((rc = check_hip(ctx, hipGetLastError(), "k_prune gen"))) return rc;
         KTimer kf(ctx, "filter_final");
-        hipLaunchKernelGGL(k_prune, dim3((W * 64 + 255) / 256, (uint32_t)order.size()), dim3(256), 0, ctx->stream,
-                           g->dev.nbr, g->dev.loff, n, L, lab0, dgen.as<uint64_t>(), dfin.as<uint64_t>(),
-                           dgen.as<uint64_t>(), cnt_gen, W, dus.as<uint32_t>() + nfinal_start,
-                           dpo.as<uint32_t>() + nfinal_start, dpars.as<Par>(), cnt_fin);
+        hipLaunchKernelGGL(k_prune, dim3((W * 64 + 255) / 256, nfinal), dim3(256), 0, ctx->stream, g->dev.nbr,
+                           g->dev.loff, n, L, lab0, dgen.as<uint64_t>(), dfin.as<uint64_t>(), dgen.as<uint64_t>(),
+                           cnt_gen, W, dus.as<uint32_t>() + final_start, dpo.as<uint32_t>() + final_start,
+                           dpars.as<Par>(), cnt_fin);
         kf.stop();
         if ((rc = check_hip(ctx, hipGetLastError(), "k_prune final"))) return rc;
     }
-    std::vector<uint32_t> fin(nq);
-    if ((rc = download(ctx, fin.data(), cnt_fin, sizeof(uint32_t) * nq))) return rc;
-    for (uint32_t u = 0; u < nq; ++u)
-        if (fin[u] == 0) {
-            cs->filter_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            return ASDM_E_EMPTY_CANDIDATES;
-        }
-    cs->csize = fin;
-    cs->cand_off.assign(nq + 1, 0);
-    for (uint32_t u = 0; u < nq; ++u) cs->cand_off[u + 1] = cs->cand_off[u] + fin[u];
+    std::vector<uint32_t> fin(S);
+    if ((rc = download(ctx, fin.data(), cnt_fin, sizeof(uint32_t) * S))) return rc;
 
-    // compaction into sorted candidate lists + per-word ranks
-    DevBuf dcoff(ctx);
-    rc = dev_alloc(ctx, (void **)&cs->d_cand, sizeof(uint32_t) * cs->cand_off[nq]);
-    if (!rc) rc = dev_alloc(ctx, (void **)&cs->d_rank, sizeof(uint32_t) * (size_t)W * nq);
-    if (!rc) rc = dev_alloc(ctx, &dcoff.p, sizeof(uint64_t) * (nq + 1));
-    if (!rc) rc = upload(ctx, dcoff.p, cs->cand_off.data(), sizeof(uint64_t) * (nq + 1));
-    if (rc) return rc;
+    // non-empty queries keep their slots; the others get no candidate storage
+    std::vector<char> ok(nb, 1);
+    for (uint32_t i = 0; i < nb; ++i)
+        for (uint32_t s2 = sb[i]; s2 < sb[i + 1]; ++s2) ok[i] &= fin[s2] != 0;
+    std::vector<uint64_t> cand_off(S + 1, 0);
+    std::vector<uint32_t> live;
+    for (uint32_t i = 0; i < nb; ++i)
+        for (uint32_t s2 = sb[i]; s2 < sb[i + 1]; ++s2) {
+            cand_off[s2 + 1] = cand_off[s2] + (ok[i] ? fin[s2] : 0);
+            if (ok[i]) live.push_back(s2);
+        }
+    auto store = new asdm_cs::Store();
+    store->refs = 0;
+    auto drop_store = [&]() {
+        dev_free(ctx, store->d_cand);
+        dev_free(ctx, store->d_bits);
+        dev_free(ctx, store->d_rank);
+        dev_free(ctx, store->d_ce_off);
+        dev_free(ctx, store->d_ce_val);
+        delete store;
+    };
+    if (live.empty()) {
+        delete store;
+        for (uint32_t i = 0; i < nb; ++i) status[i] = ASDM_E_EMPTY_CANDIDATES;
+        return ASDM_OK;
+    }
+    DevBuf dcoff(ctx), dlive(ctx);
+    rc = dev_alloc(ctx, (void **)&store->d_cand, sizeof(uint32_t) * std::max<uint64_t>(cand_off[S], 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&store->d_rank, sizeof(uint32_t) * (size_t)W * S);
+    if (!rc) rc = dev_alloc(ctx, &dcoff.p, sizeof(uint64_t) * (S + 1));
+    if (!rc) rc = dev_alloc(ctx, &dlive.p, sizeof(uint32_t) * live.size());
+    if (!rc) rc = upload(ctx, dcoff.p, cand_off.data(), sizeof(uint64_t) * (S + 1));
+    if (!rc) rc = upload(ctx, dlive.p, live.data(), sizeof(uint32_t) * live.size());
+    if (rc) {
+        drop_store();
+        return rc;
+    }
     {
         KTimer kt(ctx, "filter_compact");
-        hipLaunchKernelGGL(k_compact, dim3(nq), dim3(1024), 0, ctx->stream, dfin.as<uint64_t>(), W, cs->d_rank,
-                           dcoff.as<uint64_t>(), cs->d_cand);
+        hipLaunchKernelGGL(k_compact, dim3((uint32_t)live.size()), dim3(1024), 0, ctx->stream, dfin.as<uint64_t>(), W,
+                           store->d_rank, dcoff.as<uint64_t>(), store->d_cand, dlive.as<uint32_t>());
         kt.stop();
-        if ((rc = check_hip(ctx, hipGetLastError(), "k_compact"))) return rc;
-    }
-    cs->d_bits = dfin.as<uint64_t>();
-    dfin.p = nullptr;  // ownership moves to cs
-
-    // candidate edges
-    const uint32_t narcs = (uint32_t)q->m;
-    cs->narcs = narcs;
-    cs->arc_u.resize(narcs);
-    cs->arc_u2.resize(narcs);
-    std::vector<ArcInfo> arcs(narcs);
-    cs->ce_base.assign(narcs + 1, 0);
-    for (uint32_t u = 0; u < nq; ++u)
-        for (uint32_t a = q->off[u]; a < q->off[u + 1]; ++a) {
-            uint32_t u2 = q->nbr[a];
-            cs->arc_u[a] = u;
-            cs->arc_u2[a] = u2;
-            arcs[a] = {u, u2, q->lab[u2], cs->ce_base[a], cs->cand_off[u]};
-            cs->ce_base[a + 1] = cs->ce_base[a] + fin[u];
+        if ((rc = check_hip(ctx, hipGetLastError(), "k_compact"))) {
+            drop_store();
+            return rc;
         }
-    const uint64_t nitems = cs->ce_base[narcs];
-    DevBuf darcs(ctx), dcounts(ctx);
+    }
+    store->d_bits = dfin.as<uint64_t>();
+    dfin.p = nullptr;  // ownership moves to the store
+
+    // candidate edges over every arc of every non-empty query (slot ids, global item space)
+    std::vector<ArcInfo> arcs;
+    std::vector<uint64_t> arc_first;   // first item of each arc (global)
+    std::vector<uint32_t> qarc0(nb + 1, 0);
+    uint64_t nitems = 0;
+    for (uint32_t i = 0; i < nb; ++i) {
+        qarc0[i] = (uint32_t)arcs.size();
+        if (!ok[i]) continue;
+        const asdm_graph *q = qs[i];
+        for (uint32_t u = 0; u < q->n; ++u)
+            for (uint32_t a = q->off[u]; a < q->off[u + 1]; ++a) {
+                const uint32_t u2 = q->nbr[a];
+                arcs.push_back({sb[i] + u, sb[i] + u2, q->lab[u2], nitems, cand_off[sb[i] + u]});
+                arc_first.push_back(nitems);
+                nitems += fin[sb[i] + u];
+            }
+    }
+    qarc0[nb] = (uint32_t)arcs.size();
+    const uint32_t narcs = (uint32_t)arcs.size();
+    arc_first.push_back(nitems);
+    DevBuf darcs(ctx), dcounts(ctx), dfirst(ctx), dstart(ctx);
     rc = dev_alloc(ctx, &darcs.p, sizeof(ArcInfo) * std::max<uint32_t>(narcs, 1));
     if (!rc && narcs) rc = upload(ctx, darcs.p, arcs.data(), sizeof(ArcInfo) * narcs);
     if (!rc) rc = dev_alloc(ctx, &dcounts.p, sizeof(uint32_t) * std::max<uint64_t>(nitems, 1));
-    if (!rc) rc = dev_alloc(ctx, (void **)&cs->d_ce_off, sizeof(uint64_t) * (nitems + 1));
-    if (rc) return rc;
+    if (!rc) rc = dev_alloc(ctx, (void **)&store->d_ce_off, sizeof(uint64_t) * (nitems + 1));
+    if (!rc) rc = dev_alloc(ctx, &dfirst.p, sizeof(uint64_t) * (narcs + 1));
+    if (!rc) rc = dev_alloc(ctx, &dstart.p, sizeof(uint64_t) * (narcs + 1));
+    if (!rc) rc = upload(ctx, dfirst.p, arc_first.data(), sizeof(uint64_t) * (narcs + 1));
+    if (rc) {
+        drop_store();
+        return rc;
+    }
     const unsigned ce_grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nitems * 16 + 255) / 256, 1u << 20));
     if (nitems) {
         KTimer kt(ctx, "cand_edges_count");
         hipLaunchKernelGGL((k_cand_edges<16, false>), dim3(ce_grid), dim3(256), 0, ctx->stream, g->dev.nbr, g->dev.loff, L,
-                           cs->d_cand, darcs.as<ArcInfo>(), narcs, nitems, cs->d_bits, cs->d_rank, W,
+                           store->d_cand, darcs.as<ArcInfo>(), narcs, nitems, store->d_bits, store->d_rank, W,
                            dcounts.as<uint32_t>(), nullptr, nullptr);
         kt.stop();
-        if ((rc = check_hip(ctx, hipGetLastError(), "k_cand_edges count"))) return rc;
-    }
-    if ((rc = scan_u32_to_u64(ctx, dcounts.as<uint32_t>(), cs->d_ce_off, nitems))) return rc;
-    std::vector<uint64_t> arc_ends(narcs + 1);
-    // per-arc totals: offsets at each arc's first item
-    {
-        std::vector<uint64_t> hoff(narcs + 1);
-        for (uint32_t a = 0; a <= narcs; ++a) {
-            if ((rc = check_hip(ctx, hipMemcpyAsync(&hoff[a], cs->d_ce_off + cs->ce_base[a], sizeof(uint64_t),
-                                                    hipMemcpyDeviceToHost, ctx->stream), "D2H")))
-                return rc;
+        if ((rc = check_hip(ctx, hipGetLastError(), "k_cand_edges count"))) {
+            drop_store();
+            return rc;
         }
-        if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) return rc;
-        cs->arc_weight.resize(narcs);
-        for (uint32_t a = 0; a < narcs; ++a) cs->arc_weight[a] = hoff[a + 1] - hoff[a];
-        cs->ce_total = hoff[narcs];
     }
-    if ((rc = dev_alloc(ctx, (void **)&cs->d_ce_val, sizeof(uint32_t) * std::max<uint64_t>(cs->ce_total, 1)))) return rc;
+    if ((rc = scan_u32_to_u64(ctx, dcounts.as<uint32_t>(), store->d_ce_off, nitems))) {
+        drop_store();
+        return rc;
+    }
+    hipLaunchKernelGGL(k_gather_u64, dim3((narcs + 1 + 255) / 256), dim3(256), 0, ctx->stream, store->d_ce_off,
+                       dfirst.as<uint64_t>(), narcs + 1, dstart.as<uint64_t>());
+    std::vector<uint64_t> astart(narcs + 1);
+    if ((rc = download(ctx, astart.data(), dstart.p, sizeof(uint64_t) * (narcs + 1)))) {
+        drop_store();
+        return rc;
+    }
+    const uint64_t ce_total = astart[narcs];
+    if ((rc = dev_alloc(ctx, (void **)&store->d_ce_val, sizeof(uint32_t) * std::max<uint64_t>(ce_total, 1)))) {
+        drop_store();
+        return rc;
+    }
     if (nitems) {
         KTimer kt(ctx, "cand_edges_fill");
         hipLaunchKernelGGL((k_cand_edges<16, true>), dim3(ce_grid), dim3(256), 0, ctx->stream, g->dev.nbr, g->dev.loff, L,
-                           cs->d_cand, darcs.as<ArcInfo>(), narcs, nitems, cs->d_bits, cs->d_rank, W, nullptr,
-                           cs->d_ce_off, cs->d_ce_val);
+                           store->d_cand, darcs.as<ArcInfo>(), narcs, nitems, store->d_bits, store->d_rank, W, nullptr,
+                           store->d_ce_off, store->d_ce_val);
         kt.stop();
-        if ((rc = check_hip(ctx, hipGetLastError(), "k_cand_edges fill"))) return rc;
+        if ((rc = check_hip(ctx, hipGetLastError(), "k_cand_edges fill"))) {
+            drop_store();
+            return rc;
+        }
     }
-    if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "filter sync"))) return rc;
-    cs->filter_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    ctx->stats.filter_ms = cs->filter_ms;
-    *out = cs.release();
+    if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "filter sync"))) {
+        drop_store();
+        return rc;
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats.filter_ms = ms;
+
+    // per-query views into the shared store
+    for (uint32_t i = 0; i < nb; ++i) {
+        if (!ok[i]) {
+            status[i] = ASDM_E_EMPTY_CANDIDATES;
+            continue;
+        }
+        const asdm_graph *q = qs[i];
+        auto cs = new asdm_cs();
+        cs->ctx = ctx;
+        cs->store = store;
+        store->refs++;
+        cs->nq = q->n;
+        cs->nv = n;
+        cs->W = W;
+        cs->root = roots[i];
+        cs->qlab = q->lab;
+        cs->qoff = q->off;
+        cs->qnbr = q->nbr;
+        const uint64_t c0 = cand_off[sb[i]];
+        cs->csize.assign(fin.begin() + sb[i], fin.begin() + sb[i + 1]);
+        cs->cand_off.resize(q->n + 1);
+        for (uint32_t u = 0; u <= q->n; ++u) cs->cand_off[u] = cand_off[sb[i] + u] - c0;
+        cs->d_cand = store->d_cand + c0;
+        cs->d_bits = store->d_bits + (size_t)sb[i] * W;
+        cs->d_rank = store->d_rank + (size_t)sb[i] * W;
+        cs->narcs = (uint32_t)q->m;
+        cs->arc_u.resize(cs->narcs);
+        cs->arc_u2.resize(cs->narcs);
+        cs->ce_base.assign(cs->narcs + 1, 0);
+        cs->arc_weight.resize(cs->narcs);
+        const uint32_t a0 = qarc0[i];
+        const uint64_t item0 = arc_first[a0];
+        for (uint32_t a = 0; a < cs->narcs; ++a) {
+            cs->arc_u[a] = arcs[a0 + a].u - sb[i];
+            cs->arc_u2[a] = arcs[a0 + a].u2 - sb[i];
+            cs->ce_base[a] = arc_first[a0 + a] - item0;
+            cs->arc_weight[a] = astart[a0 + a + 1] - astart[a0 + a];
+        }
+        cs->ce_base[cs->narcs] = arc_first[a0 + cs->narcs] - item0;
+        cs->d_ce_off = store->d_ce_off + item0;
+        cs->d_ce_val = store->d_ce_val;  // list offsets are global into the shared values
+        cs->ce_total = astart[a0 + cs->narcs] - astart[a0];
+        cs->filter_ms = ms;
+        out[i] = cs;
+        status[i] = ASDM_OK;
+    }
+    if (store->refs == 0) drop_store();
     return ASDM_OK;
+}
+
+}  // namespace asdm
+
+extern "C" {
+
+int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asdm_cs **out) {
+    if (!ctx || !q || !g || !out) return ASDM_E_INVALID;
+    *out = nullptr;
+    int status = ASDM_E_INVALID;
+    int rc = filter_many(ctx, g, &q, 1, out, &status);
+    if (rc) return rc;
+    return status;
 }
 
 uint32_t asdm_cs_root(const asdm_cs *cs) { return cs ? cs->root : 0; }
