@@ -99,6 +99,7 @@ _sig("asdm_plan_estimate", C.c_int, _P, C.POINTER(C.c_double), C.POINTER(C.c_dou
 _sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
 _sig("asdm_local_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p, _u64p)
 _sig("asdm_intersect_k", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
+_sig("asdm_batch_count", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _u64p, C.POINTER(C.c_float))
 
 
 def version() -> str:
@@ -236,6 +237,18 @@ class Context:
         self._check(_lib.asdm_local_count(self._h, plan._h, cs._h, budget_bytes, _ptr(mat, C.c_uint64),
                                           C.byref(out)), "local_count")
         return mat, out.value
+
+    def batch_count(self, g: Graph, qs, budget_bytes: int = 0):
+        """Exact counts of many queries against one data graph in one batched pass; returns
+        (counts uint64[n], completion time of each query in ms from the call start)."""
+        self.upload(g)
+        n = len(qs)
+        hs = (_P * max(n, 1))(*[q._h for q in qs])
+        counts = np.zeros(max(n, 1), np.uint64)
+        lat = np.zeros(max(n, 1), np.float32)
+        self._check(_lib.asdm_batch_count(self._h, g._h, hs, n, budget_bytes, _ptr(counts, C.c_uint64),
+                                          lat.ctypes.data_as(C.POINTER(C.c_float))), "batch_count")
+        return counts[:n], lat[:n]
 
     def stats(self) -> Stats:
         s = Stats()

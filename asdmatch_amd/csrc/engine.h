@@ -38,6 +38,7 @@ struct DPlan {
     uint32_t tri[kMaxQ];
     uint64_t tri_base[kMaxQ];
     const uint32_t *tri_tab[kMaxQ];
+    uint32_t fused;                  // depth D-1 evaluated by the fused leaf path (batch engine)
     uint32_t gsize[kMaxQ];
     uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
     // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient

@@ -96,6 +96,98 @@ __device__ inline uint32_t same_hits(const SameLabel &S, const uint32_t *__restr
 }
 
 // -------------------------------------------------------------------------- expand
+// Row-level device functions shared by the single-query kernels and the batch kernels.  A lane
+// group of G aligned lanes handles one row; all lanes of the group call these together.
+
+// Number of valid depth-d candidates of a row (reduced over the group; every lane gets it).
+template <int G>
+__device__ inline uint32_t expand_count_row(const DPlan *__restrict__ P, uint32_t d, const uint32_t *__restrict__ row,
+                                            const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                            const uint32_t *__restrict__ cand, uint64_t &bytes) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint32_t nb = P->nb[d];
+    const uint64_t cand_d = P->coff[d];
+    uint32_t pj = 0;
+    ListRef pr = list_of(P, ce_off, row, d, 0);
+    for (uint32_t j = 1; j < nb; ++j) {
+        ListRef r = list_of(P, ce_off, row, d, j);
+        if (r.len < pr.len) {
+            pr = r;
+            pj = j;
+        }
+    }
+    SameLabel S;
+    load_same(S, P, cand, row, d);
+    const uint32_t *probe = ce_val + pr.lo;
+    uint32_t c;
+    if (nb == 1) {
+        c = lg == 0 ? pr.len - same_hits(S, cand, cand_d, P->csz[d], probe, pr.len) : 0;
+        bytes += lg == 0 ? 4ull * d + 16 : 0;
+    } else {
+        c = 0;
+        for (uint32_t t = lg; t < pr.len; t += G) {
+            const uint32_t x = probe[t];
+            bool ok = true;
+            for (uint32_t j = 0; j < nb && ok; ++j) {
+                if (j == pj) continue;
+                ListRef r = list_of(P, ce_off, row, d, j);
+                ok = contains_u32(ce_val + r.lo, r.len, x);
+            }
+            if (ok) ok = injective(S, cand, cand[cand_d + x]);
+            c += ok;
+        }
+        bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+        if (lg == 0) bytes += 4ull * d + 16ull * nb;
+    }
+    return (uint32_t)group_sum<G>(c);
+}
+
+// Writes the row's children (width d + 1) at out[o ...] in candidate order.
+template <int G>
+__device__ inline void expand_fill_row(const DPlan *__restrict__ P, uint32_t d, const uint32_t *__restrict__ row,
+                                       const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                       const uint32_t *__restrict__ cand, uint64_t o, uint64_t nout,
+                                       uint32_t *__restrict__ out, uint64_t &bytes) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint32_t nb = P->nb[d];
+    const uint64_t cand_d = P->coff[d];
+    uint32_t pj = 0;
+    ListRef pr = list_of(P, ce_off, row, d, 0);
+    for (uint32_t j = 1; j < nb; ++j) {
+        ListRef r = list_of(P, ce_off, row, d, j);
+        if (r.len < pr.len) {
+            pr = r;
+            pj = j;
+        }
+    }
+    SameLabel S;
+    load_same(S, P, cand, row, d);
+    const uint32_t *probe = ce_val + pr.lo;
+    for (uint32_t t0 = 0; t0 < pr.len; t0 += G) {
+        const uint32_t t = t0 + lg;
+        bool ok = t < pr.len;
+        uint32_t x = 0;
+        if (ok) {
+            x = probe[t];
+            for (uint32_t j = 0; j < nb && ok; ++j) {
+                if (j == pj) continue;
+                ListRef r = list_of(P, ce_off, row, d, j);
+                ok = contains_u32(ce_val + r.lo, r.len, x);
+            }
+            if (ok) ok = injective(S, cand, cand[cand_d + x]);
+        }
+        const uint64_t gm = group_bits<G>(__ballot(ok));
+        if (ok) {
+            uint32_t *dst = out + (o + __popcll(gm & ((1ull << lg) - 1))) * (d + 1);
+            for (uint32_t k = 0; k < d; ++k) dst[k] = row[k];
+            dst[d] = x;
+        }
+        o += __popcll(gm);
+    }
+    bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+    if (lg == 0) bytes += 4ull * d + 16ull * nb + 4ull * (d + 1) * nout;
+}
+
 // Rows of width d -> candidates of depth d.  Count mode writes counts[row]; fill mode writes rows
 // of width d + 1 at out[(outoff[row] - out_base) ...] in (row, candidate) order.
 template <int G, bool FILL>
@@ -107,73 +199,15 @@ __global__ void __launch_bounds__(256) k_expand(const DPlan *__restrict__ P, uin
                                                 uint32_t *__restrict__ out, unsigned long long *__restrict__ bytes_ctr) {
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
-    const uint32_t nb = P->nb[d];
-    const uint64_t cand_d = P->coff[d];
-    const uint32_t csz = P->csz[d];
     uint64_t bytes = 0;
     for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
         const uint32_t *row = rows + it * d;
-        uint32_t pj = 0;
-        ListRef pr = list_of(P, ce_off, row, d, 0);
-        for (uint32_t j = 1; j < nb; ++j) {
-            ListRef r = list_of(P, ce_off, row, d, j);
-            if (r.len < pr.len) {
-                pr = r;
-                pj = j;
-            }
-        }
-        SameLabel S;
-        load_same(S, P, cand, row, d);
-        const uint32_t *probe = ce_val + pr.lo;
         if constexpr (!FILL) {
-            uint32_t c;
-            if (nb == 1) {
-                c = lg == 0 ? pr.len - same_hits(S, cand, cand_d, csz, probe, pr.len) : 0;
-                bytes += lg == 0 ? 4ull * d + 16 : 0;
-            } else {
-                c = 0;
-                for (uint32_t t = lg; t < pr.len; t += G) {
-                    const uint32_t x = probe[t];
-                    bool ok = true;
-                    for (uint32_t j = 0; j < nb && ok; ++j) {
-                        if (j == pj) continue;
-                        ListRef r = list_of(P, ce_off, row, d, j);
-                        ok = contains_u32(ce_val + r.lo, r.len, x);
-                    }
-                    if (ok) ok = injective(S, cand, cand[cand_d + x]);
-                    c += ok;
-                }
-                bytes += 4ull * ((pr.len + G - 1 - lg) / G);
-                if (lg == 0) bytes += 4ull * d + 16ull * nb;
-            }
-            c = (uint32_t)group_sum<G>(c);
+            const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
             if (lg == 0) counts[it] = c;
         } else {
-            uint64_t o = outoff[it] - out_base;
-            for (uint32_t t0 = 0; t0 < pr.len; t0 += G) {
-                const uint32_t t = t0 + lg;
-                bool ok = t < pr.len;
-                uint32_t x = 0;
-                if (ok) {
-                    x = probe[t];
-                    for (uint32_t j = 0; j < nb && ok; ++j) {
-                        if (j == pj) continue;
-                        ListRef r = list_of(P, ce_off, row, d, j);
-                        ok = contains_u32(ce_val + r.lo, r.len, x);
-                    }
-                    if (ok) ok = injective(S, cand, cand[cand_d + x]);
-                }
-                const uint64_t gm = group_bits<G>(__ballot(ok));
-                if (ok) {
-                    uint32_t *dst = out + (o + __popcll(gm & ((1ull << lg) - 1))) * (d + 1);
-                    for (uint32_t k = 0; k < d; ++k) dst[k] = row[k];
-                    dst[d] = x;
-                }
-                o += __popcll(gm);
-            }
-            const uint64_t nout = outoff[it + 1] - outoff[it];
-            bytes += 4ull * ((pr.len + G - 1 - lg) / G);
-            if (lg == 0) bytes += 4ull * d + 16ull * nb + 4ull * (d + 1) * nout;
+            expand_fill_row<G>(P, d, row, ce_off, ce_val, cand, outoff[it] - out_base, outoff[it + 1] - outoff[it], out,
+                               bytes);
         }
     }
     bytes = group_sum<64>(bytes);
@@ -321,6 +355,21 @@ __device__ inline uint64_t group_count(const DPlan *__restrict__ P, uint32_t g, 
     return tot;
 }
 
+// Product of the leaf-group counts of a row of width D (group-uniform value).
+template <int G>
+__device__ inline uint64_t leaf_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row,
+                                    const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                    const uint32_t *__restrict__ cand, uint64_t &bytes) {
+    uint64_t prod = 1;
+    for (uint32_t g = 0; g < P->ngroups; ++g) {
+        uint64_t c = group_count<G>(P, g, row, ce_off, ce_val, cand, bytes);
+        c = __shfl(c, (int)((threadIdx.x & 63) & ~(G - 1)), 64);  // broadcast the group leader's value
+        prod *= c;
+        if (prod == 0) break;
+    }
+    return prod;
+}
+
 // Rows of width D -> sum over rows of the product of the leaf-group counts (mod 2^64).
 template <int G>
 __global__ void __launch_bounds__(256) k_leaf(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
@@ -330,17 +379,10 @@ __global__ void __launch_bounds__(256) k_leaf(const DPlan *__restrict__ P, const
                                               unsigned long long *__restrict__ bytes_ctr) {
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
-    const uint32_t D = P->D, ng = P->ngroups;
+    const uint32_t D = P->D;
     uint64_t acc = 0, bytes = 0;
     for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
-        const uint32_t *row = rows + it * D;
-        uint64_t prod = 1;
-        for (uint32_t g = 0; g < ng; ++g) {
-            uint64_t c = group_count<G>(P, g, row, ce_off, ce_val, cand, bytes);
-            c = __shfl(c, (int)((threadIdx.x & 63) & ~(G - 1)), 64);  // broadcast the group leader's value
-            prod *= c;
-            if (prod == 0) break;
-        }
+        const uint64_t prod = leaf_row<G>(P, rows + it * D, ce_off, ce_val, cand, bytes);
         if (lg == 0) {
             acc += prod;
             bytes += 4ull * D;
@@ -520,6 +562,50 @@ __device__ inline uint64_t group_count_lane(const DPlan *__restrict__ P, uint32_
     return tot;
 }
 
+// This lane's share of a row of width D-1: the sum over its depth-(D-1) candidates x (lanes of
+// the group split the row's candidate list) of the leaf product of (row, x).  Not reduced.
+template <int G>
+__device__ inline uint64_t fused_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row,
+                                     const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                     const uint32_t *__restrict__ cand, uint64_t &bytes) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint32_t d = P->D - 1, ng = P->ngroups;
+    const uint32_t nb = P->nb[d];
+    const uint64_t cand_d = P->coff[d];
+    uint32_t pj = 0;
+    ListRef pr = list_of(P, ce_off, row, d, 0);
+    for (uint32_t j = 1; j < nb; ++j) {
+        ListRef r = list_of(P, ce_off, row, d, j);
+        if (r.len < pr.len) {
+            pr = r;
+            pj = j;
+        }
+    }
+    SameLabel S;
+    load_same(S, P, cand, row, d);
+    const uint32_t *probe = ce_val + pr.lo;
+    uint64_t acc = 0;
+    for (uint32_t t = lg; t < pr.len; t += G) {
+        const uint32_t x = probe[t];
+        bool ok = true;
+        for (uint32_t j = 0; j < nb && ok; ++j) {
+            if (j == pj) continue;
+            ListRef r = list_of(P, ce_off, row, d, j);
+            ok = contains_u32(ce_val + r.lo, r.len, x);
+        }
+        if (ok) ok = injective(S, cand, cand[cand_d + x]);
+        if (!ok) continue;
+        const RowView R{row, d, x};
+        uint64_t prod = 1;
+        for (uint32_t g = 0; g < ng && prod; ++g) prod *= group_count_lane(P, g, R, ce_off, ce_val, cand);
+        acc += prod;
+        bytes += 4ull + 16ull * P->nadj_last;
+    }
+    bytes += 4ull * ((pr.len + G - 1 - lg) / G);
+    if (lg == 0) bytes += 4ull * d + 16ull * nb + 16ull * (ng - P->nadj_last);
+    return acc;
+}
+
 // Rows of width D-1 -> sum over their valid depth-(D-1) candidates x of the leaf product of
 // (row, x).  Lanes of a G-group split the row's candidate list; each lane evaluates its own
 // candidates' leaf products, so the depth-D frontier is never written.
@@ -530,45 +616,11 @@ __global__ void __launch_bounds__(256) k_leaf_fused(const DPlan *__restrict__ P,
                                                     const uint32_t *__restrict__ cand,
                                                     unsigned long long *__restrict__ total,
                                                     unsigned long long *__restrict__ bytes_ctr) {
-    const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
-    const uint32_t d = P->D - 1, ng = P->ngroups;
-    const uint32_t nb = P->nb[d];
-    const uint64_t cand_d = P->coff[d];
+    const uint32_t d = P->D - 1;
     uint64_t acc = 0, bytes = 0;
-    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
-        const uint32_t *row = rows + it * d;
-        uint32_t pj = 0;
-        ListRef pr = list_of(P, ce_off, row, d, 0);
-        for (uint32_t j = 1; j < nb; ++j) {
-            ListRef r = list_of(P, ce_off, row, d, j);
-            if (r.len < pr.len) {
-                pr = r;
-                pj = j;
-            }
-        }
-        SameLabel S;
-        load_same(S, P, cand, row, d);
-        const uint32_t *probe = ce_val + pr.lo;
-        for (uint32_t t = lg; t < pr.len; t += G) {
-            const uint32_t x = probe[t];
-            bool ok = true;
-            for (uint32_t j = 0; j < nb && ok; ++j) {
-                if (j == pj) continue;
-                ListRef r = list_of(P, ce_off, row, d, j);
-                ok = contains_u32(ce_val + r.lo, r.len, x);
-            }
-            if (ok) ok = injective(S, cand, cand[cand_d + x]);
-            if (!ok) continue;
-            const RowView R{row, d, x};
-            uint64_t prod = 1;
-            for (uint32_t g = 0; g < ng && prod; ++g) prod *= group_count_lane(P, g, R, ce_off, ce_val, cand);
-            acc += prod;
-            bytes += 4ull + 16ull * P->nadj_last;
-        }
-        bytes += 4ull * ((pr.len + G - 1 - lg) / G);
-        if (lg == 0) bytes += 4ull * d + 16ull * nb + 16ull * (ng - P->nadj_last);
-    }
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride)
+        acc += fused_row<G>(P, rows + it * d, ce_off, ce_val, cand, bytes);
     acc = group_sum<64>(acc);
     bytes = group_sum<64>(bytes);
     if ((threadIdx.x & 63) == 0) {
@@ -582,6 +634,90 @@ __global__ void k_iota(uint32_t *p, uint64_t n) {
         p[i] = (uint32_t)i;
 }
 
+// -------------------------------------------------------------------------- batch kernels
+// A batch frontier holds the rows of many queries at one depth d, query by query: segment s
+// covers rows [seg_start[s], seg_start[s+1]) of query seg_q[s] (segments may be empty).  Plans
+// are re-based onto the shared candidate arrays of the batch, so every row only needs its plan.
+
+__device__ inline uint32_t seg_of(const uint64_t *__restrict__ seg_start, uint32_t nseg, uint64_t it) {
+    uint32_t lo = 0, hi = nseg;  // last s with seg_start[s] <= it
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg_start[mid] <= it) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Per row: finish it (leaf depth, or fused last level) into qcount[query] and count 0 children,
+// or count its children.
+template <int G>
+__global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
+                                                const uint32_t *__restrict__ seg_q, uint32_t nseg, uint32_t d,
+                                                const uint32_t *__restrict__ rows, uint64_t nrows,
+                                                const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                                const uint32_t *__restrict__ cand, uint32_t *__restrict__ counts,
+                                                unsigned long long *__restrict__ qcount,
+                                                unsigned long long *__restrict__ bytes_ctr) {
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    uint64_t bytes = 0;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+        const uint32_t q = seg_q[seg_of(seg_start, nseg, it)];
+        const DPlan *P = plans + q;
+        const uint32_t *row = rows + it * d;
+        if (d == P->D) {
+            const uint64_t prod = leaf_row<G>(P, row, ce_off, ce_val, cand, bytes);
+            if (lg == 0) {
+                if (prod) atomicAdd(&qcount[q], (unsigned long long)prod);
+                counts[it] = 0;
+            }
+        } else if (P->fused && d + 1 == P->D) {
+            const uint64_t part = group_sum<G>(fused_row<G>(P, row, ce_off, ce_val, cand, bytes));
+            if (lg == 0) {
+                if (part) atomicAdd(&qcount[q], (unsigned long long)part);
+                counts[it] = 0;
+            }
+        } else {
+            const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
+            if (lg == 0) counts[it] = c;
+        }
+    }
+    bytes = group_sum<64>(bytes);
+    if ((threadIdx.x & 63) == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+}
+
+template <int G>
+__global__ void __launch_bounds__(256) k_bfill(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
+                                               const uint32_t *__restrict__ seg_q, uint32_t nseg, uint32_t d,
+                                               const uint32_t *__restrict__ rows, uint64_t nrows,
+                                               const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                               const uint32_t *__restrict__ cand, const uint64_t *__restrict__ offs,
+                                               uint32_t *__restrict__ out, unsigned long long *__restrict__ bytes_ctr) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    uint64_t bytes = 0;
+    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+        const uint64_t nout = offs[it + 1] - offs[it];
+        if (nout == 0) continue;  // finished rows and rows without children (group-uniform)
+        const DPlan *P = plans + seg_q[seg_of(seg_start, nseg, it)];
+        expand_fill_row<G>(P, d, rows + it * d, ce_off, ce_val, cand, offs[it], nout, out, bytes);
+    }
+    bytes = group_sum<64>(bytes);
+    if ((threadIdx.x & 63) == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+}
+
+// Depth-1 rows: position it - seg_start[s] in C(u_0) of the segment's query.
+__global__ void k_biota(const uint64_t *__restrict__ seg_start, uint32_t nseg, uint32_t *__restrict__ rows, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        rows[i] = (uint32_t)(i - seg_start[seg_of(seg_start, nseg, i)]);
+}
+
+__global__ void k_bgather(const uint64_t *__restrict__ offs, const uint64_t *__restrict__ seg_start, uint32_t nseg,
+                          uint64_t *__restrict__ out) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += gridDim.x * blockDim.x)
+        out[s] = offs[seg_start[s]];
+}
+
 // -------------------------------------------------------------------------- host driver
 namespace {
 
@@ -592,6 +728,47 @@ unsigned groups_grid(uint64_t items, int G) {
     uint64_t threads = items * (uint64_t)G;
     uint64_t blocks = (threads + 255) / 256;
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 16));
+}
+
+// Builds the per-candidate-edge intersection table of every triangle leaf of the plan and points
+// hp.tri_tab at them (tables appended to tabs; the caller frees them).
+int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
+    const DPlan &P = plan->h;
+    for (uint32_t e = P.D; e < P.nq; ++e) {
+        if (!P.tri[e]) continue;
+        const uint32_t p0 = plan->order[P.bdep[e][0]], p1 = plan->order[P.bdep[e][1]];
+        const uint32_t a = cs->arc_index(p0, p1);
+        const uint64_t nedges = cs->arc_weight[a];
+        uint32_t *tab = nullptr;
+        int rc = dev_alloc(ctx, (void **)&tab, 4 * std::max<uint64_t>(nedges, 1));
+        if (rc) return rc;
+        tabs.push_back(tab);
+        hp.tri_tab[e] = tab;
+        uint64_t first = 0;
+        rc = check_hip(ctx, hipMemcpyAsync(&first, cs->d_ce_off + cs->ce_base[a], 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (rc) return rc;
+        if (nedges) {
+            KTimer kt(ctx, "tri_table");
+            const unsigned grid = (unsigned)std::min<uint64_t>((nedges + 255) / 256, 1u << 16);
+            hipLaunchKernelGGL(k_tri_table, dim3(grid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
+                               cs->ce_base[a], cs->csize[p0], first, nedges, P.bbase[e][0], P.bbase[e][1], tab);
+            kt.stop(8ull * nedges);
+            if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;
+        }
+    }
+    return ASDM_OK;
+}
+
+// Fusion pays when each leaf count is a few lookups: every leaf group is a single leaf with one
+// candidate-edge list or a triangle leaf.
+bool plan_fusable(const asdm_plan *plan) {
+    if (plan->D < 2 || plan->local) return false;
+    for (uint32_t g = 0; g < plan->h.ngroups; ++g) {
+        const uint32_t e = plan->h.gleaf[g][0];
+        if (plan->h.gsize[g] != 1 || !(plan->h.nb[e] == 1 || plan->h.tri[e])) return false;
+    }
+    return true;
 }
 
 struct Engine {
@@ -624,38 +801,10 @@ struct Engine {
     DPlan hplan;                       // plan copy with the triangle-table pointers filled in
     std::vector<uint32_t *> tri_tabs;
 
-    int build_tri_tables() {
-        const DPlan &P = plan->h;
-        for (uint32_t e = P.D; e < P.nq; ++e) {
-            if (!P.tri[e]) continue;
-            const uint32_t p0 = plan->order[P.bdep[e][0]], p1 = plan->order[P.bdep[e][1]];
-            const uint32_t a = cs->arc_index(p0, p1);
-            const uint64_t nedges = cs->arc_weight[a];
-            uint32_t *tab = nullptr;
-            int rc = dev_alloc(ctx, (void **)&tab, 4 * std::max<uint64_t>(nedges, 1));
-            if (rc) return rc;
-            tri_tabs.push_back(tab);
-            hplan.tri_tab[e] = tab;
-            uint64_t first = 0;
-            rc = check_hip(ctx, hipMemcpyAsync(&first, cs->d_ce_off + cs->ce_base[a], 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-            if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-            if (rc) return rc;
-            if (nedges) {
-                KTimer kt(ctx, "tri_table");
-                const unsigned grid = (unsigned)std::min<uint64_t>((nedges + 255) / 256, 1u << 16);
-                hipLaunchKernelGGL(k_tri_table, dim3(grid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
-                                   cs->ce_base[a], cs->csize[p0], first, nedges, P.bbase[e][0], P.bbase[e][1], tab);
-                kt.stop(8ull * nedges);
-                if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;
-            }
-        }
-        return ASDM_OK;
-    }
-
     int init() {
         D = plan->D;
         hplan = plan->h;
-        int rc = build_tri_tables();
+        int rc = build_tri_tables(ctx, plan, cs, hplan, tri_tabs);
         if (!rc) rc = dev_alloc(ctx, (void **)&dplan, sizeof(DPlan));
         if (!rc) rc = check_hip(ctx, hipMemcpyAsync(dplan, &hplan, sizeof(DPlan), hipMemcpyHostToDevice, ctx->stream), "plan upload");
         if (rc) return rc;
@@ -849,13 +998,7 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     E.t_start = t0;
     // Fuse the last enumerated level into leaf counting when there is one (count mode only); the
     // lane-group width follows the expected list length of that level (smallest arc fan-out).
-    // Fusion pays when each leaf count is a list length minus a few injectivity hits: every leaf
-    // group must be a single leaf with a single candidate-edge list.
-    bool cheap_leaves = true;
-    for (uint32_t g = 0; g < plan->h.ngroups; ++g)
-        cheap_leaves &= plan->h.gsize[g] == 1 &&
-                        (plan->h.nb[plan->h.gleaf[g][0]] == 1 || plan->h.tri[plan->h.gleaf[g][0]]);
-    if (!mat_host && plan->D >= 2 && cheap_leaves) {
+    if (!mat_host && plan_fusable(plan)) {
         E.fused = true;
         const uint32_t d = plan->D - 1;
         double best = 1e300;
@@ -910,6 +1053,125 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
 }  // namespace
 }  // namespace asdm
 
+namespace asdm {
+namespace {
+
+constexpr int kGB = 8;  // lanes per row in the batch kernels (batched queries have short lists)
+
+// Level-synchronous batch engine over queries whose candidate spaces share one filter store.
+// All queries advance one depth per step: one count launch, one scan, one fill launch per depth
+// for the whole batch, and one host synchronization per depth.
+int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css, const std::vector<asdm_plan *> &plans,
+                 uint64_t budget, uint64_t *counts) {
+    const uint32_t Q = (uint32_t)css.size();
+    if (Q == 0) return ASDM_OK;
+    const asdm_cs::Store *st = css[0]->store;
+    std::vector<DPlan> hp(Q);
+    std::vector<uint32_t *> tabs;
+    auto cleanup = [&](void *a, void *b, void *c, void *d2, void *e, void *f, void *g2) {
+        for (auto t : tabs) dev_free(ctx, t);
+        for (void *p : {a, b, c, d2, e, f, g2}) dev_free(ctx, p);
+    };
+    int rc = ASDM_OK;
+    uint32_t maxD = 0;
+    std::vector<uint64_t> seg_start_h(Q + 1, 0);
+    for (uint32_t i = 0; i < Q; ++i) {
+        const asdm_cs *cs = css[i];
+        if (cs->store != st) return ctx->fail(ASDM_E_INVALID, "batch candidate spaces must come from one filter call");
+        hp[i] = plans[i]->h;
+        if ((rc = build_tri_tables(ctx, plans[i], cs, hp[i], tabs))) {
+            cleanup(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            return rc;
+        }
+        const uint64_t cb = (uint64_t)(cs->d_cand - st->d_cand), ib = (uint64_t)(cs->d_ce_off - st->d_ce_off);
+        for (uint32_t d = 0; d < hp[i].nq; ++d) {
+            hp[i].coff[d] += cb;
+            for (uint32_t j = 0; j < hp[i].nb[d]; ++j) hp[i].bbase[d][j] += ib;
+            if (hp[i].tri[d]) hp[i].tri_base[d] += ib;
+        }
+        hp[i].fused = plan_fusable(plans[i]);
+        maxD = std::max(maxD, hp[i].D);
+        seg_start_h[i + 1] = seg_start_h[i] + cs->csize[plans[i]->order[0]];
+    }
+    DPlan *dplans = nullptr;
+    uint64_t *seg_start = nullptr, *seg_next = nullptr;
+    uint32_t *seg_q = nullptr, *counts_d = nullptr;
+    uint64_t *offs = nullptr;
+    uint32_t *rows = nullptr;
+    rc = dev_alloc(ctx, (void **)&dplans, sizeof(DPlan) * Q);
+    if (!rc) rc = dev_alloc(ctx, (void **)&seg_start, 8 * (Q + 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&seg_next, 8 * (Q + 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&seg_q, 4 * Q);
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(dplans, hp.data(), sizeof(DPlan) * Q, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    std::vector<uint32_t> seg_q_h(Q);
+    for (uint32_t i = 0; i < Q; ++i) seg_q_h[i] = i;
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(seg_q, seg_q_h.data(), 4 * Q, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(seg_start, seg_start_h.data(), 8 * (Q + 1), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    if (!rc) rc = fill_u64(ctx, ctx->d_counters, 8, 0);
+    unsigned long long *qcount = nullptr;
+    if (!rc) rc = dev_alloc(ctx, (void **)&qcount, 8 * Q);
+    if (!rc) rc = fill_u64(ctx, (uint64_t *)qcount, Q, 0);
+    uint64_t n = seg_start_h[Q];
+    if (!rc) rc = dev_alloc(ctx, (void **)&rows, 4 * std::max<uint64_t>(n, 1));
+    if (!rc && n) {
+        hipLaunchKernelGGL(k_biota, dim3(groups_grid(n, 1)), dim3(256), 0, ctx->stream, seg_start, Q, rows, n);
+        rc = check_hip(ctx, hipGetLastError(), "biota");
+    }
+    unsigned long long *d_bytes = (unsigned long long *)ctx->d_counters + 1;
+    for (uint32_t d = 1; d <= maxD && n && !rc; ++d) {
+        dev_free(ctx, counts_d);
+        dev_free(ctx, offs);
+        counts_d = nullptr;
+        offs = nullptr;
+        rc = dev_alloc(ctx, (void **)&counts_d, 4 * n);
+        if (!rc) rc = dev_alloc(ctx, (void **)&offs, 8 * (n + 1));
+        if (rc) break;
+        {
+            KTimer kt(ctx, "batch_count");
+            hipLaunchKernelGGL((k_bcount<kGB>), dim3(groups_grid(n, kGB)), dim3(256), 0, ctx->stream, dplans, seg_start,
+                               seg_q, Q, d, rows, n, st->d_ce_off, st->d_ce_val, st->d_cand, counts_d, qcount, d_bytes);
+            kt.stop();
+        }
+        if ((rc = check_hip(ctx, hipGetLastError(), "batch count"))) break;
+        if ((rc = scan_u32_to_u64(ctx, counts_d, offs, n))) break;
+        hipLaunchKernelGGL(k_bgather, dim3(1), dim3(256), 0, ctx->stream, offs, seg_start, Q, seg_next);
+        std::vector<uint64_t> next_h(Q + 1);
+        if ((rc = check_hip(ctx, hipMemcpyAsync(next_h.data(), seg_next, 8 * (Q + 1), hipMemcpyDeviceToHost, ctx->stream), "D2H"))) break;
+        if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) break;
+        const uint64_t total = next_h[Q];
+        if (total == 0 || d == maxD) break;
+        if (4ull * (d + 1) * total > budget) {
+            rc = ctx->fail(ASDM_E_BUDGET, "batch frontier exceeds the budget");
+            break;
+        }
+        uint32_t *next_rows = nullptr;
+        if ((rc = dev_alloc(ctx, (void **)&next_rows, 4ull * (d + 1) * total))) break;
+        {
+            KTimer kt(ctx, "batch_fill");
+            hipLaunchKernelGGL((k_bfill<kGB>), dim3(groups_grid(n, kGB)), dim3(256), 0, ctx->stream, dplans, seg_start,
+                               seg_q, Q, d, rows, n, st->d_ce_off, st->d_ce_val, st->d_cand, offs, next_rows, d_bytes + 1);
+            kt.stop();
+        }
+        if ((rc = check_hip(ctx, hipGetLastError(), "batch fill"))) break;
+        dev_free(ctx, rows);
+        rows = next_rows;
+        std::swap(seg_start, seg_next);
+        n = total;
+    }
+    std::vector<uint64_t> qc(Q);
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!rc) std::memcpy(counts, qc.data(), 8 * Q);
+    resolve_timers(ctx);
+    cleanup(dplans, seg_start, seg_next, seg_q, counts_d, offs, rows);
+    dev_free(ctx, qcount);
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+}
+
+}  // namespace
+}  // namespace asdm
+
 using namespace asdm;
 
 extern "C" {
@@ -924,4 +1186,62 @@ int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, ui
     return run_engine(ctx, plan, cs, budget_bytes, mat, count);
 }
 
+
+// Batched multi-query count: the filter of all queries runs as one batched pass (filter_many);
+// the queries are then counted together by the level-synchronous batch engine.  A query whose
+// plan is not supported by the batch engine, or a batch whose frontier exceeds the budget, is
+// counted with the single-query engine instead.  lat_ms[i] is the time at which query i's count
+// was known, from the call start.
+int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq, uint64_t budget_bytes,
+                     uint64_t *counts, float *lat_ms) {
+    if (!ctx || !g || (!qs && nq) || (!counts && nq)) return ASDM_E_INVALID;
+    if (g->dev.device != ctx->device) return ctx->fail(ASDM_E_INVALID, "data graph not uploaded to this device");
+    (void)hipSetDevice(ctx->device);
+    auto t0 = std::chrono::steady_clock::now();
+    auto now_ms = [&]() {
+        return (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    std::vector<asdm_cs *> cs(nq, nullptr);
+    std::vector<int> status(nq, ASDM_E_INVALID);
+    int rc = filter_many(ctx, g, qs, nq, cs.data(), status.data());
+    if (rc) return rc;
+    std::vector<asdm_cs *> bcs;
+    std::vector<asdm_plan *> bplans;
+    std::vector<uint32_t> bidx;
+    for (uint32_t i = 0; i < nq; ++i) {
+        counts[i] = 0;
+        if (status[i] == ASDM_E_EMPTY_CANDIDATES) {
+            if (lat_ms) lat_ms[i] = now_ms();
+            continue;
+        }
+        if (status[i] != ASDM_OK) {
+            rc = status[i];
+            break;
+        }
+        asdm_plan *p = nullptr;
+        if ((rc = asdm_plan_create(qs[i], cs[i], nullptr, 0, &p))) break;
+        bcs.push_back(cs[i]);
+        bplans.push_back(p);
+        bidx.push_back(i);
+    }
+    if (!rc) {
+        const uint64_t budget = budget_bytes ? budget_bytes : (16ull << 30);
+        std::vector<uint64_t> bc(bcs.size(), 0);
+        rc = batch_engine(ctx, bcs, bplans, budget, bc.data());
+        if (rc == ASDM_E_BUDGET) {  // fall back to one query at a time
+            rc = ASDM_OK;
+            for (size_t k = 0; k < bcs.size() && !rc; ++k) {
+                rc = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
+                if (lat_ms) lat_ms[bidx[k]] = now_ms();
+            }
+        } else if (lat_ms) {
+            const float t = now_ms();
+            for (uint32_t k : bidx) lat_ms[k] = t;
+        }
+        for (size_t k = 0; k < bcs.size(); ++k) counts[bidx[k]] = bc[k];
+    }
+    for (auto p : bplans) asdm_plan_free(p);
+    for (auto c : cs) asdm_cs_free(c);
+    return rc;
+}
 }  // extern "C"

@@ -1,0 +1,46 @@
+"""Batched multi-query engine (asdm_batch_count): the same exact counts as the reference's golden
+outputs and as the single-query engine."""
+import numpy as np
+import pytest
+
+import asdmatch_amd as A
+from tests import common as T
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_matches_reference_on_yeast(ctx):
+    gold = T.yeast_golden()["queries"]
+    g = A.Graph.load(T.yeast_data_path())
+    names = list(gold)
+    qs = [A.Graph.load(T.yeast_query_path(n)) for n in names]
+    counts, lat = ctx.batch_count(g, qs)
+    assert [int(c) for c in counts] == [gold[n]["count"] for n in names]
+    assert np.all(lat > 0)
+
+
+def test_batch_matches_reference_on_synthetic(ctx):
+    for case in T.synth_golden():
+        g = T.make_graph(case)
+        recs = [r for r in case["queries"] if r["count"] is not None]
+        qs = [g.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"]) for r in recs]
+        counts, _ = ctx.batch_count(g, qs)
+        assert [int(c) for c in counts] == [r["count"] for r in recs], case["name"]
+
+
+def test_batch_config4_admitted_queries(ctx):
+    from asdmatch_amd import workloads as W
+
+    g = W.graph("c4")
+    adm = W.admitted("c4")
+    qs = [W.query(g, s, seed) for s, seed, _ in adm]
+    counts, _ = ctx.batch_count(g, qs)
+    assert [int(c) for c in counts] == [c for _, _, c in adm]
+
+
+def test_batch_budget_fallback_gives_same_counts(ctx):
+    case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
+    g = T.make_graph(case)
+    qs = [g.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"]) for r in case["queries"]]
+    counts, _ = ctx.batch_count(g, qs, budget_bytes=1 << 12)  # forces the per-query fallback
+    assert [int(c) for c in counts] == [r["count"] for r in case["queries"]]
