@@ -214,6 +214,7 @@ void release_device_graph(asdm_graph &g) {
     (void)hipFree(d.loff);
     (void)hipFree(d.vbl_off);
     (void)hipFree(d.vbl);
+    (void)hipFree(d.labbits);
     (void)hipSetDevice(cur);
     d = DeviceGraph{};
 }
@@ -364,6 +365,13 @@ int asdm_graph_upload(asdm_ctx *ctx, asdm_graph *g) {
     if (!rc) rc = up(&d.loff, loff.data(), loff.size());
     if (!rc) rc = up(&d.vbl_off, g->vbl_off.data(), g->vbl_off.size());
     if (!rc) rc = up(&d.vbl, g->vbl.data(), g->vbl.size());
+    if (!rc) {  // per-label vertex bitmaps: the base filter walks only the vertices of a label
+        const size_t W = ((size_t)g->n + 63) / 64;
+        std::vector<uint64_t> lb(std::max<size_t>(W * g->nlab, 1), 0);
+        for (uint32_t v = 0; v < g->n; ++v) lb[(size_t)g->lab[v] * W + (v >> 6)] |= 1ull << (v & 63);
+        if (hipMalloc((void **)&d.labbits, 8 * lb.size()) != hipSuccess) rc = ASDM_E_OOM;
+        else if (hipMemcpy(d.labbits, lb.data(), 8 * lb.size(), hipMemcpyHostToDevice) != hipSuccess) rc = ASDM_E_HIP;
+    }
     g->dev = d;
     if (rc) {
         release_device_graph(*g);

@@ -221,8 +221,9 @@ __global__ void __launch_bounds__(256) k_expand(const DPlan *__restrict__ P, uin
 // materialization of the bag {p0, p1, e} -- instead of once per partial embedding.
 // One lane per candidate edge k (global index base.. of the arc's lists).
 __global__ void k_tri_table(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
-                            uint64_t arc_item_base, uint32_t nitems /* |C(p0)| */, uint64_t e_first,
-                            uint64_t nedges, uint64_t l0_base, uint64_t l1_base, uint32_t *__restrict__ tab) {
+                            uint64_t arc_item_base, uint32_t nitems /* |C(p0)| */, uint64_t nedges, uint64_t l0_base,
+                            uint64_t l1_base, uint32_t *__restrict__ tab) {
+    const uint64_t e_first = ce_off[arc_item_base];  // first candidate edge of the arc
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nedges; k += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t ek = e_first + k;
         // i = position of f(p0): last item whose list starts at or before ek
@@ -654,7 +655,7 @@ __device__ inline uint32_t seg_of(const uint64_t *__restrict__ seg_start, uint32
 template <int G>
 __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                 const uint32_t *__restrict__ seg_q, uint32_t nseg, uint32_t d,
-                                                const uint32_t *__restrict__ rows, uint64_t nrows,
+                                                const uint32_t *__restrict__ rows, uint64_t it0, uint64_t it1,
                                                 const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                                                 const uint32_t *__restrict__ cand, uint32_t *__restrict__ counts,
                                                 unsigned long long *__restrict__ qcount,
@@ -662,7 +663,7 @@ __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans,
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     uint64_t bytes = 0;
-    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+    for (uint64_t it = it0 + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < it1; it += stride) {
         const uint32_t q = seg_q[seg_of(seg_start, nseg, it)];
         const DPlan *P = plans + q;
         const uint32_t *row = rows + it * d;
@@ -690,13 +691,13 @@ __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans,
 template <int G>
 __global__ void __launch_bounds__(256) k_bfill(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                const uint32_t *__restrict__ seg_q, uint32_t nseg, uint32_t d,
-                                               const uint32_t *__restrict__ rows, uint64_t nrows,
+                                               const uint32_t *__restrict__ rows, uint64_t it0, uint64_t it1,
                                                const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                                                const uint32_t *__restrict__ cand, const uint64_t *__restrict__ offs,
                                                uint32_t *__restrict__ out, unsigned long long *__restrict__ bytes_ctr) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     uint64_t bytes = 0;
-    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride) {
+    for (uint64_t it = it0 + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < it1; it += stride) {
         const uint64_t nout = offs[it + 1] - offs[it];
         if (nout == 0) continue;  // finished rows and rows without children (group-uniform)
         const DPlan *P = plans + seg_q[seg_of(seg_start, nseg, it)];
@@ -744,15 +745,11 @@ int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
         if (rc) return rc;
         tabs.push_back(tab);
         hp.tri_tab[e] = tab;
-        uint64_t first = 0;
-        rc = check_hip(ctx, hipMemcpyAsync(&first, cs->d_ce_off + cs->ce_base[a], 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-        if (rc) return rc;
         if (nedges) {
             KTimer kt(ctx, "tri_table");
             const unsigned grid = (unsigned)std::min<uint64_t>((nedges + 255) / 256, 1u << 16);
             hipLaunchKernelGGL(k_tri_table, dim3(grid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
-                               cs->ce_base[a], cs->csize[p0], first, nedges, P.bbase[e][0], P.bbase[e][1], tab);
+                               cs->ce_base[a], cs->csize[p0], nedges, P.bbase[e][0], P.bbase[e][1], tab);
             kt.stop(8ull * nedges);
             if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;
         }
@@ -1056,15 +1053,67 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
 namespace asdm {
 namespace {
 
-constexpr int kGB = 8;  // lanes per row in the batch kernels (batched queries have short lists)
+// Lane-group width class of a query: its expected candidate-list length at the deepest
+// enumerated level (smallest fan-out among that level's matched neighbours).
+uint32_t lane_class(const asdm_plan *plan, const asdm_cs *cs) {
+    const uint32_t d = plan->D >= 2 ? plan->D - 1 : 0;
+    double best = 1e300;
+    for (uint32_t j = 0; j < plan->h.nb[d]; ++j) {
+        const uint32_t p = plan->order[plan->h.bdep[d][j]];
+        const uint32_t a = cs->arc_index(p, plan->order[d]);
+        best = std::min(best, (double)cs->arc_weight[a] / std::max(1.0, (double)cs->csize[p]));
+    }
+    if (plan->h.nb[d] == 0) best = cs->csize[plan->order[0]];
+    return best < 3 ? 1 : best < 12 ? 4 : best < 48 ? 16 : 64;
+}
+
+template <int G>
+void launch_bcount(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, const uint32_t *seg_q, uint32_t Q,
+                   uint32_t d, const uint32_t *rows, uint64_t i0, uint64_t i1, const asdm_cs::Store *st,
+                   uint32_t *counts, unsigned long long *qcount, unsigned long long *bytes) {
+    hipLaunchKernelGGL((k_bcount<G>), dim3(groups_grid(i1 - i0, G)), dim3(256), 0, ctx->stream, plans, seg_start, seg_q, Q,
+                       d, rows, i0, i1, st->d_ce_off, st->d_ce_val, st->d_cand, counts, qcount, bytes);
+}
+
+template <int G>
+void launch_bfill(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, const uint32_t *seg_q, uint32_t Q,
+                  uint32_t d, const uint32_t *rows, uint64_t i0, uint64_t i1, const asdm_cs::Store *st,
+                  const uint64_t *offs, uint32_t *out, unsigned long long *bytes) {
+    hipLaunchKernelGGL((k_bfill<G>), dim3(groups_grid(i1 - i0, G)), dim3(256), 0, ctx->stream, plans, seg_start, seg_q, Q,
+                       d, rows, i0, i1, st->d_ce_off, st->d_ce_val, st->d_cand, offs, out, bytes);
+}
+
+template <class F1, class F4, class F16, class F64>
+void by_class(uint32_t g, F1 f1, F4 f4, F16 f16, F64 f64) {
+    if (g == 1) f1();
+    else if (g == 4) f4();
+    else if (g == 16) f16();
+    else f64();
+}
 
 // Level-synchronous batch engine over queries whose candidate spaces share one filter store.
 // All queries advance one depth per step: one count launch, one scan, one fill launch per depth
 // for the whole batch, and one host synchronization per depth.
-int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css, const std::vector<asdm_plan *> &plans,
-                 uint64_t budget, uint64_t *counts) {
-    const uint32_t Q = (uint32_t)css.size();
+int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std::vector<asdm_plan *> &plans_in,
+                 uint64_t budget, uint64_t *counts_out) {
+    const uint32_t Q = (uint32_t)css_in.size();
     if (Q == 0) return ASDM_OK;
+    // queries grouped by lane class, so that each class is a contiguous row range at every depth
+    std::vector<uint32_t> perm(Q), cls(Q);
+    for (uint32_t i = 0; i < Q; ++i) {
+        perm[i] = i;
+        cls[i] = lane_class(plans_in[i], css_in[i]);
+    }
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return cls[a] < cls[b]; });
+    std::vector<asdm_cs *> css(Q);
+    std::vector<asdm_plan *> plans(Q);
+    std::vector<uint32_t> qcls(Q);
+    for (uint32_t i = 0; i < Q; ++i) {
+        css[i] = css_in[perm[i]];
+        plans[i] = plans_in[perm[i]];
+        qcls[i] = cls[perm[i]];
+    }
+    std::vector<uint64_t> counts(Q);
     const asdm_cs::Store *st = css[0]->store;
     std::vector<DPlan> hp(Q);
     std::vector<uint32_t *> tabs;
@@ -1128,8 +1177,18 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css, const std::ve
         if (rc) break;
         {
             KTimer kt(ctx, "batch_count");
-            hipLaunchKernelGGL((k_bcount<kGB>), dim3(groups_grid(n, kGB)), dim3(256), 0, ctx->stream, dplans, seg_start,
-                               seg_q, Q, d, rows, n, st->d_ce_off, st->d_ce_val, st->d_cand, counts_d, qcount, d_bytes);
+            for (uint32_t a = 0; a < Q;) {  // one launch per lane class with rows at this depth
+                uint32_t b = a;
+                while (b < Q && qcls[b] == qcls[a]) ++b;
+                const uint64_t i0 = seg_start_h[a], i1 = seg_start_h[b];
+                if (i1 > i0)
+                    by_class(qcls[a],
+                             [&] { launch_bcount<1>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
+                             [&] { launch_bcount<4>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
+                             [&] { launch_bcount<16>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
+                             [&] { launch_bcount<64>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); });
+                a = b;
+            }
             kt.stop();
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch count"))) break;
@@ -1148,20 +1207,32 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css, const std::ve
         if ((rc = dev_alloc(ctx, (void **)&next_rows, 4ull * (d + 1) * total))) break;
         {
             KTimer kt(ctx, "batch_fill");
-            hipLaunchKernelGGL((k_bfill<kGB>), dim3(groups_grid(n, kGB)), dim3(256), 0, ctx->stream, dplans, seg_start,
-                               seg_q, Q, d, rows, n, st->d_ce_off, st->d_ce_val, st->d_cand, offs, next_rows, d_bytes + 1);
+            for (uint32_t a = 0; a < Q;) {
+                uint32_t b = a;
+                while (b < Q && qcls[b] == qcls[a]) ++b;
+                const uint64_t i0 = seg_start_h[a], i1 = seg_start_h[b];
+                if (i1 > i0)
+                    by_class(qcls[a],
+                             [&] { launch_bfill<1>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
+                             [&] { launch_bfill<4>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
+                             [&] { launch_bfill<16>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
+                             [&] { launch_bfill<64>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); });
+                a = b;
+            }
             kt.stop();
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch fill"))) break;
         dev_free(ctx, rows);
         rows = next_rows;
         std::swap(seg_start, seg_next);
+        seg_start_h = next_h;
         n = total;
     }
     std::vector<uint64_t> qc(Q);
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-    if (!rc) std::memcpy(counts, qc.data(), 8 * Q);
+    if (!rc)
+        for (uint32_t i = 0; i < Q; ++i) counts_out[perm[i]] = qc[i];
     resolve_timers(ctx);
     cleanup(dplans, seg_start, seg_next, seg_q, counts_d, offs, rows);
     dev_free(ctx, qcount);

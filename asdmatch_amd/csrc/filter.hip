@@ -38,29 +38,37 @@ struct Par {  // one parent test of a prune pass
     uint32_t labcount; // |V_L(p)|
 };
 
-__global__ void k_base(const uint32_t *__restrict__ glab, const uint32_t *__restrict__ goff,
-                       const uint32_t *__restrict__ loff, uint32_t n, uint32_t L, const QV *__restrict__ qv,
+// Word-parallel filter kernels: one thread per 64-vertex bitmap word of one query vertex (slot).
+// A thread visits only the set bits of its source word (the label bitmap for the base filter, the
+// previous pass for a prune), assembles its output word in a register and stores it; popcounts go
+// to the slot's counter with one atomic per wave.
+
+__global__ void k_base(const uint64_t *__restrict__ labbits, const uint32_t *__restrict__ goff,
+                       const uint32_t *__restrict__ loff, uint32_t L, const QV *__restrict__ qv,
                        uint64_t *__restrict__ bits, uint32_t W, uint32_t *__restrict__ cnt) {
     const uint32_t u = blockIdx.y;
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     const QV &q = qv[u];
-    bool pass = false;
-    if (v < n && glab[v] == q.lab && goff[v + 1] - goff[v] >= q.deg) {
-        pass = true;
-        const uint32_t *lo = loff + (size_t)v * (L + 1);
-        for (uint32_t k = 0; k < q.nnlc; ++k) {
-            uint32_t l = q.nlc_lab[k];
-            if (l >= L || lo[l + 1] - lo[l] < q.nlc_cnt[k]) {
-                pass = false;
-                break;
+    uint64_t res = 0;
+    if (w < W && q.lab < L) {
+        uint64_t word = labbits[(size_t)q.lab * W + w];
+        while (word) {
+            const int j = __ffsll((unsigned long long)word) - 1;
+            word &= word - 1;
+            const uint32_t v = w * 64 + (uint32_t)j;
+            if (goff[v + 1] - goff[v] < q.deg) continue;
+            const uint32_t *lo = loff + (size_t)v * (L + 1);
+            bool pass = true;
+            for (uint32_t k = 0; k < q.nnlc && pass; ++k) {
+                const uint32_t l = q.nlc_lab[k];
+                pass = l < L && lo[l + 1] - lo[l] >= q.nlc_cnt[k];
             }
+            if (pass) res |= 1ull << j;
         }
     }
-    uint64_t m = __ballot(pass);
-    if ((threadIdx.x & 63) == 0 && (v >> 6) < W) {
-        bits[(size_t)u * W + (v >> 6)] = m;
-        if (m) atomicAdd(&cnt[u], (uint32_t)__popcll(m));
-    }
+    if (w < W) bits[(size_t)u * W + w] = res;
+    const uint64_t c = group_sum<64>((uint64_t)__popcll(res));
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&cnt[u], (uint32_t)c);
 }
 
 // dst(u) = src(u) ∩ every parent test listed for u.  grid.y indexes (u, parent-list) pairs.
@@ -71,35 +79,41 @@ __global__ void k_prune(const uint32_t *__restrict__ nbr, const uint32_t *__rest
                         const Par *__restrict__ pars, uint32_t *__restrict__ dcnt) {
     const uint32_t slot = blockIdx.y;
     const uint32_t u = us[slot];
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    bool pass = false;
-    if (v < n) pass = (src[(size_t)u * W + (v >> 6)] >> (v & 63)) & 1ull;
-    if (pass) {
-        const uint32_t *lo = loff + (size_t)v * (L + 1);
-        // v adjacent to data vertex 0: 0 would lead v's label-L(0) slice (slices ascend by id)
-        const bool adj0 = lab0 < L && lo[lab0 + 1] > lo[lab0] && nbr[lo[lab0]] == 0;
-        for (uint32_t k = par_off[slot]; k < par_off[slot + 1] && pass; ++k) {
-            const Par pr = pars[k];
-            const bool pad = gcnt[pr.p] < pr.labcount;
-            bool hit = pad && adj0;
-            if (!hit && pr.lab < L) {
-                const uint64_t *gb = genbits + (size_t)pr.p * W;
-                for (uint32_t i = lo[pr.lab], e = lo[pr.lab + 1]; i < e; ++i) {
-                    uint32_t w = nbr[i];
-                    if ((gb[w >> 6] >> (w & 63)) & 1ull) {
-                        hit = true;
-                        break;
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t res = 0;
+    if (w < W) {
+        uint64_t word = src[(size_t)u * W + w];
+        const uint32_t p0 = par_off[slot], p1 = par_off[slot + 1];
+        while (word) {
+            const int j = __ffsll((unsigned long long)word) - 1;
+            word &= word - 1;
+            const uint32_t v = w * 64 + (uint32_t)j;
+            const uint32_t *lo = loff + (size_t)v * (L + 1);
+            // v adjacent to data vertex 0: 0 would lead v's label-L(0) slice (slices ascend by id)
+            const bool adj0 = lab0 < L && lo[lab0 + 1] > lo[lab0] && nbr[lo[lab0]] == 0;
+            bool pass = true;
+            for (uint32_t k = p0; k < p1 && pass; ++k) {
+                const Par pr = pars[k];
+                const bool pad = gcnt[pr.p] < pr.labcount;
+                bool hit = pad && adj0;
+                if (!hit && pr.lab < L) {
+                    const uint64_t *gb = genbits + (size_t)pr.p * W;
+                    for (uint32_t i = lo[pr.lab], e = lo[pr.lab + 1]; i < e; ++i) {
+                        const uint32_t x = nbr[i];
+                        if ((gb[x >> 6] >> (x & 63)) & 1ull) {
+                            hit = true;
+                            break;
+                        }
                     }
                 }
+                pass = hit;
             }
-            pass = hit;
+            if (pass) res |= 1ull << j;
         }
+        dst[(size_t)u * W + w] = res;
     }
-    uint64_t m = __ballot(pass);
-    if ((threadIdx.x & 63) == 0 && (v >> 6) < W) {
-        dst[(size_t)u * W + (v >> 6)] = m;
-        if (m) atomicAdd(&dcnt[u], (uint32_t)__popcll(m));
-    }
+    const uint64_t c = group_sum<64>((uint64_t)__popcll(res));
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&dcnt[u], (uint32_t)c);
 }
 
 // Per query vertex: rank[w] = set bits before word w; cand[cand_off[u] + rank] = ascending ids.
@@ -306,9 +320,9 @@ int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs,
     if ((rc = fill_u64(ctx, dfin.as<uint64_t>(), (uint64_t)W * S, 0))) return rc;
     {
         KTimer kt(ctx, "filter_base");
-        hipLaunchKernelGGL(k_base, dim3((W * 64 + 255) / 256, S), dim3(256), 0, ctx->stream, g->dev.lab, g->dev.off,
-                           g->dev.loff, n, L, dqv.as<QV>(), dbase.as<uint64_t>(), W, cnt_base);
-        kt.stop((uint64_t)n * 8 * S);
+        hipLaunchKernelGGL(k_base, dim3((W + 255) / 256, S), dim3(256), 0, ctx->stream, g->dev.labbits, g->dev.off,
+                           g->dev.loff, L, dqv.as<QV>(), dbase.as<uint64_t>(), W, cnt_base);
+        kt.stop((uint64_t)W * 8 * S);
         if ((rc = check_hip(ctx, hipGetLastError(), "k_base"))) return rc;
     }
     std::vector<uint32_t> base_cnt(S);
@@ -364,14 +378,14 @@ int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs,
         KTimer kt(ctx, "filter_gen");
         for (uint32_t k = 0; k < maxlen; ++k) {
             const uint32_t a = launch_start[k], cnt = launch_start[k + 1] - a;
-            hipLaunchKernelGGL(k_prune, dim3((W * 64 + 255) / 256, cnt), dim3(256), 0, ctx->stream, g->dev.nbr,
+            hipLaunchKernelGGL(k_prune, dim3((W + 255) / 256, cnt), dim3(256), 0, ctx->stream, g->dev.nbr,
                                g->dev.loff, n, L, lab0, dbase.as<uint64_t>(), dgen.as<uint64_t>(), dgen.as<uint64_t>(),
                                cnt_gen, W, dus.as<uint32_t>() + a, dpo.as<uint32_t>() + a, dpars.as<Par>(), cnt_gen);
         }
         kt.stop();
         if ((rc = check_hip(ctx, hipGetLastError(), "k_prune gen"))) return rc;
         KTimer kf(ctx, "filter_final");
-        hipLaunchKernelGGL(k_prune, dim3((W * 64 + 255) / 256, nfinal), dim3(256), 0, ctx->stream, g->dev.nbr,
+        hipLaunchKernelGGL(k_prune, dim3((W + 255) / 256, nfinal), dim3(256), 0, ctx->stream, g->dev.nbr,
                            g->dev.loff, n, L, lab0, dgen.as<uint64_t>(), dfin.as<uint64_t>(), dgen.as<uint64_t>(),
                            cnt_gen, W, dus.as<uint32_t>() + final_start, dpo.as<uint32_t>() + final_start,
                            dpars.as<Par>(), cnt_fin);

@@ -37,6 +37,11 @@ void finish_graph(asdm_graph &g) {
     g.vbl.resize(g.n);
     std::vector<uint32_t> pos(g.vbl_off.begin(), g.vbl_off.end());
     for (uint32_t v = 0; v < g.n; ++v) g.vbl[pos[g.lab[v]]++] = v;
+    std::vector<double> degsum(g.nlab, 0.0);
+    for (uint32_t v = 0; v < g.n; ++v) degsum[g.lab[v]] += g.off[v + 1] - g.off[v];
+    g.lab_avgdeg.assign(g.nlab, 0.f);
+    for (uint32_t l = 0; l < g.nlab; ++l)
+        if (g.vbl_off[l + 1] > g.vbl_off[l]) g.lab_avgdeg[l] = (float)(degsum[l] / (g.vbl_off[l + 1] - g.vbl_off[l]));
 }
 
 asdm_graph *graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges) {

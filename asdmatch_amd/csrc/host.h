@@ -19,6 +19,7 @@ struct DeviceGraph {
     uint32_t *loff = nullptr;     // [n * (nlab + 1)] start of each label run in nbr (absolute)
     uint32_t *vbl_off = nullptr;  // [nlab + 1]
     uint32_t *vbl = nullptr;      // [n] vertices grouped by label, ascending id
+    uint64_t *labbits = nullptr;  // [nlab][W] membership bitmap of each label (W = ceil(n / 64))
 };
 
 }  // namespace asdm
@@ -29,6 +30,7 @@ struct asdm_graph {
     uint32_t nlab = 0;
     std::vector<uint32_t> lab, off, nbr;
     std::vector<uint32_t> vbl_off, vbl;
+    std::vector<float> lab_avgdeg;  // mean degree of the vertices of each label
     asdm::DeviceGraph dev;
 
     uint32_t deg(uint32_t v) const { return off[v + 1] - off[v]; }

@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("c2", "c4"), default="c2",
+                    help="c2: config-2 queries one by one (default); c4: config-4 batch, batched vs plain")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
     args = ap.parse_args()
@@ -109,6 +111,9 @@ def main():
 
     import asdmatch_amd as A
     from asdmatch_amd import workloads as W
+
+    if args.workload == "c4":
+        return bench_batch(args)
 
     t0 = time.time()
     g = W.graph("c2")
@@ -195,6 +200,69 @@ def main():
         "cpu_baseline": cpu,
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats],
         "setup_s": round(gen_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """Config-4 batch: the admitted queries counted (a) one by one through the plain path and
+    (b) in one asdm_batch_count call; queries/s of both, per-query latency of the batched call."""
+    import asdmatch_amd as A
+    from asdmatch_amd import workloads as W
+
+    g = W.graph("c4")
+    adm = W.admitted("c4")
+    qs = [W.query(g, s, seed) for s, seed, _ in adm]
+    want = [c for _, _, c in adm]
+    ctx = A.Context(0)
+    ctx.upload(g)
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+
+    def plain():
+        out = []
+        for q in qs:
+            cs = ctx.filter(q, g)
+            out.append(0 if cs is None else ctx.count(cs.plan(q), cs))
+        return out
+
+    for _ in range(args.warmup):
+        assert plain() == want
+        c, _ = ctx.batch_count(g, qs)
+        assert [int(x) for x in c] == want
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        got = plain()
+    hip.hipDeviceSynchronize()
+    t1 = time.perf_counter()
+    if got != want:
+        raise SystemExit("plain count mismatch")
+    lats = []
+    for _ in range(args.steps):
+        c, lat = ctx.batch_count(g, qs)
+        lats.extend(lat.tolist())
+    hip.hipDeviceSynchronize()
+    t2 = time.perf_counter()
+    if [int(x) for x in c] != want:
+        raise SystemExit("batch count mismatch")
+    nq = len(qs) * args.steps
+    plain_qps, batch_qps = nq / (t1 - t0), nq / (t2 - t1)
+    lats.sort()
+    out = {
+        "metric": "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts",
+        "value": round(batch_qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round((t2 - t1) / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
+        "config": {"workload": "config 4: R-MAT scale 20, edge factor 10, 16 uniform labels; admitted random-walk "
+                               "queries, one asdm_batch_count call per step",
+                   "queries_per_step": len(qs), "query_sizes": sorted({q.n for q in qs}),
+                   "admission_row_limit": W.load_manifest()["c4"]["row_limit"]},
+        "plain_gpu_queries_per_s": round(plain_qps, 3),
+        "batched_vs_plain": round(batch_qps / plain_qps, 3),
+        "embeddings_per_s": round(sum(want) * args.steps / (t2 - t1), 1),
+        "latency_ms_p50": round(lats[len(lats) // 2], 3), "latency_ms_p99": round(lats[int(len(lats) * 0.99)], 3),
     }
     print(json.dumps(out), flush=True)
 
