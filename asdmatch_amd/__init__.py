@@ -100,6 +100,27 @@ _sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
 _sig("asdm_local_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p, _u64p)
 _sig("asdm_intersect_k", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_batch_count", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _u64p, C.POINTER(C.c_float))
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int)
+_sig("asdm_filter_many", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.POINTER(_P), _i32p)
+_sig("asdm_batch_count_learned", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _P, _u64p, _i32p, _f32p)
+_sig("asdm_mpnn_create", C.c_int, _P, C.c_uint64, C.POINTER(_P))
+_sig("asdm_mpnn_free", None, _P)
+_sig("asdm_mpnn_num_params", C.c_uint32)
+_sig("asdm_mpnn_get_params", C.c_int, _P, _f32p)
+_sig("asdm_mpnn_set_params", C.c_int, _P, _f32p)
+_sig("asdm_mpnn_save", C.c_int, _P, C.c_char_p)
+_sig("asdm_mpnn_load", C.c_int, _P, C.c_char_p)
+_sig("asdm_mpnn_forward", C.c_int, _P, C.c_uint32, _u32p, _f32p, _u32p, _u32p, _f32p)
+_sig("asdm_mpnn_grad", C.c_int, _P, C.c_uint32, _u32p, _f32p, _u32p, _u32p, _f32p, _f32p, _f32p)
+_sig("asdm_mpnn_train_step", C.c_int, _P, C.c_uint32, _u32p, _f32p, _u32p, _u32p, _f32p, C.c_float, _f32p)
+_sig("asdm_mpnn_features", C.c_int, _P, _P, _u64p, C.c_uint32, _u32p, _f32p, _u32p, _u32p, _u32p, _u32p)
+_sig("asdm_mpnn_estimate", C.c_int, _P, _P, _P, _u64p, C.c_uint32, C.POINTER(C.c_double))
+_sig("asdm_plan_create_learned", C.c_int, _P, _P, _P, _P, C.c_uint32, C.POINTER(_P))
+_sig("asdm_plain_estimate", C.c_int, _P, _u64p, C.c_uint32, C.POINTER(C.c_double))
+
+MPNN_FEATURES = 8
+MPNN_HIDDEN = 64
 
 
 def version() -> str:
@@ -250,6 +271,35 @@ class Context:
                                           lat.ctypes.data_as(C.POINTER(C.c_float))), "batch_count")
         return counts[:n], lat[:n]
 
+    def filter_many(self, qs, g: Graph):
+        """Batched filter: one candidate space per query (None where a candidate set is empty)."""
+        self.upload(g)
+        n = len(qs)
+        hs = (_P * max(n, 1))(*[q._h for q in qs])
+        out = (_P * max(n, 1))()
+        st = np.zeros(max(n, 1), np.int32)
+        self._check(_lib.asdm_filter_many(self._h, g._h, hs, n, out, st.ctypes.data_as(_i32p)), "filter_many")
+        res = [CandSpace(_P(out[i]), qs[i].n, self) if out[i] else None for i in range(n)]
+        bad = [i for i in range(n) if st[i] not in (ASDM_OK, ASDM_E_EMPTY_CANDIDATES)]
+        if bad:
+            raise AsdmError(int(st[bad[0]]), f"filter_many query {bad[0]}")
+        return res
+
+    def batch_count_learned(self, g: Graph, qs, mpnn: Optional["Mpnn"] = None, budget_bytes: int = 0):
+        """batch_count with learned plans (plain plans when mpnn is None) and per-query statuses:
+        returns (counts uint64[n], status int32[n], completion ms[n]).  A query over the row limit
+        has status ASDM_E_TIMEOUT and count 0."""
+        self.upload(g)
+        n = len(qs)
+        hs = (_P * max(n, 1))(*[q._h for q in qs])
+        counts = np.zeros(max(n, 1), np.uint64)
+        st = np.zeros(max(n, 1), np.int32)
+        lat = np.zeros(max(n, 1), np.float32)
+        self._check(_lib.asdm_batch_count_learned(self._h, g._h, hs, n, budget_bytes, mpnn._h if mpnn else None,
+                                                  _ptr(counts, C.c_uint64), st.ctypes.data_as(_i32p),
+                                                  lat.ctypes.data_as(_f32p)), "batch_count_learned")
+        return counts[:n], st[:n], lat[:n]
+
     def stats(self) -> Stats:
         s = Stats()
         _lib.asdm_last_stats(self._h, C.byref(s))
@@ -342,6 +392,150 @@ class CandSpace:
         if rc != ASDM_OK:
             raise AsdmError(rc, "plan_create")
         return Plan(h, self.nq)
+
+
+    def plan_learned(self, q: Graph, g: Graph, mpnn: "Mpnn", local: bool = False) -> "Plan":
+        """Plan ordered greedily by the learned estimator's cardinalities."""
+        h = _P()
+        rc = _lib.asdm_plan_create_learned(mpnn._h, g._h, q._h, self._h, ASDM_PLAN_LOCAL if local else 0,
+                                           C.byref(h))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "plan_create_learned")
+        return Plan(h, self.nq)
+
+
+class GraphBatch:
+    """A batch of small graphs for the estimator: node_off[B+1], x[N, F] float32, adj_off[N+1],
+    adj[E] (global node ids, symmetric adjacency)."""
+
+    def __init__(self, node_off, x, adj_off, adj):
+        self.node_off = _u32(node_off)
+        self.x = np.ascontiguousarray(x, np.float32).reshape(-1, MPNN_FEATURES)
+        self.adj_off = _u32(adj_off)
+        self.adj = _u32(adj)
+
+    @property
+    def size(self) -> int:
+        return len(self.node_off) - 1
+
+    def _args(self):
+        adj = self.adj if len(self.adj) else np.zeros(1, np.uint32)
+        return (self.size, _ptr(self.node_off), self.x.ctypes.data_as(_f32p), _ptr(self.adj_off), _ptr(adj))
+
+    @staticmethod
+    def concat(batches) -> "GraphBatch":
+        node_off, x, adj_off, adj = [np.zeros(1, np.uint32)], [], [np.zeros(1, np.uint32)], []
+        nbase = ebase = 0
+        for b in batches:
+            node_off.append(b.node_off[1:] + nbase)
+            adj_off.append(b.adj_off[1:] + ebase)
+            adj.append(b.adj + nbase)
+            x.append(b.x)
+            nbase += int(b.node_off[-1])
+            ebase += int(b.adj_off[-1])
+        return GraphBatch(np.concatenate(node_off), np.concatenate(x) if x else np.zeros((0, MPNN_FEATURES)),
+                          np.concatenate(adj_off), np.concatenate(adj) if adj else np.zeros(0, np.uint32))
+
+    def subset(self, idx) -> "GraphBatch":
+        return GraphBatch.concat([self._one(int(i)) for i in idx])
+
+    def _one(self, i: int) -> "GraphBatch":
+        n0, n1 = int(self.node_off[i]), int(self.node_off[i + 1])
+        e0, e1 = int(self.adj_off[n0]), int(self.adj_off[n1])
+        return GraphBatch(np.array([0, n1 - n0], np.uint32), self.x[n0:n1], self.adj_off[n0:n1 + 1] - e0,
+                          self.adj[e0:e1] - n0)
+
+
+class Mpnn:
+    """Learned cardinality estimator on a context's device (bf16 weights, fp32 masters and
+    accumulation).  Predicts log2(1 + #embeddings) of each graph of a GraphBatch."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        h = _P()
+        ctx._check(_lib.asdm_mpnn_create(ctx._h, seed, C.byref(h)), "mpnn_create")
+        self._h = h
+        self._ctx = ctx
+        self.num_params = int(_lib.asdm_mpnn_num_params())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and getattr(self._ctx, "_h", None):
+            _lib.asdm_mpnn_free(self._h)
+        self._h = None
+
+    def _check(self, rc, what):
+        self._ctx._check(rc, what)
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        self._check(_lib.asdm_mpnn_get_params(self._h, p.ctypes.data_as(_f32p)), "mpnn_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        self._check(_lib.asdm_mpnn_set_params(self._h, p.ctypes.data_as(_f32p)), "mpnn_set_params")
+
+    def save(self, path: str):
+        self._check(_lib.asdm_mpnn_save(self._h, os.fsencode(path)), f"mpnn_save {path}")
+
+    def load(self, path: str):
+        self._check(_lib.asdm_mpnn_load(self._h, os.fsencode(path)), f"mpnn_load {path}")
+
+    def forward(self, b: GraphBatch) -> np.ndarray:
+        y = np.zeros(max(b.size, 1), np.float32)
+        self._check(_lib.asdm_mpnn_forward(self._h, *b._args(), y.ctypes.data_as(_f32p)), "mpnn_forward")
+        return y[: b.size]
+
+    def grad(self, b: GraphBatch, targets):
+        t = np.ascontiguousarray(targets, np.float32)
+        g = np.zeros(self.num_params, np.float32)
+        y = np.zeros(max(b.size, 1), np.float32)
+        self._check(_lib.asdm_mpnn_grad(self._h, *b._args(), t.ctypes.data_as(_f32p), g.ctypes.data_as(_f32p),
+                                        y.ctypes.data_as(_f32p)), "mpnn_grad")
+        return g, y[: b.size]
+
+    def train_step(self, b: GraphBatch, targets, lr: float) -> float:
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = C.c_float()
+        self._check(_lib.asdm_mpnn_train_step(self._h, *b._args(), t.ctypes.data_as(_f32p), float(lr),
+                                              C.byref(loss)), "mpnn_train_step")
+        return loss.value
+
+    def estimate(self, g: Graph, cs: "CandSpace", masks) -> np.ndarray:
+        m = np.ascontiguousarray(masks, np.uint64)
+        out = np.zeros(max(len(m), 1), np.float64)
+        self._check(_lib.asdm_mpnn_estimate(self._h, g._h, cs._h, _ptr(m, C.c_uint64), len(m),
+                                            out.ctypes.data_as(C.POINTER(C.c_double))), "mpnn_estimate")
+        return out[: len(m)]
+
+
+def mpnn_features(g: Graph, cs: "CandSpace", masks) -> GraphBatch:
+    """Estimator inputs of the sub-queries induced by masks on a candidate space."""
+    m = np.ascontiguousarray(masks, np.uint64)
+    nn, na = C.c_uint32(), C.c_uint32()
+    rc = _lib.asdm_mpnn_features(g._h, cs._h, _ptr(m, C.c_uint64), len(m), None, None, None, None,
+                                 C.byref(nn), C.byref(na))
+    if rc != ASDM_OK:
+        raise AsdmError(rc, "mpnn_features")
+    node_off = np.zeros(len(m) + 1, np.uint32)
+    x = np.zeros((max(nn.value, 1), MPNN_FEATURES), np.float32)
+    adj_off = np.zeros(nn.value + 1, np.uint32)
+    adj = np.zeros(max(na.value, 1), np.uint32)
+    rc = _lib.asdm_mpnn_features(g._h, cs._h, _ptr(m, C.c_uint64), len(m), _ptr(node_off),
+                                 x.ctypes.data_as(_f32p), _ptr(adj_off), _ptr(adj), C.byref(nn), C.byref(na))
+    if rc != ASDM_OK:
+        raise AsdmError(rc, "mpnn_features")
+    return GraphBatch(node_off, x[: nn.value], adj_off, adj[: na.value])
+
+
+def plain_estimate(cs: "CandSpace", masks) -> np.ndarray:
+    """The planner's own independence-model cardinalities of the sub-queries induced by masks."""
+    m = np.ascontiguousarray(masks, np.uint64)
+    out = np.zeros(max(len(m), 1), np.float64)
+    rc = _lib.asdm_plain_estimate(cs._h, _ptr(m, C.c_uint64), len(m), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != ASDM_OK:
+        raise AsdmError(rc, "plain_estimate")
+    return out[: len(m)]
 
 
 class Plan:

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -150,9 +151,23 @@ int fill_u32(asdm_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
 int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nb, asdm_cs **out,
                 int *status);
 
-// Planner (plan.cpp)
-int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, const uint64_t *masks,
-               const double *cards, uint32_t ncards, asdm_plan *out, std::string &err);
+// Planner (plan.cpp).  card_of(mask, fallback) returns the cardinality estimate of the
+// sub-query induced by `mask` (the subsetToCard role), or `fallback` when it has none.
+using CardFn = std::function<double(uint64_t mask, double fallback)>;
+int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, const CardFn &card_of,
+               asdm_plan *out, std::string &err);
+// Pieces of the automatic plan, shared with the learned planner: the leaf set, the independence
+// model's growth factor, the vertices of `rest` outside S adjacent to S, and the device tables of
+// a chosen order whose first D vertices are enumerated.
+uint64_t choose_leaves(const asdm_cs *cs, uint32_t flags);
+double growth(const asdm_cs *cs, uint64_t S, uint32_t u);
+uint64_t frontier_of(const asdm_cs *cs, uint64_t S, uint64_t rest);
+uint32_t smallest_candidates(const asdm_cs *cs, uint64_t rest);
+int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint32_t flags, const CardFn &card_of,
+                asdm_plan *out, std::string &err);
+// Plans of a batch of candidate spaces from the learned estimator (learned.cpp).
+int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asdm_cs *> &css, uint32_t flags,
+                  std::vector<asdm_plan *> &out);
 // CFL root choice and BFS order (plan.cpp): base_count[u] = |{v : label, degree, NLF ok}|.
 uint32_t cfl_root(const asdm_graph *q, const asdm_graph *g, const std::vector<uint32_t> &base_count);
 std::vector<uint32_t> bfs_order(const asdm_graph *q, uint32_t root);

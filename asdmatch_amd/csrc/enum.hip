@@ -1094,8 +1094,11 @@ void by_class(uint32_t g, F1 f1, F4 f4, F16 f16, F64 f64) {
 // Level-synchronous batch engine over queries whose candidate spaces share one filter store.
 // All queries advance one depth per step: one count launch, one scan, one fill launch per depth
 // for the whole batch, and one host synchronization per depth.
+// qstatus_out (optional) receives ASDM_OK or, for a query whose partial embeddings exceed the
+// context's row limit, ASDM_E_TIMEOUT (its rows are dropped and its count is 0).  Without it a
+// query over the limit fails the whole call with ASDM_E_TIMEOUT.
 int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std::vector<asdm_plan *> &plans_in,
-                 uint64_t budget, uint64_t *counts_out) {
+                 uint64_t budget, uint64_t *counts_out, int *qstatus_out) {
     const uint32_t Q = (uint32_t)css_in.size();
     if (Q == 0) return ASDM_OK;
     // queries grouped by lane class, so that each class is a contiguous row range at every depth
@@ -1167,6 +1170,8 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         rc = check_hip(ctx, hipGetLastError(), "biota");
     }
     unsigned long long *d_bytes = (unsigned long long *)ctx->d_counters + 1;
+    std::vector<uint64_t> qrows(Q, 0);
+    std::vector<char> dropped(Q, 0);
     for (uint32_t d = 1; d <= maxD && n && !rc; ++d) {
         dev_free(ctx, counts_d);
         dev_free(ctx, offs);
@@ -1197,6 +1202,30 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         std::vector<uint64_t> next_h(Q + 1);
         if ((rc = check_hip(ctx, hipMemcpyAsync(next_h.data(), seg_next, 8 * (Q + 1), hipMemcpyDeviceToHost, ctx->stream), "D2H"))) break;
         if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) break;
+        if (ctx->row_limit && d < maxD) {  // per-query row limit: drop the queries over it
+            std::vector<uint32_t> over;
+            for (uint32_t i = 0; i < Q; ++i) {
+                if (dropped[i]) continue;
+                qrows[i] += next_h[i + 1] - next_h[i];
+                if (qrows[i] > ctx->row_limit) over.push_back(i);
+            }
+            if (!over.empty()) {
+                if (!qstatus_out) {
+                    rc = ctx->fail(ASDM_E_TIMEOUT, "row limit reached in a batched query");
+                    break;
+                }
+                for (uint32_t i : over) {
+                    dropped[i] = 1;
+                    if (seg_start_h[i + 1] > seg_start_h[i] &&
+                        (rc = fill_u32(ctx, counts_d + seg_start_h[i], seg_start_h[i + 1] - seg_start_h[i], 0)))
+                        break;
+                }
+                if (rc || (rc = scan_u32_to_u64(ctx, counts_d, offs, n))) break;
+                hipLaunchKernelGGL(k_bgather, dim3(1), dim3(256), 0, ctx->stream, offs, seg_start, Q, seg_next);
+                if ((rc = check_hip(ctx, hipMemcpyAsync(next_h.data(), seg_next, 8 * (Q + 1), hipMemcpyDeviceToHost, ctx->stream), "D2H"))) break;
+                if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) break;
+            }
+        }
         const uint64_t total = next_h[Q];
         if (total == 0 || d == maxD) break;
         if (4ull * (d + 1) * total > budget) {
@@ -1232,7 +1261,10 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (!rc)
-        for (uint32_t i = 0; i < Q; ++i) counts_out[perm[i]] = qc[i];
+        for (uint32_t i = 0; i < Q; ++i) {
+            counts_out[perm[i]] = dropped[i] ? 0 : qc[i];
+            if (qstatus_out) qstatus_out[perm[i]] = dropped[i] ? ASDM_E_TIMEOUT : ASDM_OK;
+        }
     resolve_timers(ctx);
     cleanup(dplans, seg_start, seg_next, seg_q, counts_d, offs, rows);
     dev_free(ctx, qcount);
@@ -1265,6 +1297,14 @@ int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, ui
 // was known, from the call start.
 int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq, uint64_t budget_bytes,
                      uint64_t *counts, float *lat_ms) {
+    return asdm_batch_count_learned(ctx, g, qs, nq, budget_bytes, nullptr, counts, nullptr, lat_ms);
+}
+
+// The same with the plans of all queries ordered by the learned estimator (one lock-step greedy
+// over the batch); m == NULL gives the plain planner.  With qstatus, a query that fails on its own
+// (row limit, unsupported shape) gets its status there and the others are still counted.
+int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
+                             uint64_t budget_bytes, asdm_mpnn *m, uint64_t *counts, int *qstatus, float *lat_ms) {
     if (!ctx || !g || (!qs && nq) || (!counts && nq)) return ASDM_E_INVALID;
     if (g->dev.device != ctx->device) return ctx->fail(ASDM_E_INVALID, "data graph not uploaded to this device");
     (void)hipSetDevice(ctx->device);
@@ -1272,6 +1312,7 @@ int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const
     auto now_ms = [&]() {
         return (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
+    auto per_query = [](int code) { return code == ASDM_E_TIMEOUT || code == ASDM_E_UNSUPPORTED; };
     std::vector<asdm_cs *> cs(nq, nullptr);
     std::vector<int> status(nq, ASDM_E_INVALID);
     int rc = filter_many(ctx, g, qs, nq, cs.data(), status.data());
@@ -1281,35 +1322,55 @@ int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const
     std::vector<uint32_t> bidx;
     for (uint32_t i = 0; i < nq; ++i) {
         counts[i] = 0;
+        if (qstatus) qstatus[i] = status[i];
         if (status[i] == ASDM_E_EMPTY_CANDIDATES) {
             if (lat_ms) lat_ms[i] = now_ms();
             continue;
         }
         if (status[i] != ASDM_OK) {
+            if (qstatus && per_query(status[i])) continue;
             rc = status[i];
             break;
         }
-        asdm_plan *p = nullptr;
-        if ((rc = asdm_plan_create(qs[i], cs[i], nullptr, 0, &p))) break;
+        if (!m) {
+            asdm_plan *p = nullptr;
+            if ((rc = asdm_plan_create(qs[i], cs[i], nullptr, 0, &p))) {
+                if (qstatus && per_query(rc)) {
+                    qstatus[i] = rc;
+                    rc = ASDM_OK;
+                    continue;
+                }
+                break;
+            }
+            bplans.push_back(p);
+        }
         bcs.push_back(cs[i]);
-        bplans.push_back(p);
         bidx.push_back(i);
     }
+    if (!rc && m) rc = learned_plans(m, g, std::vector<const asdm_cs *>(bcs.begin(), bcs.end()), 0, bplans);
     if (!rc) {
         const uint64_t budget = budget_bytes ? budget_bytes : (16ull << 30);
         std::vector<uint64_t> bc(bcs.size(), 0);
-        rc = batch_engine(ctx, bcs, bplans, budget, bc.data());
+        std::vector<int> bst(bcs.size(), ASDM_OK);
+        rc = batch_engine(ctx, bcs, bplans, budget, bc.data(), qstatus ? bst.data() : nullptr);
         if (rc == ASDM_E_BUDGET) {  // fall back to one query at a time
             rc = ASDM_OK;
             for (size_t k = 0; k < bcs.size() && !rc; ++k) {
-                rc = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
+                bst[k] = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
+                if (bst[k] != ASDM_OK) {
+                    bc[k] = 0;
+                    if (!qstatus || !per_query(bst[k])) rc = bst[k];
+                }
                 if (lat_ms) lat_ms[bidx[k]] = now_ms();
             }
         } else if (lat_ms) {
             const float t = now_ms();
             for (uint32_t k : bidx) lat_ms[k] = t;
         }
-        for (size_t k = 0; k < bcs.size(); ++k) counts[bidx[k]] = bc[k];
+        for (size_t k = 0; k < bcs.size(); ++k) {
+            counts[bidx[k]] = bc[k];
+            if (qstatus) qstatus[bidx[k]] = bst[k];
+        }
     }
     for (auto p : bplans) asdm_plan_free(p);
     for (auto c : cs) asdm_cs_free(c);

@@ -586,6 +586,14 @@ int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asd
     return status;
 }
 
+int asdm_filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq, asdm_cs **cs,
+                     int *status) {
+    if (!ctx || !g || (nq && (!qs || !cs || !status))) return ASDM_E_INVALID;
+    for (uint32_t i = 0; i < nq; ++i)
+        if (!qs[i]) return ASDM_E_INVALID;
+    return filter_many(ctx, g, qs, nq, cs, status);
+}
+
 uint32_t asdm_cs_root(const asdm_cs *cs) { return cs ? cs->root : 0; }
 
 int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes) {

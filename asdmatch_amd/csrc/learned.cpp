@@ -1,0 +1,264 @@
+// Learned planner: features of (sub-)query graphs for the message-passing estimator, batched
+// cardinality estimates, and the greedy matching order driven by them.
+//
+// The reference fills subsetToCard (plan/estimator.cpp:7) with sampling estimates for the
+// optimizer (plan/estimator.cpp:173-291) and orders by them.  Here the same role -- a cardinality
+// per vertex subset, consumed by the greedy prefix order -- is played by the network.  The greedy
+// runs in lock step over a whole batch of queries, so each step is one forward launch for all
+// queries' candidate subsets.  Whatever order it picks, the engine's count is exact.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <unordered_map>
+
+#include "engine.h"
+
+namespace asdm {
+namespace {
+
+constexpr uint32_t kFeat = 8;
+
+struct FeatBatch {
+    std::vector<uint32_t> node_off{0};
+    std::vector<float> x;
+    std::vector<uint32_t> adj_off{0};
+    std::vector<uint32_t> adj;
+};
+
+inline float lg(double v) { return (float)std::log2(1.0 + std::max(0.0, v)); }
+
+// Node features of the sub-query induced by `mask` on the candidate space `cs`:
+//   0 log2(1+|C(u)|)/16            candidate-set size
+//   1 log2(1+|V_L(u)|)/16          label frequency in the data graph
+//   2 log2(1+mean deg of L(u))/8   label degree in the data graph
+//   3 deg_S(u)/8                   degree inside the sub-query
+//   4 #same-label vertices in S/8  (injectivity pressure)
+//   5 mean over S-neighbours w of log2(1 + W(w->u)/|C(w)|)/8   expected fan-out into u
+//   6 |S|/32, 7 |E(S)|/64          size of the sub-query
+void featurize(const asdm_graph *g, const asdm_cs *cs, uint64_t mask, FeatBatch &fb) {
+    uint32_t local[64];
+    uint32_t k = 0;
+    for (uint64_t r = mask; r; r &= r - 1) local[__builtin_ctzll(r)] = k++;
+    const uint32_t base = fb.node_off.back();
+    uint32_t edges2 = 0;
+    for (uint64_t r = mask; r; r &= r - 1) {
+        const uint32_t u = (uint32_t)__builtin_ctzll(r);
+        for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) edges2 += (mask >> cs->qnbr[a]) & 1;
+    }
+    for (uint64_t r = mask; r; r &= r - 1) {
+        const uint32_t u = (uint32_t)__builtin_ctzll(r);
+        const uint32_t l = cs->qlab[u];
+        uint32_t deg = 0, same = 0;
+        double fo = 0;
+        for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) {
+            const uint32_t w = cs->qnbr[a];
+            if (!(mask >> w & 1)) continue;
+            ++deg;
+            const uint32_t aw = cs->arc_index(w, u);
+            fo += lg((double)cs->arc_weight[aw] / std::max(1.0, (double)cs->csize[w]));
+            fb.adj.push_back(base + local[w]);
+        }
+        for (uint64_t r2 = mask; r2; r2 &= r2 - 1) same += cs->qlab[__builtin_ctzll(r2)] == l;
+        const float f[kFeat] = {lg(cs->csize[u]) / 16.f,
+                                lg(g->label_count(l)) / 16.f,
+                                l < g->lab_avgdeg.size() ? lg(g->lab_avgdeg[l]) / 8.f : 0.f,
+                                deg / 8.f,
+                                same / 8.f,
+                                deg ? (float)(fo / deg) / 8.f : 0.f,
+                                k / 32.f,
+                                (edges2 / 2) / 64.f};
+        fb.x.insert(fb.x.end(), f, f + kFeat);
+        fb.adj_off.push_back((uint32_t)fb.adj.size());
+    }
+    fb.node_off.push_back(base + k);
+}
+
+int forward(asdm_mpnn *m, const FeatBatch &fb, std::vector<float> &y) {
+    const uint32_t B = (uint32_t)fb.node_off.size() - 1;
+    y.assign(B, 0.f);
+    if (!B) return ASDM_OK;
+    return asdm_mpnn_forward(m, B, fb.node_off.data(), fb.x.data(), fb.adj_off.data(),
+                             fb.adj.empty() ? nullptr : fb.adj.data(), y.data());
+}
+
+inline double card_from_log(float y) { return std::exp2(std::min(64.f, std::max(0.f, y))) - 1.0; }
+
+// Lock-step greedy over a batch of candidate spaces: every step estimates S | {u} for every
+// frontier vertex u of every unfinished query in one forward launch and extends each query by its
+// smallest estimate (first vertex on ties).  Singletons use the exact |C(u)|.
+int learned_orders(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asdm_cs *> &css, uint32_t flags,
+                   std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
+                   std::vector<std::unordered_map<uint64_t, double>> &cards) {
+    const size_t Q = css.size();
+    orders.assign(Q, {});
+    Ds.assign(Q, 0);
+    cards.assign(Q, {});
+    std::vector<uint64_t> rest(Q), S(Q), leaves(Q);
+    for (size_t i = 0; i < Q; ++i) {
+        const asdm_cs *cs = css[i];
+        const uint64_t all = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
+        leaves[i] = choose_leaves(cs, flags);
+        rest[i] = all & ~leaves[i];
+        const uint32_t first = smallest_candidates(cs, rest[i]);
+        orders[i].push_back(first);
+        S[i] = 1ull << first;
+        cards[i][S[i]] = cs->csize[first];
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> who;  // (query, vertex) of each estimated subset
+    std::vector<float> y;
+    for (;;) {
+        FeatBatch fb;
+        who.clear();
+        for (size_t i = 0; i < Q; ++i) {
+            if (S[i] == rest[i]) continue;
+            for (uint64_t r = frontier_of(css[i], S[i], rest[i]); r; r &= r - 1) {
+                const uint32_t u = (uint32_t)__builtin_ctzll(r);
+                featurize(g, css[i], S[i] | (1ull << u), fb);
+                who.emplace_back((uint32_t)i, u);
+            }
+        }
+        if (who.empty()) break;
+        if (int rc = forward(m, fb, y)) return rc;
+        std::vector<uint32_t> best(Q, UINT32_MAX);
+        std::vector<double> bestc(Q, 0);
+        for (size_t k = 0; k < who.size(); ++k) {
+            const uint32_t i = who[k].first, u = who[k].second;
+            const double c = card_from_log(y[k]);
+            cards[i][S[i] | (1ull << u)] = c;
+            if (best[i] == UINT32_MAX || c < bestc[i]) {
+                best[i] = u;
+                bestc[i] = c;
+            }
+        }
+        for (size_t i = 0; i < Q; ++i) {
+            if (best[i] == UINT32_MAX) continue;
+            orders[i].push_back(best[i]);
+            S[i] |= 1ull << best[i];
+        }
+    }
+    for (size_t i = 0; i < Q; ++i) {
+        Ds[i] = (uint32_t)orders[i].size();
+        for (uint64_t l = leaves[i]; l; l &= l - 1) orders[i].push_back((uint32_t)__builtin_ctzll(l));
+    }
+    return ASDM_OK;
+}
+
+}  // namespace
+
+// Plans for a batch of candidate spaces from the learned estimates; a query whose learned plan
+// cannot be built (or every query, when the network cannot run) gets the plain plan.
+int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asdm_cs *> &css, uint32_t flags,
+                  std::vector<asdm_plan *> &out) {
+    const size_t Q = css.size();
+    out.assign(Q, nullptr);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::vector<uint32_t>> orders;
+    std::vector<uint32_t> Ds;
+    std::vector<std::unordered_map<uint64_t, double>> cards;
+    const bool have = m && learned_orders(m, g, css, flags, orders, Ds, cards) == ASDM_OK;
+    const double share = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                         std::max<size_t>(Q, 1);
+    for (size_t i = 0; i < Q; ++i) {
+        auto t1 = std::chrono::steady_clock::now();
+        auto p = std::make_unique<asdm_plan>();
+        std::string err;
+        int rc = ASDM_E_INVALID;
+        if (have) {
+            const auto &cm = cards[i];
+            rc = finish_plan(
+                css[i], orders[i], Ds[i], flags,
+                [&cm](uint64_t mask, double fallback) {
+                    auto it = cm.find(mask);
+                    return it == cm.end() ? fallback : it->second;
+                },
+                p.get(), err);
+        }
+        if (rc) {
+            p = std::make_unique<asdm_plan>();
+            rc = build_plan(nullptr, css[i], nullptr, flags, [](uint64_t, double fb) { return fb; }, p.get(), err);
+        }
+        if (rc) {
+            for (auto &q : out) {
+                asdm_plan_free(q);
+                q = nullptr;
+            }
+            css[i]->ctx->err = err;
+            return rc;
+        }
+        p->plan_ms = share + std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+        out[i] = p.release();
+    }
+    return ASDM_OK;
+}
+
+}  // namespace asdm
+
+using namespace asdm;
+
+extern "C" {
+
+int asdm_mpnn_features(const asdm_graph *g, const asdm_cs *cs, const uint64_t *masks, uint32_t n, uint32_t *node_off,
+                       float *x, uint32_t *adj_off, uint32_t *adj, uint32_t *num_nodes, uint32_t *num_adj) {
+    if (!g || !cs || (!masks && n) || !num_nodes || !num_adj) return ASDM_E_INVALID;
+    FeatBatch fb;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t valid = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
+        if (!masks[i] || (masks[i] & ~valid) || __builtin_popcountll(masks[i]) > 32) return ASDM_E_INVALID;
+        featurize(g, cs, masks[i], fb);
+    }
+    const uint32_t N = fb.node_off.back(), E = (uint32_t)fb.adj.size();
+    if (node_off && x && adj_off && (adj || !E)) {
+        if (*num_nodes < N || *num_adj < E) return ASDM_E_INVALID;
+        std::copy(fb.node_off.begin(), fb.node_off.end(), node_off);
+        std::copy(fb.x.begin(), fb.x.end(), x);
+        std::copy(fb.adj_off.begin(), fb.adj_off.end(), adj_off);
+        std::copy(fb.adj.begin(), fb.adj.end(), adj);
+    }
+    *num_nodes = N;
+    *num_adj = E;
+    return ASDM_OK;
+}
+
+int asdm_mpnn_estimate(asdm_mpnn *m, const asdm_graph *g, const asdm_cs *cs, const uint64_t *masks, uint32_t n,
+                       double *cards) {
+    if (!m || !g || !cs || (!masks && n) || (!cards && n)) return ASDM_E_INVALID;
+    FeatBatch fb;
+    const uint64_t valid = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!masks[i] || (masks[i] & ~valid)) return ASDM_E_INVALID;
+        featurize(g, cs, masks[i], fb);
+    }
+    std::vector<float> y;
+    if (int rc = forward(m, fb, y)) return rc;
+    for (uint32_t i = 0; i < n; ++i) cards[i] = card_from_log(y[i]);
+    return ASDM_OK;
+}
+
+int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, double *cards) {
+    if (!cs || (n && (!masks || !cards))) return ASDM_E_INVALID;
+    const uint64_t valid = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!masks[i] || (masks[i] & ~valid)) return ASDM_E_INVALID;
+        double est = 1.0;
+        uint64_t S = 0;
+        for (uint64_t r = masks[i]; r; r &= r - 1) {
+            const uint32_t u = (uint32_t)__builtin_ctzll(r);
+            est *= growth(cs, S, u);
+            S |= 1ull << u;
+        }
+        cards[i] = est;
+    }
+    return ASDM_OK;
+}
+
+int asdm_plan_create_learned(asdm_mpnn *m, const asdm_graph *g, const asdm_graph *q, const asdm_cs *cs,
+                             uint32_t flags, asdm_plan **out) {
+    if (!m || !g || !q || !cs || !out || q->n != cs->nq) return ASDM_E_INVALID;
+    std::vector<asdm_plan *> plans;
+    int rc = learned_plans(m, g, {cs}, flags, plans);
+    *out = rc ? nullptr : plans[0];
+    return rc;
+}
+
+}  // extern "C"

@@ -84,17 +84,6 @@ double fanout(const asdm_cs *cs, uint32_t p, uint32_t u) {
     return (double)cs->arc_weight[a] / std::max<double>(1.0, cs->csize[p]);
 }
 
-// Estimated number of partial embeddings when u joins the matched set S (independence model):
-// |C(u)| * prod over matched neighbours p of W(p->u) / (|C(p)| |C(u)|).
-double growth(const asdm_cs *cs, uint64_t S, uint32_t u) {
-    double g = cs->csize[u];
-    for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) {
-        uint32_t p = cs->qnbr[a];
-        if (S >> p & 1) g *= (double)cs->arc_weight[cs->arc_index(p, u)] / std::max(1.0, (double)cs->csize[p] * cs->csize[u]);
-    }
-    return g;
-}
-
 void partitions(uint32_t k, std::vector<std::vector<uint32_t>> &out, std::vector<int64_t> &coef) {
     // restricted growth strings over k elements
     std::vector<uint32_t> a(k, 0);
@@ -123,21 +112,92 @@ void partitions(uint32_t k, std::vector<std::vector<uint32_t>> &out, std::vector
 
 }  // namespace
 
-int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_order, uint32_t flags,
-               const uint64_t *masks, const double *cards, uint32_t ncards, asdm_plan *out, std::string &err) {
+// Estimated number of partial embeddings when u joins the matched set S (independence model):
+// |C(u)| * prod over matched neighbours p of W(p->u) / (|C(p)| |C(u)|).
+double growth(const asdm_cs *cs, uint64_t S, uint32_t u) {
+    double g = cs->csize[u];
+    for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) {
+        uint32_t p = cs->qnbr[a];
+        if (S >> p & 1) g *= (double)cs->arc_weight[cs->arc_index(p, u)] / std::max(1.0, (double)cs->csize[p] * cs->csize[u]);
+    }
+    return g;
+}
+
+// Leaves: an independent set whose removal keeps the rest connected, preferring large expected
+// fan-out (cheap to count, expensive to enumerate).
+uint64_t choose_leaves(const asdm_cs *cs, uint32_t flags) {
     const uint32_t nq = cs->nq;
-    const bool local = flags & ASDM_PLAN_LOCAL;
     const bool nofactor = flags & ASDM_PLAN_NO_FACTOR;
-    const uint32_t max_group = local ? 1 : kMaxGroup;
+    const uint32_t max_group = (flags & ASDM_PLAN_LOCAL) ? 1 : kMaxGroup;
+    const uint64_t all = nq == 64 ? ~0ull : ((1ull << nq) - 1);
+    uint64_t leaves = 0;
+    std::vector<std::pair<double, uint32_t>> cand;
+    for (uint32_t u = 0; u < nq; ++u) {
+        double f = 1e300;
+        for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) f = std::min(f, fanout(cs, cs->qnbr[a], u));
+        cand.emplace_back(-f, u);
+    }
+    std::sort(cand.begin(), cand.end());
+    // Pass 0 keeps only leaves that the fused last-level kernel counts with a few lookups -- one
+    // query neighbour, or two adjacent ones (a triangle, counted from a per-candidate-edge table) --
+    // with pairwise distinct labels, so no inclusion-exclusion pair intersections are needed per
+    // partial embedding.  Pass 1 (taken when pass 0 admits only the last vertex) is the plain rule:
+    // any independent set within the label-group limit.
+    auto cheap_leaf = [&](uint32_t u) {
+        const uint32_t d = cs->qoff[u + 1] - cs->qoff[u];
+        return d == 1 || (d == 2 && adjacent(cs, cs->qnbr[cs->qoff[u]], cs->qnbr[cs->qoff[u] + 1]));
+    };
+    for (int pass = 0; pass < 2 && nq > 1; ++pass) {
+        leaves = 0;
+        for (auto &c : cand) {
+            if (nofactor && leaves) break;
+            uint32_t u = c.second;
+            if (pass == 0 && !cheap_leaf(u)) continue;
+            bool indep = true;
+            for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) indep &= !(leaves >> cs->qnbr[a] & 1);
+            if (!indep) continue;
+            uint32_t same = 1;
+            for (uint64_t l = leaves; l; l &= l - 1) same += cs->qlab[__builtin_ctzll(l)] == cs->qlab[u];
+            if (same > (pass == 0 ? 1u : max_group)) continue;
+            uint64_t rest = all & ~(leaves | (1ull << u));
+            if (!connected(cs, rest)) continue;
+            leaves |= 1ull << u;
+        }
+        if (__builtin_popcountll(leaves) >= 2 || nofactor) break;
+    }
+    return leaves;
+}
+
+uint64_t frontier_of(const asdm_cs *cs, uint64_t S, uint64_t rest) {
+    uint64_t f = 0;
+    for (uint64_t r = rest & ~S; r; r &= r - 1) {
+        uint32_t u = (uint32_t)__builtin_ctzll(r);
+        for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a)
+            if (S >> cs->qnbr[a] & 1) {
+                f |= 1ull << u;
+                break;
+            }
+    }
+    return f;
+}
+
+uint32_t smallest_candidates(const asdm_cs *cs, uint64_t rest) {
+    uint32_t first = UINT32_MAX;
+    for (uint64_t r = rest; r; r &= r - 1) {
+        uint32_t u = (uint32_t)__builtin_ctzll(r);
+        if (first == UINT32_MAX || cs->csize[u] < cs->csize[first]) first = u;
+    }
+    return first;
+}
+
+int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_order, uint32_t flags,
+               const CardFn &card_of, asdm_plan *out, std::string &err) {
+    (void)q;
+    const uint32_t nq = cs->nq;
+    const bool nofactor = flags & ASDM_PLAN_NO_FACTOR;
+    const uint32_t max_group = (flags & ASDM_PLAN_LOCAL) ? 1 : kMaxGroup;
     std::vector<uint32_t> order;
     uint32_t D = 0;
-
-    // optional cardinalities keyed by subset mask (subsetToCard analogue)
-    auto card_of = [&](uint64_t mask, double fallback) {
-        for (uint32_t i = 0; i < ncards; ++i)
-            if (masks[i] == mask) return cards[i];
-        return fallback;
-    };
 
     if (given_order) {
         order.assign(given_order, given_order + nq);
@@ -164,63 +224,20 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         if (D == nq) D = nq - 1 > 0 ? nq - 1 : 1;
         if (nq == 1) D = 1;
     } else {
-        // 1) leaves: an independent set whose removal keeps the rest connected, preferring large
-        //    expected fan-out (cheap to count, expensive to enumerate)
-        uint64_t all = nq == 64 ? ~0ull : ((1ull << nq) - 1);
-        uint64_t leaves = 0;
-        std::vector<std::pair<double, uint32_t>> cand;
-        for (uint32_t u = 0; u < nq; ++u) {
-            double f = 1e300;
-            for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) f = std::min(f, fanout(cs, cs->qnbr[a], u));
-            cand.emplace_back(-f, u);
-        }
-        std::sort(cand.begin(), cand.end());
-        // Pass 0 keeps only leaves that the fused last-level kernel counts with a few lookups --
-        // one query neighbour, or two adjacent ones (a triangle, counted from a per-candidate-edge
-        // table) -- with pairwise distinct labels, so no inclusion-exclusion pair intersections
-        // are needed per partial embedding.  Pass 1 (taken when pass 0 admits only the last
-        // vertex) is the plain rule: any independent set within the label-group limit.
-        auto cheap_leaf = [&](uint32_t u) {
-            const uint32_t d = cs->qoff[u + 1] - cs->qoff[u];
-            return d == 1 || (d == 2 && adjacent(cs, cs->qnbr[cs->qoff[u]], cs->qnbr[cs->qoff[u] + 1]));
-        };
-        for (int pass = 0; pass < 2 && nq > 1; ++pass) {
-            leaves = 0;
-            for (auto &c : cand) {
-                if (nofactor && leaves) break;
-                uint32_t u = c.second;
-                if (pass == 0 && !cheap_leaf(u)) continue;
-                bool indep = true;
-                for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) indep &= !(leaves >> cs->qnbr[a] & 1);
-                if (!indep) continue;
-                uint32_t same = 1;
-                for (uint64_t l = leaves; l; l &= l - 1) same += cs->qlab[__builtin_ctzll(l)] == cs->qlab[u];
-                if (same > (pass == 0 ? 1u : max_group)) continue;
-                uint64_t rest = all & ~(leaves | (1ull << u));
-                if (!connected(cs, rest)) continue;
-                leaves |= 1ull << u;
-            }
-            if (__builtin_popcountll(leaves) >= 2 || nofactor) break;
-        }
-        // 2) prefix order over the rest: smallest candidate set first, then greedy smallest
-        //    estimated growth among vertices adjacent to the matched set
-        uint64_t rest = all & ~leaves;
-        uint32_t first = UINT32_MAX;
-        for (uint64_t r = rest; r; r &= r - 1) {
-            uint32_t u = (uint32_t)__builtin_ctzll(r);
-            if (first == UINT32_MAX || cs->csize[u] < cs->csize[first]) first = u;
-        }
+        const uint64_t all = nq == 64 ? ~0ull : ((1ull << nq) - 1);
+        const uint64_t leaves = choose_leaves(cs, flags);
+        // prefix order over the rest: smallest candidate set first, then greedy smallest
+        // estimated cardinality among vertices adjacent to the matched set
+        const uint64_t rest = all & ~leaves;
+        const uint32_t first = smallest_candidates(cs, rest);
         order.push_back(first);
         uint64_t S = 1ull << first;
         double est = cs->csize[first];
         while (S != rest) {
             uint32_t best = UINT32_MAX;
             double bestg = 0;
-            for (uint64_t r = rest & ~S; r; r &= r - 1) {
+            for (uint64_t r = frontier_of(cs, S, rest); r; r &= r - 1) {
                 uint32_t u = (uint32_t)__builtin_ctzll(r);
-                bool conn = false;
-                for (uint32_t a = cs->qoff[u]; a < cs->qoff[u + 1]; ++a) conn |= (S >> cs->qnbr[a] & 1) != 0;
-                if (!conn) continue;
                 double gr = card_of(S | (1ull << u), est * growth(cs, S, u));
                 if (best == UINT32_MAX || gr < bestg) {
                     best = u;
@@ -234,7 +251,13 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         D = (uint32_t)order.size();
         for (uint64_t l = leaves; l; l &= l - 1) order.push_back((uint32_t)__builtin_ctzll(l));
     }
+    return finish_plan(cs, std::move(order), D, flags, card_of, out, err);
+}
 
+int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint32_t flags, const CardFn &card_of,
+                asdm_plan *out, std::string &err) {
+    const uint32_t nq = cs->nq;
+    const bool local = flags & ASDM_PLAN_LOCAL;
     // check connectivity of the enumerated prefix (every depth >= 1 needs a backward neighbour)
     for (uint32_t d = 1; d < nq; ++d) {
         bool has = false;
@@ -245,27 +268,21 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         }
     }
 
-    // estimated frontier sizes along the final order
+    // estimated frontier sizes along the final order, and the prefix cardinalities the plan
+    // relies on (its subsetToCard entries)
     {
         double est = cs->csize[order[0]], tot = est;
         uint64_t S = 1ull << order[0];
+        out->prefix_cards.clear();
+        out->prefix_cards.emplace_back(S, est);
         for (uint32_t d = 1; d < D; ++d) {
             est = card_of(S | (1ull << order[d]), est * growth(cs, S, order[d]));
             S |= 1ull << order[d];
             tot += est;
+            out->prefix_cards.emplace_back(S, est);
         }
         out->est_leaf = est;
         out->est_total = tot;
-        // prefix cardinalities along the order (the subsetToCard entries this plan relies on)
-        out->prefix_cards.clear();
-        double e2 = cs->csize[order[0]];
-        uint64_t S2 = 1ull << order[0];
-        out->prefix_cards.emplace_back(S2, e2);
-        for (uint32_t d = 1; d < D; ++d) {
-            e2 = card_of(S2 | (1ull << order[d]), e2 * growth(cs, S2, order[d]));
-            S2 |= 1ull << order[d];
-            out->prefix_cards.emplace_back(S2, e2);
-        }
     }
 
     DPlan &P = out->h;
@@ -346,7 +363,13 @@ int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const ui
     auto t0 = std::chrono::steady_clock::now();
     auto p = std::make_unique<asdm_plan>();
     std::string err;
-    int rc = build_plan(q, cs, nullptr, flags, masks, cards, ncards, p.get(), err);
+    if (ncards && (!masks || !cards)) return ASDM_E_INVALID;
+    auto card_of = [&](uint64_t mask, double fallback) {
+        for (uint32_t i = 0; i < ncards; ++i)
+            if (masks[i] == mask) return cards[i];
+        return fallback;
+    };
+    int rc = build_plan(q, cs, nullptr, flags, card_of, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;
@@ -362,7 +385,7 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
     auto t0 = std::chrono::steady_clock::now();
     auto p = std::make_unique<asdm_plan>();
     std::string err;
-    int rc = build_plan(q, cs, order, flags, nullptr, nullptr, 0, p.get(), err);
+    int rc = build_plan(q, cs, order, flags, [](uint64_t, double fallback) { return fallback; }, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;

@@ -21,6 +21,11 @@
  *                               (ALL_LEVEL + LOCAL_COUNT build)     main.cpp:105-107,135-146
  *   asdm_intersect_k            ComputeSetIntersection::LeapfrogJoin utility/compute_set_intersection.cpp:193-233
  *   asdm_batch_count            (no counterpart: many queries per launch; see asdm_batch_count)
+ *   asdm_filter_many            buildCandCFL for many queries       relation/candidate_space.cpp:550-721
+ *   asdm_mpnn_*                 cardinality estimates for subsetToCard, in place of the sampling
+ *                               estimator cardEstimateLabeled       plan/estimator.cpp:7, :173-291
+ *   asdm_plan_create_learned    optimizer ordering by subsetToCard  plan/optimizer.cpp:7-200
+ *   asdm_batch_count_learned    (no counterpart: asdm_batch_count with learned plans)
  */
 #ifndef ASDM_H
 #define ASDM_H
@@ -48,6 +53,7 @@ typedef struct asdm_ctx asdm_ctx;     /* device, stream, scratch arena */
 typedef struct asdm_graph asdm_graph; /* host graph (label-sorted CSR), optionally on device */
 typedef struct asdm_cs asdm_cs;       /* device candidate space of (query, data) */
 typedef struct asdm_plan asdm_plan;   /* host matching plan consumed by the engine */
+typedef struct asdm_mpnn asdm_mpnn;   /* learned cardinality estimator (device weights) */
 
 /* ---- context ---- */
 int asdm_ctx_create(int device, asdm_ctx **out);
@@ -167,6 +173,59 @@ int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out);
  * lat_ms[nq] (optional) the completion time of each query measured from the call start. */
 int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
                      uint64_t budget_bytes, uint64_t *counts, float *lat_ms);
+
+/* Batched filter: candidate spaces of nq queries against one uploaded data graph in one pass.
+ * cs[i] is NULL with status[i] = ASDM_E_EMPTY_CANDIDATES when a candidate set of query i is
+ * empty; the views share device storage that the last asdm_cs_free releases. */
+int asdm_filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq, asdm_cs **cs,
+                     int *status);
+/* asdm_batch_count with plans ordered by the learned estimator m (the plain planner when m is
+ * NULL).  Counts are the same exact values whatever the plans.  With status[nq] (optional) a query
+ * that fails on its own -- the context's row limit (ASDM_E_TIMEOUT) or an unsupported shape --
+ * gets its code there and count 0 while the rest are counted; status[i] = ASDM_E_EMPTY_CANDIDATES
+ * for a query with an empty candidate set (count 0).  Without it such a query fails the call. */
+int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
+                             uint64_t budget_bytes, asdm_mpnn *m, uint64_t *counts, int *status, float *lat_ms);
+
+/* ---- learned cardinality estimator ----
+ * A 3-layer message-passing network (hidden 64, node features ASDM_MPNN_FEATURES, graphs of at
+ * most 32 nodes) predicting y = log2(1 + #embeddings) of a (sub-)query.  Weights are bf16 with
+ * fp32 masters; all products accumulate in fp32.  A batch of B graphs is given as node_off[B+1]
+ * (node ranges), x[N * ASDM_MPNN_FEATURES] (row-major node features), adj_off[N+1] and adj (global
+ * node ids of each node's neighbours; the adjacency must be symmetric). */
+#define ASDM_MPNN_FEATURES 8
+#define ASDM_MPNN_HIDDEN 64
+int asdm_mpnn_create(asdm_ctx *ctx, uint64_t seed, asdm_mpnn **out);
+void asdm_mpnn_free(asdm_mpnn *m); /* before asdm_ctx_destroy of its context */
+uint32_t asdm_mpnn_num_params(void);
+int asdm_mpnn_get_params(asdm_mpnn *m, float *params);       /* fp32 masters */
+int asdm_mpnn_set_params(asdm_mpnn *m, const float *params); /* also refreshes the bf16 copy */
+int asdm_mpnn_save(asdm_mpnn *m, const char *path);
+int asdm_mpnn_load(asdm_mpnn *m, const char *path);
+int asdm_mpnn_forward(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const float *x, const uint32_t *adj_off,
+                      const uint32_t *adj, float *y);
+/* Gradient (fp32, w.r.t. the bf16 weights) of 0.5 * mean_b (y_b - t_b)^2, without an update. */
+int asdm_mpnn_grad(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const float *x, const uint32_t *adj_off,
+                   const uint32_t *adj, const float *targets, float *grad, float *y);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; *loss is the loss before the step. */
+int asdm_mpnn_train_step(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const float *x,
+                         const uint32_t *adj_off, const uint32_t *adj, const float *targets, float lr, float *loss);
+/* Features of the sub-queries induced by masks[n] on a candidate space (data graph g gives the
+ * label statistics).  With node_off/x/adj_off/adj NULL only *num_nodes / *num_adj are returned;
+ * otherwise they are capacities on input and sizes on output. */
+int asdm_mpnn_features(const asdm_graph *g, const asdm_cs *cs, const uint64_t *masks, uint32_t n,
+                       uint32_t *node_off, float *x, uint32_t *adj_off, uint32_t *adj, uint32_t *num_nodes,
+                       uint32_t *num_adj);
+/* Estimated cardinalities 2^y - 1 of the sub-queries induced by masks[n]. */
+int asdm_mpnn_estimate(asdm_mpnn *m, const asdm_graph *g, const asdm_cs *cs, const uint64_t *masks, uint32_t n,
+                       double *cards);
+/* The plain planner's independence-model cardinalities of the same sub-queries:
+ * prod_u |C(u)| * prod over sub-query edges (u, w) of W(u->w) / (|C(u)| |C(w)|). */
+int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, double *cards);
+/* Plan whose enumeration order is chosen greedily by the learned estimates (the plain planner's
+ * leaf set and first vertex; the plain estimator when the network cannot run). */
+int asdm_plan_create_learned(asdm_mpnn *m, const asdm_graph *g, const asdm_graph *q, const asdm_cs *cs,
+                             uint32_t flags, asdm_plan **out);
 
 /* ---- .card planner files ---- */
 /* Reads the subsetToCard map (and the stored planning time) of a .card file written by the

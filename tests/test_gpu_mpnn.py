@@ -1,0 +1,220 @@
+"""Learned cardinality estimator on the GPU: forward / gradient / Adam kernels against the CPU
+float64 oracle, features against their definition, plan-independence of the counts under learned
+plans, and what training gains over the plain independence estimator."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import asdmatch_amd as A
+from asdmatch_amd import learned as LR
+from oracle import mpnn_oracle as M
+from tests import common as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(seed, B, max_nodes):
+    no, x, ao, adj = M.random_batch(np.random.default_rng(seed), B, max_nodes)
+    return A.GraphBatch(no, x, ao, adj), (no, x, ao, adj)
+
+
+def _params(seed):
+    p = M.init_params(seed)
+    p[M.B_IN:M.B_IN + M.H] = 0.05
+    p[M.B_OUT] = 0.5
+    return p
+
+
+def test_forward_matches_oracle(ctx):
+    m = A.Mpnn(ctx, seed=1)
+    assert m.num_params == M.NPARAMS
+    p = _params(2)
+    m.set_params(p)
+    assert np.array_equal(m.params(), p)
+    for seed, B, mx in ((3, 1, 1), (4, 37, 12), (5, 300, 32)):
+        b, raw = _batch(seed, B, mx)
+        y = m.forward(b)
+        yo, _ = M.forward(p, *raw)
+        # fp32 accumulation of bf16 weights vs float64 with the same bf16 weights
+        assert np.allclose(y, yo, rtol=2e-5, atol=2e-5 * (1 + np.abs(yo).max())), (seed, np.abs(y - yo).max())
+
+
+def test_gradient_matches_oracle(ctx):
+    m = A.Mpnn(ctx, seed=1)
+    p = _params(6)
+    m.set_params(p)
+    b, raw = _batch(7, 64, 16)
+    t = np.random.default_rng(8).uniform(0, 10, 64).astype(np.float32)
+    g, y = m.grad(b, t)
+    go, yo = M.grad(p, *raw, t)
+    assert np.allclose(y, yo, rtol=2e-5, atol=1e-4)
+    scale = np.abs(go).max()
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    assert np.allclose(g, go, rtol=1e-3, atol=1e-5 * scale)
+    # every block of the parameter vector is exercised
+    for lo, hi in ((M.W_IN, M.B_IN), (M.B_IN, M.LAYER0), (M.LAYER0, M.LAYER0 + M.LAYER),
+                   (M.LAYER0 + 2 * M.LAYER, M.W_OUT), (M.W_OUT, M.NPARAMS)):
+        assert np.abs(go[lo:hi]).max() > 0
+
+
+def test_adam_step_matches_definition(ctx):
+    m = A.Mpnn(ctx, seed=1)
+    p = _params(9)
+    m.set_params(p)
+    b, raw = _batch(10, 32, 10)
+    t = np.random.default_rng(11).uniform(0, 10, 32).astype(np.float32)
+    go, yo = M.grad(p, *raw, t)
+    lr = 1e-3
+    loss = m.train_step(b, t, lr)
+    assert abs(loss - 0.5 * np.mean((yo - t) ** 2)) <= 1e-4 * (1 + loss)
+    p1 = m.params().astype(np.float64)
+    # first Adam step: m_hat = g, v_hat = g^2, so p1 = p - lr * g / (|g| + eps)
+    big = np.abs(go) > 1e-4 * np.abs(go).max()
+    expect = p.astype(np.float64) - lr * go / (np.abs(go) + 1e-8)
+    assert np.allclose(p1[big], expect[big], rtol=1e-5, atol=1e-6)
+    assert np.all(np.abs(p1 - p) <= lr * 1.0001)
+
+
+def test_save_load_roundtrip(ctx):
+    m = A.Mpnn(ctx, seed=3)
+    p = _params(12)
+    m.set_params(p)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "w.bin")
+        m.save(path)
+        m2 = A.Mpnn(ctx, seed=4)
+        m2.load(path)
+        assert np.array_equal(m2.params(), p)
+        b, _ = _batch(13, 20, 8)
+        assert np.array_equal(m.forward(b), m2.forward(b))
+
+
+def _features_restated(g, q, cs, mask):
+    """Node features and adjacency of the sub-query induced by mask, from their definition."""
+    goff, _, glab = g.csr()
+    deg = np.diff(goff.astype(np.int64))
+    off, nbr, lab = q.csr()
+    sizes = cs.sizes().astype(np.float64)
+    ids = [u for u in range(q.n) if (mask >> u) & 1]
+    pos = {u: i for i, u in enumerate(ids)}
+    lg = lambda v: np.log2(1.0 + max(0.0, v))  # noqa: E731
+    e2 = sum(1 for u in ids for w in nbr[off[u]:off[u + 1]] if int(w) in pos)
+    x, adj_off, adj = [], [0], []
+    for u in ids:
+        l = int(lab[u])
+        cnt = int(np.sum(glab == l))
+        avg = float(deg[glab == l].mean()) if cnt else 0.0
+        d = 0
+        fo = 0.0
+        for a in range(off[u], off[u + 1]):
+            w = int(nbr[a])
+            if w not in pos:
+                continue
+            d += 1
+            aw = [b for b in range(off[w], off[w + 1]) if int(nbr[b]) == u][0]
+            fo += lg(cs.arc_weight(aw) / max(1.0, sizes[w]))
+            adj.append(pos[w])
+        same = sum(1 for v in ids if int(lab[v]) == l)
+        x.append([lg(sizes[u]) / 16, lg(cnt) / 16, lg(avg) / 8, d / 8, same / 8, (fo / d / 8) if d else 0.0,
+                  len(ids) / 32, (e2 // 2) / 64])
+        adj_off.append(len(adj))
+    return np.array(x), adj_off, adj
+
+
+def test_features_match_definition(ctx):
+    case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
+    g = T.make_graph(case)
+    r = case["queries"][0]
+    q = g.random_walk_query(max(r["k"], 6), 1.0, 0xFFFFFFFF, r["seed"])
+    cs = ctx.filter(q, g)
+    assert cs is not None
+    off, nbr, _ = q.csr()
+    masks = LR.connected_masks(off, nbr, np.random.default_rng(1), 6)
+    fb = A.mpnn_features(g, cs, masks)
+    assert fb.size == len(masks)
+    for i, mask in enumerate(masks):
+        x, ao, adj = _features_restated(g, q, cs, mask)
+        n0, n1 = int(fb.node_off[i]), int(fb.node_off[i + 1])
+        assert n1 - n0 == bin(mask).count("1")
+        assert np.allclose(fb.x[n0:n1], x, rtol=1e-6, atol=1e-7)
+        e0 = int(fb.adj_off[n0])
+        assert list(fb.adj_off[n0:n1 + 1] - e0) == ao
+        assert list(fb.adj[e0:int(fb.adj_off[n1])] - n0) == adj
+
+
+def test_plain_estimate_matches_independence_model(ctx):
+    g = A.Graph.load(T.yeast_data_path())
+    name = sorted(T.yeast_golden()["queries"])[0]
+    q = A.Graph.load(T.yeast_query_path(name))
+    cs = ctx.filter(q, g)
+    off, nbr, _ = q.csr()
+    s = cs.sizes().astype(np.float64)
+    full = (1 << q.n) - 1
+    est = float(np.prod(s))
+    for u in range(q.n):
+        for a in range(off[u], off[u + 1]):
+            w = int(nbr[a])
+            if u < w:
+                est *= cs.arc_weight(a) / (s[u] * s[w])
+    got = A.plain_estimate(cs, [full])[0]
+    assert np.isclose(got, est, rtol=1e-9)
+    assert np.allclose(A.plain_estimate(cs, [1 << u for u in range(q.n)]), s)
+
+
+def test_learned_plans_keep_reference_counts(ctx):
+    # an untrained network picks arbitrary orders: the counts must still be the reference's
+    m = A.Mpnn(ctx, seed=21)
+    gold = T.yeast_golden()["queries"]
+    g = A.Graph.load(T.yeast_data_path())
+    names = list(gold)
+    qs = [A.Graph.load(T.yeast_query_path(n)) for n in names]
+    counts, st, _ = ctx.batch_count_learned(g, qs, m)
+    assert list(st) == [0] * len(qs)
+    assert [int(c) for c in counts] == [gold[n]["count"] for n in names]
+    plain, _ = ctx.batch_count(g, qs)
+    assert np.array_equal(counts, plain)
+    # single-query learned plan through the same path
+    cs = ctx.filter(qs[0], g)
+    plan = cs.plan_learned(qs[0], g, m)
+    assert ctx.count(plan, cs) == gold[names[0]]["count"]
+    for case in T.synth_golden():
+        gg = T.make_graph(case)
+        recs = [r for r in case["queries"] if r["count"] is not None]
+        qq = [gg.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"]) for r in recs]
+        c2, s2, _ = ctx.batch_count_learned(gg, qq, m)
+        assert [int(c) for c in c2] == [r["count"] for r in recs], case["name"]
+
+
+def test_row_limit_drops_only_the_query_over_it(ctx):
+    case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
+    g = T.make_graph(case)
+    recs = [r for r in case["queries"] if r["count"] is not None]
+    qs = [g.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"]) for r in recs]
+    want = [r["count"] for r in recs]
+    ctx.set_row_limit(64)
+    try:
+        counts, st, _ = ctx.batch_count_learned(g, qs)
+    finally:
+        ctx.set_row_limit(0)
+    assert set(st.tolist()) <= {0, -8}
+    for c, s, w in zip(counts, st, want):
+        assert (int(c) == w) if s == 0 else (c == 0)
+
+
+def test_training_beats_plain_estimator(ctx):
+    g = A.Graph.chung_lu(30_000, 240_000, 2.5, 8, 1.0, 0x5EED01)
+    qs = LR.random_queries(g, 160, 3, 8, 77)
+    split = int(len(qs) * 0.8)
+    rng = np.random.default_rng(5)
+    tr = LR.make_pairs(ctx, g, qs[:split], 12, rng, row_limit=1 << 22)
+    te = LR.make_pairs(ctx, g, qs[split:], 12, rng, row_limit=1 << 22)
+    assert len(tr) > 500 and len(te) > 100
+    m = A.Mpnn(ctx, seed=3)
+    losses = LR.train(m, tr, epochs=40, lr=3e-3, batch=256)
+    assert losses[-1] < 0.5 * losses[0]
+    ev = LR.evaluate(m, te)
+    print(ev)
+    assert ev["learned_median"] < ev["plain_median"]
+    assert ev["learned_p95"] < ev["plain_p95"]
