@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -663,29 +664,51 @@ __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans,
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     uint64_t bytes = 0;
+    // Finished rows add into their query's count.  Each group leader keeps a running sum for the
+    // query it is on and flushes it when the query changes; at the end the wave combines the
+    // leaders' sums, with one atomic when they all belong to the same query (the common case,
+    // since a query's rows are contiguous), so the per-query counters see few atomics.
+    uint32_t cur_q = 0xffffffffu;
+    uint64_t acc = 0;
     for (uint64_t it = it0 + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < it1; it += stride) {
         const uint32_t q = seg_q[seg_of(seg_start, nseg, it)];
         const DPlan *P = plans + q;
         const uint32_t *row = rows + it * d;
+        bool done = true;
+        uint64_t v = 0;
         if (d == P->D) {
-            const uint64_t prod = leaf_row<G>(P, row, ce_off, ce_val, cand, bytes);
-            if (lg == 0) {
-                if (prod) atomicAdd(&qcount[q], (unsigned long long)prod);
-                counts[it] = 0;
-            }
+            v = leaf_row<G>(P, row, ce_off, ce_val, cand, bytes);
         } else if (P->fused && d + 1 == P->D) {
-            const uint64_t part = group_sum<G>(fused_row<G>(P, row, ce_off, ce_val, cand, bytes));
-            if (lg == 0) {
-                if (part) atomicAdd(&qcount[q], (unsigned long long)part);
-                counts[it] = 0;
-            }
+            v = group_sum<G>(fused_row<G>(P, row, ce_off, ce_val, cand, bytes));
         } else {
+            done = false;
             const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
             if (lg == 0) counts[it] = c;
         }
+        if (done && lg == 0) {
+            counts[it] = 0;
+            if (q != cur_q) {
+                if (acc) atomicAdd(&qcount[cur_q], (unsigned long long)acc);
+                cur_q = q;
+                acc = 0;
+            }
+            acc += v;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t has = __ballot(acc != 0);
+    if (has) {
+        const uint32_t q0 = __shfl(cur_q, __builtin_ctzll(has), 64);
+        const bool same = __ballot(acc != 0 && cur_q != q0) == 0;
+        if (same) {
+            const uint64_t tot = group_sum<64>(acc);
+            if (lane == 0) atomicAdd(&qcount[q0], (unsigned long long)tot);
+        } else if (acc) {
+            atomicAdd(&qcount[cur_q], (unsigned long long)acc);
+        }
     }
     bytes = group_sum<64>(bytes);
-    if ((threadIdx.x & 63) == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+    if (lane == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
 }
 
 template <int G>
@@ -723,6 +746,7 @@ __global__ void k_bgather(const uint64_t *__restrict__ offs, const uint64_t *__r
 namespace {
 
 constexpr int kG = 16;                      // lanes per row
+constexpr int kLeafLanes = 16;              // lanes per leaf-depth row in the batch engine
 constexpr uint64_t kLeafChunk = 1ull << 22;  // rows per leaf-count launch
 
 unsigned groups_grid(uint64_t items, int G) {
@@ -1107,7 +1131,9 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         perm[i] = i;
         cls[i] = lane_class(plans_in[i], css_in[i]);
     }
-    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return cls[a] < cls[b]; });
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+        return cls[a] != cls[b] ? cls[a] < cls[b] : plans_in[a]->D < plans_in[b]->D;
+    });
     std::vector<asdm_cs *> css(Q);
     std::vector<asdm_plan *> plans(Q);
     std::vector<uint32_t> qcls(Q);
@@ -1170,6 +1196,22 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         rc = check_hip(ctx, hipGetLastError(), "biota");
     }
     unsigned long long *d_bytes = (unsigned long long *)ctx->d_counters + 1;
+    // Calls launch(class, first row, end row) for each maximal run of queries (in batch order) that
+    // share a class; queries without rows at this depth, or whose rows need no launch (skip), join
+    // the run around them.
+    auto for_runs = [&](auto cls_of, auto skip, auto launch) {
+        uint32_t a = 0;
+        auto idle = [&](uint32_t i) { return seg_start_h[i + 1] == seg_start_h[i] || skip(i); };
+        while (a < Q) {
+            while (a < Q && idle(a)) ++a;
+            if (a >= Q) break;
+            const uint32_t c = cls_of(a);
+            uint32_t b = a + 1;
+            while (b < Q && (idle(b) || cls_of(b) == c)) ++b;
+            launch(c, seg_start_h[a], seg_start_h[b]);
+            a = b;
+        }
+    };
     std::vector<uint64_t> qrows(Q, 0);
     std::vector<char> dropped(Q, 0);
     for (uint32_t d = 1; d <= maxD && n && !rc; ++d) {
@@ -1182,18 +1224,18 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         if (rc) break;
         {
             KTimer kt(ctx, "batch_count");
-            for (uint32_t a = 0; a < Q;) {  // one launch per lane class with rows at this depth
-                uint32_t b = a;
-                while (b < Q && qcls[b] == qcls[a]) ++b;
-                const uint64_t i0 = seg_start_h[a], i1 = seg_start_h[b];
-                if (i1 > i0)
-                    by_class(qcls[a],
+            // one launch per run of queries with the same lane width at this depth: leaf-depth
+            // rows (inclusion-exclusion over leaf lists) use kLeafLanes, the others their class
+            for_runs(
+                [&](uint32_t i) { return hp[i].D == d ? (uint32_t)kLeafLanes : qcls[i]; },
+                [](uint32_t) { return false; },
+                [&](uint32_t c, uint64_t i0, uint64_t i1) {
+                    by_class(c,
                              [&] { launch_bcount<1>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
                              [&] { launch_bcount<4>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
                              [&] { launch_bcount<16>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
                              [&] { launch_bcount<64>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); });
-                a = b;
-            }
+                });
             kt.stop();
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch count"))) break;
@@ -1236,18 +1278,17 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         if ((rc = dev_alloc(ctx, (void **)&next_rows, 4ull * (d + 1) * total))) break;
         {
             KTimer kt(ctx, "batch_fill");
-            for (uint32_t a = 0; a < Q;) {
-                uint32_t b = a;
-                while (b < Q && qcls[b] == qcls[a]) ++b;
-                const uint64_t i0 = seg_start_h[a], i1 = seg_start_h[b];
-                if (i1 > i0)
-                    by_class(qcls[a],
+            // rows of leaf-depth and fused queries have no children: they join any run
+            for_runs(
+                [&](uint32_t i) { return qcls[i]; },
+                [&](uint32_t i) { return hp[i].D == d || (hp[i].fused && d + 1 == hp[i].D); },
+                [&](uint32_t c, uint64_t i0, uint64_t i1) {
+                    by_class(c,
                              [&] { launch_bfill<1>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
                              [&] { launch_bfill<4>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
                              [&] { launch_bfill<16>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
                              [&] { launch_bfill<64>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); });
-                a = b;
-            }
+                });
             kt.stop();
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch fill"))) break;
@@ -1352,21 +1393,31 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
         const uint64_t budget = budget_bytes ? budget_bytes : (16ull << 30);
         std::vector<uint64_t> bc(bcs.size(), 0);
         std::vector<int> bst(bcs.size(), ASDM_OK);
-        rc = batch_engine(ctx, bcs, bplans, budget, bc.data(), qstatus ? bst.data() : nullptr);
-        if (rc == ASDM_E_BUDGET) {  // fall back to one query at a time
-            rc = ASDM_OK;
-            for (size_t k = 0; k < bcs.size() && !rc; ++k) {
-                bst[k] = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
-                if (bst[k] != ASDM_OK) {
-                    bc[k] = 0;
-                    if (!qstatus || !per_query(bst[k])) rc = bst[k];
-                }
-                if (lat_ms) lat_ms[bidx[k]] = now_ms();
+        // A (sub-)batch whose frontier exceeds the budget is split in halves; a single query that
+        // still exceeds it runs on the single-query engine, which bounds memory by chunking.
+        std::function<int(size_t, size_t)> run = [&](size_t lo, size_t hi) -> int {
+            std::vector<asdm_cs *> c(bcs.begin() + lo, bcs.begin() + hi);
+            std::vector<asdm_plan *> p(bplans.begin() + lo, bplans.begin() + hi);
+            int r = batch_engine(ctx, c, p, budget, bc.data() + lo, qstatus ? bst.data() + lo : nullptr);
+            if (r == ASDM_E_BUDGET && hi - lo > 1) {
+                const size_t mid = lo + (hi - lo) / 2;
+                if ((r = run(lo, mid))) return r;
+                return run(mid, hi);
             }
-        } else if (lat_ms) {
-            const float t = now_ms();
-            for (uint32_t k : bidx) lat_ms[k] = t;
-        }
+            if (r == ASDM_E_BUDGET) {
+                bst[lo] = asdm_count(ctx, bplans[lo], bcs[lo], budget_bytes, &bc[lo]);
+                if (lat_ms) lat_ms[bidx[lo]] = now_ms();
+                if (bst[lo] == ASDM_OK) return ASDM_OK;
+                bc[lo] = 0;
+                return qstatus && per_query(bst[lo]) ? ASDM_OK : bst[lo];
+            }
+            if (!r && lat_ms) {
+                const float t = now_ms();
+                for (size_t k = lo; k < hi; ++k) lat_ms[bidx[k]] = t;
+            }
+            return r;
+        };
+        if (!bcs.empty()) rc = run(0, bcs.size());
         for (size_t k = 0; k < bcs.size(); ++k) {
             counts[bidx[k]] = bc[k];
             if (qstatus) qstatus[bidx[k]] = bst[k];

@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+ESTIMATOR = os.path.join(ROOT, "asdmatch_amd", "data", "mpnn_estimator.bin")
 REF_LOCAL = os.path.join(ROOT, "oracle", "_ref", "ref_driver_local")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_traffic.txt")
 
@@ -249,19 +250,43 @@ def bench_batch(args):
         raise SystemExit("batch count mismatch")
     nq = len(qs) * args.steps
     plain_qps, batch_qps = nq / (t1 - t0), nq / (t2 - t1)
+    # batched + learned plans (the trained estimator orders every query's enumeration)
+    learned_qps = None
+    if os.path.exists(ESTIMATOR):
+        m = A.Mpnn(ctx)
+        m.load(ESTIMATOR)
+        for _ in range(args.warmup):
+            c, st, _ = ctx.batch_count_learned(g, qs, m)
+        hip.hipDeviceSynchronize()
+        t3 = time.perf_counter()
+        lats = []
+        for _ in range(args.steps):
+            c, st, lat = ctx.batch_count_learned(g, qs, m)
+            lats.extend(lat.tolist())
+        hip.hipDeviceSynchronize()
+        t4 = time.perf_counter()
+        if [int(x) for x in c] != want or any(st):
+            raise SystemExit("learned-plan batch count mismatch")
+        learned_qps = nq / (t4 - t3)
+    best = learned_qps if learned_qps is not None else batch_qps
+    step_s = nq / best / args.steps
     lats.sort()
     out = {
         "metric": "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts",
-        "value": round(batch_qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round((t2 - t1) / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "value": round(best, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
         "config": {"workload": "config 4: R-MAT scale 20, edge factor 10, 16 uniform labels; admitted random-walk "
-                               "queries, one asdm_batch_count call per step",
+                               "queries, one asdm_batch_count_learned call per step (learned plans when the "
+                               "trained estimator is present, else plain plans)",
+                   "learned_plans": learned_qps is not None,
                    "queries_per_step": len(qs), "query_sizes": sorted({q.n for q in qs}),
                    "admission_row_limit": W.load_manifest()["c4"]["row_limit"]},
         "plain_gpu_queries_per_s": round(plain_qps, 3),
-        "batched_vs_plain": round(batch_qps / plain_qps, 3),
-        "embeddings_per_s": round(sum(want) * args.steps / (t2 - t1), 1),
+        "batched_plain_plan_queries_per_s": round(batch_qps, 3),
+        "batched_learned_plan_queries_per_s": None if learned_qps is None else round(learned_qps, 3),
+        "batched_vs_plain": round(best / plain_qps, 3),
+        "embeddings_per_s": round(sum(want) / step_s, 1),
         "latency_ms_p50": round(lats[len(lats) // 2], 3), "latency_ms_p99": round(lats[int(len(lats) * 0.99)], 3),
     }
     print(json.dumps(out), flush=True)
