@@ -22,6 +22,7 @@
 #include <functional>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "engine.h"
@@ -564,6 +565,21 @@ __device__ inline uint64_t group_count_lane(const DPlan *__restrict__ P, uint32_
     return tot;
 }
 
+// One lane: completions of a single-leaf group of a fusable plan (its leaf has one list, or two
+// forming a triangle, and at most kFusedSame same-label matched vertices).
+constexpr uint32_t kFusedSame = 8;
+
+__device__ inline uint64_t single_leaf_lane(const DPlan *__restrict__ P, uint32_t g, const RowView &R,
+                                            const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                            const uint32_t *__restrict__ cand) {
+    const uint32_t e = P->gleaf[g][0];
+    const uint32_t ns = P->ns[e];
+    uint32_t y[kFusedSame];
+#pragma unroll
+    for (uint32_t s = 0; s < kFusedSame; ++s) y[s] = s < ns ? matched_id(P, cand, R, P->sdep[e][s]) : 0xffffffffu;
+    return leaf_set_count_lane(P, g, 1u, R, ce_off, ce_val, cand, y, ns);
+}
+
 // This lane's share of a row of width D-1: the sum over its depth-(D-1) candidates x (lanes of
 // the group split the row's candidate list) of the leaf product of (row, x).  Not reduced.
 template <int G>
@@ -599,7 +615,7 @@ __device__ inline uint64_t fused_row(const DPlan *__restrict__ P, const uint32_t
         if (!ok) continue;
         const RowView R{row, d, x};
         uint64_t prod = 1;
-        for (uint32_t g = 0; g < ng && prod; ++g) prod *= group_count_lane(P, g, R, ce_off, ce_val, cand);
+        for (uint32_t g = 0; g < ng && prod; ++g) prod *= single_leaf_lane(P, g, R, ce_off, ce_val, cand);
         acc += prod;
         bytes += 4ull + 16ull * P->nadj_last;
     }
@@ -651,83 +667,109 @@ __device__ inline uint32_t seg_of(const uint64_t *__restrict__ seg_start, uint32
     return lo;
 }
 
+// Row kinds of a batch level; one kernel instantiation per kind keeps each kernel's registers to
+// what its own row function needs (a kernel holding all three paths needs twice the registers of
+// the largest and runs at half the occupancy).
+constexpr int kExpandRows = 0, kFusedRows = 1, kLeafRows = 2;
+
+// Wave tiles.  A launch covers the rows of queries [qa, qb); each query's rows are cut into tiles
+// of tile_rows consecutive rows, and tile_pref[i] is the number of tiles before query i.  A wave
+// takes whole tiles, so the query -- and the plan every row function reads -- is uniform across
+// the wave and its fields are loaded once into scalar registers instead of once per lane.
+__device__ inline uint32_t tile_query(const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb, uint64_t t) {
+    uint32_t lo = qa, hi = qb;  // last i in [qa, qb) with tile_pref[i] <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tile_pref[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+}
+
 // Per row: finish it (leaf depth, or fused last level) into qcount[query] and count 0 children,
-// or count its children.
-template <int G>
+// or count its children.  Every row of a launch is of the given kind.
+template <int G, int KIND>
 __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
-                                                const uint32_t *__restrict__ seg_q, uint32_t nseg, uint32_t d,
-                                                const uint32_t *__restrict__ rows, uint64_t it0, uint64_t it1,
+                                                const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
+                                                uint32_t tile_rows, uint32_t d, const uint32_t *__restrict__ rows,
                                                 const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                                                 const uint32_t *__restrict__ cand, uint32_t *__restrict__ counts,
                                                 unsigned long long *__restrict__ qcount,
                                                 unsigned long long *__restrict__ bytes_ctr) {
-    const int lg = threadIdx.x & (G - 1);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const int lane = threadIdx.x & 63;
+    const int lg = lane & (G - 1);
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t t0 = tile_pref[qa], t1 = tile_pref[qb];
     uint64_t bytes = 0;
-    // Finished rows add into their query's count.  Each group leader keeps a running sum for the
-    // query it is on and flushes it when the query changes; at the end the wave combines the
-    // leaders' sums, with one atomic when they all belong to the same query (the common case,
-    // since a query's rows are contiguous), so the per-query counters see few atomics.
+    // Finished rows add into a running per-lane sum for the wave's current query, which the wave
+    // reduces and adds with one atomic when its query changes and at the end.
     uint32_t cur_q = 0xffffffffu;
     uint64_t acc = 0;
-    for (uint64_t it = it0 + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < it1; it += stride) {
-        const uint32_t q = seg_q[seg_of(seg_start, nseg, it)];
-        const DPlan *P = plans + q;
-        const uint32_t *row = rows + it * d;
-        bool done = true;
-        uint64_t v = 0;
-        if (d == P->D) {
-            v = leaf_row<G>(P, row, ce_off, ce_val, cand, bytes);
-        } else if (P->fused && d + 1 == P->D) {
-            v = group_sum<G>(fused_row<G>(P, row, ce_off, ce_val, cand, bytes));
-        } else {
-            done = false;
-            const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
-            if (lg == 0) counts[it] = c;
-        }
-        if (done && lg == 0) {
-            counts[it] = 0;
-            if (q != cur_q) {
-                if (acc) atomicAdd(&qcount[cur_q], (unsigned long long)acc);
-                cur_q = q;
-                acc = 0;
-            }
-            acc += v;
-        }
-    }
-    const int lane = threadIdx.x & 63;
-    const uint64_t has = __ballot(acc != 0);
-    if (has) {
-        const uint32_t q0 = __shfl(cur_q, __builtin_ctzll(has), 64);
-        const bool same = __ballot(acc != 0 && cur_q != q0) == 0;
-        if (same) {
+    auto flush = [&]() {
+        if (KIND != kExpandRows && cur_q != 0xffffffffu) {
             const uint64_t tot = group_sum<64>(acc);
-            if (lane == 0) atomicAdd(&qcount[q0], (unsigned long long)tot);
-        } else if (acc) {
-            atomicAdd(&qcount[cur_q], (unsigned long long)acc);
+            if (lane == 0 && tot) atomicAdd(&qcount[cur_q], (unsigned long long)tot);
+        }
+        acc = 0;
+    };
+    for (uint64_t t = t0 + wave; t < t1; t += nwaves) {
+        const uint32_t q = tile_query(tile_pref, qa, qb, t);
+        if (q != cur_q) {
+            flush();
+            cur_q = q;
+        }
+        const DPlan *P = plans + q;
+        const uint64_t r0 = seg_start[q] + (t - tile_pref[q]) * tile_rows;
+        const uint64_t r1 = min(r0 + tile_rows, seg_start[q + 1]);
+        for (uint64_t it = r0 + (uint64_t)(lane / G); it < r1; it += 64 / G) {
+            const uint32_t *row = rows + it * d;
+            if constexpr (KIND == kLeafRows) {
+                const uint64_t v = leaf_row<G>(P, row, ce_off, ce_val, cand, bytes);
+                if (lg == 0) {
+                    acc += v;
+                    counts[it] = 0;
+                }
+            } else if constexpr (KIND == kFusedRows) {
+                const uint64_t v = fused_row<G>(P, row, ce_off, ce_val, cand, bytes);  // lane share
+                acc += v;
+                if (lg == 0) counts[it] = 0;
+            } else {
+                const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
+                if (lg == 0) counts[it] = c;
+            }
         }
     }
+    flush();
     bytes = group_sum<64>(bytes);
     if (lane == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
 }
 
 template <int G>
 __global__ void __launch_bounds__(256) k_bfill(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
-                                               const uint32_t *__restrict__ seg_q, uint32_t nseg, uint32_t d,
-                                               const uint32_t *__restrict__ rows, uint64_t it0, uint64_t it1,
+                                               const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
+                                               uint32_t tile_rows, uint32_t d, const uint32_t *__restrict__ rows,
                                                const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                                                const uint32_t *__restrict__ cand, const uint64_t *__restrict__ offs,
                                                uint32_t *__restrict__ out, unsigned long long *__restrict__ bytes_ctr) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t t0 = tile_pref[qa], t1 = tile_pref[qb];
     uint64_t bytes = 0;
-    for (uint64_t it = it0 + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < it1; it += stride) {
-        const uint64_t nout = offs[it + 1] - offs[it];
-        if (nout == 0) continue;  // finished rows and rows without children (group-uniform)
-        const DPlan *P = plans + seg_q[seg_of(seg_start, nseg, it)];
-        expand_fill_row<G>(P, d, rows + it * d, ce_off, ce_val, cand, offs[it], nout, out, bytes);
+    for (uint64_t t = t0 + wave; t < t1; t += nwaves) {
+        const uint32_t q = tile_query(tile_pref, qa, qb, t);
+        const DPlan *P = plans + q;
+        const uint64_t r0 = seg_start[q] + (t - tile_pref[q]) * tile_rows;
+        const uint64_t r1 = min(r0 + tile_rows, seg_start[q + 1]);
+        for (uint64_t it = r0 + (uint64_t)(lane / G); it < r1; it += 64 / G) {
+            const uint64_t nout = offs[it + 1] - offs[it];
+            if (nout == 0) continue;  // finished rows and rows without children (group-uniform)
+            expand_fill_row<G>(P, d, rows + it * d, ce_off, ce_val, cand, offs[it], nout, out, bytes);
+        }
     }
     bytes = group_sum<64>(bytes);
-    if ((threadIdx.x & 63) == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+    if (lane == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
 }
 
 // Depth-1 rows: position it - seg_start[s] in C(u_0) of the segment's query.
@@ -787,7 +829,8 @@ bool plan_fusable(const asdm_plan *plan) {
     if (plan->D < 2 || plan->local) return false;
     for (uint32_t g = 0; g < plan->h.ngroups; ++g) {
         const uint32_t e = plan->h.gleaf[g][0];
-        if (plan->h.gsize[g] != 1 || !(plan->h.nb[e] == 1 || plan->h.tri[e])) return false;
+        if (plan->h.gsize[g] != 1 || !(plan->h.nb[e] == 1 || plan->h.tri[e]) || plan->h.ns[e] > kFusedSame)
+            return false;
     }
     return true;
 }
@@ -1091,20 +1134,29 @@ uint32_t lane_class(const asdm_plan *plan, const asdm_cs *cs) {
     return best < 3 ? 1 : best < 12 ? 4 : best < 48 ? 16 : 64;
 }
 
-template <int G>
-void launch_bcount(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, const uint32_t *seg_q, uint32_t Q,
-                   uint32_t d, const uint32_t *rows, uint64_t i0, uint64_t i1, const asdm_cs::Store *st,
+// Rows per wave tile for lane width G: four rows per lane group.
+constexpr uint32_t kTileIters = 4;
+inline uint32_t tile_rows_of(uint32_t G) { return 64u / G * kTileIters; }
+
+inline unsigned tiles_grid(uint64_t tiles) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((tiles + 3) / 4, 1u << 16));
+}
+
+template <int G, int KIND>
+void launch_bcount(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, const uint64_t *tile_pref, uint32_t qa,
+                   uint32_t qb, uint64_t tiles, uint32_t d, const uint32_t *rows, const asdm_cs::Store *st,
                    uint32_t *counts, unsigned long long *qcount, unsigned long long *bytes) {
-    hipLaunchKernelGGL((k_bcount<G>), dim3(groups_grid(i1 - i0, G)), dim3(256), 0, ctx->stream, plans, seg_start, seg_q, Q,
-                       d, rows, i0, i1, st->d_ce_off, st->d_ce_val, st->d_cand, counts, qcount, bytes);
+    hipLaunchKernelGGL((k_bcount<G, KIND>), dim3(tiles_grid(tiles)), dim3(256), 0, ctx->stream, plans, seg_start,
+                       tile_pref, qa, qb, tile_rows_of(G), d, rows, st->d_ce_off, st->d_ce_val, st->d_cand, counts,
+                       qcount, bytes);
 }
 
 template <int G>
-void launch_bfill(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, const uint32_t *seg_q, uint32_t Q,
-                  uint32_t d, const uint32_t *rows, uint64_t i0, uint64_t i1, const asdm_cs::Store *st,
+void launch_bfill(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, const uint64_t *tile_pref, uint32_t qa,
+                  uint32_t qb, uint64_t tiles, uint32_t d, const uint32_t *rows, const asdm_cs::Store *st,
                   const uint64_t *offs, uint32_t *out, unsigned long long *bytes) {
-    hipLaunchKernelGGL((k_bfill<G>), dim3(groups_grid(i1 - i0, G)), dim3(256), 0, ctx->stream, plans, seg_start, seg_q, Q,
-                       d, rows, i0, i1, st->d_ce_off, st->d_ce_val, st->d_cand, offs, out, bytes);
+    hipLaunchKernelGGL((k_bfill<G>), dim3(tiles_grid(tiles)), dim3(256), 0, ctx->stream, plans, seg_start, tile_pref,
+                       qa, qb, tile_rows_of(G), d, rows, st->d_ce_off, st->d_ce_val, st->d_cand, offs, out, bytes);
 }
 
 template <class F1, class F4, class F16, class F64>
@@ -1196,9 +1248,9 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         rc = check_hip(ctx, hipGetLastError(), "biota");
     }
     unsigned long long *d_bytes = (unsigned long long *)ctx->d_counters + 1;
-    // Calls launch(class, first row, end row) for each maximal run of queries (in batch order) that
-    // share a class; queries without rows at this depth, or whose rows need no launch (skip), join
-    // the run around them.
+    // Calls launch(class, first query, end query) for each maximal run of queries (in batch order)
+    // that share a class; queries without rows at this depth, or whose rows need no launch (skip),
+    // join the run around them.
     auto for_runs = [&](auto cls_of, auto skip, auto launch) {
         uint32_t a = 0;
         auto idle = [&](uint32_t i) { return seg_start_h[i + 1] == seg_start_h[i] || skip(i); };
@@ -1208,10 +1260,25 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
             const uint32_t c = cls_of(a);
             uint32_t b = a + 1;
             while (b < Q && (idle(b) || cls_of(b) == c)) ++b;
-            launch(c, seg_start_h[a], seg_start_h[b]);
+            launch(c, a, b);
             a = b;
         }
     };
+    // Tile prefix of one phase: query i's rows in tiles of tile_rows_of(lanes(i)); skipped queries
+    // get no tiles.  Uploaded into buf from host (kept alive until the next stream sync).
+    auto tile_prefix = [&](auto lanes, auto skip, std::vector<uint64_t> &h, uint64_t *buf) {
+        h.assign(Q + 1, 0);
+        for (uint32_t i = 0; i < Q; ++i) {
+            const uint64_t r = seg_start_h[i + 1] - seg_start_h[i];
+            const uint64_t tr = tile_rows_of(lanes(i));
+            h[i + 1] = h[i] + (skip(i) ? 0 : (r + tr - 1) / tr);
+        }
+        return check_hip(ctx, hipMemcpyAsync(buf, h.data(), 8 * (Q + 1), hipMemcpyHostToDevice, ctx->stream), "H2D");
+    };
+    std::vector<uint64_t> tp_count_h, tp_fill_h;
+    uint64_t *tp_count = nullptr, *tp_fill = nullptr;
+    if (!rc) rc = dev_alloc(ctx, (void **)&tp_count, 8 * (Q + 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&tp_fill, 8 * (Q + 1));
     std::vector<uint64_t> qrows(Q, 0);
     std::vector<char> dropped(Q, 0);
     for (uint32_t d = 1; d <= maxD && n && !rc; ++d) {
@@ -1226,15 +1293,31 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
             KTimer kt(ctx, "batch_count");
             // one launch per run of queries with the same lane width at this depth: leaf-depth
             // rows (inclusion-exclusion over leaf lists) use kLeafLanes, the others their class
+            // one launch per run of queries with the same row kind and lane width at this depth:
+            // leaf-depth rows (inclusion-exclusion over leaf lists) use kLeafLanes, the others
+            // their class
+            auto kind_of = [&](uint32_t i) {
+                return hp[i].D == d ? kLeafRows : (hp[i].fused && d + 1 == hp[i].D) ? kFusedRows : kExpandRows;
+            };
+            auto lanes = [&](uint32_t i) { return kind_of(i) == kLeafRows ? (uint32_t)kLeafLanes : qcls[i]; };
+            if ((rc = tile_prefix(lanes, [](uint32_t) { return false; }, tp_count_h, tp_count))) break;
             for_runs(
-                [&](uint32_t i) { return hp[i].D == d ? (uint32_t)kLeafLanes : qcls[i]; },
+                [&](uint32_t i) { return 256u * kind_of(i) + lanes(i); },
                 [](uint32_t) { return false; },
-                [&](uint32_t c, uint64_t i0, uint64_t i1) {
-                    by_class(c,
-                             [&] { launch_bcount<1>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
-                             [&] { launch_bcount<4>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
-                             [&] { launch_bcount<16>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); },
-                             [&] { launch_bcount<64>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, counts_d, qcount, d_bytes); });
+                [&](uint32_t key, uint32_t qa, uint32_t qb) {
+                    const uint32_t c = key % 256;
+                    const uint64_t tiles = tp_count_h[qb] - tp_count_h[qa];
+                    auto go = [&](auto kind) {
+                        constexpr int K = decltype(kind)::value;
+                        by_class(c,
+                                 [&] { launch_bcount<1, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); },
+                                 [&] { launch_bcount<4, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); },
+                                 [&] { launch_bcount<16, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); },
+                                 [&] { launch_bcount<64, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); });
+                    };
+                    if (key / 256 == kLeafRows) go(std::integral_constant<int, kLeafRows>{});
+                    else if (key / 256 == kFusedRows) go(std::integral_constant<int, kFusedRows>{});
+                    else go(std::integral_constant<int, kExpandRows>{});
                 });
             kt.stop();
         }
@@ -1278,16 +1361,19 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         if ((rc = dev_alloc(ctx, (void **)&next_rows, 4ull * (d + 1) * total))) break;
         {
             KTimer kt(ctx, "batch_fill");
-            // rows of leaf-depth and fused queries have no children: they join any run
+            // rows of leaf-depth and fused queries have no children: they get no tiles and join
+            // any run
+            auto finished = [&](uint32_t i) { return hp[i].D == d || (hp[i].fused && d + 1 == hp[i].D); };
+            if ((rc = tile_prefix([&](uint32_t i) { return qcls[i]; }, finished, tp_fill_h, tp_fill))) break;
             for_runs(
-                [&](uint32_t i) { return qcls[i]; },
-                [&](uint32_t i) { return hp[i].D == d || (hp[i].fused && d + 1 == hp[i].D); },
-                [&](uint32_t c, uint64_t i0, uint64_t i1) {
+                [&](uint32_t i) { return qcls[i]; }, finished,
+                [&](uint32_t c, uint32_t qa, uint32_t qb) {
+                    const uint64_t tiles = tp_fill_h[qb] - tp_fill_h[qa];
                     by_class(c,
-                             [&] { launch_bfill<1>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
-                             [&] { launch_bfill<4>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
-                             [&] { launch_bfill<16>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); },
-                             [&] { launch_bfill<64>(ctx, dplans, seg_start, seg_q, Q, d, rows, i0, i1, st, offs, next_rows, d_bytes + 1); });
+                             [&] { launch_bfill<1>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); },
+                             [&] { launch_bfill<4>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); },
+                             [&] { launch_bfill<16>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); },
+                             [&] { launch_bfill<64>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); });
                 });
             kt.stop();
         }
@@ -1309,6 +1395,8 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     resolve_timers(ctx);
     cleanup(dplans, seg_start, seg_next, seg_q, counts_d, offs, rows);
     dev_free(ctx, qcount);
+    dev_free(ctx, tp_count);
+    dev_free(ctx, tp_fill);
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
 }
