@@ -222,28 +222,49 @@ __global__ void __launch_bounds__(256) k_expand(const DPlan *__restrict__ P, uin
 // (f(p0), f(p1)).  It is computed once per candidate edge of arc (p0 -> p1) -- the
 // materialization of the bag {p0, p1, e} -- instead of once per partial embedding.
 // One lane per candidate edge k (global index base.. of the arc's lists).
+// The two lists of candidate edge ek of the arc, shorter one first.
+struct TriLists {
+    const uint32_t *A, *B;
+    uint32_t na, nb;
+};
+
+__device__ inline TriLists tri_lists(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                     uint64_t arc_item_base, uint32_t nitems, uint64_t ek, uint64_t l0_base,
+                                     uint64_t l1_base) {
+    // i0 = position of f(p0): last item whose list starts at or before ek
+    uint32_t lo = 0, hi = nitems;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (ce_off[arc_item_base + mid] <= ek) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t i0 = lo, i1 = ce_val[ek];
+    const uint64_t a0 = ce_off[l0_base + i0], a1 = ce_off[l0_base + i0 + 1];
+    const uint64_t b0 = ce_off[l1_base + i1], b1 = ce_off[l1_base + i1 + 1];
+    TriLists L{ce_val + a0, ce_val + b0, (uint32_t)(a1 - a0), (uint32_t)(b1 - b0)};
+    if (L.na > L.nb) {
+        const uint32_t *t = L.A;
+        L.A = L.B;
+        L.B = t;
+        const uint32_t tn = L.na;
+        L.na = L.nb;
+        L.nb = tn;
+    }
+    return L;
+}
+
+// Edges whose shorter list holds more than kTriWide entries are left to k_tri_table_wide.
+constexpr uint32_t kTriWide = 64;
+
 __global__ void k_tri_table(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                             uint64_t arc_item_base, uint32_t nitems /* |C(p0)| */, uint64_t nedges, uint64_t l0_base,
                             uint64_t l1_base, uint32_t *__restrict__ tab) {
     const uint64_t e_first = ce_off[arc_item_base];  // first candidate edge of the arc
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nedges; k += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t ek = e_first + k;
-        // i = position of f(p0): last item whose list starts at or before ek
-        uint32_t lo = 0, hi = nitems;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (ce_off[arc_item_base + mid] <= ek) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t i0 = lo, i1 = ce_val[ek];
-        const uint64_t a0 = ce_off[l0_base + i0], a1 = ce_off[l0_base + i0 + 1];
-        const uint64_t b0 = ce_off[l1_base + i1], b1 = ce_off[l1_base + i1 + 1];
-        const uint32_t *A = ce_val + a0, *B = ce_val + b0;
-        uint32_t na = (uint32_t)(a1 - a0), nb = (uint32_t)(b1 - b0);
-        if (na > nb) {
-            const uint32_t *t = A; A = B; B = t;
-            uint32_t tn = na; na = nb; nb = tn;
-        }
+        const TriLists L = tri_lists(ce_off, ce_val, arc_item_base, nitems, e_first + k, l0_base, l1_base);
+        if (L.na > kTriWide) continue;
+        const uint32_t *A = L.A, *B = L.B;
+        const uint32_t na = L.na, nb = L.nb;
         uint32_t c = 0, p = 0;
         for (uint32_t t = 0; t < na; ++t) {  // probe the shorter list, galloping in the longer one
             const uint32_t x = A[t];
@@ -254,6 +275,25 @@ __global__ void k_tri_table(const uint64_t *__restrict__ ce_off, const uint32_t 
             if (p < nb && B[p] == x) ++c;
         }
         tab[k] = c;
+    }
+}
+
+// The edges k_tri_table leaves: one wave per edge, lanes split the shorter list and binary-search
+// the longer one.
+__global__ void __launch_bounds__(256) k_tri_table_wide(const uint64_t *__restrict__ ce_off,
+                                                        const uint32_t *__restrict__ ce_val, uint64_t arc_item_base,
+                                                        uint32_t nitems, uint64_t nedges, uint64_t l0_base,
+                                                        uint64_t l1_base, uint32_t *__restrict__ tab) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t e_first = ce_off[arc_item_base];
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nedges; k += nwaves) {
+        const TriLists L = tri_lists(ce_off, ce_val, arc_item_base, nitems, e_first + k, l0_base, l1_base);
+        if (L.na <= kTriWide) continue;  // wave-uniform
+        uint64_t c = 0;
+        for (uint32_t t = lane; t < L.na; t += 64) c += contains_u32(L.B, L.nb, L.A[t]);
+        c = group_sum<64>(c);
+        if (lane == 0) tab[k] = (uint32_t)c;
     }
 }
 
@@ -815,6 +855,9 @@ int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
             KTimer kt(ctx, "tri_table");
             const unsigned grid = (unsigned)std::min<uint64_t>((nedges + 255) / 256, 1u << 16);
             hipLaunchKernelGGL(k_tri_table, dim3(grid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
+                               cs->ce_base[a], cs->csize[p0], nedges, P.bbase[e][0], P.bbase[e][1], tab);
+            const unsigned wgrid = (unsigned)std::min<uint64_t>((nedges + 3) / 4, 1u << 14);
+            hipLaunchKernelGGL(k_tri_table_wide, dim3(wgrid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
                                cs->ce_base[a], cs->csize[p0], nedges, P.bbase[e][0], P.bbase[e][1], tab);
             kt.stop(8ull * nedges);
             if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;

@@ -91,10 +91,13 @@ class PairSet:
 
 
 def make_pairs(ctx: Context, g: Graph, queries: List[Graph], per_query: int, rng: np.random.Generator,
-               row_limit: int = 1 << 24, chunk: int = 512, log: Optional[Callable] = None) -> PairSet:
+               row_limit: int = 1 << 24, chunk: int = 512, log: Optional[Callable] = None,
+               seconds: Optional[float] = None) -> PairSet:
     """(sub-query, exact count) pairs: per_query connected induced sub-queries of each query (and
     the query itself), counted by the batched engine; sub-queries over row_limit are dropped, as
-    are those with an empty candidate set (the planner never sees them)."""
+    are those with an empty candidate set (the planner never sees them).  With `seconds`, no new
+    chunk starts after that much time."""
+    t_start = time.perf_counter()
     subs = []
     for q in queries:
         off, nbr, _ = q.csr()
@@ -105,6 +108,10 @@ def make_pairs(ctx: Context, g: Graph, queries: List[Graph], per_query: int, rng
     ctx.set_row_limit(row_limit)
     try:
         for c0 in range(0, len(subs), chunk):
+            if seconds is not None and time.perf_counter() - t_start > seconds:
+                if log:
+                    log(f"  time budget reached after {c0} of {len(subs)} sub-queries")
+                break
             sq = subs[c0:c0 + chunk]
             t0 = time.perf_counter()
             css = ctx.filter_many(sq, g)

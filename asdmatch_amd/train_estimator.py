@@ -45,23 +45,27 @@ def c2_queries(g, per_slot: int):
     return out
 
 
-def collect(ctx, target_pairs: int, seed: int):
+def collect(ctx, target_pairs: int, seed: int, seconds: float):
     rng = np.random.default_rng(seed)
-    sources = []  # (name, graph factory, query factory, share of the pairs)
-    sources.append(("c2", lambda: W.graph("c2"), lambda g: c2_queries(g, 12), 0.4))
+    # (name, graph factory, query factory, share of the pairs, row limit, chunk); the config-2
+    # graph's sub-queries are the costliest to count exactly, so they get a lower row limit
+    sources = [("c2", lambda: W.graph("c2"), lambda g: c2_queries(g, 12), 0.1, 1 << 21, 256)]
     for k, (name, mk) in enumerate(LR.TRAIN_GRAPHS.items()):
-        sources.append((name, mk, lambda g, k=k: LR.random_queries(g, 1200, 3, 12, 9100 + k), 0.2))
+        sources.append((name, mk, lambda g, k=k: LR.random_queries(g, 4000, 3, 12, 9100 + k), 0.3, 1 << 22, 2048))
     train, held = [], []
-    for name, mk, mkq, share in sources:
+    for name, mk, mkq, share, limit, chunk in sources:
         t0 = time.perf_counter()
         g = mk()
         qs = mkq(g)
         cut = int(len(qs) * 0.9)
         want = int(target_pairs * share)
         per_q = max(2, int(np.ceil(want / max(1, len(qs)))))
+        budget = seconds * share
         log(f"{name}: {len(qs)} queries, {per_q} sub-queries each (graph {time.perf_counter() - t0:.1f} s)")
-        train.append(LR.make_pairs(ctx, g, qs[:cut], per_q, rng, row_limit=1 << 24, chunk=1024, log=log))
-        held.append(LR.make_pairs(ctx, g, qs[cut:], per_q, rng, row_limit=1 << 24, chunk=1024, log=log))
+        train.append(LR.make_pairs(ctx, g, qs[:cut], per_q, rng, row_limit=limit, chunk=chunk, log=log,
+                                   seconds=0.9 * budget))
+        held.append(LR.make_pairs(ctx, g, qs[cut:], per_q, rng, row_limit=limit, chunk=chunk, log=log,
+                                  seconds=0.1 * budget))
         log(f"{name}: {len(train[-1])} train / {len(held[-1])} held-out pairs in {time.perf_counter() - t0:.1f} s")
         del g
     return LR.PairSet.concat(train), LR.PairSet.concat(held)
@@ -98,10 +102,12 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--data-seconds", type=float, default=600.0,
+                    help="time budget of pair generation, split across the sources by their share")
     a = ap.parse_args(argv)
     ctx = Context(0)
     t0 = time.perf_counter()
-    tr, te = collect(ctx, a.pairs, a.seed)
+    tr, te = collect(ctx, a.pairs, a.seed, a.data_seconds)
     t_data = time.perf_counter() - t0
     log(f"pairs: {len(tr)} train, {len(te)} held-out ({t_data:.1f} s)")
     m = Mpnn(ctx, seed=a.seed)
