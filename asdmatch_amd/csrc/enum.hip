@@ -711,6 +711,10 @@ __device__ inline uint32_t seg_of(const uint64_t *__restrict__ seg_start, uint32
 // what its own row function needs (a kernel holding all three paths needs twice the registers of
 // the largest and runs at half the occupancy).
 constexpr int kExpandRows = 0, kFusedRows = 1, kLeafRows = 2;
+// Kernel-table names and algorithmic-byte counter slots (offsets from d_counters + 1) per kind.
+constexpr const char *kKindTimer[3] = {"batch_count_expand", "batch_count_fused", "batch_count_leaf"};
+constexpr int kKindBytesSlot[3] = {0, 2, 3};
+constexpr int kFillBytesSlot = 1;
 
 // Wave tiles.  A launch covers the rows of queries [qa, qb); each query's rows are cut into tiles
 // of tile_rows consecutive rows, and tile_pref[i] is the number of tiles before query i.  A wave
@@ -1333,9 +1337,6 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         if (!rc) rc = dev_alloc(ctx, (void **)&offs, 8 * (n + 1));
         if (rc) break;
         {
-            KTimer kt(ctx, "batch_count");
-            // one launch per run of queries with the same lane width at this depth: leaf-depth
-            // rows (inclusion-exclusion over leaf lists) use kLeafLanes, the others their class
             // one launch per run of queries with the same row kind and lane width at this depth:
             // leaf-depth rows (inclusion-exclusion over leaf lists) use kLeafLanes, the others
             // their class
@@ -1352,17 +1353,19 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                     const uint64_t tiles = tp_count_h[qb] - tp_count_h[qa];
                     auto go = [&](auto kind) {
                         constexpr int K = decltype(kind)::value;
+                        unsigned long long *kb = d_bytes + kKindBytesSlot[K];
+                        KTimer kt(ctx, kKindTimer[K]);
                         by_class(c,
-                                 [&] { launch_bcount<1, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); },
-                                 [&] { launch_bcount<4, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); },
-                                 [&] { launch_bcount<16, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); },
-                                 [&] { launch_bcount<64, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, d_bytes); });
+                                 [&] { launch_bcount<1, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, kb); },
+                                 [&] { launch_bcount<4, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, kb); },
+                                 [&] { launch_bcount<16, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, kb); },
+                                 [&] { launch_bcount<64, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, kb); });
+                        kt.stop();
                     };
                     if (key / 256 == kLeafRows) go(std::integral_constant<int, kLeafRows>{});
                     else if (key / 256 == kFusedRows) go(std::integral_constant<int, kFusedRows>{});
                     else go(std::integral_constant<int, kExpandRows>{});
                 });
-            kt.stop();
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch count"))) break;
         if ((rc = scan_u32_to_u64(ctx, counts_d, offs, n))) break;
@@ -1403,7 +1406,6 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         uint32_t *next_rows = nullptr;
         if ((rc = dev_alloc(ctx, (void **)&next_rows, 4ull * (d + 1) * total))) break;
         {
-            KTimer kt(ctx, "batch_fill");
             // rows of leaf-depth and fused queries have no children: they get no tiles and join
             // any run
             auto finished = [&](uint32_t i) { return hp[i].D == d || (hp[i].fused && d + 1 == hp[i].D); };
@@ -1412,13 +1414,15 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                 [&](uint32_t i) { return qcls[i]; }, finished,
                 [&](uint32_t c, uint32_t qa, uint32_t qb) {
                     const uint64_t tiles = tp_fill_h[qb] - tp_fill_h[qa];
+                    unsigned long long *fb = d_bytes + kFillBytesSlot;
+                    KTimer kt(ctx, "batch_fill");
                     by_class(c,
-                             [&] { launch_bfill<1>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); },
-                             [&] { launch_bfill<4>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); },
-                             [&] { launch_bfill<16>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); },
-                             [&] { launch_bfill<64>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, d_bytes + 1); });
+                             [&] { launch_bfill<1>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, fb); },
+                             [&] { launch_bfill<4>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, fb); },
+                             [&] { launch_bfill<16>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, fb); },
+                             [&] { launch_bfill<64>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, fb); });
+                    kt.stop();
                 });
-            kt.stop();
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch fill"))) break;
         dev_free(ctx, rows);
@@ -1428,8 +1432,14 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         n = total;
     }
     std::vector<uint64_t> qc(Q);
+    uint64_t kbytes[8] = {0};
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(kbytes, ctx->d_counters, 8 * 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!rc) {
+        for (int k = 0; k < 3; ++k) add_kernel_bytes(ctx, kKindTimer[k], kbytes[1 + kKindBytesSlot[k]]);
+        add_kernel_bytes(ctx, "batch_fill", kbytes[1 + kFillBytesSlot]);
+    }
     if (!rc)
         for (uint32_t i = 0; i < Q; ++i) {
             counts_out[perm[i]] = dropped[i] ? 0 : qc[i];

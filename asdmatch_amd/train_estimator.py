@@ -49,9 +49,9 @@ def collect(ctx, target_pairs: int, seed: int, seconds: float):
     rng = np.random.default_rng(seed)
     # (name, graph factory, query factory, share of the pairs, row limit, chunk); the config-2
     # graph's sub-queries are the costliest to count exactly, so they get a lower row limit
-    sources = [("c2", lambda: W.graph("c2"), lambda g: c2_queries(g, 12), 0.1, 1 << 21, 256)]
+    sources = [("c2", lambda: W.graph("c2"), lambda g: c2_queries(g, 12), 0.1, 1 << 22, 256)]
     for k, (name, mk) in enumerate(LR.TRAIN_GRAPHS.items()):
-        sources.append((name, mk, lambda g, k=k: LR.random_queries(g, 4000, 3, 12, 9100 + k), 0.3, 1 << 22, 2048))
+        sources.append((name, mk, lambda g, k=k: LR.random_queries(g, 8000, 3, 12, 9100 + k), 0.3, 1 << 24, 1024))
     train, held = [], []
     for name, mk, mkq, share, limit, chunk in sources:
         t0 = time.perf_counter()

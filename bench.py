@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the matching hot path on one MI355X.
 
-Workload (config 2): seeded Chung-Lu graph, 1,000,000 vertices, 15,000,000 undirected edges,
-beta 2.5, 32 labels Zipf(1.2), resident in HBM; the admitted random-walk queries of the config-2
-query stream (asdmatch_amd/workload_manifest.json; admission rule in asdmatch_amd/workloads.py).
-One step = filter (CFL + candidate edges) + plan + exact count of every admitted query, through
-the C ABI.  Every count is checked against the manifest's exact count.
+Default workload (config 4, the batched multi-query configuration): seeded R-MAT graph, scale 20,
+edge factor 10, (a, b, c) = (.57, .19, .19), deduplicated, 16 uniform labels, resident in HBM; the
+admitted random-walk queries of the config-4 query stream (asdmatch_amd/workload_manifest.json;
+admission rule in asdmatch_amd/workloads.py).  One step = every admitted query filtered, planned
+and counted exactly.  Timed side by side:
+  - plain GPU drop-in: one filter + plan + count call sequence per query (asdm_filter_cfl,
+    asdm_plan_create, asdm_count), as the reference's main.cpp runs a query;
+  - batched + learned plans (the value): one asdm_batch_count_learned call per step, plans
+    ordered by the trained estimator (asdmatch_amd/data/mpnn_estimator.bin).
+Every count is checked against the manifest's exact count.
 
 Prints one JSON line: queries/s (value), embeddings/s, the roofline of the dominant kernel (its
 algorithmic bytes over its HIP-event-timed launches in the timed region) and the reference CPU
 path timed on this host on a bounded sample of the same workload.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--workload c4|c2]
+
+--workload c2 runs the config-2 queries (Chung-Lu 1M / 15M) one by one through the plain path.
 """
 import argparse
 import json
@@ -28,7 +35,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ESTIMATOR = os.path.join(ROOT, "asdmatch_amd", "data", "mpnn_estimator.bin")
 REF_LOCAL = os.path.join(ROOT, "oracle", "_ref", "ref_driver_local")
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c2_traffic.txt")
+TRAFFIC_FILES = {"c2": os.path.join(ROOT, "profiles", "c2_traffic.txt"),
+                 "c4": os.path.join(ROOT, "profiles", "c4_traffic.txt")}
 
 
 def start_cpu_baseline(g, q, tmpdir):
@@ -66,12 +74,13 @@ def finish_cpu_baseline(proc, expect, timeout):
             "embeddings_per_s": (vals.get("count") or 0) / secs}
 
 
-def read_traffic(kernel):
+def read_traffic(kernel, workload="c2"):
     """Calibrated HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
-    if not os.path.exists(TRAFFIC_FILE):
+    path = TRAFFIC_FILES[workload]
+    if not os.path.exists(path):
         return None
     vals = {}
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             if ":" in line:
                 k, v = line.split(":", 1)
@@ -87,8 +96,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c4"), default="c2",
-                    help="c2: config-2 queries one by one (default); c4: config-4 batch, batched vs plain")
+    ap.add_argument("--workload", choices=("c2", "c4"), default="c4",
+                    help="c4: config-4 batch, batched + learned plans vs plain (default); c2: config-2 queries "
+                         "one by one through the plain path")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
     args = ap.parse_args()
@@ -160,17 +170,7 @@ def main():
 
     elapsed = t2 - t1
     nq = len(queries) * args.steps
-    dom = max(kstats, key=lambda k: k["ms"]) if kstats else None
-    roofline = None
-    if dom and dom["ms"] > 0 and dom["launches"]:
-        bytes_per_launch = dom["bytes"] / dom["launches"]
-        avg_s = dom["ms"] / 1e3 / dom["launches"]
-        achieved = bytes_per_launch / avg_s / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(dom["name"]),
-                    "kernel": dom["name"],
-                    "alg_bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4),
-                    "launches": dom["launches"]}
+    roofline = roofline_of(kstats, "c2")
 
     cpu = finish_cpu_baseline(proc, min(queries, key=lambda t: t[2])[2] if queries else None, timeout=600)
     if cpu and "value" in cpu:
@@ -205,21 +205,84 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def roofline_of(kstats, workload):
+    """Roofline object of the dominant kernel (most device time) of a kernel table."""
+    ks = [k for k in kstats if k["launches"] and k["ms"] > 0]
+    if not ks:
+        return None
+    dom = max(ks, key=lambda k: k["ms"])
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    avg_s = dom["ms"] / 1e3 / dom["launches"]
+    achieved = bytes_per_launch / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(dom["name"], workload),
+            "kernel": dom["name"], "alg_bytes_per_launch": round(bytes_per_launch),
+            "avg_launch_ms": round(avg_s * 1e3, 4), "launches": dom["launches"]}
+
+
+def start_cpu_batch_baseline(g, sample, tmpdir):
+    """Reference ALL_LEVEL build (its exact path), one single-threaded process per sampled query,
+    the processes run side by side on the host cores."""
+    if not os.path.exists(REF_LOCAL) or not sample:
+        return None
+    dpath = os.path.join(tmpdir, "c4.graph")
+    g.write(dpath)
+    procs = []
+    for k, (q, _) in enumerate(sample):
+        qpath = os.path.join(tmpdir, f"c4_query_{k}.graph")
+        q.write(qpath)
+        procs.append(subprocess.Popen([REF_LOCAL, "-q", qpath, "-d", dpath], stdout=subprocess.PIPE,
+                                      stderr=subprocess.STDOUT, text=True))
+    return procs
+
+
+def finish_cpu_batch_baseline(procs, sample, total_embeddings, nq, timeout):
+    if not procs:
+        return None
+    secs, ok, emb = 0.0, True, 0
+    for p, (_, expect) in zip(procs, sample):
+        r = finish_cpu_baseline(p, expect, timeout)
+        if not r or "value" not in r:
+            return r
+        secs += r["seconds"]
+        ok &= bool(r["count_matches_gpu"])
+        emb += expect
+    rate = emb / secs  # embeddings per second of the reference's exact path
+    return {"value": round(len(sample) / secs, 6), "unit": "queries/s", "cores": 1, "kind": "reference",
+            "seconds": round(secs, 3), "queries": len(sample), "counts_match_gpu": ok,
+            "embeddings_per_s": round(rate, 1),
+            "full_batch_queries_per_s_extrapolated": round(nq / (total_embeddings / rate), 9)}
+
+
 def bench_batch(args):
     """Config-4 batch: the admitted queries counted (a) one by one through the plain path and
-    (b) in one asdm_batch_count call; queries/s of both, per-query latency of the batched call."""
+    (b) in one asdm_batch_count_learned call per step with learned plans (the value)."""
     import asdmatch_amd as A
     from asdmatch_amd import workloads as W
 
+    t0 = time.time()
     g = W.graph("c4")
     adm = W.admitted("c4")
     qs = [W.query(g, s, seed) for s, seed, _ in adm]
     want = [c for _, _, c in adm]
+    gen_s = time.time() - t0
+
+    # CPU baseline sample: the four admitted queries with the fewest embeddings (the reference
+    # enumerates every embedding; the whole batch would take it days), run while the GPU works
+    tmpdir = tempfile.mkdtemp(prefix="asdm_bench_")
+    order = sorted(range(len(qs)), key=lambda i: want[i])
+    sample = [(qs[i], want[i]) for i in order[:4]]
+    procs = None if args.no_cpu_baseline else start_cpu_batch_baseline(g, sample, tmpdir)
+
     ctx = A.Context(0)
     ctx.upload(g)
     import ctypes
 
     hip = ctypes.CDLL("libamdhip64.so")
+    m = None
+    if os.path.exists(ESTIMATOR):
+        m = A.Mpnn(ctx)
+        m.load(ESTIMATOR)
 
     def plain():
         out = []
@@ -228,66 +291,70 @@ def bench_batch(args):
             out.append(0 if cs is None else ctx.count(cs.plan(q), cs))
         return out
 
+    def batched(mm):
+        c, st, lat = ctx.batch_count_learned(g, qs, mm)
+        if [int(x) for x in c] != want or any(st):
+            raise SystemExit("batched count mismatch")
+        return lat
+
     for _ in range(args.warmup):
-        assert plain() == want
-        c, _ = ctx.batch_count(g, qs)
-        assert [int(x) for x in c] == want
+        if plain() != want:
+            raise SystemExit("plain count mismatch")
+        batched(None)
+        batched(m)
     hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
+    t1 = time.perf_counter()
     for _ in range(args.steps):
         got = plain()
     hip.hipDeviceSynchronize()
-    t1 = time.perf_counter()
+    t2 = time.perf_counter()
     if got != want:
         raise SystemExit("plain count mismatch")
-    lats = []
     for _ in range(args.steps):
-        c, lat = ctx.batch_count(g, qs)
-        lats.extend(lat.tolist())
+        batched(None)
     hip.hipDeviceSynchronize()
-    t2 = time.perf_counter()
-    if [int(x) for x in c] != want:
-        raise SystemExit("batch count mismatch")
+    t3 = time.perf_counter()
+    # the value: batched + learned plans, HIP events around every launch (resolved afterwards)
+    ctx.set_profiling(True)
+    lats = []
+    hip.hipDeviceSynchronize()
+    t4 = time.perf_counter()
+    for _ in range(args.steps):
+        lats.extend(batched(m).tolist())
+    hip.hipDeviceSynchronize()
+    t5 = time.perf_counter()
+    kstats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+
     nq = len(qs) * args.steps
-    plain_qps, batch_qps = nq / (t1 - t0), nq / (t2 - t1)
-    # batched + learned plans (the trained estimator orders every query's enumeration)
-    learned_qps = None
-    if os.path.exists(ESTIMATOR):
-        m = A.Mpnn(ctx)
-        m.load(ESTIMATOR)
-        for _ in range(args.warmup):
-            c, st, _ = ctx.batch_count_learned(g, qs, m)
-        hip.hipDeviceSynchronize()
-        t3 = time.perf_counter()
-        lats = []
-        for _ in range(args.steps):
-            c, st, lat = ctx.batch_count_learned(g, qs, m)
-            lats.extend(lat.tolist())
-        hip.hipDeviceSynchronize()
-        t4 = time.perf_counter()
-        if [int(x) for x in c] != want or any(st):
-            raise SystemExit("learned-plan batch count mismatch")
-        learned_qps = nq / (t4 - t3)
-    best = learned_qps if learned_qps is not None else batch_qps
-    step_s = nq / best / args.steps
+    plain_qps, batch_qps, learned_qps = nq / (t2 - t1), nq / (t3 - t2), nq / (t5 - t4)
     lats.sort()
+    cpu = finish_cpu_batch_baseline(procs, sample, sum(want), len(qs), timeout=600)
+    if cpu and "value" in cpu:
+        cpu["sample"] = ("reference ALL_LEVEL build (exact enumeration path), one thread per query, on the four "
+                         "admitted config-4 queries with the fewest embeddings; time = the sum of their "
+                         "Filtering + Planning + Execution lines (graph load excluded, as the GPU graph is "
+                         "resident); the extrapolation divides the batch's embeddings by the sample's rate")
     out = {
         "metric": "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts",
-        "value": round(best, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(step_s * 1e3, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
-        "config": {"workload": "config 4: R-MAT scale 20, edge factor 10, 16 uniform labels; admitted random-walk "
-                               "queries, one asdm_batch_count_learned call per step (learned plans when the "
-                               "trained estimator is present, else plain plans)",
-                   "learned_plans": learned_qps is not None,
+        "value": round(learned_qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round((t5 - t4) / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
+        "config": {"workload": "config 4: R-MAT scale 20, edge factor 10, (a,b,c)=(.57,.19,.19), 16 uniform "
+                               "labels; admitted random-walk queries, one asdm_batch_count_learned call per step",
                    "queries_per_step": len(qs), "query_sizes": sorted({q.n for q in qs}),
-                   "admission_row_limit": W.load_manifest()["c4"]["row_limit"]},
+                   "admission_row_limit": W.load_manifest()["c4"]["row_limit"],
+                   "learned_plans": m is not None},
         "plain_gpu_queries_per_s": round(plain_qps, 3),
         "batched_plain_plan_queries_per_s": round(batch_qps, 3),
-        "batched_learned_plan_queries_per_s": None if learned_qps is None else round(learned_qps, 3),
-        "batched_vs_plain": round(best / plain_qps, 3),
-        "embeddings_per_s": round(sum(want) / step_s, 1),
+        "batched_learned_plan_queries_per_s": round(learned_qps, 3),
+        "batched_learned_vs_plain": round(learned_qps / plain_qps, 3),
+        "embeddings_per_s": round(sum(want) * args.steps / (t5 - t4), 1),
         "latency_ms_p50": round(lats[len(lats) // 2], 3), "latency_ms_p99": round(lats[int(len(lats) * 0.99)], 3),
+        "roofline": roofline_of(kstats, "c4"),
+        "cpu_baseline": cpu,
+        "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats],
+        "setup_s": round(gen_s, 2),
     }
     print(json.dumps(out), flush=True)
 

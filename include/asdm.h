@@ -15,7 +15,7 @@
  *   asdm_cs_candidates          CandidateSpace::candidateSet        relation/candidate_space.h:62
  *   asdm_cs_arc_weight          setQueryGraphWeights' edge weight   relation/candidate_space.cpp:507-524
  *   asdm_plan_create            simplePlan / optCostPlan            plan/optimizer.cpp:7-200, :380-548
- *   asdm_plan_set_cardinalities subsetToCard                        plan/estimator.cpp:7
+ *   asdm_plan_create_with_cards subsetToCard consumed by the order  plan/estimator.cpp:7
  *   asdm_count                  adaptiveShareJoin (count build)     matching/adaptive.cpp:1179-2279
  *   asdm_local_count            adaptiveShareJoin + produceResult   matching/join.cpp:1016-1041,
  *                               (ALL_LEVEL + LOCAL_COUNT build)     main.cpp:105-107,135-146
@@ -26,6 +26,8 @@
  *                               estimator cardEstimateLabeled       plan/estimator.cpp:7, :173-291
  *   asdm_plan_create_learned    optimizer ordering by subsetToCard  plan/optimizer.cpp:7-200
  *   asdm_batch_count_learned    (no counterpart: asdm_batch_count with learned plans)
+ *   asdm_plain_estimate         (no counterpart: this planner's own independence-model estimate,
+ *                               the learned estimator's fallback)
  */
 #ifndef ASDM_H
 #define ASDM_H
