@@ -439,17 +439,28 @@ __global__ void __launch_bounds__(256) k_leaf(const DPlan *__restrict__ P, const
     }
 }
 
-// LOCAL_COUNT leaves (every leaf group has one leaf): per row, each matched vertex receives the
-// number of completions, and each leaf candidate x receives the product of the other leaves'
-// counts.  mat is |Vg| x |Vq| row-major; additions are integer atomics (order-independent).
-template <int G>
-__global__ void __launch_bounds__(256) k_leaf_local(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
-                                                    uint64_t nrows, const uint64_t *__restrict__ ce_off,
-                                                    const uint32_t *__restrict__ ce_val,
-                                                    const uint32_t *__restrict__ cand,
-                                                    unsigned long long *__restrict__ mat,
-                                                    unsigned long long *__restrict__ total) {
-    const int lg = threadIdx.x & (G - 1);
+// LOCAL_COUNT leaves (every leaf group has one leaf), aggregated without atomics on the matrix.
+// Per row, each matched vertex receives the number of completions and each leaf candidate x the
+// product of the other leaves' counts.  A row's contributions are (cell, value) pairs: each
+// matched vertex v_j gets (v_j * |Vq| + u_j, product of the leaf-group counts) and each valid
+// candidate x of leaf e gets (x * |Vq| + u_e, product of the other groups' counts).  The count pass
+// (FILL = false) writes the number of pairs of every row; after an exclusive scan the fill pass
+// (FILL = true) writes them at the row's offset (ordered ballot compaction within the lane group).
+// The pairs are then radix-sorted and reduced by key, and every cell receives one plain add.
+template <int G, bool FILL>
+__global__ void __launch_bounds__(256) k_leaf_local_pairs(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
+                                                          uint64_t nrows, const uint64_t *__restrict__ ce_off,
+                                                          const uint32_t *__restrict__ ce_val,
+                                                          const uint32_t *__restrict__ cand,
+                                                          uint32_t *__restrict__ npairs, const uint64_t *__restrict__ poff,
+                                                          uint64_t pbase, uint64_t *__restrict__ keys,
+                                                          uint64_t *__restrict__ vals,
+                                                          unsigned long long *__restrict__ total,
+                                                          uint32_t *__restrict__ overflow) {
+    const int lane = threadIdx.x & 63;
+    const int lg = lane & (G - 1);
+    const int leader = lane & ~(G - 1);
+    const uint64_t below = lg ? ((1ull << lg) - 1) : 0ull;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     const uint32_t D = P->D, ng = P->ngroups, nq = P->nq;
     uint64_t acc = 0, dummy = 0;
@@ -459,16 +470,33 @@ __global__ void __launch_bounds__(256) k_leaf_local(const DPlan *__restrict__ P,
         uint64_t prod = 1;
         for (uint32_t g = 0; g < ng; ++g) {
             uint64_t c = group_count<G>(P, g, row, ce_off, ce_val, cand, dummy);
-            c = __shfl(c, (int)((threadIdx.x & 63) & ~(G - 1)), 64);
+            c = __shfl(c, leader, 64);
             cnt[g] = c;
             prod *= c;
         }
-        if (prod == 0) continue;
-        if (lg == 0) acc += prod;
-        for (uint32_t j = lg; j < D; j += G) {
-            const uint32_t v = cand[P->coff[j] + row[j]];
-            atomicAdd(&mat[(uint64_t)v * nq + P->u[j]], (unsigned long long)prod);
+        if constexpr (!FILL) {
+            if (lg == 0) {
+                uint64_t np = 0;
+                if (prod) {
+                    np = D;
+                    for (uint32_t g = 0; g < ng; ++g) np += cnt[g];
+                }
+                if (np > 0xffffffffull) {
+                    *overflow = 1;
+                    np = 0;
+                }
+                npairs[it] = (uint32_t)np;
+                acc += prod;
+            }
+            continue;
         }
+        if (prod == 0) continue;
+        uint64_t o = poff[it] - pbase;
+        for (uint32_t j = lg; j < D; j += G) {
+            keys[o + j] = (uint64_t)cand[P->coff[j] + row[j]] * nq + P->u[j];
+            vals[o + j] = prod;
+        }
+        o += D;
         for (uint32_t g = 0; g < ng; ++g) {
             uint64_t others = 1;
             for (uint32_t h = 0; h < ng; ++h)
@@ -485,22 +513,42 @@ __global__ void __launch_bounds__(256) k_leaf_local(const DPlan *__restrict__ P,
                     pj = j;
                 }
             }
-            for (uint32_t t = lg; t < pr.len; t += G) {
-                const uint32_t x = ce_val[pr.lo + t];
-                bool ok = true;
-                for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
-                    if (j == pj) continue;
-                    ListRef r = list_of(P, ce_off, row, e, j);
-                    ok = contains_u32(ce_val + r.lo, r.len, x);
+            for (uint32_t t0 = 0; t0 < pr.len; t0 += G) {
+                const uint32_t t = t0 + lg;
+                bool ok = t < pr.len;
+                uint32_t xid = 0;
+                if (ok) {
+                    const uint32_t x = ce_val[pr.lo + t];
+                    for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
+                        if (j == pj) continue;
+                        ListRef r = list_of(P, ce_off, row, e, j);
+                        ok = contains_u32(ce_val + r.lo, r.len, x);
+                    }
+                    xid = cand[P->coff[e] + x];
+                    if (ok) ok = injective(S, cand, xid);
                 }
-                const uint32_t xid = cand[P->coff[e] + x];
-                if (ok) ok = injective(S, cand, xid);
-                if (ok) atomicAdd(&mat[(uint64_t)xid * nq + P->u[e]], (unsigned long long)others);
+                const uint64_t m = group_bits<G>(__ballot(ok));
+                if (ok) {
+                    const uint64_t k = o + (uint64_t)__builtin_popcountll(m & below);
+                    keys[k] = (uint64_t)xid * nq + P->u[e];
+                    vals[k] = others;
+                }
+                o += (uint64_t)__builtin_popcountll(m);
             }
         }
     }
-    acc = group_sum<64>(acc);
-    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(total, (unsigned long long)acc);
+    if constexpr (!FILL) {
+        acc = group_sum<64>(acc);
+        if (lane == 0 && acc) atomicAdd(total, (unsigned long long)acc);
+    }
+}
+
+// One plain add per reduced cell (keys are unique within a batch).
+__global__ void k_mat_add(const uint64_t *__restrict__ ukeys, const uint64_t *__restrict__ usums,
+                          const uint32_t *__restrict__ nruns, unsigned long long *__restrict__ mat) {
+    const uint32_t n = *nruns;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        mat[ukeys[i]] += (unsigned long long)usums[i];
 }
 
 // -------------------------------------------------------------------------- fused last level
@@ -834,6 +882,7 @@ namespace {
 constexpr int kG = 16;                      // lanes per row
 constexpr int kLeafLanes = 16;              // lanes per leaf-depth row in the batch engine
 constexpr uint64_t kLeafChunk = 1ull << 22;  // rows per leaf-count launch
+constexpr uint64_t kPairCap = 1ull << 26;    // local-count (cell, value) pairs per sort batch
 
 unsigned groups_grid(uint64_t items, int G) {
     uint64_t threads = items * (uint64_t)G;
@@ -981,20 +1030,113 @@ struct Engine {
         return ASDM_OK;
     }
 
+    // LOCAL_COUNT of leaf rows r[0, m) without atomics on the matrix: count pass, scan, then per
+    // batch of at most `cap` pairs a fill pass, a radix sort + reduce by key, and one plain add
+    // per distinct cell.
+    int leaf_local(uint32_t *r, uint64_t m) {
+        uint32_t *np = nullptr, *flag = nullptr, *nruns = nullptr;
+        uint64_t *po = nullptr;
+        std::vector<void *> tmp;
+        auto alloc = [&](void **p, size_t bytes) {
+            int rc = dev_alloc(ctx, p, std::max<size_t>(bytes, 8));
+            if (!rc) tmp.push_back(*p);
+            return rc;
+        };
+        auto done = [&](int rc) {
+            for (void *p : tmp) dev_free(ctx, p);
+            return rc;
+        };
+        int rc = alloc((void **)&np, 4 * m);
+        if (!rc) rc = alloc((void **)&po, 8 * (m + 1));
+        if (!rc) rc = alloc((void **)&flag, 4);
+        if (!rc) rc = alloc((void **)&nruns, 4);
+        if (!rc) rc = fill_u32(ctx, flag, 1, 0);
+        if (rc) return done(rc);
+        {
+            KTimer kt(ctx, "leaf_local_count");
+            hipLaunchKernelGGL((k_leaf_local_pairs<kG, false>), dim3(groups_grid(m, kG)), dim3(256), 0, ctx->stream, dplan,
+                               r, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, np, nullptr, 0, nullptr, nullptr, d_total,
+                               flag);
+            kt.stop();
+        }
+        if ((rc = check_hip(ctx, hipGetLastError(), "local pair count"))) return done(rc);
+        if ((rc = scan_u32_to_u64(ctx, np, po, m))) return done(rc);
+        std::vector<uint64_t> hpo(m + 1);
+        uint32_t hflag = 0;
+        rc = check_hip(ctx, hipMemcpyAsync(hpo.data(), po, 8 * (m + 1), hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (rc) return done(rc);
+        if (hflag) return done(ctx->fail(ASDM_E_UNSUPPORTED, "more than 2^32 local-count pairs in one row"));
+        const uint64_t cells = (uint64_t)cs->nv * cs->nq;
+        const int end_bit = cells > 1 ? 64 - __builtin_clzll(cells - 1) : 1;
+        // six 8-byte buffers per pair (pairs, sorted pairs, reduced cells) within the frontier budget
+        const uint64_t cap = std::max<uint64_t>(1024, std::min<uint64_t>(kPairCap, budget / 48));
+        for (uint64_t a = 0; a < m;) {
+            // largest b > a with at most cap pairs in rows [a, b) (one row may exceed the cap)
+            uint64_t lo = a + 1, hi = m;
+            while (lo < hi) {
+                const uint64_t mid = lo + (hi - lo + 1) / 2;
+                if (hpo[mid] - hpo[a] <= cap) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint64_t b = lo, n = hpo[b] - hpo[a];
+            if (n) {
+                uint64_t *keys = nullptr, *vals = nullptr, *ks = nullptr, *vs = nullptr, *uk = nullptr, *us = nullptr;
+                std::vector<void *> bt;
+                auto balloc = [&](void **p, size_t bytes) {
+                    int r2 = dev_alloc(ctx, p, bytes);
+                    if (!r2) bt.push_back(*p);
+                    return r2;
+                };
+                rc = balloc((void **)&keys, 8 * n);
+                if (!rc) rc = balloc((void **)&vals, 8 * n);
+                if (!rc) rc = balloc((void **)&ks, 8 * n);
+                if (!rc) rc = balloc((void **)&vs, 8 * n);
+                if (!rc) rc = balloc((void **)&uk, 8 * n);
+                if (!rc) rc = balloc((void **)&us, 8 * n);
+                if (!rc) {
+                    KTimer kt(ctx, "leaf_local_fill");
+                    hipLaunchKernelGGL((k_leaf_local_pairs<kG, true>), dim3(groups_grid(b - a, kG)), dim3(256), 0,
+                                       ctx->stream, dplan, r + a * D, b - a, cs->d_ce_off, cs->d_ce_val, cs->d_cand,
+                                       nullptr, po + a, hpo[a], keys, vals, nullptr, nullptr);
+                    kt.stop();
+                    rc = check_hip(ctx, hipGetLastError(), "local pair fill");
+                }
+                if (!rc) {
+                    KTimer kt(ctx, "local_sort_reduce");
+                    rc = sort_reduce_pairs(ctx, keys, vals, n, end_bit, ks, vs, uk, us, nruns);
+                    kt.stop();
+                }
+                if (!rc) {
+                    KTimer kt(ctx, "local_mat_add");
+                    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16);
+                    hipLaunchKernelGGL(k_mat_add, dim3(grid), dim3(256), 0, ctx->stream, uk, us, nruns, d_mat);
+                    kt.stop();
+                    rc = check_hip(ctx, hipGetLastError(), "matrix add");
+                }
+                for (void *p : bt) dev_free(ctx, p);
+                if (rc) return done(rc);
+            }
+            a = b;
+        }
+        return done(ASDM_OK);
+    }
+
     // Leaf counting in launches of at most kLeafChunk rows, so that the limits are checked often.
     int leaf(uint32_t *r, uint64_t n) {
         for (uint64_t i0 = 0; i0 < n; i0 += kLeafChunk) {
             const uint64_t m = std::min<uint64_t>(kLeafChunk, n - i0);
             if (i0 && timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
             ++launches;
-            KTimer kt(ctx, d_mat ? "leaf_local" : "leaf_count");
-            if (d_mat)
-                hipLaunchKernelGGL((k_leaf_local<kG>), dim3(groups_grid(m, kG)), dim3(256), 0, ctx->stream, dplan,
-                                   r + i0 * D, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_mat, d_total);
-            else
+            if (d_mat) {
+                if (int rc = leaf_local(r + i0 * D, m)) return rc;
+            } else {
+                KTimer kt(ctx, "leaf_count");
                 hipLaunchKernelGGL((k_leaf<kG>), dim3(groups_grid(m, kG)), dim3(256), 0, ctx->stream, dplan, r + i0 * D,
                                    m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2);
-            kt.stop();
+                kt.stop();
+            }
             int rc = check_hip(ctx, hipGetLastError(), "leaf kernel");
             if (rc) return rc;
             if (i0 + m < n) {  // let the device catch up so the wall-clock check means something

@@ -171,6 +171,27 @@ def test_local_counts_match_reference(ctx):
             assert all(int(s) == c for s in mat.sum(axis=0))
 
 
+def test_local_counts_same_under_many_pair_batches(ctx):
+    """The atomic-free aggregation sorts and reduces (cell, value) pairs in batches bounded by the
+    frontier budget; a tiny budget splits every leaf chunk into many batches and must give the
+    same matrices as the reference."""
+    done = 0
+    for case in T.synth_golden():
+        if not case["local"]:
+            continue
+        g = T.make_graph(case)
+        for rec in case["queries"]:
+            if not rec.get("local_npz") or done >= 6:
+                continue
+            q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+            mat, c = A.local_count(ctx, q, g, budget_bytes=1 << 16)
+            ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
+            assert c == rec["count"]
+            assert np.array_equal(mat, ref), (case["name"], rec["seed"])
+            done += 1
+    assert done > 0
+
+
 def test_config2_count_matches_reference(ctx):
     """Config-2 scale (1M vertices, 15M edges): the reference's exact build counted this admitted
     query (2,265,571,276 embeddings, tests/golden/c2_reference.json)."""
