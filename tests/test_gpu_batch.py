@@ -44,3 +44,22 @@ def test_batch_budget_fallback_gives_same_counts(ctx):
     qs = [g.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"]) for r in case["queries"]]
     counts, _ = ctx.batch_count(g, qs, budget_bytes=1 << 12)  # forces the per-query fallback
     assert [int(c) for c in counts] == [r["count"] for r in case["queries"]]
+
+
+def test_config4_heaviest_queries_single_and_batched(ctx):
+    """The heaviest admitted config-4 queries (millions of leaf rows, hub-heavy lists): the
+    single-query engine, the batched engine and the batched engine with an untrained estimator's
+    arbitrary plans must agree with the manifest's counts."""
+    from asdmatch_amd import workloads as W
+
+    g = W.graph("c4")
+    adm = W.admitted("c4")
+    heavy = sorted(adm, key=lambda t: -t[2])[:8]
+    qs = [W.query(g, s, seed) for s, seed, _ in heavy]
+    want = [c for _, _, c in heavy]
+    assert [A.count_embeddings(ctx, q, g) for q in qs] == want
+    counts, _ = ctx.batch_count(g, qs)
+    assert [int(c) for c in counts] == want
+    m = A.Mpnn(ctx, seed=5)
+    counts, status, _ = ctx.batch_count_learned(g, qs, m)
+    assert [int(c) for c in counts] == want and not any(status)
