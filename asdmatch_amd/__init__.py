@@ -118,6 +118,10 @@ _sig("asdm_mpnn_features", C.c_int, _P, _P, _u64p, C.c_uint32, _u32p, _f32p, _u3
 _sig("asdm_mpnn_estimate", C.c_int, _P, _P, _P, _u64p, C.c_uint32, C.POINTER(C.c_double))
 _sig("asdm_plan_create_learned", C.c_int, _P, _P, _P, _P, C.c_uint32, C.POINTER(_P))
 _sig("asdm_plain_estimate", C.c_int, _P, _u64p, C.c_uint32, C.POINTER(C.c_double))
+_sig("asdm_ctx_set_batch_tuning", C.c_int, _P, C.c_uint32, C.c_double, C.c_uint32)
+_sig("asdm_ctx_get_batch_tuning", C.c_int, _P, _u32p, C.POINTER(C.c_double), _u32p)
+_sig("asdm_batch_autotune", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _P, C.c_uint32, _u32p,
+     C.POINTER(C.c_double), _u32p, _f32p)
 
 MPNN_FEATURES = 8
 MPNN_HIDDEN = 64
@@ -299,6 +303,30 @@ class Context:
                                                   _ptr(counts, C.c_uint64), st.ctypes.data_as(_i32p),
                                                   lat.ctypes.data_as(_f32p)), "batch_count_learned")
         return counts[:n], st[:n], lat[:n]
+
+    def batch_tuning(self):
+        """(lanes per leaf-depth row, lane-class threshold factor, rows per lane group in a tile)."""
+        l, t = C.c_uint32(), C.c_uint32()
+        sc = C.c_double()
+        self._check(_lib.asdm_ctx_get_batch_tuning(self._h, C.byref(l), C.byref(sc), C.byref(t)), "get_batch_tuning")
+        return l.value, sc.value, t.value
+
+    def set_batch_tuning(self, leaf_lanes: int, class_scale: float, tile_iters: int):
+        self._check(_lib.asdm_ctx_set_batch_tuning(self._h, leaf_lanes, float(class_scale), tile_iters),
+                    "set_batch_tuning")
+
+    def batch_autotune(self, g: Graph, qs, mpnn: Optional["Mpnn"] = None, reps: int = 2, budget_bytes: int = 0):
+        """Times the batch under each launch shape of the tuner's grid and keeps the fastest;
+        returns (leaf lanes, class factor, tile rows, best median ms)."""
+        self.upload(g)
+        n = len(qs)
+        hs = (_P * max(n, 1))(*[q._h for q in qs])
+        l, t = C.c_uint32(), C.c_uint32()
+        sc = C.c_double()
+        ms = C.c_float()
+        self._check(_lib.asdm_batch_autotune(self._h, g._h, hs, n, budget_bytes, mpnn._h if mpnn else None, reps,
+                                             C.byref(l), C.byref(sc), C.byref(t), C.byref(ms)), "batch_autotune")
+        return l.value, sc.value, t.value, ms.value
 
     def stats(self) -> Stats:
         s = Stats()

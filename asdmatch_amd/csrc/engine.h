@@ -59,6 +59,12 @@ struct asdm_ctx {
     bool profiling = false;
     double time_limit_ms = 0;  // 0 = no limit
     uint64_t row_limit = 0;    // 0 = no limit
+    // Batch-engine launch shape (asdm_ctx_set_batch_tuning / asdm_batch_autotune): lanes per
+    // leaf-depth row, the factor on the list-length thresholds of the lane classes (3, 12, 48
+    // expected entries for 4, 16, 64 lanes), and rows per lane group in a wave tile.
+    uint32_t tune_leaf_lanes = 16;
+    double tune_class_scale = 1.0;
+    uint32_t tune_tile_iters = 4;
     // per-kernel device time (ms), launches and algorithmic bytes, accumulated while profiling
     struct KStat {
         std::string name;

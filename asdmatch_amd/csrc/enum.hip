@@ -880,7 +880,6 @@ __global__ void k_bgather(const uint64_t *__restrict__ offs, const uint64_t *__r
 namespace {
 
 constexpr int kG = 16;                      // lanes per row
-constexpr int kLeafLanes = 16;              // lanes per leaf-depth row in the batch engine
 constexpr uint64_t kLeafChunk = 1ull << 22;  // rows per leaf-count launch
 constexpr uint64_t kPairCap = 1ull << 26;    // local-count (cell, value) pairs per sort batch
 
@@ -1311,7 +1310,7 @@ namespace {
 
 // Lane-group width class of a query: its expected candidate-list length at the deepest
 // enumerated level (smallest fan-out among that level's matched neighbours).
-uint32_t lane_class(const asdm_plan *plan, const asdm_cs *cs) {
+uint32_t lane_class(const asdm_plan *plan, const asdm_cs *cs, double scale) {
     const uint32_t d = plan->D >= 2 ? plan->D - 1 : 0;
     double best = 1e300;
     for (uint32_t j = 0; j < plan->h.nb[d]; ++j) {
@@ -1320,12 +1319,11 @@ uint32_t lane_class(const asdm_plan *plan, const asdm_cs *cs) {
         best = std::min(best, (double)cs->arc_weight[a] / std::max(1.0, (double)cs->csize[p]));
     }
     if (plan->h.nb[d] == 0) best = cs->csize[plan->order[0]];
-    return best < 3 ? 1 : best < 12 ? 4 : best < 48 ? 16 : 64;
+    return best < 3 * scale ? 1 : best < 12 * scale ? 4 : best < 48 * scale ? 16 : 64;
 }
 
-// Rows per wave tile for lane width G: four rows per lane group.
-constexpr uint32_t kTileIters = 4;
-inline uint32_t tile_rows_of(uint32_t G) { return 64u / G * kTileIters; }
+// Rows per wave tile for lane width G: `iters` rows per lane group.
+inline uint32_t tile_rows_of(uint32_t G, uint32_t iters) { return 64u / G * iters; }
 
 inline unsigned tiles_grid(uint64_t tiles) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((tiles + 3) / 4, 1u << 16));
@@ -1336,7 +1334,7 @@ void launch_bcount(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start,
                    uint32_t qb, uint64_t tiles, uint32_t d, const uint32_t *rows, const asdm_cs::Store *st,
                    uint32_t *counts, unsigned long long *qcount, unsigned long long *bytes) {
     hipLaunchKernelGGL((k_bcount<G, KIND>), dim3(tiles_grid(tiles)), dim3(256), 0, ctx->stream, plans, seg_start,
-                       tile_pref, qa, qb, tile_rows_of(G), d, rows, st->d_ce_off, st->d_ce_val, st->d_cand, counts,
+                       tile_pref, qa, qb, tile_rows_of(G, ctx->tune_tile_iters), d, rows, st->d_ce_off, st->d_ce_val, st->d_cand, counts,
                        qcount, bytes);
 }
 
@@ -1345,7 +1343,7 @@ void launch_bfill(asdm_ctx *ctx, const DPlan *plans, const uint64_t *seg_start, 
                   uint32_t qb, uint64_t tiles, uint32_t d, const uint32_t *rows, const asdm_cs::Store *st,
                   const uint64_t *offs, uint32_t *out, unsigned long long *bytes) {
     hipLaunchKernelGGL((k_bfill<G>), dim3(tiles_grid(tiles)), dim3(256), 0, ctx->stream, plans, seg_start, tile_pref,
-                       qa, qb, tile_rows_of(G), d, rows, st->d_ce_off, st->d_ce_val, st->d_cand, offs, out, bytes);
+                       qa, qb, tile_rows_of(G, ctx->tune_tile_iters), d, rows, st->d_ce_off, st->d_ce_val, st->d_cand, offs, out, bytes);
 }
 
 template <class F1, class F4, class F16, class F64>
@@ -1370,7 +1368,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     std::vector<uint32_t> perm(Q), cls(Q);
     for (uint32_t i = 0; i < Q; ++i) {
         perm[i] = i;
-        cls[i] = lane_class(plans_in[i], css_in[i]);
+        cls[i] = lane_class(plans_in[i], css_in[i], ctx->tune_class_scale);
     }
     std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
         return cls[a] != cls[b] ? cls[a] < cls[b] : plans_in[a]->D < plans_in[b]->D;
@@ -1459,7 +1457,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         h.assign(Q + 1, 0);
         for (uint32_t i = 0; i < Q; ++i) {
             const uint64_t r = seg_start_h[i + 1] - seg_start_h[i];
-            const uint64_t tr = tile_rows_of(lanes(i));
+            const uint64_t tr = tile_rows_of(lanes(i), ctx->tune_tile_iters);
             h[i + 1] = h[i] + (skip(i) ? 0 : (r + tr - 1) / tr);
         }
         return check_hip(ctx, hipMemcpyAsync(buf, h.data(), 8 * (Q + 1), hipMemcpyHostToDevice, ctx->stream), "H2D");
@@ -1480,12 +1478,12 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         if (rc) break;
         {
             // one launch per run of queries with the same row kind and lane width at this depth:
-            // leaf-depth rows (inclusion-exclusion over leaf lists) use kLeafLanes, the others
+            // leaf-depth rows (inclusion-exclusion over leaf lists) use tune_leaf_lanes, the others
             // their class
             auto kind_of = [&](uint32_t i) {
                 return hp[i].D == d ? kLeafRows : (hp[i].fused && d + 1 == hp[i].D) ? kFusedRows : kExpandRows;
             };
-            auto lanes = [&](uint32_t i) { return kind_of(i) == kLeafRows ? (uint32_t)kLeafLanes : qcls[i]; };
+            auto lanes = [&](uint32_t i) { return kind_of(i) == kLeafRows ? ctx->tune_leaf_lanes : qcls[i]; };
             if ((rc = tile_prefix(lanes, [](uint32_t) { return false; }, tp_count_h, tp_count))) break;
             for_runs(
                 [&](uint32_t i) { return 256u * kind_of(i) + lanes(i); },
@@ -1602,6 +1600,74 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
 using namespace asdm;
 
 extern "C" {
+
+int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_scale, uint32_t tile_iters) {
+    if (!ctx || !(leaf_lanes == 1 || leaf_lanes == 4 || leaf_lanes == 16 || leaf_lanes == 64) || !(class_scale > 0) ||
+        tile_iters < 1 || tile_iters > 64)
+        return ASDM_E_INVALID;
+    ctx->tune_leaf_lanes = leaf_lanes;
+    ctx->tune_class_scale = class_scale;
+    ctx->tune_tile_iters = tile_iters;
+    return ASDM_OK;
+}
+
+int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale, uint32_t *tile_iters) {
+    if (!ctx) return ASDM_E_INVALID;
+    if (leaf_lanes) *leaf_lanes = ctx->tune_leaf_lanes;
+    if (class_scale) *class_scale = ctx->tune_class_scale;
+    if (tile_iters) *tile_iters = ctx->tune_tile_iters;
+    return ASDM_OK;
+}
+
+// Autotuner of the batch launch shape: every shape of a small grid runs the caller's batch `reps`
+// times after one untimed run; the shape with the smallest median time is kept on the context.
+// Every run's counts must equal the first run's (they are plan- and shape-independent).
+int asdm_batch_autotune(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
+                        uint64_t budget_bytes, asdm_mpnn *m, uint32_t reps, uint32_t *leaf_lanes, double *class_scale,
+                        uint32_t *tile_iters, float *ms_best) {
+    if (!ctx || !g || (!qs && nq)) return ASDM_E_INVALID;
+    reps = std::max<uint32_t>(1, reps);
+    const uint32_t save_l = ctx->tune_leaf_lanes, save_t = ctx->tune_tile_iters;
+    const double save_s = ctx->tune_class_scale;
+    std::vector<uint64_t> ref(nq), cur(nq);
+    int rc = asdm_batch_count_learned(ctx, g, qs, nq, budget_bytes, m, ref.data(), nullptr, nullptr);
+    if (rc) return rc;
+    double best_ms = 1e300;
+    uint32_t bl = save_l, bt = save_t;
+    double bs = save_s;
+    for (uint32_t l : {4u, 16u, 64u})
+        for (double sc : {0.25, 0.5, 1.0, 2.0})
+            for (uint32_t ti : {2u, 4u, 8u}) {
+                ctx->tune_leaf_lanes = l;
+                ctx->tune_class_scale = sc;
+                ctx->tune_tile_iters = ti;
+                std::vector<double> t;
+                for (uint32_t r = 0; r <= reps && !rc; ++r) {
+                    auto t0 = std::chrono::steady_clock::now();
+                    rc = asdm_batch_count_learned(ctx, g, qs, nq, budget_bytes, m, cur.data(), nullptr, nullptr);
+                    if (r) t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+                    if (!rc && cur != ref) rc = ctx->fail(ASDM_E_INVALID, "autotune: counts differ between launch shapes");
+                }
+                if (rc) break;
+                std::sort(t.begin(), t.end());
+                const double med = t[t.size() / 2];
+                if (med < best_ms) {
+                    best_ms = med;
+                    bl = l;
+                    bs = sc;
+                    bt = ti;
+                }
+            }
+    ctx->tune_leaf_lanes = rc ? save_l : bl;
+    ctx->tune_class_scale = rc ? save_s : bs;
+    ctx->tune_tile_iters = rc ? save_t : bt;
+    if (rc) return rc;
+    if (leaf_lanes) *leaf_lanes = bl;
+    if (class_scale) *class_scale = bs;
+    if (tile_iters) *tile_iters = bt;
+    if (ms_best) *ms_best = (float)best_ms;
+    return ASDM_OK;
+}
 
 int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint64_t *count) {
     return run_engine(ctx, plan, cs, budget_bytes, nullptr, count);

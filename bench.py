@@ -302,6 +302,8 @@ def bench_batch(args):
             raise SystemExit("plain count mismatch")
         batched(None)
         batched(m)
+    # launch shape of the batched engine, picked by the library's autotuner on this batch (untimed)
+    tune = ctx.batch_autotune(g, qs, m, reps=1) if args.warmup else ctx.batch_tuning() + (None,)
     hip.hipDeviceSynchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
@@ -344,7 +346,9 @@ def bench_batch(args):
                                "labels; admitted random-walk queries, one asdm_batch_count_learned call per step",
                    "queries_per_step": len(qs), "query_sizes": sorted({q.n for q in qs}),
                    "admission_row_limit": W.load_manifest()["c4"]["row_limit"],
-                   "learned_plans": m is not None},
+                   "learned_plans": m is not None,
+                   "batch_launch_shape": {"leaf_lanes": tune[0], "class_scale": tune[1], "tile_iters": tune[2],
+                                          "autotuned": tune[3] is not None}},
         "plain_gpu_queries_per_s": round(plain_qps, 3),
         "batched_plain_plan_queries_per_s": round(batch_qps, 3),
         "batched_learned_plan_queries_per_s": round(learned_qps, 3),

@@ -26,6 +26,8 @@
  *                               estimator cardEstimateLabeled       plan/estimator.cpp:7, :173-291
  *   asdm_plan_create_learned    optimizer ordering by subsetToCard  plan/optimizer.cpp:7-200
  *   asdm_batch_count_learned    (no counterpart: asdm_batch_count with learned plans)
+ *   asdm_batch_autotune,        (no counterpart: launch shape of the batched engine)
+ *   asdm_ctx_{set,get}_batch_tuning
  *   asdm_plain_estimate         (no counterpart: this planner's own independence-model estimate,
  *                               the learned estimator's fallback)
  */
@@ -188,6 +190,20 @@ int asdm_filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const
  * for a query with an empty candidate set (count 0).  Without it such a query fails the call. */
 int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
                              uint64_t budget_bytes, asdm_mpnn *m, uint64_t *counts, int *status, float *lat_ms);
+
+/* Launch shape of the batched engine: lanes per leaf-depth row (1, 4, 16 or 64), a factor (> 0) on
+ * the expected-list-length thresholds that pick 4, 16 or 64 lanes per row for the other rows, and
+ * rows per lane group in a wave tile (1..64).  Defaults 16, 1.0, 4.  Counts do not depend on it. */
+int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_scale, uint32_t tile_iters);
+int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale,
+                              uint32_t *tile_iters);
+/* Autotuner: runs the batch once untimed, then `reps` timed runs for each shape of the grid leaf
+ * lanes {4, 16, 64} x class factor {0.25, 0.5, 1, 2} x tile rows {2, 4, 8}; keeps the shape with the
+ * smallest median time on the context and returns it (ms_best: that median).  Fails with
+ * ASDM_E_INVALID if any run's counts differ from the first run's. */
+int asdm_batch_autotune(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
+                        uint64_t budget_bytes, asdm_mpnn *m, uint32_t reps, uint32_t *leaf_lanes,
+                        double *class_scale, uint32_t *tile_iters, float *ms_best);
 
 /* ---- learned cardinality estimator ----
  * A 3-layer message-passing network (hidden 64, node features ASDM_MPNN_FEATURES, graphs of at

@@ -63,3 +63,30 @@ def test_config4_heaviest_queries_single_and_batched(ctx):
     m = A.Mpnn(ctx, seed=5)
     counts, status, _ = ctx.batch_count_learned(g, qs, m)
     assert [int(c) for c in counts] == want and not any(status)
+
+
+def test_autotune_and_launch_shapes_keep_counts(ctx):
+    """Every launch shape gives the same counts (the autotuner checks this across its whole grid)
+    and the tuner keeps a shape from that grid on the context."""
+    case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
+    g = T.make_graph(case)
+    recs = [r for r in case["queries"] if r["count"] is not None]
+    qs = [g.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"]) for r in recs]
+    want = [r["count"] for r in recs]
+    default = ctx.batch_tuning()
+    try:
+        for shape in ((1, 0.5, 1), (64, 2.0, 8), (4, 1.0, 64)):
+            ctx.set_batch_tuning(*shape)
+            assert ctx.batch_tuning() == shape
+            counts, _ = ctx.batch_count(g, qs)
+            assert [int(c) for c in counts] == want, shape
+        for bad in ((3, 1.0, 4), (16, 0.0, 4), (16, 1.0, 0), (16, 1.0, 65)):
+            with pytest.raises(A.AsdmError):
+                ctx.set_batch_tuning(*bad)
+        lanes, scale, iters, ms = ctx.batch_autotune(g, qs, reps=1)
+        assert lanes in (4, 16, 64) and scale in (0.25, 0.5, 1.0, 2.0) and iters in (2, 4, 8) and ms > 0
+        assert ctx.batch_tuning() == (lanes, scale, iters)
+        counts, _ = ctx.batch_count(g, qs)
+        assert [int(c) for c in counts] == want
+    finally:
+        ctx.set_batch_tuning(*default)
