@@ -94,6 +94,7 @@ _sig("asdm_plan_create", C.c_int, _P, _P, _u32p, C.c_uint32, C.POINTER(_P))
 _sig("asdm_plan_create_with_cards", C.c_int, _P, _P, _u64p, C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
      C.POINTER(_P))
 _sig("asdm_plan_free", None, _P)
+_sig("asdm_plan_create_ordered", C.c_int, _P, _P, _u32p, C.c_uint32, C.c_uint32, C.POINTER(_P))
 _sig("asdm_plan_order", C.c_int, _P, _u32p, _u32p)
 _sig("asdm_plan_estimate", C.c_int, _P, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
@@ -405,9 +406,15 @@ class CandSpace:
         _lib.asdm_cs_arc_weight(self._h, a, C.byref(w))
         return w.value
 
-    def plan(self, q: Graph, order=None, local: bool = False, cards=None, factor: bool = True) -> "Plan":
+    def plan(self, q: Graph, order=None, local: bool = False, cards=None, factor: bool = True,
+             num_enumerated: Optional[int] = None) -> "Plan":
         h = _P()
         flags = (ASDM_PLAN_LOCAL if local else 0) | (0 if factor else ASDM_PLAN_NO_FACTOR)
+        if num_enumerated is not None:
+            rc = _lib.asdm_plan_create_ordered(q._h, self._h, _ptr(_u32(order)), num_enumerated, flags, C.byref(h))
+            if rc != ASDM_OK:
+                raise AsdmError(rc, "plan_create_ordered")
+            return Plan(h, self.nq)
         if cards:
             masks = np.array(list(cards.keys()), np.uint64)
             vals = np.array(list(cards.values()), np.float64)

@@ -47,6 +47,11 @@ struct DPlan {
     uint32_t pnblk[kMaxGroup + 1][kMaxParts];
     uint32_t pblk[kMaxGroup + 1][kMaxParts][kMaxGroup];
     int64_t pcoef[kMaxGroup + 1][kMaxParts];
+    // candidate-set membership bitmaps of the query's vertices ([nq][W] words, row = query vertex)
+    // and their per-word rank prefixes: data id -> position in C(u) in two loads
+    const uint64_t *bits;
+    const uint32_t *rank;
+    uint32_t W;
 };
 
 }  // namespace asdm

@@ -75,10 +75,22 @@ __device__ inline bool injective(const SameLabel &S, const uint32_t *__restrict_
     return ok;
 }
 
+// Position of data vertex xid in C(u_d), the candidate set of depth d's query vertex, or
+// 0xffffffff when xid is not a candidate: one membership-bitmap word and its rank prefix.
+__device__ inline uint32_t cand_pos(const DPlan *__restrict__ P, uint32_t d, uint32_t xid) {
+    const uint64_t idx = (uint64_t)P->u[d] * P->W + (xid >> 6);
+    const uint64_t w = P->bits[idx];
+    const uint32_t b = xid & 63;
+    if (!((w >> b) & 1)) return 0xffffffffu;
+    return P->rank[idx] + (uint32_t)__builtin_popcountll(w & ((1ull << b) - 1));
+}
+
 // Number of list elements (positions in C(u_d)) whose data id is one of the same-label matched
-// vertices: each such vertex is located in C(u_d) and then in the list by binary search.
-__device__ inline uint32_t same_hits(const SameLabel &S, const uint32_t *__restrict__ cand, uint64_t cand_d,
-                                     uint32_t csz, const uint32_t *__restrict__ list, uint32_t len) {
+// vertices: each such vertex is located in C(u_d) through the bitmap, then in the list by binary
+// search.
+__device__ inline uint32_t same_hits(const SameLabel &S, const DPlan *__restrict__ P, uint32_t d,
+                                     const uint32_t *__restrict__ cand, const uint32_t *__restrict__ list,
+                                     uint32_t len) {
     uint32_t h = 0;
     for (uint32_t s = 0; s < S.ns; ++s) {
         uint32_t y;
@@ -91,8 +103,8 @@ __device__ inline uint32_t same_hits(const SameLabel &S, const uint32_t *__restr
             uint32_t dep = S.sdep[s];
             y = cand[S.coff[dep] + S.row[dep]];
         }
-        const uint32_t x = lower_bound_u32(cand + cand_d, csz, y);
-        if (x < csz && cand[cand_d + x] == y && contains_u32(list, len, x)) ++h;
+        const uint32_t x = cand_pos(P, d, y);
+        if (x != 0xffffffffu && contains_u32(list, len, x)) ++h;
     }
     return h;
 }
@@ -123,7 +135,7 @@ __device__ inline uint32_t expand_count_row(const DPlan *__restrict__ P, uint32_
     const uint32_t *probe = ce_val + pr.lo;
     uint32_t c;
     if (nb == 1) {
-        c = lg == 0 ? pr.len - same_hits(S, cand, cand_d, P->csz[d], probe, pr.len) : 0;
+        c = lg == 0 ? pr.len - same_hits(S, P, d, cand, probe, pr.len) : 0;
         bytes += lg == 0 ? 4ull * d + 16 : 0;
     } else {
         c = 0;
@@ -307,11 +319,9 @@ __device__ inline uint64_t tri_count(const DPlan *__restrict__ P, uint32_t e, ui
     const uint32_t len = (uint32_t)(ce_off[P->tri_base[e] + i0 + 1] - lo);
     const uint32_t k = lower_bound_u32(ce_val + lo, len, i1);  // (f(p0), f(p1)) is a candidate edge
     uint64_t c = P->tri_tab[e][lo + k - ce_off[P->tri_base[e]]];
-    const uint64_t ce = P->coff[e];
     for (uint32_t s = 0; s < ns; ++s) {
-        const uint32_t x = lower_bound_u32(cand + ce, P->csz[e], y[s]);
-        if (x < P->csz[e] && cand[ce + x] == y[s] && contains_u32(ce_val + l0.lo, l0.len, x) &&
-            contains_u32(ce_val + l1.lo, l1.len, x))
+        const uint32_t x = cand_pos(P, e, y[s]);
+        if (x != 0xffffffffu && contains_u32(ce_val + l0.lo, l0.len, x) && contains_u32(ce_val + l1.lo, l1.len, x))
             --c;
     }
     return c;
@@ -348,7 +358,7 @@ __device__ inline uint64_t leaf_set_count(const DPlan *__restrict__ P, uint32_t 
     if (nlists == 1) {
         if (lg == 0) {
             bytes += 16 + 8ull * S.ns;
-            return pr.len - same_hits(S, cand, cand_p, P->csz[pe], probe, pr.len);
+            return pr.len - same_hits(S, P, pe, cand, probe, pr.len);
         }
         return 0;
     }
@@ -363,8 +373,8 @@ __device__ inline uint64_t leaf_set_count(const DPlan *__restrict__ P, uint32_t 
             const uint32_t e = P->gleaf[g][i];
             uint32_t pos = x;
             if (e != pe) {
-                pos = lower_bound_u32(cand + P->coff[e], P->csz[e], xid);
-                ok = pos < P->csz[e] && cand[P->coff[e] + pos] == xid;
+                pos = cand_pos(P, e, xid);
+                ok = pos != 0xffffffffu;
             }
             for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
                 if (e == pe && j == pj) continue;
@@ -602,8 +612,8 @@ __device__ inline uint64_t leaf_set_count_lane(const DPlan *__restrict__ P, uint
     if (nlists == 1) {
         uint32_t h = 0;
         for (uint32_t s = 0; s < ns; ++s) {
-            const uint32_t x = lower_bound_u32(cand + cand_p, P->csz[pe], y[s]);
-            if (x < P->csz[pe] && cand[cand_p + x] == y[s] && contains_u32(probe, pr.len, x)) ++h;
+            const uint32_t x = cand_pos(P, pe, y[s]);
+            if (x != 0xffffffffu && contains_u32(probe, pr.len, x)) ++h;
         }
         return pr.len - h;
     }
@@ -618,8 +628,8 @@ __device__ inline uint64_t leaf_set_count_lane(const DPlan *__restrict__ P, uint
             const uint32_t e = P->gleaf[g][i];
             uint32_t pos = x;
             if (e != pe) {
-                pos = lower_bound_u32(cand + P->coff[e], P->csz[e], xid);
-                ok = pos < P->csz[e] && cand[P->coff[e] + pos] == xid;
+                pos = cand_pos(P, e, xid);
+                ok = pos != 0xffffffffu;
             }
             for (uint32_t j = 0; j < P->nb[e] && ok; ++j) {
                 if (e == pe && j == pj) continue;

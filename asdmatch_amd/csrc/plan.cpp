@@ -344,6 +344,9 @@ int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint
             P.pcoef[k][i] = coef[i];
         }
     }
+    P.bits = cs->d_bits;
+    P.rank = cs->d_rank;
+    P.W = cs->W;
     out->order = order;
     out->D = D;
     out->local = local;
@@ -386,6 +389,43 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
     auto p = std::make_unique<asdm_plan>();
     std::string err;
     int rc = build_plan(q, cs, order, flags, [](uint64_t, double fallback) { return fallback; }, p.get(), err);
+    if (rc) {
+        cs->ctx->err = err;
+        return rc;
+    }
+    p->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = p.release();
+    return ASDM_OK;
+}
+
+int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t num_enumerated,
+                             uint32_t flags, asdm_plan **out) {
+    if (!q || !cs || !order || !out || q->n != cs->nq || num_enumerated < 1 || num_enumerated > cs->nq)
+        return ASDM_E_INVALID;
+    *out = nullptr;
+    const uint32_t nq = cs->nq, D = num_enumerated;
+    const uint32_t max_group = (flags & ASDM_PLAN_LOCAL) ? 1 : kMaxGroup;
+    std::vector<uint32_t> o(order, order + nq);
+    std::vector<char> seen(nq, 0);
+    for (uint32_t u : o) {
+        if (u >= nq || seen[u]) return cs->ctx->fail(ASDM_E_INVALID, "order is not a permutation of the query vertices");
+        seen[u] = 1;
+    }
+    // the leaves (depths D..nq-1) must be pairwise non-adjacent, within the label-group limit
+    for (uint32_t i = D; i < nq; ++i) {
+        uint32_t same = 0;
+        for (uint32_t j = D; j < nq; ++j) {
+            if (j != i && adjacent(cs, o[i], o[j]))
+                return cs->ctx->fail(ASDM_E_UNSUPPORTED, "leaves of the order are adjacent");
+            same += cs->qlab[o[j]] == cs->qlab[o[i]];
+        }
+        if (same > max_group) return cs->ctx->fail(ASDM_E_UNSUPPORTED, "leaf label group larger than supported");
+    }
+    if (D == nq && nq > 1) return cs->ctx->fail(ASDM_E_INVALID, "at least one vertex must be a leaf");
+    auto t0 = std::chrono::steady_clock::now();
+    auto p = std::make_unique<asdm_plan>();
+    std::string err;
+    int rc = finish_plan(cs, o, D, flags, [](uint64_t, double fb) { return fb; }, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;

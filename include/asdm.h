@@ -16,6 +16,7 @@
  *   asdm_cs_arc_weight          setQueryGraphWeights' edge weight   relation/candidate_space.cpp:507-524
  *   asdm_plan_create            simplePlan / optCostPlan            plan/optimizer.cpp:7-200, :380-548
  *   asdm_plan_create_with_cards subsetToCard consumed by the order  plan/estimator.cpp:7
+ *   asdm_plan_create_ordered    plan from a given attribute order   plan/optimizer.cpp:7-200
  *   asdm_count                  adaptiveShareJoin (count build)     matching/adaptive.cpp:1179-2279
  *   asdm_local_count            adaptiveShareJoin + produceResult   matching/join.cpp:1016-1041,
  *                               (ALL_LEVEL + LOCAL_COUNT build)     main.cpp:105-107,135-146
@@ -133,6 +134,11 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
 #define ASDM_PLAN_NO_FACTOR 2u
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
+/* Plan with an explicit order whose first num_enumerated vertices are enumerated and the rest are
+ * leaves (pairwise non-adjacent, within the label-group limit); the analogue of building a plan
+ * from a given attribute order (plan/optimizer.cpp:7-200). */
+int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order,
+                             uint32_t num_enumerated, uint32_t flags, asdm_plan **out);
 void asdm_plan_free(asdm_plan *p);
 /* Planner's estimate (independence model over candidate-edge list sizes, or the supplied
  * cardinalities) of the partial embeddings materialized at the leaf depth and over all depths. */
