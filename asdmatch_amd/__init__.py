@@ -27,6 +27,7 @@ ASDM_OK = 0
 ASDM_E_EMPTY_CANDIDATES = 1
 ASDM_PLAN_LOCAL = 1
 ASDM_PLAN_NO_FACTOR = 2
+ASDM_PLAN_COST = 4
 
 _ERRNAMES = {
     -1: "ASDM_E_INVALID", -2: "ASDM_E_IO", -3: "ASDM_E_OOM", -4: "ASDM_E_HIP",
@@ -407,9 +408,11 @@ class CandSpace:
         return w.value
 
     def plan(self, q: Graph, order=None, local: bool = False, cards=None, factor: bool = True,
-             num_enumerated: Optional[int] = None) -> "Plan":
+             num_enumerated: Optional[int] = None, cost: bool = False) -> "Plan":
+        """cost: multi-start cost-based order search with the independence model's cardinalities."""
         h = _P()
-        flags = (ASDM_PLAN_LOCAL if local else 0) | (0 if factor else ASDM_PLAN_NO_FACTOR)
+        flags = ((ASDM_PLAN_LOCAL if local else 0) | (0 if factor else ASDM_PLAN_NO_FACTOR) |
+                 (ASDM_PLAN_COST if cost else 0))
         if num_enumerated is not None:
             rc = _lib.asdm_plan_create_ordered(q._h, self._h, _ptr(_u32(order)), num_enumerated, flags, C.byref(h))
             if rc != ASDM_OK:

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <functional>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -181,6 +182,15 @@ uint64_t frontier_of(const asdm_cs *cs, uint64_t S, uint64_t rest);
 uint32_t smallest_candidates(const asdm_cs *cs, uint64_t rest);
 int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint32_t flags, const CardFn &card_of,
                 asdm_plan *out, std::string &err);
+// Cost-based order search shared by the plain (ASDM_PLAN_COST) and learned planners (learned.cpp):
+// est(requests, out) returns the estimated cardinality of each (query index, vertex subset).
+using BatchEstimator =
+    std::function<int(const std::vector<std::pair<uint32_t, uint64_t>> &reqs, std::vector<double> &out)>;
+int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, const BatchEstimator &est,
+                std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
+                std::vector<std::unordered_map<uint64_t, double>> &cards);
+// The independence model's cardinality of the sub-query induced by mask.
+double indep_card(const asdm_cs *cs, uint64_t mask);
 // Plans of a batch of candidate spaces from the learned estimator (learned.cpp).
 int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asdm_cs *> &css, uint32_t flags,
                   std::vector<asdm_plan *> &out);

@@ -84,59 +84,84 @@ int forward(asdm_mpnn *m, const FeatBatch &fb, std::vector<float> &y) {
 
 inline double card_from_log(float y) { return std::exp2(std::min(64.f, std::max(0.f, y))) - 1.0; }
 
-// Lock-step greedy over a batch of candidate spaces: every step estimates S | {u} for every
-// frontier vertex u of every unfinished query in one forward launch and extends each query by its
-// smallest estimate (first vertex on ties).  Singletons use the exact |C(u)|.
-int learned_orders(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asdm_cs *> &css, uint32_t flags,
-                   std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
-                   std::vector<std::unordered_map<uint64_t, double>> &cards) {
+}  // namespace
+
+// Cost-based order search over a batch of candidate spaces.  The leaf set is the plain planner's.
+// For every query a greedy order is grown from each vertex of the enumerated part: each step adds
+// the frontier vertex whose extended prefix has the smallest estimated cardinality.  The order
+// whose summed prefix cardinalities -- the estimated partial embeddings the engine materializes --
+// is smallest wins (the subsetToCard-driven choice of plan/optimizer.cpp:7-200, over greedy
+// orders).  All greedy runs of all queries advance in lock step, with one estimator call per step
+// for every subset they ask about.  Singletons use the exact |C(u)|.
+int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, const BatchEstimator &est,
+                std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
+                std::vector<std::unordered_map<uint64_t, double>> &cards) {
     const size_t Q = css.size();
     orders.assign(Q, {});
     Ds.assign(Q, 0);
     cards.assign(Q, {});
-    std::vector<uint64_t> rest(Q), S(Q), leaves(Q);
+    struct Run {
+        uint32_t q;
+        uint64_t S;
+        std::vector<uint32_t> order;
+        double cost;
+    };
+    std::vector<Run> runs;
+    std::vector<uint64_t> rest(Q), leaves(Q);
     for (size_t i = 0; i < Q; ++i) {
         const asdm_cs *cs = css[i];
         const uint64_t all = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
         leaves[i] = choose_leaves(cs, flags);
         rest[i] = all & ~leaves[i];
-        const uint32_t first = smallest_candidates(cs, rest[i]);
-        orders[i].push_back(first);
-        S[i] = 1ull << first;
-        cards[i][S[i]] = cs->csize[first];
+        for (uint64_t r = rest[i]; r; r &= r - 1) {
+            const uint32_t u = (uint32_t)__builtin_ctzll(r);
+            runs.push_back({(uint32_t)i, 1ull << u, {u}, (double)cs->csize[u]});
+            cards[i][1ull << u] = cs->csize[u];
+        }
     }
-    std::vector<std::pair<uint32_t, uint32_t>> who;  // (query, vertex) of each estimated subset
-    std::vector<float> y;
+    std::vector<std::pair<uint32_t, uint64_t>> reqs;  // (query, subset) asked this step
+    std::vector<std::pair<size_t, uint32_t>> who;     // (run, vertex) of each request
+    std::vector<double> y;
     for (;;) {
-        FeatBatch fb;
+        reqs.clear();
         who.clear();
-        for (size_t i = 0; i < Q; ++i) {
-            if (S[i] == rest[i]) continue;
-            for (uint64_t r = frontier_of(css[i], S[i], rest[i]); r; r &= r - 1) {
-                const uint32_t u = (uint32_t)__builtin_ctzll(r);
-                featurize(g, css[i], S[i] | (1ull << u), fb);
-                who.emplace_back((uint32_t)i, u);
+        for (size_t k = 0; k < runs.size(); ++k) {
+            Run &r = runs[k];
+            if (r.S == rest[r.q]) continue;
+            for (uint64_t f = frontier_of(css[r.q], r.S, rest[r.q]); f; f &= f - 1) {
+                const uint32_t u = (uint32_t)__builtin_ctzll(f);
+                reqs.emplace_back(r.q, r.S | (1ull << u));
+                who.emplace_back(k, u);
             }
         }
-        if (who.empty()) break;
-        if (int rc = forward(m, fb, y)) return rc;
-        std::vector<uint32_t> best(Q, UINT32_MAX);
-        std::vector<double> bestc(Q, 0);
+        if (reqs.empty()) break;
+        if (int rc = est(reqs, y)) return rc;
+        std::vector<uint32_t> best(runs.size(), UINT32_MAX);
+        std::vector<double> bestc(runs.size(), 0);
         for (size_t k = 0; k < who.size(); ++k) {
-            const uint32_t i = who[k].first, u = who[k].second;
-            const double c = card_from_log(y[k]);
-            cards[i][S[i] | (1ull << u)] = c;
-            if (best[i] == UINT32_MAX || c < bestc[i]) {
-                best[i] = u;
-                bestc[i] = c;
+            const size_t ri = who[k].first;
+            const double c = y[k];
+            cards[reqs[k].first][reqs[k].second] = c;
+            if (best[ri] == UINT32_MAX || c < bestc[ri]) {
+                best[ri] = who[k].second;
+                bestc[ri] = c;
             }
         }
-        for (size_t i = 0; i < Q; ++i) {
-            if (best[i] == UINT32_MAX) continue;
-            orders[i].push_back(best[i]);
-            S[i] |= 1ull << best[i];
+        for (size_t k = 0; k < runs.size(); ++k) {
+            if (best[k] == UINT32_MAX) continue;
+            runs[k].order.push_back(best[k]);
+            runs[k].S |= 1ull << best[k];
+            runs[k].cost += bestc[k];
         }
     }
+    std::vector<double> bestcost(Q, 0);
+    std::vector<char> have(Q, 0);
+    for (const Run &r : runs)  // first run (smallest start vertex) on ties
+        if (!have[r.q] || r.cost < bestcost[r.q]) {
+            have[r.q] = 1;
+            bestcost[r.q] = r.cost;
+            orders[r.q] = r.order;
+        }
     for (size_t i = 0; i < Q; ++i) {
         Ds[i] = (uint32_t)orders[i].size();
         for (uint64_t l = leaves[i]; l; l &= l - 1) orders[i].push_back((uint32_t)__builtin_ctzll(l));
@@ -144,7 +169,18 @@ int learned_orders(asdm_mpnn *m, const asdm_graph *g, const std::vector<const as
     return ASDM_OK;
 }
 
-}  // namespace
+// The independence model's cardinality of the sub-query induced by mask (the plain planner's
+// estimate): prod_u |C(u)| * prod over edges (u, w) of W(u->w) / (|C(u)| |C(w)|).
+double indep_card(const asdm_cs *cs, uint64_t mask) {
+    double e = 1.0;
+    uint64_t S = 0;
+    for (uint64_t r = mask; r; r &= r - 1) {
+        const uint32_t u = (uint32_t)__builtin_ctzll(r);
+        e *= growth(cs, S, u);
+        S |= 1ull << u;
+    }
+    return e;
+}
 
 // Plans for a batch of candidate spaces from the learned estimates; a query whose learned plan
 // cannot be built (or every query, when the network cannot run) gets the plain plan.
@@ -156,7 +192,16 @@ int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asd
     std::vector<std::vector<uint32_t>> orders;
     std::vector<uint32_t> Ds;
     std::vector<std::unordered_map<uint64_t, double>> cards;
-    const bool have = m && learned_orders(m, g, css, flags, orders, Ds, cards) == ASDM_OK;
+    BatchEstimator learned = [&](const std::vector<std::pair<uint32_t, uint64_t>> &reqs, std::vector<double> &out) {
+        FeatBatch fb;
+        for (const auto &r : reqs) featurize(g, css[r.first], r.second, fb);
+        std::vector<float> y;
+        if (int rc = forward(m, fb, y)) return rc;
+        out.resize(reqs.size());
+        for (size_t k = 0; k < reqs.size(); ++k) out[k] = card_from_log(y[k]);
+        return ASDM_OK;
+    };
+    const bool have = m && cost_orders(css, flags, learned, orders, Ds, cards) == ASDM_OK;
     const double share = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
                          std::max<size_t>(Q, 1);
     for (size_t i = 0; i < Q; ++i) {

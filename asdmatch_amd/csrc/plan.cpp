@@ -223,6 +223,20 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
         }
         if (D == nq) D = nq - 1 > 0 ? nq - 1 : 1;
         if (nq == 1) D = 1;
+    } else if (flags & ASDM_PLAN_COST) {
+        // multi-start cost-based search with the independence model's cardinalities
+        std::vector<std::vector<uint32_t>> orders;
+        std::vector<uint32_t> Ds;
+        std::vector<std::unordered_map<uint64_t, double>> cards;
+        BatchEstimator plain = [cs](const std::vector<std::pair<uint32_t, uint64_t>> &reqs, std::vector<double> &o) {
+            o.resize(reqs.size());
+            for (size_t k = 0; k < reqs.size(); ++k) o[k] = indep_card(cs, reqs[k].second);
+            return ASDM_OK;
+        };
+        int rc = cost_orders({cs}, flags, plain, orders, Ds, cards);
+        if (rc) return rc;
+        order = orders[0];
+        D = Ds[0];
     } else {
         const uint64_t all = nq == 64 ? ~0ull : ((1ull << nq) - 1);
         const uint64_t leaves = choose_leaves(cs, flags);

@@ -132,6 +132,7 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
 /* Count only the last vertex in closed form (no multi-leaf factorization): a plain enumeration
  * used to cross-check the leaf factorization. */
 #define ASDM_PLAN_NO_FACTOR 2u
+#define ASDM_PLAN_COST 4u      /* multi-start cost-based order search (independence-model cardinalities) */
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
 /* Plan with an explicit order whose first num_enumerated vertices are enumerated and the rest are
