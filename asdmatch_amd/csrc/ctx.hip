@@ -273,7 +273,7 @@ int asdm_ctx_create(int device, asdm_ctx **out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) return ASDM_E_HIP;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return ASDM_E_HIP;
-    if (hipMalloc((void **)&ctx->d_counters, sizeof(uint64_t) * 8) != hipSuccess) return ASDM_E_OOM;
+    if (hipMalloc((void **)&ctx->d_counters, sizeof(uint64_t) * kNumCounters) != hipSuccess) return ASDM_E_OOM;
     *out = ctx.release();
     return ASDM_OK;
 }

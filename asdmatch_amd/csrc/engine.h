@@ -14,6 +14,8 @@
 
 namespace asdm {
 
+constexpr int kNumCounters = 32;  // device counters per context
+
 constexpr uint32_t kMaxQ = 32;      // MAX_PATTERN_SIZE (config/config.h:31)
 constexpr uint32_t kMaxGroup = 4;   // largest same-label leaf group counted by inclusion-exclusion
 constexpr uint32_t kMaxParts = 15;  // Bell(4)
@@ -84,8 +86,8 @@ struct asdm_ctx {
         hipEvent_t a, b;
     };
     std::vector<PendingTimer> pending;  // recorded, resolved after the stream has drained
-    uint64_t *d_counters = nullptr;  // [8] device counters: 0 count, 1..3 algorithmic bytes of
-                                     // expand_count, expand_fill, leaf kernels
+    uint64_t *d_counters = nullptr;  // [kNumCounters] device counters: 0 count, then algorithmic
+                                     // bytes per kernel (single engine: 1..3; batch engine: 1..16)
     // caching allocator: freed blocks are kept for reuse (all work is ordered on `stream`, so a
     // block freed by one launch sequence may be handed to the next one immediately)
     struct Block {

@@ -769,10 +769,16 @@ __device__ inline uint32_t seg_of(const uint64_t *__restrict__ seg_start, uint32
 // what its own row function needs (a kernel holding all three paths needs twice the registers of
 // the largest and runs at half the occupancy).
 constexpr int kExpandRows = 0, kFusedRows = 1, kLeafRows = 2;
-// Kernel-table names and algorithmic-byte counter slots (offsets from d_counters + 1) per kind.
-constexpr const char *kKindTimer[3] = {"batch_count_expand", "batch_count_fused", "batch_count_leaf"};
-constexpr int kKindBytesSlot[3] = {0, 2, 3};
-constexpr int kFillBytesSlot = 1;
+// Kernel-table names and algorithmic-byte counter slots (offsets from d_counters + 1) of the count
+// launches, one per (row kind, lane width), so that each entry is one kernel instantiation.
+constexpr const char *kKindTimer[3][4] = {
+    {"batch_count_expand_g1", "batch_count_expand_g4", "batch_count_expand_g16", "batch_count_expand_g64"},
+    {"batch_count_fused_g1", "batch_count_fused_g4", "batch_count_fused_g16", "batch_count_fused_g64"},
+    {"batch_count_leaf_g1", "batch_count_leaf_g4", "batch_count_leaf_g16", "batch_count_leaf_g64"}};
+constexpr const char *kFillTimer[4] = {"batch_fill_g1", "batch_fill_g4", "batch_fill_g16", "batch_fill_g64"};
+inline int lane_index(uint32_t G) { return G == 1 ? 0 : G == 4 ? 1 : G == 16 ? 2 : 3; }
+inline int count_slot(int kind, uint32_t G) { return kind * 4 + lane_index(G); }  // 0..11
+inline int fill_slot(uint32_t G) { return 12 + lane_index(G); }                    // 12..15
 
 // Wave tiles.  A launch covers the rows of queries [qa, qb); each query's rows are cut into tiles
 // of tile_rows consecutive rows, and tile_pref[i] is the number of tiles before query i.  A wave
@@ -987,7 +993,7 @@ struct Engine {
             cap[d] = c;
         }
         if ((rc = ensure_rows(1))) return rc;
-        rc = fill_u64(ctx, ctx->d_counters, 8, 0);
+        rc = fill_u64(ctx, ctx->d_counters, kNumCounters, 0);
         d_total = (unsigned long long *)ctx->d_counters;
         d_bytes = (unsigned long long *)ctx->d_counters + 1;
         return rc;
@@ -1434,7 +1440,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     for (uint32_t i = 0; i < Q; ++i) seg_q_h[i] = i;
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(seg_q, seg_q_h.data(), 4 * Q, hipMemcpyHostToDevice, ctx->stream), "H2D");
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(seg_start, seg_start_h.data(), 8 * (Q + 1), hipMemcpyHostToDevice, ctx->stream), "H2D");
-    if (!rc) rc = fill_u64(ctx, ctx->d_counters, 8, 0);
+    if (!rc) rc = fill_u64(ctx, ctx->d_counters, kNumCounters, 0);
     unsigned long long *qcount = nullptr;
     if (!rc) rc = dev_alloc(ctx, (void **)&qcount, 8 * Q);
     if (!rc) rc = fill_u64(ctx, (uint64_t *)qcount, Q, 0);
@@ -1503,8 +1509,8 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                     const uint64_t tiles = tp_count_h[qb] - tp_count_h[qa];
                     auto go = [&](auto kind) {
                         constexpr int K = decltype(kind)::value;
-                        unsigned long long *kb = d_bytes + kKindBytesSlot[K];
-                        KTimer kt(ctx, kKindTimer[K]);
+                        unsigned long long *kb = d_bytes + count_slot(K, c);
+                        KTimer kt(ctx, kKindTimer[K][lane_index(c)]);
                         by_class(c,
                                  [&] { launch_bcount<1, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, kb); },
                                  [&] { launch_bcount<4, K>(ctx, dplans, seg_start, tp_count, qa, qb, tiles, d, rows, st, counts_d, qcount, kb); },
@@ -1564,8 +1570,8 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                 [&](uint32_t i) { return qcls[i]; }, finished,
                 [&](uint32_t c, uint32_t qa, uint32_t qb) {
                     const uint64_t tiles = tp_fill_h[qb] - tp_fill_h[qa];
-                    unsigned long long *fb = d_bytes + kFillBytesSlot;
-                    KTimer kt(ctx, "batch_fill");
+                    unsigned long long *fb = d_bytes + fill_slot(c);
+                    KTimer kt(ctx, kFillTimer[lane_index(c)]);
                     by_class(c,
                              [&] { launch_bfill<1>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, fb); },
                              [&] { launch_bfill<4>(ctx, dplans, seg_start, tp_fill, qa, qb, tiles, d, rows, st, offs, next_rows, fb); },
@@ -1582,13 +1588,16 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         n = total;
     }
     std::vector<uint64_t> qc(Q);
-    uint64_t kbytes[8] = {0};
+    uint64_t kbytes[kNumCounters] = {0};
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(kbytes, ctx->d_counters, 8 * 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(kbytes, ctx->d_counters, 8 * kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-    if (!rc) {
-        for (int k = 0; k < 3; ++k) add_kernel_bytes(ctx, kKindTimer[k], kbytes[1 + kKindBytesSlot[k]]);
-        add_kernel_bytes(ctx, "batch_fill", kbytes[1 + kFillBytesSlot]);
+    if (!rc && ctx->profiling) {
+        for (int k = 0; k < 3; ++k)
+            for (int gi = 0; gi < 4; ++gi)
+                if (kbytes[1 + k * 4 + gi]) add_kernel_bytes(ctx, kKindTimer[k][gi], kbytes[1 + k * 4 + gi]);
+        for (int gi = 0; gi < 4; ++gi)
+            if (kbytes[13 + gi]) add_kernel_bytes(ctx, kFillTimer[gi], kbytes[13 + gi]);
     }
     if (!rc)
         for (uint32_t i = 0; i < Q; ++i) {
@@ -1629,8 +1638,8 @@ int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double 
     return ASDM_OK;
 }
 
-// Autotuner of the batch launch shape: every shape of a small grid runs the caller's batch `reps`
-// times after one untimed run; the shape with the smallest median time is kept on the context.
+// Autotuner of the batch launch shape: after one untimed run of the caller's batch, every shape of
+// a small grid runs it `reps` times; the shape with the smallest median time is kept on the context.
 // Every run's counts must equal the first run's (they are plan- and shape-independent).
 int asdm_batch_autotune(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
                         uint64_t budget_bytes, asdm_mpnn *m, uint32_t reps, uint32_t *leaf_lanes, double *class_scale,
@@ -1652,10 +1661,10 @@ int asdm_batch_autotune(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *co
                 ctx->tune_class_scale = sc;
                 ctx->tune_tile_iters = ti;
                 std::vector<double> t;
-                for (uint32_t r = 0; r <= reps && !rc; ++r) {
+                for (uint32_t r = 0; r < reps && !rc; ++r) {
                     auto t0 = std::chrono::steady_clock::now();
                     rc = asdm_batch_count_learned(ctx, g, qs, nq, budget_bytes, m, cur.data(), nullptr, nullptr);
-                    if (r) t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+                    t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
                     if (!rc && cur != ref) rc = ctx->fail(ASDM_E_INVALID, "autotune: counts differ between launch shapes");
                 }
                 if (rc) break;

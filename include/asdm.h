@@ -204,7 +204,7 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
 int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_scale, uint32_t tile_iters);
 int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale,
                               uint32_t *tile_iters);
-/* Autotuner: runs the batch once untimed, then `reps` timed runs for each shape of the grid leaf
+/* Autotuner: runs the batch once untimed, then `reps` timed runs (at least one) for each shape of the grid leaf
  * lanes {4, 16, 64} x class factor {0.25, 0.5, 1, 2} x tile rows {2, 4, 8}; keeps the shape with the
  * smallest median time on the context and returns it (ms_best: that median).  Fails with
  * ASDM_E_INVALID if any run's counts differ from the first run's. */
