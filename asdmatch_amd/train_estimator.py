@@ -104,8 +104,22 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--data-seconds", type=float, default=600.0,
                     help="time budget of pair generation, split across the sources by their share")
+    ap.add_argument("--eval-only", default=None, metavar="WEIGHTS",
+                    help="only evaluate these weights on config 4 (admitted queries and their sub-queries)")
     a = ap.parse_args(argv)
     ctx = Context(0)
+    if a.eval_only:
+        m = Mpnn(ctx)
+        m.load(a.eval_only)
+        c4_full, c4_sub = c4_eval(ctx, m, np.random.default_rng(a.seed + 1))
+        rep = {"weights": os.path.basename(a.eval_only), "c4_admitted_full": c4_full, "c4_subqueries": c4_sub}
+        log(json.dumps(rep, indent=1))
+        if a.report:
+            with open(a.report, "w") as f:
+                json.dump(rep, f, indent=1)
+        del m
+        ctx.close()
+        return
     t0 = time.perf_counter()
     tr, te = collect(ctx, a.pairs, a.seed, a.data_seconds)
     t_data = time.perf_counter() - t0
