@@ -432,11 +432,12 @@ class CandSpace:
         return Plan(h, self.nq)
 
 
-    def plan_learned(self, q: Graph, g: Graph, mpnn: "Mpnn", local: bool = False) -> "Plan":
-        """Plan ordered greedily by the learned estimator's cardinalities."""
+    def plan_learned(self, q: Graph, g: Graph, mpnn: "Mpnn", local: bool = False, cost: bool = False) -> "Plan":
+        """Plan ordered greedily by the learned estimator's cardinalities (cost: the best greedy
+        order over all start vertices by summed estimated prefix cardinalities)."""
         h = _P()
-        rc = _lib.asdm_plan_create_learned(mpnn._h, g._h, q._h, self._h, ASDM_PLAN_LOCAL if local else 0,
-                                           C.byref(h))
+        flags = (ASDM_PLAN_LOCAL if local else 0) | (ASDM_PLAN_COST if cost else 0)
+        rc = _lib.asdm_plan_create_learned(mpnn._h, g._h, q._h, self._h, flags, C.byref(h))
         if rc != ASDM_OK:
             raise AsdmError(rc, "plan_create_learned")
         return Plan(h, self.nq)

@@ -188,7 +188,7 @@ int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint
 // est(requests, out) returns the estimated cardinality of each (query index, vertex subset).
 using BatchEstimator =
     std::function<int(const std::vector<std::pair<uint32_t, uint64_t>> &reqs, std::vector<double> &out)>;
-int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, const BatchEstimator &est,
+int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, bool multi_start, const BatchEstimator &est,
                 std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
                 std::vector<std::unordered_map<uint64_t, double>> &cards);
 // The independence model's cardinality of the sub-query induced by mask.

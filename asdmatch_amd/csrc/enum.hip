@@ -678,6 +678,25 @@ __device__ inline uint64_t single_leaf_lane(const DPlan *__restrict__ P, uint32_
     return leaf_set_count_lane(P, g, 1u, R, ce_off, ce_val, cand, y, ns);
 }
 
+// Up to kRowLeaves leaf groups whose leaf has no matched neighbour at depth D-1 are counted once per
+// row instead of once per candidate x: their lists do not depend on x, and x can only matter to
+// them as a same-label vertex, which one membership test per candidate corrects.
+constexpr uint32_t kRowLeaves = 4;
+
+// Is data vertex xid in the candidate list set S_e of the single-leaf group g?
+__device__ inline bool in_leaf_lists(const DPlan *__restrict__ P, uint32_t g, const RowView &R,
+                                     const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                     uint32_t xid) {
+    const uint32_t e = P->gleaf[g][0];
+    const uint32_t pos = cand_pos(P, e, xid);
+    bool in = pos != 0xffffffffu;
+    for (uint32_t j = 0; j < P->nb[e] && in; ++j) {
+        const ListRef r = list_of_v(P, ce_off, R, e, j);
+        in = contains_u32(ce_val + r.lo, r.len, pos);
+    }
+    return in;
+}
+
 // This lane's share of a row of width D-1: the sum over its depth-(D-1) candidates x (lanes of
 // the group split the row's candidate list) of the leaf product of (row, x).  Not reduced.
 template <int G>
@@ -700,6 +719,30 @@ __device__ inline uint64_t fused_row(const DPlan *__restrict__ P, const uint32_t
     SameLabel S;
     load_same(S, P, cand, row, d);
     const uint32_t *probe = ce_val + pr.lo;
+    // row-level counts of the leaf groups not adjacent to depth d (every lane computes the same)
+    uint64_t hc[kRowLeaves];
+    uint32_t hg[kRowLeaves], hsame = 0, nh = 0, hoisted = 0;
+    if (pr.len) {
+        const RowView R0{row, d, 0};  // depth d is read by none of these leaves' lists
+        for (uint32_t g = 0; g < ng && nh < kRowLeaves; ++g) {
+            const uint32_t e = P->gleaf[g][0];
+            bool adj = false;
+            for (uint32_t j = 0; j < P->nb[e]; ++j) adj |= P->bdep[e][j] == d;
+            if (adj) continue;
+            uint32_t y[kFusedSame], ns = 0;
+            bool same_d = false;
+            for (uint32_t s2 = 0; s2 < P->ns[e]; ++s2) {
+                const uint32_t dep = P->sdep[e][s2];
+                if (dep == d) same_d = true;
+                else y[ns++] = matched_id(P, cand, R0, dep);
+            }
+            hc[nh] = leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+            hg[nh] = g;
+            hsame |= (same_d ? 1u : 0u) << nh;
+            hoisted |= 1u << g;
+            ++nh;
+        }
+    }
     uint64_t acc = 0;
     for (uint32_t t = lg; t < pr.len; t += G) {
         const uint32_t x = probe[t];
@@ -709,11 +752,18 @@ __device__ inline uint64_t fused_row(const DPlan *__restrict__ P, const uint32_t
             ListRef r = list_of(P, ce_off, row, d, j);
             ok = contains_u32(ce_val + r.lo, r.len, x);
         }
-        if (ok) ok = injective(S, cand, cand[cand_d + x]);
+        const uint32_t xid = cand[cand_d + x];
+        if (ok) ok = injective(S, cand, xid);
         if (!ok) continue;
         const RowView R{row, d, x};
         uint64_t prod = 1;
-        for (uint32_t g = 0; g < ng && prod; ++g) prod *= single_leaf_lane(P, g, R, ce_off, ce_val, cand);
+        for (uint32_t k = 0; k < nh && prod; ++k) {
+            uint64_t c = hc[k];
+            if ((hsame >> k & 1) && c && in_leaf_lists(P, hg[k], R, ce_off, ce_val, xid)) --c;
+            prod *= c;
+        }
+        for (uint32_t g = 0; g < ng && prod; ++g)
+            if (!(hoisted >> g & 1)) prod *= single_leaf_lane(P, g, R, ce_off, ce_val, cand);
         acc += prod;
         bytes += 4ull + 16ull * P->nadj_last;
     }

@@ -87,13 +87,13 @@ inline double card_from_log(float y) { return std::exp2(std::min(64.f, std::max(
 }  // namespace
 
 // Cost-based order search over a batch of candidate spaces.  The leaf set is the plain planner's.
-// For every query a greedy order is grown from each vertex of the enumerated part: each step adds
-// the frontier vertex whose extended prefix has the smallest estimated cardinality.  The order
-// whose summed prefix cardinalities -- the estimated partial embeddings the engine materializes --
-// is smallest wins (the subsetToCard-driven choice of plan/optimizer.cpp:7-200, over greedy
-// orders).  All greedy runs of all queries advance in lock step, with one estimator call per step
+// For every query a greedy order is grown from the plain planner's first vertex (the smallest
+// candidate set) or, with multi_start, from each vertex of the enumerated part: each step adds the
+// frontier vertex whose extended prefix has the smallest estimated cardinality.  The order whose
+// summed prefix cardinalities -- the estimated partial embeddings the engine materializes -- is
+// smallest wins (the subsetToCard-driven choice of plan/optimizer.cpp:7-200, over greedy orders).  All greedy runs of all queries advance in lock step, with one estimator call per step
 // for every subset they ask about.  Singletons use the exact |C(u)|.
-int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, const BatchEstimator &est,
+int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, bool multi_start, const BatchEstimator &est,
                 std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
                 std::vector<std::unordered_map<uint64_t, double>> &cards) {
     const size_t Q = css.size();
@@ -113,7 +113,8 @@ int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, const B
         const uint64_t all = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
         leaves[i] = choose_leaves(cs, flags);
         rest[i] = all & ~leaves[i];
-        for (uint64_t r = rest[i]; r; r &= r - 1) {
+        const uint64_t starts = multi_start ? rest[i] : (1ull << smallest_candidates(cs, rest[i]));
+        for (uint64_t r = starts; r; r &= r - 1) {
             const uint32_t u = (uint32_t)__builtin_ctzll(r);
             runs.push_back({(uint32_t)i, 1ull << u, {u}, (double)cs->csize[u]});
             cards[i][1ull << u] = cs->csize[u];
@@ -201,7 +202,7 @@ int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asd
         for (size_t k = 0; k < reqs.size(); ++k) out[k] = card_from_log(y[k]);
         return ASDM_OK;
     };
-    const bool have = m && cost_orders(css, flags, learned, orders, Ds, cards) == ASDM_OK;
+    const bool have = m && cost_orders(css, flags, (flags & ASDM_PLAN_COST) != 0, learned, orders, Ds, cards) == ASDM_OK;
     const double share = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
                          std::max<size_t>(Q, 1);
     for (size_t i = 0; i < Q; ++i) {

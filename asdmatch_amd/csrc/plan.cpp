@@ -233,7 +233,7 @@ int build_plan(const asdm_graph *q, const asdm_cs *cs, const uint32_t *given_ord
             for (size_t k = 0; k < reqs.size(); ++k) o[k] = indep_card(cs, reqs[k].second);
             return ASDM_OK;
         };
-        int rc = cost_orders({cs}, flags, plain, orders, Ds, cards);
+        int rc = cost_orders({cs}, flags, true, plain, orders, Ds, cards);
         if (rc) return rc;
         order = orders[0];
         D = Ds[0];

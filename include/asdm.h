@@ -248,7 +248,9 @@ int asdm_mpnn_estimate(asdm_mpnn *m, const asdm_graph *g, const asdm_cs *cs, con
  * prod_u |C(u)| * prod over sub-query edges (u, w) of W(u->w) / (|C(u)| |C(w)|). */
 int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, double *cards);
 /* Plan whose enumeration order is chosen greedily by the learned estimates (the plain planner's
- * leaf set and first vertex; the plain estimator when the network cannot run). */
+ * leaf set and first vertex; with ASDM_PLAN_COST in flags, the best of the greedy orders grown from
+ * every enumerated vertex by summed estimated prefix cardinalities).  The plain planner is used when
+ * the network cannot run. */
 int asdm_plan_create_learned(asdm_mpnn *m, const asdm_graph *g, const asdm_graph *q, const asdm_cs *cs,
                              uint32_t flags, asdm_plan **out);
 
