@@ -697,6 +697,94 @@ __device__ inline bool in_leaf_lists(const DPlan *__restrict__ P, uint32_t g, co
     return in;
 }
 
+// The leaf groups of a fused plan that are counted once per row: those whose leaf has no matched
+// neighbour at the last enumerated depth d (plan-level, the same for every row).
+struct HoistedGroups {
+    uint32_t g[kRowLeaves];
+    uint32_t n, mask, same;  // count, bit per group, bit k: depth d shares the label of group g[k]
+};
+
+__device__ inline HoistedGroups hoisted_groups(const DPlan *__restrict__ P, uint32_t d) {
+    HoistedGroups H{};
+    for (uint32_t g = 0; g < P->ngroups && H.n < kRowLeaves; ++g) {
+        const uint32_t e = P->gleaf[g][0];
+        bool adj = false, same_d = false;
+        for (uint32_t j = 0; j < P->nb[e]; ++j) adj |= P->bdep[e][j] == d;
+        if (adj) continue;
+        for (uint32_t s2 = 0; s2 < P->ns[e]; ++s2) same_d |= P->sdep[e][s2] == d;
+        H.g[H.n] = g;
+        H.same |= (same_d ? 1u : 0u) << H.n;
+        H.mask |= 1u << g;
+        ++H.n;
+    }
+    return H;
+}
+
+// Per-row state of a fused row: the probe list at depth d and the hoisted groups' counts, with
+// depth d left out of their same-label sets.
+struct FusedRow {
+    ListRef pr;
+    uint32_t pj;
+    uint64_t hc[kRowLeaves];  // entries beyond the hoisted count stay 0
+};
+
+__device__ inline void fused_setup(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t d,
+                                   const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
+                                   const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                   FusedRow &F) {
+    const uint32_t nb = P->nb[d];
+    F.pj = 0;
+    F.pr = list_of(P, ce_off, row, d, 0);
+    for (uint32_t j = 1; j < nb; ++j) {
+        const ListRef r = list_of(P, ce_off, row, d, j);
+        if (r.len < F.pr.len) {
+            F.pr = r;
+            F.pj = j;
+        }
+    }
+    const RowView R0{row, d, 0};  // depth d is read by none of the hoisted leaves' lists
+    for (uint32_t k = 0; k < H.n; ++k) {
+        const uint32_t g = H.g[k], e = P->gleaf[g][0];
+        uint32_t y[kFusedSame], ns = 0;
+        for (uint32_t s2 = 0; s2 < P->ns[e]; ++s2) {
+            const uint32_t dep = P->sdep[e][s2];
+            if (dep != d) y[ns++] = matched_id(P, cand, R0, dep);
+        }
+        F.hc[k] = F.pr.len ? leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns) : 0;
+    }
+}
+
+// Leaf product of (row, x) for candidate x (a position in C(u_d) from the probe list), or 0 when x
+// is not a valid extension (missing from another backward list, or a same-label matched vertex).
+__device__ inline uint64_t fused_candidate(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t d,
+                                           uint32_t pj, const uint64_t *hc, const HoistedGroups &H,
+                                           const SameLabel &S, uint32_t x, const uint64_t *__restrict__ ce_off,
+                                           const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                           uint64_t &bytes) {
+    bool ok = true;
+    for (uint32_t j = 0; j < P->nb[d] && ok; ++j) {
+        if (j == pj) continue;
+        const ListRef r = list_of(P, ce_off, row, d, j);
+        ok = contains_u32(ce_val + r.lo, r.len, x);
+    }
+    const uint32_t xid = cand[P->coff[d] + x];
+    if (ok) ok = injective(S, cand, xid);
+    if (!ok) return 0;
+    const RowView R{row, d, x};
+    uint64_t prod = 1;
+#pragma unroll
+    for (uint32_t k = 0; k < kRowLeaves; ++k) {
+        if (k >= H.n || !prod) continue;
+        uint64_t c = hc[k];
+        if ((H.same >> k & 1) && c && in_leaf_lists(P, H.g[k], R, ce_off, ce_val, xid)) --c;
+        prod *= c;
+    }
+    for (uint32_t g = 0; g < P->ngroups && prod; ++g)
+        if (!(H.mask >> g & 1)) prod *= single_leaf_lane(P, g, R, ce_off, ce_val, cand);
+    bytes += 4ull + 16ull * P->nadj_last;
+    return prod;
+}
+
 // This lane's share of a row of width D-1: the sum over its depth-(D-1) candidates x (lanes of
 // the group split the row's candidate list) of the leaf product of (row, x).  Not reduced.
 template <int G>
@@ -704,71 +792,79 @@ __device__ inline uint64_t fused_row(const DPlan *__restrict__ P, const uint32_t
                                      const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                                      const uint32_t *__restrict__ cand, uint64_t &bytes) {
     const int lg = threadIdx.x & (G - 1);
-    const uint32_t d = P->D - 1, ng = P->ngroups;
-    const uint32_t nb = P->nb[d];
-    const uint64_t cand_d = P->coff[d];
-    uint32_t pj = 0;
-    ListRef pr = list_of(P, ce_off, row, d, 0);
-    for (uint32_t j = 1; j < nb; ++j) {
-        ListRef r = list_of(P, ce_off, row, d, j);
-        if (r.len < pr.len) {
-            pr = r;
-            pj = j;
-        }
-    }
+    const uint32_t d = P->D - 1;
+    const HoistedGroups H = hoisted_groups(P, d);
+    FusedRow F{};
+    fused_setup(P, row, d, H, ce_off, ce_val, cand, F);
     SameLabel S;
     load_same(S, P, cand, row, d);
-    const uint32_t *probe = ce_val + pr.lo;
-    // row-level counts of the leaf groups not adjacent to depth d (every lane computes the same)
-    uint64_t hc[kRowLeaves];
-    uint32_t hg[kRowLeaves], hsame = 0, nh = 0, hoisted = 0;
-    if (pr.len) {
-        const RowView R0{row, d, 0};  // depth d is read by none of these leaves' lists
-        for (uint32_t g = 0; g < ng && nh < kRowLeaves; ++g) {
-            const uint32_t e = P->gleaf[g][0];
-            bool adj = false;
-            for (uint32_t j = 0; j < P->nb[e]; ++j) adj |= P->bdep[e][j] == d;
-            if (adj) continue;
-            uint32_t y[kFusedSame], ns = 0;
-            bool same_d = false;
-            for (uint32_t s2 = 0; s2 < P->ns[e]; ++s2) {
-                const uint32_t dep = P->sdep[e][s2];
-                if (dep == d) same_d = true;
-                else y[ns++] = matched_id(P, cand, R0, dep);
-            }
-            hc[nh] = leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
-            hg[nh] = g;
-            hsame |= (same_d ? 1u : 0u) << nh;
-            hoisted |= 1u << g;
-            ++nh;
-        }
-    }
     uint64_t acc = 0;
-    for (uint32_t t = lg; t < pr.len; t += G) {
-        const uint32_t x = probe[t];
-        bool ok = true;
-        for (uint32_t j = 0; j < nb && ok; ++j) {
-            if (j == pj) continue;
-            ListRef r = list_of(P, ce_off, row, d, j);
-            ok = contains_u32(ce_val + r.lo, r.len, x);
-        }
-        const uint32_t xid = cand[cand_d + x];
-        if (ok) ok = injective(S, cand, xid);
-        if (!ok) continue;
-        const RowView R{row, d, x};
-        uint64_t prod = 1;
-        for (uint32_t k = 0; k < nh && prod; ++k) {
-            uint64_t c = hc[k];
-            if ((hsame >> k & 1) && c && in_leaf_lists(P, hg[k], R, ce_off, ce_val, xid)) --c;
-            prod *= c;
-        }
-        for (uint32_t g = 0; g < ng && prod; ++g)
-            if (!(hoisted >> g & 1)) prod *= single_leaf_lane(P, g, R, ce_off, ce_val, cand);
-        acc += prod;
-        bytes += 4ull + 16ull * P->nadj_last;
+    for (uint32_t t = lg; t < F.pr.len; t += G)
+        acc += fused_candidate(P, row, d, F.pj, F.hc, H, S, ce_val[F.pr.lo + t], ce_off, ce_val, cand, bytes);
+    bytes += 4ull * ((F.pr.len + G - 1 - lg) / G);
+    if (lg == 0) bytes += 4ull * d + 16ull * P->nb[d] + 16ull * (P->ngroups - P->nadj_last);
+    return acc;
+}
+
+// Fused rows of one query tile (at most 64 rows, one per lane), flattened: lane k sets up row k,
+// a wave scan lays the rows' probe lists end to end, and the wave then takes 64 consecutive
+// candidates at a time whatever rows they belong to, reading each row's state from its lane.  A
+// tile of rows with short lists keeps every lane busy, and a hub row's list is split across the
+// whole wave.
+__device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows, uint64_t r0,
+                                      uint32_t nrows, uint32_t d, const uint64_t *__restrict__ ce_off,
+                                      const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                      uint32_t *__restrict__ counts, uint64_t &bytes) {
+    const int lane = threadIdx.x & 63;
+    const HoistedGroups H = hoisted_groups(P, d);
+    const bool mine = (uint32_t)lane < nrows;
+    const uint32_t *row = rows + (r0 + (mine ? (uint32_t)lane : 0u)) * d;
+    FusedRow F{};
+    SameLabel S{};
+    if (mine) {
+        fused_setup(P, row, d, H, ce_off, ce_val, cand, F);
+        load_same(S, P, cand, row, d);
+        counts[r0 + lane] = 0;
+        bytes += 4ull * d + 16ull * P->nb[d] + 16ull * (P->ngroups - P->nadj_last) + 4ull * F.pr.len;
     }
-    bytes += 4ull * ((pr.len + G - 1 - lg) / G);
-    if (lg == 0) bytes += 4ull * d + 16ull * nb + 16ull * (ng - P->nadj_last);
+    const uint64_t plen = mine ? F.pr.len : 0;
+    const uint64_t inc = wave_inclusive_scan(plen);
+    const uint64_t excl = inc - plen;
+    const uint64_t total = __shfl(inc, 63, 64);
+    const uint32_t ns = P->ns[d] < 8 ? P->ns[d] : 8;
+    uint64_t acc = 0;
+    for (uint64_t base = 0; base < total; base += 64) {
+        const uint64_t c = base + lane;
+        // The row whose list holds candidate c: the last k < nrows with excl_k <= c.  Six steps
+        // for every lane: a shuffle must run on the whole wave (one that reads an inactive lane
+        // returns 0), so the search cannot stop early per lane.
+        int k = 0;
+#pragma unroll
+        for (int step = 32; step; step >>= 1) {
+            const uint64_t v = __shfl(excl, (k + step) & 63, 64);
+            if ((uint32_t)(k + step) < nrows && v <= c) k += step;
+        }
+        const uint64_t plo = __shfl(F.pr.lo, k, 64);
+        const uint64_t off = __shfl(excl, k, 64);
+        const uint32_t pj = __shfl(F.pj, k, 64);
+        uint64_t hc[kRowLeaves];
+#pragma unroll
+        for (uint32_t h = 0; h < kRowLeaves; ++h) hc[h] = __shfl(F.hc[h], k, 64);
+        SameLabel Sk = S;
+#pragma unroll
+        for (uint32_t s2 = 0; s2 < 8; ++s2) {
+            const uint32_t v = __shfl(S.y[s2], k, 64);
+            Sk.y[s2] = s2 < ns ? v : 0xffffffffu;
+        }
+        if (c < total) {
+            const uint32_t *rk = rows + (r0 + (uint32_t)k) * d;
+            Sk.row = rk;
+            Sk.ns = P->ns[d];
+            Sk.sdep = P->sdep[d];
+            Sk.coff = P->coff;
+            acc += fused_candidate(P, rk, d, pj, hc, H, Sk, ce_val[plo + (c - off)], ce_off, ce_val, cand, bytes);
+        }
+    }
     return acc;
 }
 
@@ -829,6 +925,7 @@ constexpr const char *kFillTimer[4] = {"batch_fill_g1", "batch_fill_g4", "batch_
 inline int lane_index(uint32_t G) { return G == 1 ? 0 : G == 4 ? 1 : G == 16 ? 2 : 3; }
 inline int count_slot(int kind, uint32_t G) { return kind * 4 + lane_index(G); }  // 0..11
 inline int fill_slot(uint32_t G) { return 12 + lane_index(G); }                    // 12..15
+constexpr int kFlatFusedSlot = 16;
 
 // Wave tiles.  A launch covers the rows of queries [qa, qb); each query's rows are cut into tiles
 // of tile_rows consecutive rows, and tile_pref[i] is the number of tiles before query i.  A wave
@@ -926,6 +1023,44 @@ __global__ void __launch_bounds__(256) k_bfill(const DPlan *__restrict__ plans, 
             expand_fill_row<G>(P, d, rows + it * d, ce_off, ce_val, cand, offs[it], nout, out, bytes);
         }
     }
+    bytes = group_sum<64>(bytes);
+    if (lane == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
+}
+
+// Fused rows of the batch in flattened tiles of 64 rows (fused_tile); tiles as in k_bcount with
+// tile_rows = 64.
+__global__ void __launch_bounds__(256) k_bfused_flat(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
+                                                     const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
+                                                     uint32_t d, const uint32_t *__restrict__ rows,
+                                                     const uint64_t *__restrict__ ce_off,
+                                                     const uint32_t *__restrict__ ce_val,
+                                                     const uint32_t *__restrict__ cand, uint32_t *__restrict__ counts,
+                                                     unsigned long long *__restrict__ qcount,
+                                                     unsigned long long *__restrict__ bytes_ctr) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t t0 = tile_pref[qa], t1 = tile_pref[qb];
+    uint64_t bytes = 0, acc = 0;
+    uint32_t cur_q = 0xffffffffu;
+    auto flush = [&]() {
+        if (cur_q != 0xffffffffu) {
+            const uint64_t tot = group_sum<64>(acc);
+            if (lane == 0 && tot) atomicAdd(&qcount[cur_q], (unsigned long long)tot);
+        }
+        acc = 0;
+    };
+    for (uint64_t t = t0 + wave; t < t1; t += nwaves) {
+        const uint32_t q = tile_query(tile_pref, qa, qb, t);
+        if (q != cur_q) {
+            flush();
+            cur_q = q;
+        }
+        const uint64_t r0 = seg_start[q] + (t - tile_pref[q]) * 64;
+        const uint32_t nrows = (uint32_t)min<uint64_t>(64, seg_start[q + 1] - r0);
+        acc += fused_tile(plans + q, rows, r0, nrows, d, ce_off, ce_val, cand, counts, bytes);
+    }
+    flush();
     bytes = group_sum<64>(bytes);
     if (lane == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
 }
@@ -1519,11 +1654,11 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     };
     // Tile prefix of one phase: query i's rows in tiles of tile_rows_of(lanes(i)); skipped queries
     // get no tiles.  Uploaded into buf from host (kept alive until the next stream sync).
-    auto tile_prefix = [&](auto lanes, auto skip, std::vector<uint64_t> &h, uint64_t *buf) {
+    auto tile_prefix = [&](auto rows_per_tile, auto skip, std::vector<uint64_t> &h, uint64_t *buf) {
         h.assign(Q + 1, 0);
         for (uint32_t i = 0; i < Q; ++i) {
             const uint64_t r = seg_start_h[i + 1] - seg_start_h[i];
-            const uint64_t tr = tile_rows_of(lanes(i), ctx->tune_tile_iters);
+            const uint64_t tr = rows_per_tile(i);
             h[i + 1] = h[i] + (skip(i) ? 0 : (r + tr - 1) / tr);
         }
         return check_hip(ctx, hipMemcpyAsync(buf, h.data(), 8 * (Q + 1), hipMemcpyHostToDevice, ctx->stream), "H2D");
@@ -1549,14 +1684,30 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
             auto kind_of = [&](uint32_t i) {
                 return hp[i].D == d ? kLeafRows : (hp[i].fused && d + 1 == hp[i].D) ? kFusedRows : kExpandRows;
             };
-            auto lanes = [&](uint32_t i) { return kind_of(i) == kLeafRows ? ctx->tune_leaf_lanes : qcls[i]; };
-            if ((rc = tile_prefix(lanes, [](uint32_t) { return false; }, tp_count_h, tp_count))) break;
+            // fused rows run flattened (lane width 0 in the key: 64-row tiles, all lanes pooled)
+            auto lanes = [&](uint32_t i) {
+                const int k = kind_of(i);
+                return k == kLeafRows ? ctx->tune_leaf_lanes : k == kFusedRows ? 0u : qcls[i];
+            };
+            auto tile_rows = [&](uint32_t i) {
+                return lanes(i) ? (uint64_t)tile_rows_of(lanes(i), ctx->tune_tile_iters) : (uint64_t)64;
+            };
+            if ((rc = tile_prefix(tile_rows, [](uint32_t) { return false; }, tp_count_h, tp_count))) break;
             for_runs(
                 [&](uint32_t i) { return 256u * kind_of(i) + lanes(i); },
                 [](uint32_t) { return false; },
                 [&](uint32_t key, uint32_t qa, uint32_t qb) {
                     const uint32_t c = key % 256;
                     const uint64_t tiles = tp_count_h[qb] - tp_count_h[qa];
+                    if (key / 256 == kFusedRows) {
+                        unsigned long long *kb = d_bytes + kFlatFusedSlot;
+                        KTimer kt(ctx, "batch_count_fused_flat");
+                        hipLaunchKernelGGL(k_bfused_flat, dim3(tiles_grid(tiles)), dim3(256), 0, ctx->stream, dplans,
+                                           seg_start, tp_count, qa, qb, d, rows, st->d_ce_off, st->d_ce_val,
+                                           st->d_cand, counts_d, qcount, kb);
+                        kt.stop();
+                        return;
+                    }
                     auto go = [&](auto kind) {
                         constexpr int K = decltype(kind)::value;
                         unsigned long long *kb = d_bytes + count_slot(K, c);
@@ -1615,7 +1766,9 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
             // rows of leaf-depth and fused queries have no children: they get no tiles and join
             // any run
             auto finished = [&](uint32_t i) { return hp[i].D == d || (hp[i].fused && d + 1 == hp[i].D); };
-            if ((rc = tile_prefix([&](uint32_t i) { return qcls[i]; }, finished, tp_fill_h, tp_fill))) break;
+            if ((rc = tile_prefix([&](uint32_t i) { return (uint64_t)tile_rows_of(qcls[i], ctx->tune_tile_iters); },
+                                  finished, tp_fill_h, tp_fill)))
+                break;
             for_runs(
                 [&](uint32_t i) { return qcls[i]; }, finished,
                 [&](uint32_t c, uint32_t qa, uint32_t qb) {
@@ -1648,6 +1801,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                 if (kbytes[1 + k * 4 + gi]) add_kernel_bytes(ctx, kKindTimer[k][gi], kbytes[1 + k * 4 + gi]);
         for (int gi = 0; gi < 4; ++gi)
             if (kbytes[13 + gi]) add_kernel_bytes(ctx, kFillTimer[gi], kbytes[13 + gi]);
+        if (kbytes[1 + kFlatFusedSlot]) add_kernel_bytes(ctx, "batch_count_fused_flat", kbytes[1 + kFlatFusedSlot]);
     }
     if (!rc)
         for (uint32_t i = 0; i < Q; ++i) {
