@@ -785,27 +785,6 @@ __device__ inline uint64_t fused_candidate(const DPlan *__restrict__ P, const ui
     return prod;
 }
 
-// This lane's share of a row of width D-1: the sum over its depth-(D-1) candidates x (lanes of
-// the group split the row's candidate list) of the leaf product of (row, x).  Not reduced.
-template <int G>
-__device__ inline uint64_t fused_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row,
-                                     const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
-                                     const uint32_t *__restrict__ cand, uint64_t &bytes) {
-    const int lg = threadIdx.x & (G - 1);
-    const uint32_t d = P->D - 1;
-    const HoistedGroups H = hoisted_groups(P, d);
-    FusedRow F{};
-    fused_setup(P, row, d, H, ce_off, ce_val, cand, F);
-    SameLabel S;
-    load_same(S, P, cand, row, d);
-    uint64_t acc = 0;
-    for (uint32_t t = lg; t < F.pr.len; t += G)
-        acc += fused_candidate(P, row, d, F.pj, F.hc, H, S, ce_val[F.pr.lo + t], ce_off, ce_val, cand, bytes);
-    bytes += 4ull * ((F.pr.len + G - 1 - lg) / G);
-    if (lg == 0) bytes += 4ull * d + 16ull * P->nb[d] + 16ull * (P->ngroups - P->nadj_last);
-    return acc;
-}
-
 // Fused rows of one query tile (at most 64 rows, one per lane), flattened: lane k sets up row k,
 // a wave scan lays the rows' probe lists end to end, and the wave then takes 64 consecutive
 // candidates at a time whatever rows they belong to, reading each row's state from its lane.  A
@@ -824,7 +803,7 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
     if (mine) {
         fused_setup(P, row, d, H, ce_off, ce_val, cand, F);
         load_same(S, P, cand, row, d);
-        counts[r0 + lane] = 0;
+        if (counts) counts[r0 + lane] = 0;  // a fused row has no children
         bytes += 4ull * d + 16ull * P->nb[d] + 16ull * (P->ngroups - P->nadj_last) + 4ull * F.pr.len;
     }
     const uint64_t plen = mine ? F.pr.len : 0;
@@ -869,20 +848,24 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
 }
 
 // Rows of width D-1 -> sum over their valid depth-(D-1) candidates x of the leaf product of
-// (row, x).  Lanes of a G-group split the row's candidate list; each lane evaluates its own
-// candidates' leaf products, so the depth-D frontier is never written.
-template <int G>
+// (row, x), in flattened tiles of 64 rows (fused_tile), one wave per tile; the depth-D frontier is
+// never written.
 __global__ void __launch_bounds__(256) k_leaf_fused(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
                                                     uint64_t nrows, const uint64_t *__restrict__ ce_off,
                                                     const uint32_t *__restrict__ ce_val,
                                                     const uint32_t *__restrict__ cand,
                                                     unsigned long long *__restrict__ total,
                                                     unsigned long long *__restrict__ bytes_ctr) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t ntiles = (nrows + 63) / 64;
     const uint32_t d = P->D - 1;
     uint64_t acc = 0, bytes = 0;
-    for (uint64_t it = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; it < nrows; it += stride)
-        acc += fused_row<G>(P, rows + it * d, ce_off, ce_val, cand, bytes);
+    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+        const uint64_t r0 = t * 64;
+        const uint32_t n = (uint32_t)min<uint64_t>(64, nrows - r0);
+        acc += fused_tile(P, rows, r0, n, d, ce_off, ce_val, cand, nullptr, bytes);
+    }
     acc = group_sum<64>(acc);
     bytes = group_sum<64>(bytes);
     if ((threadIdx.x & 63) == 0) {
@@ -916,7 +899,8 @@ __device__ inline uint32_t seg_of(const uint64_t *__restrict__ seg_start, uint32
 // the largest and runs at half the occupancy).
 constexpr int kExpandRows = 0, kFusedRows = 1, kLeafRows = 2;
 // Kernel-table names and algorithmic-byte counter slots (offsets from d_counters + 1) of the count
-// launches, one per (row kind, lane width), so that each entry is one kernel instantiation.
+// launches, one per (row kind, lane width), so that each entry is one kernel instantiation.  Fused
+// rows run in k_bfused_flat (own name and slot); their row of the table is unused.
 constexpr const char *kKindTimer[3][4] = {
     {"batch_count_expand_g1", "batch_count_expand_g4", "batch_count_expand_g16", "batch_count_expand_g64"},
     {"batch_count_fused_g1", "batch_count_fused_g4", "batch_count_fused_g16", "batch_count_fused_g64"},
@@ -941,8 +925,9 @@ __device__ inline uint32_t tile_query(const uint64_t *__restrict__ tile_pref, ui
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
 }
 
-// Per row: finish it (leaf depth, or fused last level) into qcount[query] and count 0 children,
-// or count its children.  Every row of a launch is of the given kind.
+// Per row: finish it (leaf depth) into qcount[query] and count 0 children, or count its children.
+// Every row of a launch is of the given kind (kLeafRows or kExpandRows; fused rows run in
+// k_bfused_flat).
 template <int G, int KIND>
 __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                 const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
@@ -985,10 +970,6 @@ __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans,
                     acc += v;
                     counts[it] = 0;
                 }
-            } else if constexpr (KIND == kFusedRows) {
-                const uint64_t v = fused_row<G>(P, row, ce_off, ce_val, cand, bytes);  // lane share
-                acc += v;
-                if (lg == 0) counts[it] = 0;
             } else {
                 const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
                 if (lg == 0) counts[it] = c;
@@ -1082,6 +1063,7 @@ namespace {
 
 constexpr int kG = 16;                      // lanes per row
 constexpr uint64_t kLeafChunk = 1ull << 22;  // rows per leaf-count launch
+constexpr uint64_t kFusedMinRows = 1ull << 19;  // rows per fused launch, at least
 constexpr uint64_t kPairCap = 1ull << 26;    // local-count (cell, value) pairs per sort batch
 
 unsigned groups_grid(uint64_t items, int G) {
@@ -1149,7 +1131,6 @@ struct Engine {
     std::vector<uint64_t> hoff;       // host copy of offsets when splitting
     std::chrono::steady_clock::time_point t_start;
     bool fused = false;           // depth D-1 evaluated by k_leaf_fused
-    uint32_t g_fused = 16;        // lanes per row in k_leaf_fused
     uint64_t fanout_last = 1;     // expected candidate-list length at depth D-1
 
     bool timed_out() const {
@@ -1206,19 +1187,17 @@ struct Engine {
 
     // Fused last level: rows at depth D-1, leaf products summed over their depth-(D-1) candidates.
     int leaf_fused(uint32_t *r, uint64_t n) {
-        const uint64_t chunk = std::max<uint64_t>(1, kLeafChunk / std::max<uint64_t>(1, fanout_last));
+        // about kLeafChunk candidates per launch, but never fewer rows than fill the chip with
+        // 64-row tiles (2^19 rows = 8192 waves, 32 per CU)
+        const uint64_t chunk = std::max<uint64_t>(kFusedMinRows, kLeafChunk / std::max<uint64_t>(1, fanout_last));
         for (uint64_t i0 = 0; i0 < n; i0 += chunk) {
             const uint64_t m = std::min<uint64_t>(chunk, n - i0);
             if (i0 && timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
             ++launches;
             KTimer kt(ctx, "leaf_fused");
             const uint32_t *rr = r + i0 * (D - 1);
-            switch (g_fused) {
-            case 1: hipLaunchKernelGGL((k_leaf_fused<1>), dim3(groups_grid(m, 1)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
-            case 4: hipLaunchKernelGGL((k_leaf_fused<4>), dim3(groups_grid(m, 4)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
-            case 16: hipLaunchKernelGGL((k_leaf_fused<16>), dim3(groups_grid(m, 16)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
-            default: hipLaunchKernelGGL((k_leaf_fused<64>), dim3(groups_grid(m, 64)), dim3(256), 0, ctx->stream, dplan, rr, m, cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2); break;
-            }
+            hipLaunchKernelGGL(k_leaf_fused, dim3(groups_grid((m + 63) / 64, 64)), dim3(256), 0, ctx->stream, dplan, rr, m,
+                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2);
             kt.stop();
             int rc = check_hip(ctx, hipGetLastError(), "fused leaf kernel");
             if (rc) return rc;
@@ -1376,7 +1355,6 @@ struct Engine {
             if (ctx->row_limit && rows_total > ctx->row_limit)
                 return ctx->fail(ASDM_E_TIMEOUT, "per-query row limit exceeded");
             fanout_last = std::max<uint64_t>(1, total / n);
-            g_fused = fanout_last < 3 ? 1 : fanout_last < 12 ? 4 : fanout_last < 48 ? 16 : 64;
             return leaf_fused(r, n);
         }
         const uint64_t c = cap[d + 1];
@@ -1450,7 +1428,7 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     Engine E{ctx, plan, cs};
     E.t_start = t0;
     // Fuse the last enumerated level into leaf counting when there is one (count mode only); the
-    // lane-group width follows the expected list length of that level (smallest arc fan-out).
+    // expected list length of that level (smallest arc fan-out) sizes the fused launches' chunks.
     if (!mat_host && plan_fusable(plan)) {
         E.fused = true;
         const uint32_t d = plan->D - 1;
@@ -1461,7 +1439,6 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
             best = std::min(best, (double)cs->arc_weight[a] / std::max(1.0, (double)cs->csize[p]));
         }
         E.fanout_last = (uint64_t)std::max(1.0, best);
-        E.g_fused = best < 3 ? 1 : best < 12 ? 4 : best < 48 ? 16 : 64;
     }
     E.budget = budget ? std::max<uint64_t>(budget, 1ull << 12) : default_budget(ctx);
     int rc = E.init();
@@ -1720,7 +1697,6 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                         kt.stop();
                     };
                     if (key / 256 == kLeafRows) go(std::integral_constant<int, kLeafRows>{});
-                    else if (key / 256 == kFusedRows) go(std::integral_constant<int, kFusedRows>{});
                     else go(std::integral_constant<int, kExpandRows>{});
                 });
         }
