@@ -117,6 +117,7 @@ struct asdm_cs {
     Store *store = nullptr;
     asdm_ctx *ctx = nullptr;
     uint32_t nq = 0, nv = 0, root = 0;
+    bool loop_free = false;                  // the data graph has no self-loops
     std::vector<uint32_t> qlab, qoff, qnbr;  // query CSR copy (label-sorted)
     std::vector<uint32_t> csize;             // |C(u)|
     std::vector<uint64_t> cand_off;          // [nq + 1] offsets into d_cand

@@ -849,8 +849,8 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
 
 // Rows of width D-1 -> sum over their valid depth-(D-1) candidates x of the leaf product of
 // (row, x), in flattened tiles of 64 rows (fused_tile), one wave per tile; the depth-D frontier is
-// never written.
-__global__ void __launch_bounds__(256) k_leaf_fused(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
+// never written.  Five waves per SIMD, as k_bfused_flat.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_leaf_fused(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
                                                     uint64_t nrows, const uint64_t *__restrict__ ce_off,
                                                     const uint32_t *__restrict__ ce_val,
                                                     const uint32_t *__restrict__ cand,
@@ -1009,8 +1009,9 @@ __global__ void __launch_bounds__(256) k_bfill(const DPlan *__restrict__ plans, 
 }
 
 // Fused rows of the batch in flattened tiles of 64 rows (fused_tile); tiles as in k_bcount with
-// tile_rows = 64.
-__global__ void __launch_bounds__(256) k_bfused_flat(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
+// tile_rows = 64.  Five waves per SIMD (at most 96 VGPRs, about 100 B per lane spilled) hide more
+// of the lookups' latency than four without spills (116 VGPRs).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_bfused_flat(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                      const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
                                                      uint32_t d, const uint32_t *__restrict__ rows,
                                                      const uint64_t *__restrict__ ce_off,

@@ -536,6 +536,7 @@ int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs,
         store->refs++;
         cs->nq = q->n;
         cs->nv = n;
+        cs->loop_free = g->loop_free;
         cs->W = W;
         cs->root = roots[i];
         cs->qlab = q->lab;

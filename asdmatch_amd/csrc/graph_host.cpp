@@ -31,6 +31,13 @@ void finish_graph(asdm_graph &g) {
         }
         for (auto &t : th) t.join();
     }
+    g.loop_free = true;
+    for (uint32_t v = 0; v < g.n && g.loop_free; ++v)
+        for (uint32_t a = g.off[v]; a < g.off[v + 1]; ++a)
+            if (g.nbr[a] == v) {
+                g.loop_free = false;
+                break;
+            }
     g.vbl_off.assign(g.nlab + 1, 0);
     for (uint32_t v = 0; v < g.n; ++v) g.vbl_off[g.lab[v] + 1]++;
     for (uint32_t l = 0; l < g.nlab; ++l) g.vbl_off[l + 1] += g.vbl_off[l];

@@ -31,6 +31,7 @@ struct asdm_graph {
     std::vector<uint32_t> lab, off, nbr;
     std::vector<uint32_t> vbl_off, vbl;
     std::vector<float> lab_avgdeg;  // mean degree of the vertices of each label
+    bool loop_free = false;         // no vertex is its own neighbour
     asdm::DeviceGraph dev;
 
     uint32_t deg(uint32_t v) const { return off[v + 1] - off[v]; }

@@ -333,6 +333,31 @@ int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint
                 done[e] = 1;
             }
     }
+    // On a data graph without self-loops a vertex adjacent to v_b is never v_b itself, so a
+    // same-label depth that is a backward neighbour of an enumerated vertex (of every leaf of a
+    // group, for a leaf group) needs no injectivity test against it.
+    if (cs->loop_free) {
+        auto is_back = [&](uint32_t e, uint32_t j) {
+            for (uint32_t b = 0; b < P.nb[e]; ++b)
+                if (P.bdep[e][b] == j) return true;
+            return false;
+        };
+        auto prune = [&](uint32_t e, const std::function<bool(uint32_t)> &skip) {
+            uint32_t k = 0;
+            for (uint32_t s = 0; s < P.ns[e]; ++s)
+                if (!skip(P.sdep[e][s])) P.sdep[e][k++] = P.sdep[e][s];
+            P.ns[e] = k;
+        };
+        for (uint32_t d = 0; d < D; ++d) prune(d, [&](uint32_t j) { return is_back(d, j); });
+        for (uint32_t g = 0; g < P.ngroups; ++g) {
+            auto common = [&](uint32_t j) {
+                for (uint32_t i = 0; i < P.gsize[g]; ++i)
+                    if (!is_back(P.gleaf[g][i], j)) return false;
+                return true;
+            };
+            for (uint32_t i = 0; i < P.gsize[g]; ++i) prune(P.gleaf[g][i], common);
+        }
+    }
     for (uint32_t e = D; e < nq; ++e)
         for (uint32_t j = 0; j < P.nb[e]; ++j)
             if (P.bdep[e][j] + 1 == D) {
