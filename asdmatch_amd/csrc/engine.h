@@ -43,6 +43,12 @@ struct DPlan {
     uint64_t tri_base[kMaxQ];
     const uint32_t *tri_tab[kMaxQ];
     uint32_t fused;                  // depth D-1 evaluated by the fused leaf path (batch engine)
+    // Row-sum table of a fused level whose depth D-1 has a single backward neighbour, depth
+    // tail_dep, and no same-label constraint on its vertex: tail[p] = sum over the candidates x
+    // of D-1 adjacent to position p of C(u_tail_dep) of the product of the adjacent leaves'
+    // counts at x.  nullptr when the level has no such table.
+    const uint64_t *tail;
+    uint32_t tail_dep;
     uint32_t gsize[kMaxQ];
     uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
     // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient

@@ -785,6 +785,41 @@ __device__ inline uint64_t fused_candidate(const DPlan *__restrict__ P, const ui
     return prod;
 }
 
+// Leaves of a fused level with a row-sum table (DPlan::tail): each leaf adjacent to depth d = D-1
+// is a pendant leaf of d (count: its list length at x) or a triangle leaf on (tail_dep, d)
+// (count: its triangle-table entry at the candidate edge (p, x)).
+constexpr uint32_t kTailLeaves = 8;
+struct TailSpec {
+    uint64_t arc_base;  // list offsets of the arc (u_tail_dep -> u_d) start at ce_off[arc_base]
+    uint32_t np;        // |C(u_tail_dep)|
+    uint32_t nadj;      // adjacent leaves
+    uint64_t pend_base[kTailLeaves];       // pendant leaf: list offsets of the arc (u_d -> u_e)
+    const uint32_t *tri_tab[kTailLeaves];  // triangle leaf: its table (nullptr for a pendant leaf)
+};
+
+// tail[p] for every position p of C(u_tail_dep): a wave per position, lanes split its list.
+__global__ void __launch_bounds__(256) k_tail_sums(TailSpec T, const uint64_t *__restrict__ ce_off,
+                                                   const uint32_t *__restrict__ ce_val, uint64_t *__restrict__ tail) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t e_first = ce_off[T.arc_base];  // the arc's first candidate edge (triangle-table index 0)
+    for (uint64_t p = wave; p < T.np; p += nwaves) {
+        const uint64_t lo = ce_off[T.arc_base + p], hi = ce_off[T.arc_base + p + 1];
+        uint64_t s = 0;
+        for (uint64_t i = lo + lane; i < hi; i += 64) {
+            const uint32_t x = ce_val[i];
+            uint64_t prod = 1;
+            for (uint32_t k = 0; k < T.nadj; ++k)
+                prod *= T.tri_tab[k] ? (uint64_t)T.tri_tab[k][i - e_first]
+                                     : ce_off[T.pend_base[k] + x + 1] - ce_off[T.pend_base[k] + x];
+            s += prod;
+        }
+        s = group_sum<64>(s);
+        if (lane == 0) tail[p] = s;
+    }
+}
+
 // Fused rows of one query tile (at most 64 rows, one per lane), flattened: lane k sets up row k,
 // a wave scan lays the rows' probe lists end to end, and the wave then takes 64 consecutive
 // candidates at a time whatever rows they belong to, reading each row's state from its lane.  A
@@ -798,6 +833,25 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
     const HoistedGroups H = hoisted_groups(P, d);
     const bool mine = (uint32_t)lane < nrows;
     const uint32_t *row = rows + (r0 + (mine ? (uint32_t)lane : 0u)) * d;
+    if (P->tail) {
+        // a row's sum is its table entry times the hoisted groups' counts (every group not adjacent
+        // to d is hoisted, and none shares d's label)
+        uint64_t v = 0;
+        if (mine) {
+            v = P->tail[row[P->tail_dep]];
+            const RowView R0{row, d, 0};
+            for (uint32_t k = 0; k < H.n && v; ++k) {
+                const uint32_t g = H.g[k], e = P->gleaf[g][0];
+                uint32_t y[kFusedSame];
+                const uint32_t ns = P->ns[e];
+                for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
+                v *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+            }
+            if (counts) counts[r0 + lane] = 0;  // a fused row has no children
+            bytes += 4ull * d + 8ull + 16ull * H.n;
+        }
+        return v;
+    }
     FusedRow F{};
     SameLabel S{};
     if (mine) {
@@ -1073,8 +1127,67 @@ unsigned groups_grid(uint64_t items, int G) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 1u << 16));
 }
 
+bool plan_fusable(const asdm_plan *plan);
+
+// Row-sum table of a fused plan (DPlan::tail) when depth d = D-1 has a single backward neighbour
+// b, no earlier depth shares d's label, no leaf adjacent to d has a same-label matched vertex to
+// avoid, and every leaf group not adjacent to d is hoisted and has a label other than d's.  Then
+// no constraint of a row reaches its depth-d candidates x except adjacency to the row's vertex at
+// b, and the row's sum is tail[row[b]] times its hoisted counts.  Needs hp.tri_tab.
+int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
+    const DPlan &P = plan->h;
+    hp.tail = nullptr;
+    hp.tail_dep = 0;
+    if (!plan_fusable(plan)) return ASDM_OK;
+    const uint32_t d = P.D - 1;
+    if (P.nb[d] != 1 || P.ns[d] != 0) return ASDM_OK;
+    const uint32_t b = P.bdep[d][0];
+    TailSpec T{};
+    T.arc_base = P.bbase[d][0];
+    T.np = cs->csize[plan->order[b]];
+    uint32_t hoisted = 0;
+    for (uint32_t g = 0; g < P.ngroups; ++g) {
+        const uint32_t e = P.gleaf[g][0];
+        bool adj = false;
+        for (uint32_t j = 0; j < P.nb[e]; ++j) adj |= P.bdep[e][j] == d;
+        if (!adj) {
+            for (uint32_t s = 0; s < P.ns[e]; ++s)
+                if (P.sdep[e][s] == d) return ASDM_OK;  // shares d's label
+            if (++hoisted > kRowLeaves) return ASDM_OK;
+            continue;
+        }
+        if (P.ns[e] != 0 || T.nadj == kTailLeaves) return ASDM_OK;
+        if (P.nb[e] == 1) {
+            T.pend_base[T.nadj] = P.bbase[e][0];
+            T.tri_tab[T.nadj] = nullptr;
+        } else if (P.tri[e] && P.nb[e] == 2 && P.bdep[e][0] == b && P.bdep[e][1] == d && hp.tri_tab[e]) {
+            T.pend_base[T.nadj] = 0;
+            T.tri_tab[T.nadj] = hp.tri_tab[e];
+        } else {
+            return ASDM_OK;
+        }
+        ++T.nadj;
+    }
+    uint64_t *tab = nullptr;
+    int rc = dev_alloc(ctx, (void **)&tab, 8ull * std::max<uint32_t>(T.np, 1));
+    if (rc) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(tab));
+    if (T.np) {
+        KTimer kt(ctx, "tail_sums");
+        const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)T.np + 3) / 4, 1u << 16);
+        hipLaunchKernelGGL(k_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, T, cs->d_ce_off, cs->d_ce_val, tab);
+        const uint32_t a = cs->arc_index(plan->order[b], plan->order[d]);
+        kt.stop(24ull * T.np + (4ull + 16ull * T.nadj) * cs->arc_weight[a]);
+        if ((rc = check_hip(ctx, hipGetLastError(), "tail table"))) return rc;
+    }
+    hp.tail = tab;
+    hp.tail_dep = b;
+    return ASDM_OK;
+}
+
 // Builds the per-candidate-edge intersection table of every triangle leaf of the plan and points
-// hp.tri_tab at them (tables appended to tabs; the caller frees them).
+// hp.tri_tab at them, then the row-sum table of the fused level where it applies (tables appended
+// to tabs; the caller frees them).
 int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
     const DPlan &P = plan->h;
     for (uint32_t e = P.D; e < P.nq; ++e) {
@@ -1099,7 +1212,7 @@ int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
             if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;
         }
     }
-    return ASDM_OK;
+    return build_tail_table(ctx, plan, cs, hp, tabs);
 }
 
 // Fusion pays when each leaf count is a few lookups: every leaf group is a single leaf with one

@@ -155,6 +155,43 @@ def test_same_label_leaf_groups_against_bruteforce(ctx):
             assert A.count_embeddings(ctx, q, g) == want, (trial, ql, qe)
 
 
+def test_row_sum_table_against_bruteforce(ctx):
+    """A fused level whose last enumerated vertex has one backward neighbour and no same-label
+    constraint finishes each row from a per-position table (k_tail_sums) instead of scanning the
+    row's candidates.  Paths, stars, tadpoles and triangle fans with distinct labels near that
+    vertex take the table; the others (shared labels) take the per-candidate path.  Both engines
+    must give the brute-force count, and the table must have been used."""
+    rng = random.Random(21)
+    queries = (
+        ([0, 1, 2, 3], [(0, 1), (1, 2), (2, 3)]),
+        ([0, 1, 2, 3], [(0, 1), (0, 2), (0, 3)]),
+        ([0, 1, 2, 3], [(0, 1), (1, 2), (0, 2), (2, 3)]),
+        ([0, 1, 2, 3, 0], [(0, 1), (1, 2), (2, 3), (3, 4)]),
+        ([0, 1, 2, 3], [(0, 1), (1, 2), (0, 2), (1, 3), (2, 3)]),
+        ([0, 1, 2, 3, 1], [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4)]),
+        ([0, 1, 1, 2, 3], [(0, 1), (0, 2), (1, 3), (2, 4)]),
+    )
+    used = 0
+    for trial in range(6):
+        gl, ge = _rand_graph(rng, 80, 0.12, 4)
+        g, og = T.graph_from_lists(gl, ge)
+        qs, want = [], []
+        for ql, qe in queries:
+            q, oq = T.graph_from_lists(ql, qe)
+            qs.append(q)
+            want.append(O.bruteforce_count(oq, og))
+        ctx.set_profiling(True)
+        try:
+            got = [A.count_embeddings(ctx, q, g) for q in qs]
+            batch, _ = ctx.batch_count(g, qs)
+            used += sum(k["launches"] for k in ctx.kernel_stats() if k["name"] == "tail_sums")
+        finally:
+            ctx.set_profiling(False)
+        assert got == want, trial
+        assert [int(c) for c in batch] == want, trial
+    assert used > 0
+
+
 def test_local_counts_match_reference(ctx):
     for case in T.synth_golden():
         if not case["local"]:
