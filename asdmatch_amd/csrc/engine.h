@@ -44,9 +44,9 @@ struct DPlan {
     const uint32_t *tri_tab[kMaxQ];
     uint32_t fused;                  // depth D-1 evaluated by the fused leaf path (batch engine)
     // Row-sum table of a fused level whose depth D-1 has a single backward neighbour, depth
-    // tail_dep, and no same-label constraint on its vertex: tail[p] = sum over the candidates x
-    // of D-1 adjacent to position p of C(u_tail_dep) of the product of the adjacent leaves'
-    // counts at x.  nullptr when the level has no such table.
+    // tail_dep, and whose leaves put no same-label constraint on its vertex: tail[p] = sum over
+    // the candidates x of D-1 adjacent to position p of C(u_tail_dep) of the product of the
+    // adjacent leaves' list counts at x.  nullptr when the level has no such table.
     const uint64_t *tail;
     uint32_t tail_dep;
     uint32_t gsize[kMaxQ];

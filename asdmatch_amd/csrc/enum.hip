@@ -820,6 +820,51 @@ __global__ void __launch_bounds__(256) k_tail_sums(TailSpec T, const uint64_t *_
     }
 }
 
+// f0(x): the product of the adjacent leaves' list counts at x = L[t] (i = L.lo + t), the value the
+// row-sum table holds for x.
+__device__ inline uint64_t tail_clean(const DPlan *__restrict__ P, uint32_t x, uint64_t i, const HoistedGroups &H,
+                                      const uint64_t *__restrict__ ce_off) {
+    uint64_t prod = 1;
+    for (uint32_t g = 0; g < P->ngroups; ++g) {
+        if (H.mask >> g & 1) continue;
+        const uint32_t e = P->gleaf[g][0];
+        prod *= P->tri[e] ? (uint64_t)P->tri_tab[e][i - ce_off[P->tri_base[e]]]
+                          : ce_off[P->bbase[e][0] + x + 1] - ce_off[P->bbase[e][0] + x];
+    }
+    return prod;
+}
+
+// One row of a fused level with a row-sum table: H0 * tail[row[b]] (H0: the product of the row's
+// hoisted counts), less H0 * f0(x) for each candidate x that is the matched vertex of a
+// same-label depth (not an injective extension; there are at most ns[d] of them, and the table
+// counted them).  Exact modulo 2^64, as the per-candidate sum is.
+__device__ inline uint64_t tail_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t d,
+                                    const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
+                                    const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                    uint64_t &bytes) {
+    uint64_t s0 = P->tail[row[P->tail_dep]];
+    uint64_t H0 = 1;
+    const RowView R0{row, d, 0};
+    for (uint32_t k = 0; k < H.n && (s0 || P->ns[d]); ++k) {
+        const uint32_t g = H.g[k], e = P->gleaf[g][0];
+        uint32_t y[kFusedSame];
+        const uint32_t ns = P->ns[e];  // d is not among them: no hoisted leaf has d's label
+        for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
+        H0 *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+    }
+    bytes += 4ull * d + 8ull + 16ull * H.n;
+    if (!P->ns[d]) return s0 * H0;
+    const ListRef L = list_of(P, ce_off, row, d, 0);
+    const uint32_t *Lv = ce_val + L.lo;
+    for (uint32_t s2 = 0; s2 < P->ns[d]; ++s2) {
+        const uint32_t x = cand_pos(P, d, matched_id(P, cand, R0, P->sdep[d][s2]));
+        if (x == 0xffffffffu) continue;
+        const uint32_t t = lower_bound_u32(Lv, L.len, x);
+        if (t < L.len && Lv[t] == x) s0 -= tail_clean(P, x, L.lo + t, H, ce_off);
+    }
+    return s0 * H0;
+}
+
 // Fused rows of one query tile (at most 64 rows, one per lane), flattened: lane k sets up row k,
 // a wave scan lays the rows' probe lists end to end, and the wave then takes 64 consecutive
 // candidates at a time whatever rows they belong to, reading each row's state from its lane.  A
@@ -834,21 +879,10 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
     const bool mine = (uint32_t)lane < nrows;
     const uint32_t *row = rows + (r0 + (mine ? (uint32_t)lane : 0u)) * d;
     if (P->tail) {
-        // a row's sum is its table entry times the hoisted groups' counts (every group not adjacent
-        // to d is hoisted, and none shares d's label)
         uint64_t v = 0;
         if (mine) {
-            v = P->tail[row[P->tail_dep]];
-            const RowView R0{row, d, 0};
-            for (uint32_t k = 0; k < H.n && v; ++k) {
-                const uint32_t g = H.g[k], e = P->gleaf[g][0];
-                uint32_t y[kFusedSame];
-                const uint32_t ns = P->ns[e];
-                for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
-                v *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
-            }
+            v = tail_row(P, row, d, H, ce_off, ce_val, cand, bytes);
             if (counts) counts[r0 + lane] = 0;  // a fused row has no children
-            bytes += 4ull * d + 8ull + 16ull * H.n;
         }
         return v;
     }
@@ -1130,17 +1164,19 @@ unsigned groups_grid(uint64_t items, int G) {
 bool plan_fusable(const asdm_plan *plan);
 
 // Row-sum table of a fused plan (DPlan::tail) when depth d = D-1 has a single backward neighbour
-// b, no earlier depth shares d's label, no leaf adjacent to d has a same-label matched vertex to
-// avoid, and every leaf group not adjacent to d is hoisted and has a label other than d's.  Then
-// no constraint of a row reaches its depth-d candidates x except adjacency to the row's vertex at
-// b, and the row's sum is tail[row[b]] times its hoisted counts.  Needs hp.tri_tab.
+// b: tail[p] = sum over the candidates x adjacent to position p of C(u_b) of the product of the
+// adjacent leaves' list counts at x.  It applies when every leaf group is a pendant leaf of d or a
+// triangle leaf on (b, d) with no same-label matched vertex to avoid, or a hoisted leaf (not
+// adjacent to d; at most kRowLeaves of them) whose label is not d's.  Then the only constraint on
+// x beyond adjacency to the row's vertex at b is injectivity against the same-label depths of d,
+// which tail_row corrects, and a row's sum needs no scan of its candidates.  Needs hp.tri_tab.
 int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
     const DPlan &P = plan->h;
     hp.tail = nullptr;
     hp.tail_dep = 0;
     if (!plan_fusable(plan)) return ASDM_OK;
     const uint32_t d = P.D - 1;
-    if (P.nb[d] != 1 || P.ns[d] != 0) return ASDM_OK;
+    if (P.nb[d] != 1) return ASDM_OK;
     const uint32_t b = P.bdep[d][0];
     TailSpec T{};
     T.arc_base = P.bbase[d][0];

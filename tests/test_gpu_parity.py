@@ -192,6 +192,43 @@ def test_row_sum_table_against_bruteforce(ctx):
     assert used > 0
 
 
+def test_row_sum_table_corrections_against_bruteforce(ctx):
+    """Forced plans with same-label constraints around the last enumerated vertex d: a same-label
+    earlier depth of d (query 1: the row-sum table, corrected at that vertex), a hoisted leaf with
+    d's label (2), a same-label matched vertex of a pendant leaf (3) and of a triangle leaf (4) of
+    d, and mixes of them (5, 6).  Queries 2-6 fall back to the per-candidate path; every count must
+    be the brute force's, and the table must have been used."""
+    A_, B_, C_ = 0, 1, 2
+    cases = (  # labels, edges, order, enumerated depths
+        ([A_, B_, A_, C_], [(0, 1), (1, 2), (2, 3)], [1, 0, 2, 3], 3),
+        ([B_, A_, C_, A_], [(0, 1), (0, 2), (2, 3)], [0, 2, 3, 1], 3),
+        ([A_, B_, C_, A_], [(0, 1), (1, 2), (2, 3)], [1, 0, 2, 3], 3),
+        ([A_, B_, C_, A_], [(0, 1), (1, 2), (1, 3), (2, 3)], [1, 0, 2, 3], 3),
+        ([A_, B_, A_, A_], [(0, 1), (1, 2), (2, 3)], [1, 0, 2, 3], 3),
+        ([B_, A_, C_, A_, C_, A_], [(0, 1), (0, 2), (2, 3), (3, 4), (0, 5)], [0, 2, 5, 3, 1, 4], 4),
+    )
+    rng = random.Random(33)
+    used = 0
+    for trial in range(8):
+        gl, ge = _rand_graph(rng, 60, 0.15, 3)
+        g, og = T.graph_from_lists(gl, ge)
+        for ql, qe, order, D in cases:
+            q, oq = T.graph_from_lists(ql, qe)
+            want = O.bruteforce_count(oq, og)
+            cs = ctx.filter(q, g)
+            if cs is None:
+                assert want == 0
+                continue
+            ctx.set_profiling(True)
+            try:
+                got = ctx.count(cs.plan(q, order=order, num_enumerated=D), cs)
+                used += sum(k["launches"] for k in ctx.kernel_stats() if k["name"] == "tail_sums")
+            finally:
+                ctx.set_profiling(False)
+            assert got == want, (trial, ql, qe, order, D)
+    assert used > 0
+
+
 def test_local_counts_match_reference(ctx):
     for case in T.synth_golden():
         if not case["local"]:
