@@ -49,6 +49,12 @@ struct DPlan {
     // adjacent leaves' list counts at x.  nullptr when the level has no such table.
     const uint64_t *tail;
     uint32_t tail_dep;
+    // One correction factor of the table: tail_fix 1 = the adjacent leaf at depth tail_leaf has
+    // same-label matched vertices (its count at x drops by one for each adjacent to x; tail_rbase:
+    // list offsets of the arc (u_tail_leaf -> u_{D-1})); 2 = the hoisted leaf at depth tail_leaf
+    // has the label of depth D-1 (its count drops by one at the x in its list); 0 = none.
+    uint32_t tail_fix, tail_leaf;
+    uint64_t tail_rbase;
     uint32_t gsize[kMaxQ];
     uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
     // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient

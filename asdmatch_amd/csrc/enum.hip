@@ -820,14 +820,17 @@ __global__ void __launch_bounds__(256) k_tail_sums(TailSpec T, const uint64_t *_
     }
 }
 
+constexpr uint32_t kTailFixLeaf = 1, kTailFixHoist = 2;
+
 // f0(x): the product of the adjacent leaves' list counts at x = L[t] (i = L.lo + t), the value the
-// row-sum table holds for x.
+// row-sum table holds for x; the leaf at depth skip (if any) is left out.
 __device__ inline uint64_t tail_clean(const DPlan *__restrict__ P, uint32_t x, uint64_t i, const HoistedGroups &H,
-                                      const uint64_t *__restrict__ ce_off) {
+                                      const uint64_t *__restrict__ ce_off, uint32_t skip = 0xffffffffu) {
     uint64_t prod = 1;
     for (uint32_t g = 0; g < P->ngroups; ++g) {
         if (H.mask >> g & 1) continue;
         const uint32_t e = P->gleaf[g][0];
+        if (e == skip) continue;
         prod *= P->tri[e] ? (uint64_t)P->tri_tab[e][i - ce_off[P->tri_base[e]]]
                           : ce_off[P->bbase[e][0] + x + 1] - ce_off[P->bbase[e][0] + x];
     }
@@ -837,24 +840,32 @@ __device__ inline uint64_t tail_clean(const DPlan *__restrict__ P, uint32_t x, u
 // One row of a fused level with a row-sum table: H0 * tail[row[b]] (H0: the product of the row's
 // hoisted counts), less H0 * f0(x) for each candidate x that is the matched vertex of a
 // same-label depth (not an injective extension; there are at most ns[d] of them, and the table
-// counted them).  Exact modulo 2^64, as the per-candidate sum is.
+// counted them).  Exact modulo 2^64, as the per-candidate sum is.  For the correction factor's
+// pass it also returns the row's list L from depth tail_dep and coef: H0, or for a hoisted
+// correction leaf the product of the other hoisted counts.
 __device__ inline uint64_t tail_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t d,
                                     const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
                                     const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
-                                    uint64_t &bytes) {
+                                    uint64_t &coef, ListRef &L, uint64_t &bytes) {
     uint64_t s0 = P->tail[row[P->tail_dep]];
-    uint64_t H0 = 1;
+    uint64_t H0 = 1, Hx = 1;
     const RowView R0{row, d, 0};
-    for (uint32_t k = 0; k < H.n && (s0 || P->ns[d]); ++k) {
+    const bool need = s0 || P->ns[d] || P->tail_fix;
+    for (uint32_t k = 0; k < H.n && need; ++k) {
         const uint32_t g = H.g[k], e = P->gleaf[g][0];
-        uint32_t y[kFusedSame];
-        const uint32_t ns = P->ns[e];  // d is not among them: no hoisted leaf has d's label
-        for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
-        H0 *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+        uint32_t y[kFusedSame], ns = 0;
+        for (uint32_t s2 = 0; s2 < P->ns[e]; ++s2) {
+            const uint32_t dep = P->sdep[e][s2];
+            if (dep != d) y[ns++] = matched_id(P, cand, R0, dep);
+        }
+        const uint64_t c = leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+        H0 *= c;
+        if (!(P->tail_fix == kTailFixHoist && e == P->tail_leaf)) Hx *= c;
     }
+    coef = P->tail_fix == kTailFixHoist ? Hx : H0;
     bytes += 4ull * d + 8ull + 16ull * H.n;
-    if (!P->ns[d]) return s0 * H0;
-    const ListRef L = list_of(P, ce_off, row, d, 0);
+    if (!P->ns[d] && !P->tail_fix) return s0 * H0;
+    L = list_of(P, ce_off, row, d, 0);
     const uint32_t *Lv = ce_val + L.lo;
     for (uint32_t s2 = 0; s2 < P->ns[d]; ++s2) {
         const uint32_t x = cand_pos(P, d, matched_id(P, cand, R0, P->sdep[d][s2]));
@@ -863,6 +874,99 @@ __device__ inline uint64_t tail_row(const DPlan *__restrict__ P, const uint32_t 
         if (t < L.len && Lv[t] == x) s0 -= tail_clean(P, x, L.lo + t, H, ce_off);
     }
     return s0 * H0;
+}
+
+// Correction pass of a row-sum table with a correction factor (DPlan::tail_fix), flattened over
+// the wave as in fused_tile.  For every row of the tile and every source of the factor (each
+// same-label matched vertex y of the adjacent correction leaf, or the hoisted correction leaf),
+// the injective candidates x of the row's list L that lie in the source's set -- the candidates
+// of d adjacent to y (the reverse arc's list at y), or the hoisted leaf's list -- each take
+// coef * (product of the other adjacent leaves' counts at x) off the row's sum.  Lane k holds
+// row k's scan: the shorter of L and the set is scanned and the other probed.  Returns this
+// lane's share, for rows of any lane.
+__device__ inline uint64_t tail_fix_tile(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows, uint64_t r0,
+                                         uint32_t d, bool mine, const uint32_t *__restrict__ row, uint64_t coef,
+                                         ListRef L, const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
+                                         const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                         uint64_t &bytes) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t fix = P->tail_fix, fl = P->tail_leaf;
+    const uint32_t nsrc = fix == kTailFixLeaf ? P->ns[fl] : 1u;
+    const uint32_t skip = fix == kTailFixLeaf ? fl : 0xffffffffu;
+    uint64_t corr = 0;
+    for (uint32_t i = 0; i < nsrc; ++i) {
+        ListRef Sr{0, 0};
+        if (mine && L.len && coef) {
+            const RowView R0{row, d, 0};
+            if (fix == kTailFixLeaf) {
+                const uint32_t q = cand_pos(P, fl, matched_id(P, cand, R0, P->sdep[fl][i]));
+                bool ok = q != 0xffffffffu;
+                for (uint32_t j = 0; j < P->nb[fl] && ok; ++j) {
+                    if (P->bdep[fl][j] == d) continue;  // the other list of a triangle leaf
+                    const ListRef r = list_of_v(P, ce_off, R0, fl, j);
+                    ok = contains_u32(ce_val + r.lo, r.len, q);
+                }
+                if (ok) {
+                    Sr.lo = ce_off[P->tail_rbase + q];
+                    Sr.len = (uint32_t)(ce_off[P->tail_rbase + q + 1] - Sr.lo);
+                }
+            } else {
+                Sr = list_of_v(P, ce_off, R0, fl, 0);
+            }
+            bytes += 16ull;
+        }
+        const bool scan_l = L.len <= Sr.len;
+        const uint64_t n = Sr.len ? (scan_l ? L.len : Sr.len) : 0;
+        const uint64_t inc = wave_inclusive_scan(n);
+        const uint64_t excl = inc - n;
+        const uint64_t total = __shfl(inc, 63, 64);
+        for (uint64_t base = 0; base < total; base += 64) {
+            const uint64_t c = base + lane;
+            int k = 0;  // the row holding work item c (as in fused_tile, six steps on every lane)
+#pragma unroll
+            for (int step = 32; step; step >>= 1) {
+                const uint64_t v = __shfl(excl, (k + step) & 63, 64);
+                if (k + step < 64 && v <= c) k += step;
+            }
+            const uint64_t off = __shfl(excl, k, 64);
+            const uint64_t llo = __shfl(L.lo, k, 64), slo = __shfl(Sr.lo, k, 64), ck = __shfl(coef, k, 64);
+            const uint32_t llen = __shfl(L.len, k, 64), slen = __shfl(Sr.len, k, 64);
+            const int sl = __shfl((int)scan_l, k, 64);
+            if (c >= total) continue;
+            const uint32_t t = (uint32_t)(c - off);
+            const uint32_t *Lv = ce_val + llo;
+            uint32_t x = 0xffffffffu, idx = 0;
+            bool member = false;
+            if (sl) {
+                idx = t;
+                x = Lv[t];
+                if (fix == kTailFixLeaf) {
+                    member = contains_u32(ce_val + slo, slen, x);
+                } else {
+                    const uint32_t q = cand_pos(P, fl, cand[P->coff[d] + x]);
+                    member = q != 0xffffffffu && contains_u32(ce_val + slo, slen, q);
+                }
+            } else {
+                const uint32_t e = ce_val[slo + t];
+                x = fix == kTailFixLeaf ? e : cand_pos(P, d, cand[P->coff[fl] + e]);
+                if (x != 0xffffffffu) {
+                    idx = lower_bound_u32(Lv, llen, x);
+                    member = idx < llen && Lv[idx] == x;
+                }
+            }
+            bytes += 8ull;
+            if (!member) continue;
+            const uint32_t *rk = rows + (r0 + (uint32_t)k) * d;
+            const uint32_t xid = cand[P->coff[d] + x];
+            bool valid = true;
+            for (uint32_t s2 = 0; s2 < P->ns[d]; ++s2) {
+                const uint32_t dep = P->sdep[d][s2];
+                valid &= xid != cand[P->coff[dep] + rk[dep]];
+            }
+            if (valid) corr += ck * tail_clean(P, x, llo + idx, H, ce_off, skip);
+        }
+    }
+    return corr;
 }
 
 // Fused rows of one query tile (at most 64 rows, one per lane), flattened: lane k sets up row k,
@@ -879,11 +983,13 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
     const bool mine = (uint32_t)lane < nrows;
     const uint32_t *row = rows + (r0 + (mine ? (uint32_t)lane : 0u)) * d;
     if (P->tail) {
-        uint64_t v = 0;
+        uint64_t v = 0, coef = 0;
+        ListRef L{0, 0};
         if (mine) {
-            v = tail_row(P, row, d, H, ce_off, ce_val, cand, bytes);
+            v = tail_row(P, row, d, H, ce_off, ce_val, cand, coef, L, bytes);
             if (counts) counts[r0 + lane] = 0;  // a fused row has no children
         }
+        if (P->tail_fix) v -= tail_fix_tile(P, rows, r0, d, mine, row, coef, L, H, ce_off, ce_val, cand, bytes);
         return v;
     }
     FusedRow F{};
@@ -1166,14 +1272,19 @@ bool plan_fusable(const asdm_plan *plan);
 // Row-sum table of a fused plan (DPlan::tail) when depth d = D-1 has a single backward neighbour
 // b: tail[p] = sum over the candidates x adjacent to position p of C(u_b) of the product of the
 // adjacent leaves' list counts at x.  It applies when every leaf group is a pendant leaf of d or a
-// triangle leaf on (b, d) with no same-label matched vertex to avoid, or a hoisted leaf (not
-// adjacent to d; at most kRowLeaves of them) whose label is not d's.  Then the only constraint on
-// x beyond adjacency to the row's vertex at b is injectivity against the same-label depths of d,
-// which tail_row corrects, and a row's sum needs no scan of its candidates.  Needs hp.tri_tab.
+// triangle leaf on (b, d), or a hoisted leaf (not adjacent to d; at most kRowLeaves of them), and
+// at most one leaf is a correction factor: an adjacent leaf with same-label matched vertices to
+// avoid, or a hoisted pendant leaf with d's label.  Then a row's sum needs no scan of its
+// candidates: tail_row corrects injectivity against the same-label depths of d, and
+// tail_fix_tile the correction factor over the intersection of the row's list with one other
+// list per source.  Needs hp.tri_tab.
 int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
     const DPlan &P = plan->h;
     hp.tail = nullptr;
     hp.tail_dep = 0;
+    hp.tail_fix = 0;
+    hp.tail_leaf = 0;
+    hp.tail_rbase = 0;
     if (!plan_fusable(plan)) return ASDM_OK;
     const uint32_t d = P.D - 1;
     if (P.nb[d] != 1) return ASDM_OK;
@@ -1181,18 +1292,31 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     TailSpec T{};
     T.arc_base = P.bbase[d][0];
     T.np = cs->csize[plan->order[b]];
-    uint32_t hoisted = 0;
+    uint32_t hoisted = 0, fix = 0, fix_leaf = 0;
+    uint64_t rbase = 0;
     for (uint32_t g = 0; g < P.ngroups; ++g) {
         const uint32_t e = P.gleaf[g][0];
         bool adj = false;
         for (uint32_t j = 0; j < P.nb[e]; ++j) adj |= P.bdep[e][j] == d;
         if (!adj) {
-            for (uint32_t s = 0; s < P.ns[e]; ++s)
-                if (P.sdep[e][s] == d) return ASDM_OK;  // shares d's label
+            bool same_d = false;  // shares d's label
+            for (uint32_t s = 0; s < P.ns[e]; ++s) same_d |= P.sdep[e][s] == d;
+            if (same_d) {
+                if (fix || P.nb[e] != 1) return ASDM_OK;
+                fix = kTailFixHoist;
+                fix_leaf = e;
+            }
             if (++hoisted > kRowLeaves) return ASDM_OK;
             continue;
         }
-        if (P.ns[e] != 0 || T.nadj == kTailLeaves) return ASDM_OK;
+        if (T.nadj == kTailLeaves) return ASDM_OK;
+        if (P.ns[e] != 0) {
+            const uint32_t ra = cs->arc_index(plan->order[e], plan->order[d]);
+            if (fix || ra == UINT32_MAX) return ASDM_OK;
+            fix = kTailFixLeaf;
+            fix_leaf = e;
+            rbase = cs->ce_base[ra];
+        }
         if (P.nb[e] == 1) {
             T.pend_base[T.nadj] = P.bbase[e][0];
             T.tri_tab[T.nadj] = nullptr;
@@ -1218,6 +1342,9 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     }
     hp.tail = tab;
     hp.tail_dep = b;
+    hp.tail_fix = fix;
+    hp.tail_leaf = fix_leaf;
+    hp.tail_rbase = rbase;
     return ASDM_OK;
 }
 
@@ -1734,6 +1861,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
             for (uint32_t j = 0; j < hp[i].nb[d]; ++j) hp[i].bbase[d][j] += ib;
             if (hp[i].tri[d]) hp[i].tri_base[d] += ib;
         }
+        if (hp[i].tail_fix == kTailFixLeaf) hp[i].tail_rbase += ib;
         hp[i].fused = plan_fusable(plans[i]);
         maxD = std::max(maxD, hp[i].D);
         seg_start_h[i + 1] = seg_start_h[i] + cs->csize[plans[i]->order[0]];

@@ -193,11 +193,12 @@ def test_row_sum_table_against_bruteforce(ctx):
 
 
 def test_row_sum_table_corrections_against_bruteforce(ctx):
-    """Forced plans with same-label constraints around the last enumerated vertex d: a same-label
-    earlier depth of d (query 1: the row-sum table, corrected at that vertex), a hoisted leaf with
-    d's label (2), a same-label matched vertex of a pendant leaf (3) and of a triangle leaf (4) of
-    d, and mixes of them (5, 6).  Queries 2-6 fall back to the per-candidate path; every count must
-    be the brute force's, and the table must have been used."""
+    """Forced plans with same-label constraints around the last enumerated vertex d, all on the
+    row-sum table: a same-label earlier depth of d (query 1, corrected at that vertex); one
+    correction factor -- a hoisted leaf with d's label (2), a same-label matched vertex of a pendant
+    leaf (3) or of a triangle leaf (4) of d -- alone or with a same-label depth (5).  Query 6 has
+    two correction factors and takes the per-candidate path.  Every count must be the brute
+    force's, and the table must have been used."""
     A_, B_, C_ = 0, 1, 2
     cases = (  # labels, edges, order, enumerated depths
         ([A_, B_, A_, C_], [(0, 1), (1, 2), (2, 3)], [1, 0, 2, 3], 3),
