@@ -880,9 +880,10 @@ __device__ inline uint64_t tail_row(const DPlan *__restrict__ P, const uint32_t 
 // the wave as in fused_tile.  For every row of the tile and every source of the factor (each
 // same-label matched vertex y of the adjacent correction leaf, or the hoisted correction leaf),
 // the injective candidates x of the row's list L that lie in the source's set -- the candidates
-// of d adjacent to y (the reverse arc's list at y), or the hoisted leaf's list -- each take
+// of d adjacent to y (the reverse arc's list at y), or the hoisted leaf's lists -- each take
 // coef * (product of the other adjacent leaves' counts at x) off the row's sum.  Lane k holds
-// row k's scan: the shorter of L and the set is scanned and the other probed.  Returns this
+// row k's scan: the shorter of L and the set (a hoisted leaf's first list) is scanned and the
+// other probed (with a hoisted leaf's other lists).  Returns this
 // lane's share, for rows of any lane.
 __device__ inline uint64_t tail_fix_tile(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows, uint64_t r0,
                                          uint32_t d, bool mine, const uint32_t *__restrict__ row, uint64_t coef,
@@ -937,26 +938,39 @@ __device__ inline uint64_t tail_fix_tile(const DPlan *__restrict__ P, const uint
             const uint32_t *Lv = ce_val + llo;
             uint32_t x = 0xffffffffu, idx = 0;
             bool member = false;
+            const uint32_t *rk = rows + (r0 + (uint32_t)k) * d;
+            uint32_t q = 0xffffffffu;  // hoisted correction leaf: x's position in C(u_fl)
             if (sl) {
                 idx = t;
                 x = Lv[t];
                 if (fix == kTailFixLeaf) {
                     member = contains_u32(ce_val + slo, slen, x);
                 } else {
-                    const uint32_t q = cand_pos(P, fl, cand[P->coff[d] + x]);
+                    q = cand_pos(P, fl, cand[P->coff[d] + x]);
                     member = q != 0xffffffffu && contains_u32(ce_val + slo, slen, q);
                 }
             } else {
                 const uint32_t e = ce_val[slo + t];
-                x = fix == kTailFixLeaf ? e : cand_pos(P, d, cand[P->coff[fl] + e]);
+                if (fix == kTailFixLeaf) {
+                    x = e;
+                } else {
+                    q = e;
+                    x = cand_pos(P, d, cand[P->coff[fl] + e]);
+                }
                 if (x != 0xffffffffu) {
                     idx = lower_bound_u32(Lv, llen, x);
                     member = idx < llen && Lv[idx] == x;
                 }
             }
+            if (fix == kTailFixHoist) {  // the hoisted leaf's other lists (a triangle leaf's second)
+                const RowView Rk{rk, d, 0};
+                for (uint32_t j = 1; j < P->nb[fl] && member; ++j) {
+                    const ListRef r = list_of_v(P, ce_off, Rk, fl, j);
+                    member = contains_u32(ce_val + r.lo, r.len, q);
+                }
+            }
             bytes += 8ull;
             if (!member) continue;
-            const uint32_t *rk = rows + (r0 + (uint32_t)k) * d;
             const uint32_t xid = cand[P->coff[d] + x];
             bool valid = true;
             for (uint32_t s2 = 0; s2 < P->ns[d]; ++s2) {
@@ -1274,7 +1288,7 @@ bool plan_fusable(const asdm_plan *plan);
 // adjacent leaves' list counts at x.  It applies when every leaf group is a pendant leaf of d or a
 // triangle leaf on (b, d), or a hoisted leaf (not adjacent to d; at most kRowLeaves of them), and
 // at most one leaf is a correction factor: an adjacent leaf with same-label matched vertices to
-// avoid, or a hoisted pendant leaf with d's label.  Then a row's sum needs no scan of its
+// avoid, or a hoisted leaf with d's label.  Then a row's sum needs no scan of its
 // candidates: tail_row corrects injectivity against the same-label depths of d, and
 // tail_fix_tile the correction factor over the intersection of the row's list with one other
 // list per source.  Needs hp.tri_tab.
@@ -1302,7 +1316,7 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
             bool same_d = false;  // shares d's label
             for (uint32_t s = 0; s < P.ns[e]; ++s) same_d |= P.sdep[e][s] == d;
             if (same_d) {
-                if (fix || P.nb[e] != 1) return ASDM_OK;
+                if (fix) return ASDM_OK;
                 fix = kTailFixHoist;
                 fix_leaf = e;
             }

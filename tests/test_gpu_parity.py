@@ -196,9 +196,9 @@ def test_row_sum_table_corrections_against_bruteforce(ctx):
     """Forced plans with same-label constraints around the last enumerated vertex d, all on the
     row-sum table: a same-label earlier depth of d (query 1, corrected at that vertex); one
     correction factor -- a hoisted leaf with d's label (2), a same-label matched vertex of a pendant
-    leaf (3) or of a triangle leaf (4) of d -- alone or with a same-label depth (5).  Query 6 has
-    two correction factors and takes the per-candidate path.  Every count must be the brute
-    force's, and the table must have been used."""
+    leaf (3) or of a triangle leaf (4) of d, a hoisted triangle leaf with d's label (7) -- alone or
+    with a same-label depth (5).  Query 6 has two correction factors and takes the per-candidate
+    path.  Every count must be the brute force's, and the table must have been used."""
     A_, B_, C_ = 0, 1, 2
     cases = (  # labels, edges, order, enumerated depths
         ([A_, B_, A_, C_], [(0, 1), (1, 2), (2, 3)], [1, 0, 2, 3], 3),
@@ -207,6 +207,7 @@ def test_row_sum_table_corrections_against_bruteforce(ctx):
         ([A_, B_, C_, A_], [(0, 1), (1, 2), (1, 3), (2, 3)], [1, 0, 2, 3], 3),
         ([A_, B_, A_, A_], [(0, 1), (1, 2), (2, 3)], [1, 0, 2, 3], 3),
         ([B_, A_, C_, A_, C_, A_], [(0, 1), (0, 2), (2, 3), (3, 4), (0, 5)], [0, 2, 5, 3, 1, 4], 4),
+        ([B_, C_, A_, A_], [(0, 1), (0, 2), (1, 2), (1, 3)], [0, 1, 3, 2], 3),
     )
     rng = random.Random(33)
     used = 0
