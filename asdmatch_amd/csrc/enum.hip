@@ -1625,6 +1625,10 @@ struct Engine {
         if (ctx->row_limit && rows_total > ctx->row_limit)
             return ctx->fail(ASDM_E_TIMEOUT, "per-query row limit exceeded");
         if (d == D) return leaf(r, n);
+        // rows of width D-1 of a fused plan: the fused kernel scans (or reads from the row-sum
+        // table) their depth-(D-1) candidates itself, so they need no child count, and the rows
+        // it finishes are never materialized (nor counted against the row limit)
+        if (fused && d + 1 == D) return leaf_fused(r, n);
         int rc = ensure_scan(d);
         if (rc) return rc;
         ++launches;
@@ -1640,14 +1644,6 @@ struct Engine {
         if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&total, offs[d] + n, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
         if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (rc || total == 0) return rc;
-        if (fused && d + 1 == D) {
-            // the leaf-depth rows are evaluated inside the fused kernel, never stored
-            rows_total += total;
-            if (ctx->row_limit && rows_total > ctx->row_limit)
-                return ctx->fail(ASDM_E_TIMEOUT, "per-query row limit exceeded");
-            fanout_last = std::max<uint64_t>(1, total / n);
-            return leaf_fused(r, n);
-        }
         const uint64_t c = cap[d + 1];
         if (total <= c) return fill_and_recurse(d, r, 0, n, 0, total);
         // split the rows so that each chunk's output fits the next level
