@@ -55,6 +55,11 @@ struct DPlan {
     // has the label of depth D-1 (its count drops by one at the x in its list); 0 = none.
     uint32_t tail_fix, tail_leaf;
     uint64_t tail_rbase;
+    // Second table, one level up: when the table's backward neighbour is depth D-2, which has a
+    // single backward neighbour tail2_dep, tail2[p] = sum over the candidates x' of D-2 adjacent
+    // to position p of C(u_tail2_dep) of tail[x'], and rows of width D-2 are finished from it.
+    const uint64_t *tail2;
+    uint32_t tail2_dep;
     uint32_t gsize[kMaxQ];
     uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
     // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient

@@ -795,9 +795,11 @@ struct TailSpec {
     uint32_t nadj;      // adjacent leaves
     uint64_t pend_base[kTailLeaves];       // pendant leaf: list offsets of the arc (u_d -> u_e)
     const uint32_t *tri_tab[kTailLeaves];  // triangle leaf: its table (nullptr for a pendant leaf)
+    const uint64_t *inner;                 // second table: the entry of x is inner[x] (no leaves)
 };
 
-// tail[p] for every position p of C(u_tail_dep): a wave per position, lanes split its list.
+// tail[p] for every position p of C(u_tail_dep): a wave per position, lanes split its list.  With
+// T.inner set (the second table), the entry of a candidate x is inner[x].
 __global__ void __launch_bounds__(256) k_tail_sums(TailSpec T, const uint64_t *__restrict__ ce_off,
                                                    const uint32_t *__restrict__ ce_val, uint64_t *__restrict__ tail) {
     const int lane = threadIdx.x & 63;
@@ -809,7 +811,7 @@ __global__ void __launch_bounds__(256) k_tail_sums(TailSpec T, const uint64_t *_
         uint64_t s = 0;
         for (uint64_t i = lo + lane; i < hi; i += 64) {
             const uint32_t x = ce_val[i];
-            uint64_t prod = 1;
+            uint64_t prod = T.inner ? T.inner[x] : 1;
             for (uint32_t k = 0; k < T.nadj; ++k)
                 prod *= T.tri_tab[k] ? (uint64_t)T.tri_tab[k][i - e_first]
                                      : ce_off[T.pend_base[k] + x + 1] - ce_off[T.pend_base[k] + x];
@@ -873,6 +875,38 @@ __device__ inline uint64_t tail_row(const DPlan *__restrict__ P, const uint32_t 
         const uint32_t t = lower_bound_u32(Lv, L.len, x);
         if (t < L.len && Lv[t] == x) s0 -= tail_clean(P, x, L.lo + t, H, ce_off);
     }
+    return s0 * H0;
+}
+
+// A row of width c = D-2 of a plan with a second table: c's candidates x' depend on the row only
+// through its vertex at tail2_dep, and the row's sum is H0 * (tail2[row[tail2_dep]] less tail[x']
+// for each x' that is the matched vertex of a same-label depth of c).  Every hoisted leaf reads
+// depths below c only, and no same-label set of depth D-1 or of a leaf holds c.
+__device__ inline uint64_t tail2_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t c,
+                                     const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
+                                     const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                     uint64_t &bytes) {
+    uint64_t s0 = P->tail2[row[P->tail2_dep]];
+    const RowView R0{row, c, 0};
+    if (P->ns[c]) {
+        const ListRef L = list_of(P, ce_off, row, c, 0);
+        const uint32_t *Lv = ce_val + L.lo;
+        for (uint32_t s2 = 0; s2 < P->ns[c]; ++s2) {
+            const uint32_t x = cand_pos(P, c, matched_id(P, cand, R0, P->sdep[c][s2]));
+            if (x == 0xffffffffu) continue;
+            const uint32_t t = lower_bound_u32(Lv, L.len, x);
+            if (t < L.len && Lv[t] == x) s0 -= P->tail[x];
+        }
+    }
+    uint64_t H0 = 1;
+    for (uint32_t k = 0; k < H.n && s0; ++k) {
+        const uint32_t g = H.g[k], e = P->gleaf[g][0];
+        uint32_t y[kFusedSame];
+        const uint32_t ns = P->ns[e];
+        for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
+        H0 *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+    }
+    bytes += 4ull * c + 8ull + 16ull * H.n;
     return s0 * H0;
 }
 
@@ -993,9 +1027,17 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
                                       const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
                                       uint32_t *__restrict__ counts, uint64_t &bytes) {
     const int lane = threadIdx.x & 63;
-    const HoistedGroups H = hoisted_groups(P, d);
     const bool mine = (uint32_t)lane < nrows;
     const uint32_t *row = rows + (r0 + (mine ? (uint32_t)lane : 0u)) * d;
+    if (P->tail2 && d + 2 == P->D) {  // rows of width D-2 finished from the second table
+        uint64_t v = 0;
+        if (mine) {
+            v = tail2_row(P, row, d, hoisted_groups(P, d + 1), ce_off, ce_val, cand, bytes);
+            if (counts) counts[r0 + lane] = 0;
+        }
+        return v;
+    }
+    const HoistedGroups H = hoisted_groups(P, d);
     if (P->tail) {
         uint64_t v = 0, coef = 0;
         ListRef L{0, 0};
@@ -1055,19 +1097,19 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
     return acc;
 }
 
-// Rows of width D-1 -> sum over their valid depth-(D-1) candidates x of the leaf product of
+// Rows of width d = D-1 -> sum over their valid depth-(D-1) candidates x of the leaf product of
 // (row, x), in flattened tiles of 64 rows (fused_tile), one wave per tile; the depth-D frontier is
-// never written.  Five waves per SIMD, as k_bfused_flat.
+// never written.  With a second table the rows have width d = D-2.  Five waves per SIMD, as
+// k_bfused_flat.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_leaf_fused(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
                                                     uint64_t nrows, const uint64_t *__restrict__ ce_off,
                                                     const uint32_t *__restrict__ ce_val,
                                                     const uint32_t *__restrict__ cand,
                                                     unsigned long long *__restrict__ total,
-                                                    unsigned long long *__restrict__ bytes_ctr) {
+                                                    unsigned long long *__restrict__ bytes_ctr, uint32_t d) {
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t ntiles = (nrows + 63) / 64;
-    const uint32_t d = P->D - 1;
     uint64_t acc = 0, bytes = 0;
     for (uint64_t t = wave; t < ntiles; t += nwaves) {
         const uint64_t r0 = t * 64;
@@ -1296,6 +1338,8 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     const DPlan &P = plan->h;
     hp.tail = nullptr;
     hp.tail_dep = 0;
+    hp.tail2 = nullptr;
+    hp.tail2_dep = 0;
     hp.tail_fix = 0;
     hp.tail_leaf = 0;
     hp.tail_rbase = 0;
@@ -1359,6 +1403,39 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     hp.tail_fix = fix;
     hp.tail_leaf = fix_leaf;
     hp.tail_rbase = rbase;
+    // Second table: b = D-2 =: c has a single backward neighbour b2, no correction factor and no
+    // same-label depth of d, and no hoisted leaf reads c (as a neighbour or a same-label vertex).
+    // Then c's candidates depend on a row of width c only through its vertex at b2, and each
+    // contributes its tail entry.
+    const uint32_t c = b;
+    if (fix || P.ns[d] || c + 2 != P.D || c == 0 || P.nb[c] != 1) return ASDM_OK;
+    for (uint32_t g = 0; g < P.ngroups; ++g) {
+        const uint32_t e = P.gleaf[g][0];
+        bool adj_d = false, reads_c = false;
+        for (uint32_t j = 0; j < P.nb[e]; ++j) {
+            adj_d |= P.bdep[e][j] == d;
+            reads_c |= P.bdep[e][j] == c;
+        }
+        for (uint32_t s = 0; s < P.ns[e]; ++s) reads_c |= P.sdep[e][s] == c;
+        if (!adj_d && reads_c) return ASDM_OK;  // adjacent leaves are inside tail
+    }
+    TailSpec T2{};
+    T2.arc_base = P.bbase[c][0];
+    T2.np = cs->csize[plan->order[P.bdep[c][0]]];
+    T2.inner = tab;
+    uint64_t *tab2 = nullptr;
+    if ((rc = dev_alloc(ctx, (void **)&tab2, 8ull * std::max<uint32_t>(T2.np, 1)))) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(tab2));
+    if (T2.np) {
+        KTimer kt(ctx, "tail_sums");
+        const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)T2.np + 3) / 4, 1u << 16);
+        hipLaunchKernelGGL(k_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, T2, cs->d_ce_off, cs->d_ce_val, tab2);
+        const uint32_t a = cs->arc_index(plan->order[P.bdep[c][0]], plan->order[c]);
+        kt.stop(24ull * T2.np + 12ull * cs->arc_weight[a]);
+        if ((rc = check_hip(ctx, hipGetLastError(), "second tail table"))) return rc;
+    }
+    hp.tail2 = tab2;
+    hp.tail2_dep = P.bdep[c][0];
     return ASDM_OK;
 }
 
@@ -1477,7 +1554,8 @@ struct Engine {
     }
 
     // Fused last level: rows at depth D-1, leaf products summed over their depth-(D-1) candidates.
-    int leaf_fused(uint32_t *r, uint64_t n) {
+    // rows r[0, n) of width w (D-1, or D-2 with a second table)
+    int leaf_fused(uint32_t *r, uint64_t n, uint32_t w) {
         // about kLeafChunk candidates per launch, but never fewer rows than fill the chip with
         // 64-row tiles (2^19 rows = 8192 waves, 32 per CU)
         const uint64_t chunk = std::max<uint64_t>(kFusedMinRows, kLeafChunk / std::max<uint64_t>(1, fanout_last));
@@ -1486,9 +1564,9 @@ struct Engine {
             if (i0 && timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
             ++launches;
             KTimer kt(ctx, "leaf_fused");
-            const uint32_t *rr = r + i0 * (D - 1);
+            const uint32_t *rr = r + i0 * w;
             hipLaunchKernelGGL(k_leaf_fused, dim3(groups_grid((m + 63) / 64, 64)), dim3(256), 0, ctx->stream, dplan, rr, m,
-                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2);
+                               cs->d_ce_off, cs->d_ce_val, cs->d_cand, d_total, d_bytes + 2, w);
             kt.stop();
             int rc = check_hip(ctx, hipGetLastError(), "fused leaf kernel");
             if (rc) return rc;
@@ -1628,7 +1706,8 @@ struct Engine {
         // rows of width D-1 of a fused plan: the fused kernel scans (or reads from the row-sum
         // table) their depth-(D-1) candidates itself, so they need no child count, and the rows
         // it finishes are never materialized (nor counted against the row limit)
-        if (fused && d + 1 == D) return leaf_fused(r, n);
+        if (fused && d + 1 == D) return leaf_fused(r, n, d);
+        if (fused && hplan.tail2 && d + 2 == D) return leaf_fused(r, n, d);
         int rc = ensure_scan(d);
         if (rc) return rc;
         ++launches;
@@ -1790,6 +1869,12 @@ uint32_t lane_class(const asdm_plan *plan, const asdm_cs *cs, double scale) {
 // Rows per wave tile for lane width G: `iters` rows per lane group.
 inline uint32_t tile_rows_of(uint32_t G, uint32_t iters) { return 64u / G * iters; }
 
+// Rows of width d of this plan are finished by the fused kernel: d = D-1, or d = D-2 with a second
+// row-sum table.
+inline bool fused_at(const DPlan &P, uint32_t d) {
+    return P.fused && (d + 1 == P.D || (P.tail2 && d + 2 == P.D));
+}
+
 inline unsigned tiles_grid(uint64_t tiles) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((tiles + 3) / 4, 1u << 16));
 }
@@ -1946,9 +2031,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
             // one launch per run of queries with the same row kind and lane width at this depth:
             // leaf-depth rows (inclusion-exclusion over leaf lists) use tune_leaf_lanes, the others
             // their class
-            auto kind_of = [&](uint32_t i) {
-                return hp[i].D == d ? kLeafRows : (hp[i].fused && d + 1 == hp[i].D) ? kFusedRows : kExpandRows;
-            };
+            auto kind_of = [&](uint32_t i) { return hp[i].D == d ? kLeafRows : fused_at(hp[i], d) ? kFusedRows : kExpandRows; };
             // fused rows run flattened (lane width 0 in the key: 64-row tiles, all lanes pooled)
             auto lanes = [&](uint32_t i) {
                 const int k = kind_of(i);
@@ -2029,7 +2112,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         {
             // rows of leaf-depth and fused queries have no children: they get no tiles and join
             // any run
-            auto finished = [&](uint32_t i) { return hp[i].D == d || (hp[i].fused && d + 1 == hp[i].D); };
+            auto finished = [&](uint32_t i) { return hp[i].D == d || fused_at(hp[i], d); };
             if ((rc = tile_prefix([&](uint32_t i) { return (uint64_t)tile_rows_of(qcls[i], ctx->tune_tile_iters); },
                                   finished, tp_fill_h, tp_fill)))
                 break;
