@@ -60,6 +60,11 @@ struct DPlan {
     // to position p of C(u_tail2_dep) of tail[x'], and rows of width D-2 are finished from it.
     const uint64_t *tail2;
     uint32_t tail2_dep;
+    // Edge table of a fused level whose depth D-1 has two backward neighbours b1 < b2 that are
+    // adjacent in the query: etail[k] = sum over x in L(b1 -> D-1) ∩ L(b2 -> D-1) of the product of
+    // the adjacent leaves' list counts at x, for the k-th candidate edge (p1, p2) of the arc
+    // (u_b1 -> u_b2).  nullptr when the level has none.
+    const uint64_t *etail;
     uint32_t gsize[kMaxQ];
     uint32_t gleaf[kMaxQ][kMaxGroup];  // depth of each leaf in the group
     // set partitions of {0..k-1} for k = 2..4: blocks as leaf-index masks, Moebius coefficient

@@ -878,6 +878,107 @@ __device__ inline uint64_t tail_row(const DPlan *__restrict__ P, const uint32_t 
     return s0 * H0;
 }
 
+// Edge table (DPlan::etail) of a fused level with two adjacent backward neighbours b1 < b2 of d:
+// a wave per candidate edge (p1, p2) of the arc (u_b1 -> u_b2); lanes split the shorter of
+// L1 = L(b1 -> d)(p1) and L2 = L(b2 -> d)(p2) and binary-search the other.  A triangle leaf on
+// (b_j, d) reads its table at x's candidate edge in L_j.
+struct EdgeTailSpec {
+    uint64_t arc_base;   // list offsets of the arc (u_b1 -> u_b2)
+    uint32_t np1;        // |C(u_b1)|
+    uint64_t nedges;     // candidate edges of the arc
+    uint64_t l1_base, l2_base;  // list offsets of the arcs (u_b1 -> u_d) and (u_b2 -> u_d)
+    uint32_t nadj;
+    uint64_t pend_base[kTailLeaves];
+    const uint32_t *tri_tab[kTailLeaves];
+    uint32_t tri_list[kTailLeaves];  // triangle leaf on (b1, d): 0; on (b2, d): 1
+};
+
+__global__ void __launch_bounds__(256) k_edge_tail_sums(EdgeTailSpec T, const uint64_t *__restrict__ ce_off,
+                                                        const uint32_t *__restrict__ ce_val, uint64_t *__restrict__ tab) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t e_first = ce_off[T.arc_base];
+    const uint64_t f1 = ce_off[T.l1_base], f2 = ce_off[T.l2_base];  // first candidate edges of L1's and L2's arcs
+    for (uint64_t k = wave; k < T.nedges; k += nwaves) {
+        uint32_t lo = 0, hi = T.np1;  // p1: the last position whose list starts at or before edge k
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ce_off[T.arc_base + mid] <= e_first + k) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t p1 = lo, p2 = ce_val[e_first + k];
+        const uint64_t a0 = ce_off[T.l1_base + p1], a1 = ce_off[T.l1_base + p1 + 1];
+        const uint64_t b0 = ce_off[T.l2_base + p2], b1 = ce_off[T.l2_base + p2 + 1];
+        const bool scan1 = a1 - a0 <= b1 - b0;
+        const uint64_t s0 = scan1 ? a0 : b0, s1 = scan1 ? a1 : b1, o0 = scan1 ? b0 : a0;
+        const uint32_t olen = (uint32_t)(scan1 ? b1 - b0 : a1 - a0);
+        uint64_t s = 0;
+        for (uint64_t i = s0 + lane; i < s1; i += 64) {
+            const uint32_t x = ce_val[i];
+            const uint32_t t = lower_bound_u32(ce_val + o0, olen, x);
+            if (t >= olen || ce_val[o0 + t] != x) continue;
+            const uint64_t i1 = scan1 ? i : o0 + t, i2 = scan1 ? o0 + t : i;  // x's global index in L1, L2
+            uint64_t prod = 1;
+            for (uint32_t j = 0; j < T.nadj; ++j)
+                prod *= T.tri_tab[j] ? (uint64_t)T.tri_tab[j][T.tri_list[j] ? i2 - f2 : i1 - f1]
+                                     : ce_off[T.pend_base[j] + x + 1] - ce_off[T.pend_base[j] + x];
+            s += prod;
+        }
+        s = group_sum<64>(s);
+        if (lane == 0) tab[k] = s;
+    }
+}
+
+// A row of width d = D-1 of a plan with an edge table: H0 * etail[edge (row[b1], row[b2])], less
+// H0 * f0(x) for each x in L1 ∩ L2 that is the matched vertex of a same-label depth of d.
+__device__ inline uint64_t etail_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t d,
+                                     const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
+                                     const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
+                                     uint64_t &bytes) {
+    const uint32_t b1 = P->bdep[d][0], b2 = P->bdep[d][1];
+    uint64_t ab = 0;  // list offsets of the arc (u_b1 -> u_b2)
+    for (uint32_t j = 0; j < P->nb[b2]; ++j)
+        if (P->bdep[b2][j] == b1) ab = P->bbase[b2][j];
+    const uint64_t lo = ce_off[ab + row[b1]];
+    const uint32_t len = (uint32_t)(ce_off[ab + row[b1] + 1] - lo);
+    const uint32_t k = lower_bound_u32(ce_val + lo, len, row[b2]);  // (p1, p2) is a candidate edge
+    uint64_t s0 = P->etail[lo + k - ce_off[ab]];
+    const RowView R0{row, d, 0};
+    if (P->ns[d]) {
+        const ListRef L1 = list_of(P, ce_off, row, d, 0), L2 = list_of(P, ce_off, row, d, 1);
+        for (uint32_t s2 = 0; s2 < P->ns[d]; ++s2) {
+            const uint32_t x = cand_pos(P, d, matched_id(P, cand, R0, P->sdep[d][s2]));
+            if (x == 0xffffffffu) continue;
+            const uint32_t t1 = lower_bound_u32(ce_val + L1.lo, L1.len, x);
+            const uint32_t t2 = lower_bound_u32(ce_val + L2.lo, L2.len, x);
+            if (t1 >= L1.len || ce_val[L1.lo + t1] != x || t2 >= L2.len || ce_val[L2.lo + t2] != x) continue;
+            uint64_t f = 1;
+            for (uint32_t g = 0; g < P->ngroups; ++g) {
+                if (H.mask >> g & 1) continue;
+                const uint32_t e = P->gleaf[g][0];
+                if (P->tri[e]) {
+                    const bool on1 = P->bdep[e][0] == b1;
+                    f *= P->tri_tab[e][(on1 ? L1.lo + t1 : L2.lo + t2) - ce_off[P->tri_base[e]]];
+                } else {
+                    f *= ce_off[P->bbase[e][0] + x + 1] - ce_off[P->bbase[e][0] + x];
+                }
+            }
+            s0 -= f;
+        }
+    }
+    uint64_t H0 = 1;
+    for (uint32_t k2 = 0; k2 < H.n && s0; ++k2) {
+        const uint32_t g = H.g[k2], e = P->gleaf[g][0];
+        uint32_t y[kFusedSame];
+        const uint32_t ns = P->ns[e];  // d is not among them: no hoisted leaf has d's label
+        for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
+        H0 *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
+    }
+    bytes += 4ull * d + 16ull + 8ull + 16ull * H.n;
+    return s0 * H0;
+}
+
 // A row of width c = D-2 of a plan with a second table: c's candidates x' depend on the row only
 // through its vertex at tail2_dep, and the row's sum is H0 * (tail2[row[tail2_dep]] less tail[x']
 // for each x' that is the matched vertex of a same-label depth of c).  Every hoisted leaf reads
@@ -1038,6 +1139,14 @@ __device__ inline uint64_t fused_tile(const DPlan *__restrict__ P, const uint32_
         return v;
     }
     const HoistedGroups H = hoisted_groups(P, d);
+    if (P->etail) {
+        uint64_t v = 0;
+        if (mine) {
+            v = etail_row(P, row, d, H, ce_off, ce_val, cand, bytes);
+            if (counts) counts[r0 + lane] = 0;
+        }
+        return v;
+    }
     if (P->tail) {
         uint64_t v = 0, coef = 0;
         ListRef L{0, 0};
@@ -1325,6 +1434,66 @@ unsigned groups_grid(uint64_t items, int G) {
 
 bool plan_fusable(const asdm_plan *plan);
 
+// Edge table of a fused plan (DPlan::etail) when depth d = D-1 has two backward neighbours b1 < b2
+// that are adjacent in the query: the row's candidates are L1 ∩ L2, a function of the candidate
+// edge (row[b1], row[b2]).  Every leaf group must be a pendant leaf of d or a triangle leaf on
+// (b1, d) or (b2, d) with no same-label matched vertex to avoid, or a hoisted leaf (at most
+// kRowLeaves) whose label is not d's; same-label depths of d are corrected per row.
+int build_edge_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp,
+                          std::vector<uint32_t *> &tabs) {
+    const DPlan &P = plan->h;
+    const uint32_t d = P.D - 1, b1 = P.bdep[d][0], b2 = P.bdep[d][1];
+    uint32_t jb = UINT32_MAX;
+    for (uint32_t j = 0; j < P.nb[b2]; ++j)
+        if (P.bdep[b2][j] == b1) jb = j;
+    if (jb == UINT32_MAX) return ASDM_OK;
+    const uint32_t a12 = cs->arc_index(plan->order[b1], plan->order[b2]);
+    if (a12 == UINT32_MAX) return ASDM_OK;
+    EdgeTailSpec T{};
+    T.arc_base = P.bbase[b2][jb];
+    T.np1 = cs->csize[plan->order[b1]];
+    T.nedges = cs->arc_weight[a12];
+    T.l1_base = P.bbase[d][0];
+    T.l2_base = P.bbase[d][1];
+    uint32_t hoisted = 0;
+    for (uint32_t g = 0; g < P.ngroups; ++g) {
+        const uint32_t e = P.gleaf[g][0];
+        bool adj = false;
+        for (uint32_t j = 0; j < P.nb[e]; ++j) adj |= P.bdep[e][j] == d;
+        if (!adj) {
+            for (uint32_t s = 0; s < P.ns[e]; ++s)
+                if (P.sdep[e][s] == d) return ASDM_OK;  // shares d's label
+            if (++hoisted > kRowLeaves) return ASDM_OK;
+            continue;
+        }
+        if (P.ns[e] != 0 || T.nadj == kTailLeaves) return ASDM_OK;
+        if (P.nb[e] == 1) {
+            T.pend_base[T.nadj] = P.bbase[e][0];
+            T.tri_tab[T.nadj] = nullptr;
+        } else if (P.tri[e] && P.nb[e] == 2 && P.bdep[e][1] == d && (P.bdep[e][0] == b1 || P.bdep[e][0] == b2) &&
+                   hp.tri_tab[e]) {
+            T.tri_tab[T.nadj] = hp.tri_tab[e];
+            T.tri_list[T.nadj] = P.bdep[e][0] == b2 ? 1u : 0u;
+        } else {
+            return ASDM_OK;
+        }
+        ++T.nadj;
+    }
+    uint64_t *tab = nullptr;
+    int rc = dev_alloc(ctx, (void **)&tab, 8ull * std::max<uint64_t>(T.nedges, 1));
+    if (rc) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(tab));
+    if (T.nedges) {
+        KTimer kt(ctx, "edge_tail_sums");
+        const unsigned grid = (unsigned)std::min<uint64_t>((T.nedges + 3) / 4, 1u << 16);
+        hipLaunchKernelGGL(k_edge_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, T, cs->d_ce_off, cs->d_ce_val, tab);
+        kt.stop(40ull * T.nedges);
+        if ((rc = check_hip(ctx, hipGetLastError(), "edge tail table"))) return rc;
+    }
+    hp.etail = tab;
+    return ASDM_OK;
+}
+
 // Row-sum table of a fused plan (DPlan::tail) when depth d = D-1 has a single backward neighbour
 // b: tail[p] = sum over the candidates x adjacent to position p of C(u_b) of the product of the
 // adjacent leaves' list counts at x.  It applies when every leaf group is a pendant leaf of d or a
@@ -1340,11 +1509,13 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     hp.tail_dep = 0;
     hp.tail2 = nullptr;
     hp.tail2_dep = 0;
+    hp.etail = nullptr;
     hp.tail_fix = 0;
     hp.tail_leaf = 0;
     hp.tail_rbase = 0;
     if (!plan_fusable(plan)) return ASDM_OK;
     const uint32_t d = P.D - 1;
+    if (P.nb[d] == 2) return build_edge_tail_table(ctx, plan, cs, hp, tabs);
     if (P.nb[d] != 1) return ASDM_OK;
     const uint32_t b = P.bdep[d][0];
     TailSpec T{};

@@ -264,6 +264,40 @@ def test_second_row_sum_table_against_bruteforce(ctx):
             assert tables == (2 if second else 1), (trial, ql, tables)
 
 
+def test_edge_row_sum_table_against_bruteforce(ctx):
+    """Forced plans whose last enumerated vertex d closes a triangle with its two backward
+    neighbours b1 < b2: its candidates are a function of the candidate edge (row[b1], row[b2]), and
+    rows are finished from an edge-indexed table (one edge_tail_sums launch) -- with a pendant leaf
+    (1), a same-label earlier depth of d (2), a triangle leaf on (b1, d) (3) or on (b2, d) (4) --
+    but not when a leaf of d has a same-label matched vertex to avoid (5)."""
+    cases = (  # labels, edges, order, enumerated depths, edge table expected
+        ([0, 1, 2, 3], [(0, 1), (0, 2), (1, 2), (2, 3)], [0, 1, 2, 3], 3, True),
+        ([1, 0, 2, 1, 3], [(0, 1), (1, 2), (1, 3), (2, 3), (3, 4)], [0, 1, 2, 3, 4], 4, True),
+        ([0, 1, 2, 3], [(0, 1), (0, 2), (1, 2), (0, 3), (2, 3)], [0, 1, 2, 3], 3, True),
+        ([0, 1, 2, 3], [(0, 1), (0, 2), (1, 2), (1, 3), (2, 3)], [0, 1, 2, 3], 3, True),
+        ([0, 1, 2, 0], [(0, 1), (0, 2), (1, 2), (2, 3)], [0, 1, 2, 3], 3, False),
+    )
+    rng = random.Random(55)
+    for trial in range(6):
+        gl, ge = _rand_graph(rng, 70, 0.16, 4)
+        g, og = T.graph_from_lists(gl, ge)
+        for ql, qe, order, D, edge in cases:
+            q, oq = T.graph_from_lists(ql, qe)
+            want = O.bruteforce_count(oq, og)
+            cs = ctx.filter(q, g)
+            if cs is None:
+                assert want == 0
+                continue
+            ctx.set_profiling(True)
+            try:
+                got = ctx.count(cs.plan(q, order=order, num_enumerated=D), cs)
+                tables = sum(k["launches"] for k in ctx.kernel_stats() if k["name"] == "edge_tail_sums")
+            finally:
+                ctx.set_profiling(False)
+            assert got == want, (trial, ql, qe, order, D)
+            assert tables == (1 if edge else 0), (trial, ql, tables)
+
+
 def test_local_counts_match_reference(ctx):
     for case in T.synth_golden():
         if not case["local"]:
