@@ -1434,6 +1434,8 @@ unsigned groups_grid(uint64_t items, int G) {
 
 bool plan_fusable(const asdm_plan *plan);
 
+constexpr uint64_t kEdgeTableMax = 1ull << 27;  // candidate edges of an edge table (8 B each: 1 GiB)
+
 // Edge table of a fused plan (DPlan::etail) when depth d = D-1 has two backward neighbours b1 < b2
 // that are adjacent in the query: the row's candidates are L1 ∩ L2, a function of the candidate
 // edge (row[b1], row[b2]).  Every leaf group must be a pendant leaf of d or a triangle leaf on
@@ -1453,6 +1455,7 @@ int build_edge_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *c
     T.arc_base = P.bbase[b2][jb];
     T.np1 = cs->csize[plan->order[b1]];
     T.nedges = cs->arc_weight[a12];
+    if (T.nedges > kEdgeTableMax) return ASDM_OK;  // the per-candidate path needs no table memory
     T.l1_base = P.bbase[d][0];
     T.l2_base = P.bbase[d][1];
     uint32_t hoisted = 0;
