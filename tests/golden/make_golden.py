@@ -16,9 +16,9 @@ Fixtures (data only: inputs and the reference's outputs):
                                text the generator writes) with the reference's counts and candidate
                                set sizes; local-count matrices as .npz
 Why ALL_LEVEL: the count build's factorized path returns counts that disagree with exhaustive
-enumeration on multi-bag queries and vary between runs of the same input; the
-ALL_LEVEL build enumerates every embedding (matching/join.cpp:819-907,1016-1097) and is the
-reference's own exact path.
+enumeration on multi-bag queries and vary between runs of the same input; the ALL_LEVEL build
+enumerates every embedding (matching/join.cpp:819-907,1016-1097) and is the reference's own exact
+path.
 """
 import hashlib
 import io
