@@ -1327,7 +1327,7 @@ __global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans,
                 const uint64_t v = leaf_row<G>(P, row, ce_off, ce_val, cand, bytes);
                 if (lg == 0) {
                     acc += v;
-                    counts[it] = 0;
+                    if (counts) counts[it] = 0;  // null at a level without expansions
                 }
             } else {
                 const uint32_t c = expand_count_row<G>(P, d, row, ce_off, ce_val, cand, bytes);
@@ -2198,9 +2198,16 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         dev_free(ctx, offs);
         counts_d = nullptr;
         offs = nullptr;
-        rc = dev_alloc(ctx, (void **)&counts_d, 4 * n);
-        if (!rc) rc = dev_alloc(ctx, (void **)&offs, 8 * (n + 1));
-        if (rc) break;
+        // a level at which no query expands is the last: its rows (leaf-depth or fused) need no
+        // child counts, no scan and no next level
+        bool any_expand = false;
+        for (uint32_t i = 0; i < Q && !any_expand; ++i)
+            any_expand = seg_start_h[i + 1] > seg_start_h[i] && hp[i].D != d && !fused_at(hp[i], d);
+        if (any_expand) {
+            rc = dev_alloc(ctx, (void **)&counts_d, 4 * n);
+            if (!rc) rc = dev_alloc(ctx, (void **)&offs, 8 * (n + 1));
+            if (rc) break;
+        }
         {
             // one launch per run of queries with the same row kind and lane width at this depth:
             // leaf-depth rows (inclusion-exclusion over leaf lists) use tune_leaf_lanes, the others
@@ -2246,6 +2253,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                 });
         }
         if ((rc = check_hip(ctx, hipGetLastError(), "batch count"))) break;
+        if (!any_expand) break;
         if ((rc = scan_u32_to_u64(ctx, counts_d, offs, n))) break;
         hipLaunchKernelGGL(k_bgather, dim3(1), dim3(256), 0, ctx->stream, offs, seg_start, Q, seg_next);
         std::vector<uint64_t> next_h(Q + 1);
