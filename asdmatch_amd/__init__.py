@@ -75,6 +75,7 @@ _sig("asdm_last_stats", C.c_int, _P, C.POINTER(Stats))
 _sig("asdm_graph_load_text", C.c_int, C.c_char_p, C.POINTER(_P))
 _sig("asdm_graph_from_edges", C.c_int, C.c_uint32, _u32p, C.c_uint64, _u32p, C.POINTER(_P))
 _sig("asdm_graph_write_text", C.c_int, _P, C.c_char_p)
+_sig("asdm_graph_save_binary", C.c_int, _P, C.c_char_p)
 _sig("asdm_graph_free", None, _P)
 _sig("asdm_graph_info", C.c_int, _P, _u32p, _u64p, _u32p)
 _sig("asdm_graph_csr", C.c_int, _P, _u32p, _u32p, _u32p)
@@ -215,6 +216,10 @@ class Graph:
 
     def write(self, path: str):
         self._check(_lib.asdm_graph_write_text(self._h, os.fsencode(path)), f"write {path}")
+
+    def save_binary(self, path: str):
+        """Binary CSR cache; Graph.load reads it back without parsing or sorting."""
+        self._check(_lib.asdm_graph_save_binary(self._h, os.fsencode(path)), f"save_binary {path}")
 
 
 class Context:

@@ -45,6 +45,8 @@ namespace asdm {
 
 // Sorts adjacency by (label, id) and builds the vertices-by-label index.
 void finish_graph(asdm_graph &g);
+// The indexes over a sorted CSR: vertices by label, mean degree per label, self-loop flag.
+void index_graph(asdm_graph &g);
 asdm_graph *graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges);
 void release_device_graph(asdm_graph &g);
 

@@ -77,8 +77,16 @@ const char *asdm_version(void);
 /* ---- graphs (host side; no device needed) ---- */
 /* Reproduces the reference loader: header 't N M' (M undirected), 'v id label deg' records
  * fixing offsets from declared degrees, 'e a b [extra tokens]' inserted in both directions,
- * adjacency sorted by (label(nbr), nbr).  Declared degrees must equal the edge records. */
+ * adjacency sorted by (label(nbr), nbr).  Declared degrees must equal the edge records.
+ * Replaces Graph::loadGraphFromTextFile (relation/graph.cpp:68-148).  Files of 32 MB and more are
+ * parsed by host threads, with the same result.  A file written by asdm_graph_save_binary (it
+ * starts with "ASDMCSR1") is read as that binary CSR instead. */
 int asdm_graph_load_text(const char *path, asdm_graph **out);
+/* Binary CSR cache of a loaded graph (labels, offsets, sorted adjacency); asdm_graph_load_text
+ * reads it back without parsing or sorting.  The reference has a binary graph format
+ * (relation/graph.cpp:293-332) that its main never uses; this one is for callers that load the
+ * same large graph many times. */
+int asdm_graph_save_binary(const asdm_graph *g, const char *path);
 /* Graph from n labels and ne undirected edges (edges[2*i], edges[2*i+1]), in record order. */
 int asdm_graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges,
                           asdm_graph **out);

@@ -79,6 +79,18 @@ def test_cli_counts_and_lines(tmp_path):
 
 
 @pytest.mark.gpu
+def test_cli_accepts_a_binary_data_graph(tmp_path):
+    # the binary CSR cache of the data graph gives the same counts as its text file
+    gold = T.yeast_golden()["queries"]
+    binpath = tmp_path / "yeast.csr"
+    A.Graph.load(T.yeast_data_path()).save_binary(str(binpath))
+    for name in [n for n in gold if not gold[n]["empty"]][:6]:
+        r = run(["ASDMatch.out", "-q", T.yeast_query_path(name), "-d", str(binpath)])
+        assert r.returncode == 0, r.stderr
+        assert int(r.stdout.strip().split("\n")[5].split(":")[1]) == gold[name]["count"], name
+
+
+@pytest.mark.gpu
 def test_cli_reads_reference_card_files():
     want = json.load(open(os.path.join(CARDS, "golden.json")))["counts"]
     for q, c in want.items():

@@ -61,6 +61,59 @@ def test_extra_tokens_on_edge_lines_are_ignored():
         os.remove(p)
 
 
+def _same_csr(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a.csr(), b.csr()))
+
+
+def test_parallel_parse_of_large_files_matches_sequential_and_oracle():
+    # 3M undirected edges: about 50 MB of text, above the parallel-parse threshold (32 MB)
+    g = A.Graph.chung_lu(200_000, 3_000_000, 2.5, 16, 1.0, 0x51)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "big.graph")
+        g.write(p)
+        assert os.path.getsize(p) > 32 << 20
+        h = A.Graph.load(p)
+        assert _same_csr(g, h)
+        assert all(np.array_equal(x, y) for x, y in zip(h.csr(), O.OGraph.load(p).csr()))
+        # the same records with the vertex lines after the first edge line: the sequential scan decides
+        lines = open(p).read().split("\n")
+        nv = 1 + g.n
+        moved = [lines[0], lines[nv]] + lines[1:nv] + lines[nv + 1:]
+        q = os.path.join(d, "moved.graph")
+        open(q, "w").write("\n".join(moved))
+        with pytest.raises(A.AsdmError):  # the reference's loader needs offsets before edges
+            A.Graph.load(q)
+        # a malformed record deep inside a chunk fails as the sequential scan does
+        bad = lines[:]
+        bad[len(bad) // 2] = "e 1"
+        r = os.path.join(d, "bad.graph")
+        open(r, "w").write("\n".join(bad))
+        with pytest.raises(A.AsdmError):
+            A.Graph.load(r)
+
+
+def test_binary_cache_round_trip():
+    g = A.Graph.chung_lu(5000, 40_000, 2.5, 7, 1.0, 0x52)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "g.csr")
+        g.save_binary(p)
+        h = A.Graph.load(p)
+        assert _same_csr(g, h)
+        assert h.info() == g.info()
+        # truncated or corrupted files are rejected
+        raw = open(p, "rb").read()
+        open(p, "wb").write(raw[:-4])
+        with pytest.raises(A.AsdmError):
+            A.Graph.load(p)
+        off, nbr, lab = g.csr()
+        body = bytearray(raw)
+        i = 24 + 4 * g.n + 4 * (g.n + 1)  # first adjacency entry
+        body[i:i + 4] = np.uint32(g.n + 5).tobytes()
+        open(p, "wb").write(bytes(body))
+        with pytest.raises(A.AsdmError):
+            A.Graph.load(p)
+
+
 def test_adjacency_sorted_by_label_then_id():
     g = A.Graph.chung_lu(500, 3000, 2.5, 5, 1.0, 1)
     off, nbr, lab = g.csr()
