@@ -101,6 +101,7 @@ _sig("asdm_plan_order", C.c_int, _P, _u32p, _u32p)
 _sig("asdm_plan_estimate", C.c_int, _P, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
 _sig("asdm_local_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p, _u64p)
+_sig("asdm_enumerate", C.c_int, _P, _P, _P, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, _u64p)
 _sig("asdm_intersect_k", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_batch_count", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _u64p, C.POINTER(C.c_float))
 _f32p = C.POINTER(C.c_float)
@@ -269,6 +270,15 @@ class Context:
         self._check(_lib.asdm_local_count(self._h, plan._h, cs._h, budget_bytes, _ptr(mat, C.c_uint64),
                                           C.byref(out)), "local_count")
         return mat, out.value
+
+    def enumerate(self, plan: "Plan", cs: "CandSpace", max_rows: int, budget_bytes: int = 0) -> np.ndarray:
+        """Every embedding as a row of data ids by query vertex (uint32[count, nq]); the plan must be
+        made with factor=False.  Raises AsdmError (ASDM_E_BUDGET) past max_rows embeddings."""
+        out = np.zeros((max(max_rows, 1), cs.nq), np.uint32)
+        n = C.c_uint64()
+        self._check(_lib.asdm_enumerate(self._h, plan._h, cs._h, budget_bytes, _ptr(out), max_rows, C.byref(n)),
+                    "enumerate")
+        return out[: n.value]
 
     def batch_count(self, g: Graph, qs, budget_bytes: int = 0):
         """Exact counts of many queries against one data graph in one batched pass; returns

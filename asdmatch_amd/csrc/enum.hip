@@ -1233,6 +1233,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     }
 }
 
+// Full rows (width w = |V(q)|, positions by depth) -> data ids by query vertex:
+// out[r * w + u_j] = C(u_j)[row[j]].
+__global__ void k_emit_ids(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows, uint64_t n, uint32_t w,
+                           const uint32_t *__restrict__ cand, uint32_t *__restrict__ out) {
+    const uint64_t total = n * w;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = i / w;
+        const uint32_t j = (uint32_t)(i - r * w);
+        out[r * w + P->u[j]] = cand[P->coff[j] + rows[i]];
+    }
+}
+
 __global__ void k_iota(uint32_t *p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = (uint32_t)i;
@@ -1673,6 +1685,13 @@ struct Engine {
     std::vector<uint64_t> hoff;       // host copy of offsets when splitting
     std::chrono::steady_clock::time_point t_start;
     bool fused = false;           // depth D-1 evaluated by k_leaf_fused
+    // Enumeration (asdm_enumerate): the plan's leaf depth (at most one) is expanded like the others,
+    // D = |V(q)|, and every full row is written out as data ids (enum_out may be null: count only).
+    bool enumerate = false;
+    uint32_t *enum_out = nullptr;
+    uint64_t enum_cap = 0, enum_n = 0;
+    uint32_t *enum_dev = nullptr;
+    uint64_t enum_dev_rows = 0;
     uint64_t fanout_last = 1;     // expected candidate-list length at depth D-1
 
     bool timed_out() const {
@@ -1685,7 +1704,7 @@ struct Engine {
     std::vector<uint32_t *> tri_tabs;
 
     int init() {
-        D = plan->D;
+        D = enumerate ? plan->h.nq : plan->D;
         hplan = plan->h;
         int rc = build_tri_tables(ctx, plan, cs, hplan, tri_tabs);
         if (!rc) rc = dev_alloc(ctx, (void **)&dplan, sizeof(DPlan));
@@ -1720,6 +1739,7 @@ struct Engine {
     }
 
     void release() {
+        dev_free(ctx, enum_dev);
         for (auto t : tri_tabs) dev_free(ctx, t);
         dev_free(ctx, dplan);
         for (auto p : rows) dev_free(ctx, p);
@@ -1846,6 +1866,30 @@ struct Engine {
     }
 
     // Leaf counting in launches of at most kLeafChunk rows, so that the limits are checked often.
+    // Full rows r[0, n) to the caller's buffer, as data ids by query vertex.
+    int emit(uint32_t *r, uint64_t n) {
+        if (enum_out && enum_n + n > enum_cap)
+            return ctx->fail(ASDM_E_BUDGET, "more embeddings than the output buffer holds");
+        if (enum_out) {
+            if (enum_dev_rows < n) {
+                dev_free(ctx, enum_dev);
+                enum_dev = nullptr;
+                enum_dev_rows = 0;
+                if (int rc = dev_alloc(ctx, (void **)&enum_dev, 4ull * D * n)) return rc;
+                enum_dev_rows = n;
+            }
+            const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n * D + 255) / 256, 1u << 16));
+            hipLaunchKernelGGL(k_emit_ids, dim3(grid), dim3(256), 0, ctx->stream, dplan, r, n, D, cs->d_cand, enum_dev);
+            if (int rc = check_hip(ctx, hipGetLastError(), "emit")) return rc;
+            if (int rc = check_hip(ctx, hipMemcpyAsync(enum_out + enum_n * D, enum_dev, 4ull * D * n, hipMemcpyDeviceToHost,
+                                                       ctx->stream), "D2H embeddings"))
+                return rc;
+            if (int rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync")) return rc;
+        }
+        enum_n += n;
+        return ASDM_OK;
+    }
+
     int leaf(uint32_t *r, uint64_t n) {
         for (uint64_t i0 = 0; i0 < n; i0 += kLeafChunk) {
             const uint64_t m = std::min<uint64_t>(kLeafChunk, n - i0);
@@ -1876,7 +1920,7 @@ struct Engine {
         rows_total += n;
         if (ctx->row_limit && rows_total > ctx->row_limit)
             return ctx->fail(ASDM_E_TIMEOUT, "per-query row limit exceeded");
-        if (d == D) return leaf(r, n);
+        if (d == D) return enumerate ? emit(r, n) : leaf(r, n);
         // rows of width D-1 of a fused plan: the fused kernel scans (or reads from the row-sum
         // table) their depth-(D-1) candidates itself, so they need no child count, and the rows
         // it finishes are never materialized (nor counted against the row limit)
@@ -1959,17 +2003,23 @@ uint64_t default_budget(asdm_ctx *ctx) {
 }
 
 int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget, uint64_t *mat_host,
-               uint64_t *count) {
+               uint64_t *count, bool enumerate = false, uint32_t *enum_out = nullptr, uint64_t enum_cap = 0) {
     if (!ctx || !plan || !cs || !count) return ASDM_E_INVALID;
     if (plan->h.nq != cs->nq) return ctx->fail(ASDM_E_INVALID, "plan and candidate space disagree");
     if (mat_host && !plan->local) return ctx->fail(ASDM_E_INVALID, "local counting needs a plan made with ASDM_PLAN_LOCAL");
+    // a plan without factorization has at most one leaf (the last vertex, counted in closed form)
+    if (enumerate && plan->D + 1 != plan->h.nq && plan->D != plan->h.nq)
+        return ctx->fail(ASDM_E_INVALID, "enumeration needs a plan made with ASDM_PLAN_NO_FACTOR");
     (void)hipSetDevice(ctx->device);
     auto t0 = std::chrono::steady_clock::now();
     Engine E{ctx, plan, cs};
     E.t_start = t0;
+    E.enumerate = enumerate;
+    E.enum_out = enum_out;
+    E.enum_cap = enum_cap;
     // Fuse the last enumerated level into leaf counting when there is one (count mode only); the
     // expected list length of that level (smallest arc fan-out) sizes the fused launches' chunks.
-    if (!mat_host && plan_fusable(plan)) {
+    if (!mat_host && !enumerate && plan_fusable(plan)) {
         E.fused = true;
         const uint32_t d = plan->D - 1;
         double best = 1e300;
@@ -2005,7 +2055,7 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
     E.release();
     (void)hipStreamSynchronize(ctx->stream);
     if (rc) return rc;
-    *count = res[0];
+    *count = enumerate ? E.enum_n : res[0];
     resolve_timers(ctx);
     add_kernel_bytes(ctx, "expand_count", res[1]);
     add_kernel_bytes(ctx, "expand_fill", res[2]);
@@ -2429,6 +2479,11 @@ int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, ui
                      uint64_t *count) {
     if (!mat) return ASDM_E_INVALID;
     return run_engine(ctx, plan, cs, budget_bytes, mat, count);
+}
+
+int asdm_enumerate(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint32_t *out,
+                   uint64_t cap_rows, uint64_t *count) {
+    return run_engine(ctx, plan, cs, budget_bytes, nullptr, count, true, out, out ? cap_rows : 0);
 }
 
 

@@ -169,6 +169,13 @@ int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
  * caller-owned host memory; *count = total.  The plan must be created with ASDM_PLAN_LOCAL. */
 int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes,
                      uint64_t *mat, uint64_t *count);
+/* COLLECT_RESULT: every embedding, out[i * nq + u] = data vertex of query vertex u in the i-th one,
+ * in no particular order; *count = their number.  Replaces the result vector the reference fills
+ * when built with COLLECT_RESULT (matching/join.cpp:1032-1034,1076-1078; matching/adaptive.cpp:131).
+ * The plan must be created with ASDM_PLAN_NO_FACTOR.  out == NULL counts only; more embeddings than
+ * cap_rows returns ASDM_E_BUDGET. */
+int asdm_enumerate(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes,
+                   uint32_t *out, uint64_t cap_rows, uint64_t *count);
 
 /* Per-kernel device timing (HIP events on the context stream), collected while profiling is on.
  * Turning profiling on or off clears the table.  No reference counterpart (measurement only). */

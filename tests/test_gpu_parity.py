@@ -298,6 +298,81 @@ def test_edge_row_sum_table_against_bruteforce(ctx):
             assert tables == (1 if edge else 0), (trial, ql, tables)
 
 
+def _embeddings_bruteforce(ql, qe, gl, ge):
+    """All injective label- and edge-preserving maps, as tuples of data ids by query vertex."""
+    gadj = {v: set() for v in range(len(gl))}
+    for a, b in ge:
+        gadj[a].add(b)
+        gadj[b].add(a)
+    out, f = [], []
+
+    def rec(u):
+        if u == len(ql):
+            out.append(tuple(f))
+            return
+        for v in range(len(gl)):
+            if gl[v] != ql[u] or v in f:
+                continue
+            if all(f[w] in gadj[v] for a, b in qe for w in ((a,) if b == u else (b,) if a == u else ()) if w < u):
+                f.append(v)
+                rec(u + 1)
+                f.pop()
+    rec(0)
+    return sorted(out)
+
+
+def test_enumerate_matches_bruteforce(ctx):
+    """COLLECT_RESULT: every embedding, not just the count.  Paths, stars, triangles and a 4-cycle,
+    with shared labels, against a brute-force enumeration; the count equals asdm_count; a buffer
+    too small is an error."""
+    rng = random.Random(66)
+    queries = (
+        ([0, 1, 0], [(0, 1), (1, 2)]),
+        ([0, 1, 1, 1], [(0, 1), (0, 2), (0, 3)]),
+        ([0, 1, 2], [(0, 1), (1, 2), (0, 2)]),
+        ([0, 1, 0, 1], [(0, 1), (1, 2), (2, 3), (3, 0)]),
+        ([1, 0, 2, 0, 1], [(0, 1), (1, 2), (2, 3), (3, 4), (1, 3)]),
+    )
+    for trial in range(4):
+        gl, ge = _rand_graph(rng, 24, 0.25, 3)
+        g, _ = T.graph_from_lists(gl, ge)
+        for ql, qe in queries:
+            want = _embeddings_bruteforce(ql, qe, gl, ge)
+            q, _ = T.graph_from_lists(ql, qe)
+            cs = ctx.filter(q, g)
+            if cs is None:
+                assert want == []
+                continue
+            plan = cs.plan(q, factor=False)
+            rows = ctx.enumerate(plan, cs, max_rows=len(want) + 5)
+            assert sorted(map(tuple, rows.tolist())) == want, (trial, ql, qe)
+            assert ctx.count(cs.plan(q), cs) == len(want)
+            if len(want) > 1:
+                with pytest.raises(A.AsdmError) as e:
+                    ctx.enumerate(plan, cs, max_rows=len(want) - 1)
+                assert e.value.code == -6
+    # a real data graph: every row is an embedding, and there are as many as the reference counts
+    yeast = A.Graph.load(T.yeast_data_path())
+    goff, gnbr, glab = yeast.csr()
+    gold = T.yeast_golden()["queries"]
+    names = [n for n in gold if 0 < gold[n]["count"] <= 200_000][:3]
+    assert names
+    for name in names:
+        q = A.Graph.load(T.yeast_query_path(name))
+        cs = ctx.filter(q, yeast)
+        rows = ctx.enumerate(cs.plan(q, factor=False), cs, max_rows=gold[name]["count"])
+        assert len(rows) == gold[name]["count"], name
+        assert len({tuple(r) for r in rows.tolist()}) == len(rows)
+        qoff, qnbr, qlab = q.csr()
+        assert np.array_equal(glab[rows], np.broadcast_to(qlab, rows.shape))
+        assert all(len(set(r)) == q.n for r in rows[:2000].tolist())
+        for u in range(q.n):
+            for w in qnbr[qoff[u]:qoff[u + 1]]:
+                a, b = rows[:, u], rows[:, int(w)]
+                ok = [int(bb) in set(gnbr[goff[aa]:goff[aa + 1]].tolist()) for aa, bb in zip(a[:500], b[:500])]
+                assert all(ok), (name, u, int(w))
+
+
 def test_local_counts_match_reference(ctx):
     for case in T.synth_golden():
         if not case["local"]:
