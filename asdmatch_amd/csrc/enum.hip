@@ -979,14 +979,21 @@ __device__ inline uint64_t etail_row(const DPlan *__restrict__ P, const uint32_t
     return s0 * H0;
 }
 
-// A row of width c = D-2 of a plan with a second table: c's candidates x' depend on the row only
-// through its vertex at tail2_dep, and the row's sum is H0 * (tail2[row[tail2_dep]] less tail[x']
-// for each x' that is the matched vertex of a same-label depth of c).  Every hoisted leaf reads
-// depths below c only, and no same-label set of depth D-1 or of a leaf holds c.
+// A row of width c = D-2 of a plan with a second table.  c's candidates x' depend on the row only
+// through its vertex at tail2_dep, and each contributes inner(x') = (tail[x'] when the table's
+// backward neighbour b is c) x (the list lengths at x' of the hoisted leaves pendant on c); tail2
+// holds their sums.  The row's sum is H0 * (tail[row[b]] when b < c) * (tail2[row[tail2_dep]] less
+// inner(x') at each x' that is the matched vertex of a same-label depth of c), H0 being the
+// product of the other hoisted leaves' counts, which read depths below c only.
+__device__ inline bool pendant_on(const DPlan *__restrict__ P, uint32_t e, uint32_t c) {
+    return P->nb[e] == 1 && P->bdep[e][0] == c;
+}
+
 __device__ inline uint64_t tail2_row(const DPlan *__restrict__ P, const uint32_t *__restrict__ row, uint32_t c,
                                      const HoistedGroups &H, const uint64_t *__restrict__ ce_off,
                                      const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
                                      uint64_t &bytes) {
+    const uint32_t b = P->tail_dep;
     uint64_t s0 = P->tail2[row[P->tail2_dep]];
     const RowView R0{row, c, 0};
     if (P->ns[c]) {
@@ -996,18 +1003,26 @@ __device__ inline uint64_t tail2_row(const DPlan *__restrict__ P, const uint32_t
             const uint32_t x = cand_pos(P, c, matched_id(P, cand, R0, P->sdep[c][s2]));
             if (x == 0xffffffffu) continue;
             const uint32_t t = lower_bound_u32(Lv, L.len, x);
-            if (t < L.len && Lv[t] == x) s0 -= P->tail[x];
+            if (t >= L.len || Lv[t] != x) continue;
+            uint64_t f = b == c ? P->tail[x] : 1;
+            for (uint32_t k = 0; k < H.n; ++k) {
+                const uint32_t e = P->gleaf[H.g[k]][0];
+                if (pendant_on(P, e, c)) f *= ce_off[P->bbase[e][0] + x + 1] - ce_off[P->bbase[e][0] + x];
+            }
+            s0 -= f;
         }
     }
+    if (b < c) s0 *= P->tail[row[b]];
     uint64_t H0 = 1;
     for (uint32_t k = 0; k < H.n && s0; ++k) {
         const uint32_t g = H.g[k], e = P->gleaf[g][0];
+        if (pendant_on(P, e, c)) continue;  // inside tail2
         uint32_t y[kFusedSame];
         const uint32_t ns = P->ns[e];
         for (uint32_t s2 = 0; s2 < ns; ++s2) y[s2] = matched_id(P, cand, R0, P->sdep[e][s2]);
         H0 *= leaf_set_count_lane(P, g, 1u, R0, ce_off, ce_val, cand, y, ns);
     }
-    bytes += 4ull * c + 8ull + 16ull * H.n;
+    bytes += 4ull * c + 16ull + 16ull * H.n;
     return s0 * H0;
 }
 
@@ -1589,12 +1604,15 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     hp.tail_fix = fix;
     hp.tail_leaf = fix_leaf;
     hp.tail_rbase = rbase;
-    // Second table: b = D-2 =: c has a single backward neighbour b2, no correction factor and no
-    // same-label depth of d, and no hoisted leaf reads c (as a neighbour or a same-label vertex).
-    // Then c's candidates depend on a row of width c only through its vertex at b2, and each
-    // contributes its tail entry.
-    const uint32_t c = b;
-    if (fix || P.ns[d] || c + 2 != P.D || c == 0 || P.nb[c] != 1) return ASDM_OK;
+    // Second table: c = D-2 has a single backward neighbour b2, the table's backward neighbour b is
+    // c or earlier, no correction factor and no same-label depth of d apply, and every hoisted leaf
+    // that reads c (as a neighbour or a same-label vertex) is a pendant leaf of c with no same-label
+    // matched vertex to avoid.  Then c's candidates x' depend on a row of width c only through its
+    // vertex at b2, and each contributes (tail[x'] if b = c) times those leaves' list lengths at x'
+    // (1 when there are none): depth D-1 is never materialized.
+    const uint32_t c = P.D - 2;
+    if (fix || P.ns[d] || P.D < 3 || b > c || P.nb[c] != 1) return ASDM_OK;
+    TailSpec T2{};
     for (uint32_t g = 0; g < P.ngroups; ++g) {
         const uint32_t e = P.gleaf[g][0];
         bool adj_d = false, reads_c = false;
@@ -1603,12 +1621,15 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
             reads_c |= P.bdep[e][j] == c;
         }
         for (uint32_t s = 0; s < P.ns[e]; ++s) reads_c |= P.sdep[e][s] == c;
-        if (!adj_d && reads_c) return ASDM_OK;  // adjacent leaves are inside tail
+        if (adj_d || !reads_c) continue;  // inside tail, or part of H0
+        if (P.nb[e] != 1 || P.bdep[e][0] != c || P.ns[e] != 0 || T2.nadj == kTailLeaves) return ASDM_OK;
+        T2.pend_base[T2.nadj] = P.bbase[e][0];
+        T2.tri_tab[T2.nadj] = nullptr;
+        ++T2.nadj;
     }
-    TailSpec T2{};
     T2.arc_base = P.bbase[c][0];
     T2.np = cs->csize[plan->order[P.bdep[c][0]]];
-    T2.inner = tab;
+    T2.inner = b == c ? tab : nullptr;
     uint64_t *tab2 = nullptr;
     if ((rc = dev_alloc(ctx, (void **)&tab2, 8ull * std::max<uint32_t>(T2.np, 1)))) return rc;
     tabs.push_back(reinterpret_cast<uint32_t *>(tab2));

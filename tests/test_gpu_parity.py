@@ -232,20 +232,25 @@ def test_row_sum_table_corrections_against_bruteforce(ctx):
 
 
 def test_second_row_sum_table_against_bruteforce(ctx):
-    """Forced plans whose table's backward neighbour c = D-2 has one backward neighbour itself:
-    rows of width D-2 are finished from a second table (two table launches) -- a path (1), a path
-    whose depth c has a same-label earlier depth (2), a triangle leaf on (c, D-1) (4), a hoisted
-    leaf on depth 0 (5) -- but not when a hoisted leaf reads c through its same-label set (3)."""
+    """Forced plans where depth c = D-2 has one backward neighbour: rows of width D-2 are finished
+    from a second table (two table launches) -- a path (1), a path whose depth c has a same-label
+    earlier depth (2), a triangle leaf on (c, D-1) (4), a hoisted leaf on depth 0 (5); with the last
+    vertex's backward neighbour before c, alone (6) or with a hoisted leaf pendant on c (7); with
+    both a pendant leaf on c and the last vertex hanging on c (8) -- but not when a hoisted leaf
+    reads c through its same-label set (3)."""
     cases = (  # labels, edges, order, enumerated depths, second table expected
         ([0, 1, 2, 3], [(0, 1), (1, 2), (2, 3)], [0, 1, 2, 3], 3, True),
         ([1, 0, 1, 2, 3], [(0, 1), (1, 2), (2, 3), (3, 4)], [0, 1, 2, 3, 4], 4, True),
         ([0, 1, 2, 3, 1], [(0, 1), (1, 2), (2, 3), (0, 4)], [0, 1, 2, 3, 4], 3, False),
         ([0, 1, 2, 3], [(0, 1), (1, 2), (1, 3), (2, 3)], [0, 1, 2, 3], 3, True),
         ([0, 1, 2, 3, 0], [(0, 1), (1, 2), (2, 3), (0, 4)], [0, 1, 2, 3, 4], 3, True),
+        ([0, 1, 2, 3, 4], [(0, 1), (1, 2), (1, 3), (3, 4)], [0, 1, 2, 3, 4], 4, True),
+        ([0, 1, 2, 3, 4], [(0, 1), (1, 2), (1, 3), (2, 4)], [0, 1, 2, 3, 4], 4, True),
+        ([0, 1, 2, 3, 4], [(0, 1), (1, 2), (2, 3), (1, 4)], [0, 1, 2, 3, 4], 3, True),
     )
     rng = random.Random(44)
     for trial in range(6):
-        gl, ge = _rand_graph(rng, 70, 0.14, 4)
+        gl, ge = _rand_graph(rng, 70, 0.16, 5)
         g, og = T.graph_from_lists(gl, ge)
         for ql, qe, order, D, second in cases:
             q, oq = T.graph_from_lists(ql, qe)
