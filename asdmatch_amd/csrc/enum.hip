@@ -800,25 +800,56 @@ struct TailSpec {
 
 // tail[p] for every position p of C(u_tail_dep): a wave per position, lanes split its list.  With
 // T.inner set (the second table), the entry of a candidate x is inner[x].
+// One table entry, by the whole wave: the sum over position p's list of the product of the leaves'
+// counts (times inner[x] for a second table).
+__device__ inline void tail_sum_at(const TailSpec &T, uint64_t p, const uint64_t *__restrict__ ce_off,
+                                   const uint32_t *__restrict__ ce_val, uint64_t *__restrict__ tail) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t e_first = ce_off[T.arc_base];  // the arc's first candidate edge (triangle-table index 0)
+    const uint64_t lo = ce_off[T.arc_base + p], hi = ce_off[T.arc_base + p + 1];
+    uint64_t s = 0;
+    for (uint64_t i = lo + lane; i < hi; i += 64) {
+        const uint32_t x = ce_val[i];
+        uint64_t prod = T.inner ? T.inner[x] : 1;
+        for (uint32_t k = 0; k < T.nadj; ++k)
+            prod *= T.tri_tab[k] ? (uint64_t)T.tri_tab[k][i - e_first]
+                                 : ce_off[T.pend_base[k] + x + 1] - ce_off[T.pend_base[k] + x];
+        s += prod;
+    }
+    s = group_sum<64>(s);
+    if (lane == 0) tail[p] = s;
+}
+
 __global__ void __launch_bounds__(256) k_tail_sums(TailSpec T, const uint64_t *__restrict__ ce_off,
                                                    const uint32_t *__restrict__ ce_val, uint64_t *__restrict__ tail) {
-    const int lane = threadIdx.x & 63;
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    const uint64_t e_first = ce_off[T.arc_base];  // the arc's first candidate edge (triangle-table index 0)
-    for (uint64_t p = wave; p < T.np; p += nwaves) {
-        const uint64_t lo = ce_off[T.arc_base + p], hi = ce_off[T.arc_base + p + 1];
-        uint64_t s = 0;
-        for (uint64_t i = lo + lane; i < hi; i += 64) {
-            const uint32_t x = ce_val[i];
-            uint64_t prod = T.inner ? T.inner[x] : 1;
-            for (uint32_t k = 0; k < T.nadj; ++k)
-                prod *= T.tri_tab[k] ? (uint64_t)T.tri_tab[k][i - e_first]
-                                     : ce_off[T.pend_base[k] + x + 1] - ce_off[T.pend_base[k] + x];
-            s += prod;
+    for (uint64_t p = wave; p < T.np; p += nwaves) tail_sum_at(T, p, ce_off, ce_val, tail);
+}
+
+// The tables of many queries in one launch (batch engine): job j covers waves [pref[j], pref[j+1]),
+// one per position of its table.
+struct TailJob {
+    TailSpec T;
+    const uint64_t *ce_off;
+    const uint32_t *ce_val;
+    uint64_t *out;
+};
+
+__global__ void __launch_bounds__(256) k_tail_sums_multi(const TailJob *__restrict__ jobs,
+                                                         const uint64_t *__restrict__ pref, uint32_t njobs) {
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t total = pref[njobs];
+    for (uint64_t w = wave; w < total; w += nwaves) {
+        uint32_t lo = 0, hi = njobs;  // last job with pref[j] <= w (wave-uniform)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pref[mid] <= w) lo = mid;
+            else hi = mid;
         }
-        s = group_sum<64>(s);
-        if (lane == 0) tail[p] = s;
+        const TailJob &J = jobs[lo];
+        tail_sum_at(J.T, w - pref[lo], J.ce_off, J.ce_val, J.out);
     }
 }
 
@@ -1533,7 +1564,13 @@ int build_edge_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *c
 // candidates: tail_row corrects injectivity against the same-label depths of d, and
 // tail_fix_tile the correction factor over the intersection of the row's list with one other
 // list per source.  Needs hp.tri_tab.
-int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
+// Row-sum table builds deferred by the batch engine, to run as one launch per table level.
+struct TableJobs {
+    std::vector<TailJob> first, second;
+};
+
+int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs,
+                     TableJobs *jobs) {
     const DPlan &P = plan->h;
     hp.tail = nullptr;
     hp.tail_dep = 0;
@@ -1591,7 +1628,9 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     int rc = dev_alloc(ctx, (void **)&tab, 8ull * std::max<uint32_t>(T.np, 1));
     if (rc) return rc;
     tabs.push_back(reinterpret_cast<uint32_t *>(tab));
-    if (T.np) {
+    if (T.np && jobs) {
+        jobs->first.push_back({T, cs->d_ce_off, cs->d_ce_val, tab});
+    } else if (T.np) {
         KTimer kt(ctx, "tail_sums");
         const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)T.np + 3) / 4, 1u << 16);
         hipLaunchKernelGGL(k_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, T, cs->d_ce_off, cs->d_ce_val, tab);
@@ -1633,7 +1672,9 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     uint64_t *tab2 = nullptr;
     if ((rc = dev_alloc(ctx, (void **)&tab2, 8ull * std::max<uint32_t>(T2.np, 1)))) return rc;
     tabs.push_back(reinterpret_cast<uint32_t *>(tab2));
-    if (T2.np) {
+    if (T2.np && jobs) {
+        jobs->second.push_back({T2, cs->d_ce_off, cs->d_ce_val, tab2});
+    } else if (T2.np) {
         KTimer kt(ctx, "tail_sums");
         const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)T2.np + 3) / 4, 1u << 16);
         hipLaunchKernelGGL(k_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, T2, cs->d_ce_off, cs->d_ce_val, tab2);
@@ -1649,7 +1690,8 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
 // Builds the per-candidate-edge intersection table of every triangle leaf of the plan and points
 // hp.tri_tab at them, then the row-sum table of the fused level where it applies (tables appended
 // to tabs; the caller frees them).
-int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs) {
+int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs,
+                     TableJobs *jobs = nullptr) {
     const DPlan &P = plan->h;
     for (uint32_t e = P.D; e < P.nq; ++e) {
         if (!P.tri[e]) continue;
@@ -1673,7 +1715,29 @@ int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
             if ((rc = check_hip(ctx, hipGetLastError(), "tri table"))) return rc;
         }
     }
-    return build_tail_table(ctx, plan, cs, hp, tabs);
+    return build_tail_table(ctx, plan, cs, hp, tabs, jobs);
+}
+
+// One launch for the deferred tables of a level (job device copies are freed with tabs).
+int run_tail_jobs(asdm_ctx *ctx, const std::vector<TailJob> &jobs, std::vector<uint32_t *> &tabs) {
+    if (jobs.empty()) return ASDM_OK;
+    std::vector<uint64_t> pref(jobs.size() + 1, 0);
+    for (size_t j = 0; j < jobs.size(); ++j) pref[j + 1] = pref[j] + jobs[j].T.np;
+    TailJob *jd = nullptr;
+    uint64_t *pd = nullptr;
+    int rc = dev_alloc(ctx, (void **)&jd, sizeof(TailJob) * jobs.size());
+    if (rc) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(jd));
+    if ((rc = dev_alloc(ctx, (void **)&pd, 8 * pref.size()))) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(pd));
+    if ((rc = check_hip(ctx, hipMemcpy(jd, jobs.data(), sizeof(TailJob) * jobs.size(), hipMemcpyHostToDevice), "H2D jobs")) ||
+        (rc = check_hip(ctx, hipMemcpy(pd, pref.data(), 8 * pref.size(), hipMemcpyHostToDevice), "H2D jobs")))
+        return rc;
+    KTimer kt(ctx, "tail_sums");
+    const unsigned grid = (unsigned)std::min<uint64_t>((pref.back() + 3) / 4, 1u << 16);
+    hipLaunchKernelGGL(k_tail_sums_multi, dim3(std::max(1u, grid)), dim3(256), 0, ctx->stream, jd, pd, (uint32_t)jobs.size());
+    kt.stop();
+    return check_hip(ctx, hipGetLastError(), "tail tables");
 }
 
 // Fusion pays when each leaf count is a few lookups: every leaf group is a single leaf with one
@@ -2187,11 +2251,12 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     int rc = ASDM_OK;
     uint32_t maxD = 0;
     std::vector<uint64_t> seg_start_h(Q + 1, 0);
+    TableJobs jobs;  // every query's row-sum tables, built in two launches below
     for (uint32_t i = 0; i < Q; ++i) {
         const asdm_cs *cs = css[i];
         if (cs->store != st) return ctx->fail(ASDM_E_INVALID, "batch candidate spaces must come from one filter call");
         hp[i] = plans[i]->h;
-        if ((rc = build_tri_tables(ctx, plans[i], cs, hp[i], tabs))) {
+        if ((rc = build_tri_tables(ctx, plans[i], cs, hp[i], tabs, &jobs))) {
             cleanup(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
             return rc;
         }
@@ -2205,6 +2270,10 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         hp[i].fused = plan_fusable(plans[i]);
         maxD = std::max(maxD, hp[i].D);
         seg_start_h[i + 1] = seg_start_h[i] + cs->csize[plans[i]->order[0]];
+    }
+    if ((rc = run_tail_jobs(ctx, jobs.first, tabs)) || (rc = run_tail_jobs(ctx, jobs.second, tabs))) {
+        cleanup(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        return rc;
     }
     DPlan *dplans = nullptr;
     uint64_t *seg_start = nullptr, *seg_next = nullptr;
