@@ -268,44 +268,97 @@ __device__ inline TriLists tri_lists(const uint64_t *__restrict__ ce_off, const 
 // Edges whose shorter list holds more than kTriWide entries are left to k_tri_table_wide.
 constexpr uint32_t kTriWide = 64;
 
+// Entry k of a triangle table by one lane (shorter list at most kTriWide long) ...
+__device__ inline void tri_entry_narrow(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                        uint64_t arc_item_base, uint32_t nitems, uint64_t e_first, uint64_t k,
+                                        uint64_t l0_base, uint64_t l1_base, uint32_t *__restrict__ tab) {
+    const TriLists L = tri_lists(ce_off, ce_val, arc_item_base, nitems, e_first + k, l0_base, l1_base);
+    if (L.na > kTriWide) return;
+    const uint32_t *A = L.A, *B = L.B;
+    const uint32_t na = L.na, nb = L.nb;
+    uint32_t c = 0, p = 0;
+    for (uint32_t t = 0; t < na; ++t) {  // probe the shorter list, galloping in the longer one
+        const uint32_t x = A[t];
+        uint32_t step = 1, q = p;
+        while (q + step < nb && B[q + step] < x) { q += step; step <<= 1; }
+        uint32_t hi2 = q + step < nb ? q + step + 1 : nb;
+        p = q + lower_bound_u32(B + q, hi2 - q, x);
+        if (p < nb && B[p] == x) ++c;
+    }
+    tab[k] = c;
+}
+
+// ... and by a whole wave (the others): lanes split the shorter list and binary-search the longer.
+__device__ inline void tri_entry_wide(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
+                                      uint64_t arc_item_base, uint32_t nitems, uint64_t e_first, uint64_t k,
+                                      uint64_t l0_base, uint64_t l1_base, uint32_t *__restrict__ tab) {
+    const int lane = threadIdx.x & 63;
+    const TriLists L = tri_lists(ce_off, ce_val, arc_item_base, nitems, e_first + k, l0_base, l1_base);
+    if (L.na <= kTriWide) return;  // wave-uniform
+    uint64_t c = 0;
+    for (uint32_t t = lane; t < L.na; t += 64) c += contains_u32(L.B, L.nb, L.A[t]);
+    c = group_sum<64>(c);
+    if (lane == 0) tab[k] = (uint32_t)c;
+}
+
 __global__ void k_tri_table(const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
                             uint64_t arc_item_base, uint32_t nitems /* |C(p0)| */, uint64_t nedges, uint64_t l0_base,
                             uint64_t l1_base, uint32_t *__restrict__ tab) {
     const uint64_t e_first = ce_off[arc_item_base];  // first candidate edge of the arc
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nedges; k += (uint64_t)gridDim.x * blockDim.x) {
-        const TriLists L = tri_lists(ce_off, ce_val, arc_item_base, nitems, e_first + k, l0_base, l1_base);
-        if (L.na > kTriWide) continue;
-        const uint32_t *A = L.A, *B = L.B;
-        const uint32_t na = L.na, nb = L.nb;
-        uint32_t c = 0, p = 0;
-        for (uint32_t t = 0; t < na; ++t) {  // probe the shorter list, galloping in the longer one
-            const uint32_t x = A[t];
-            uint32_t step = 1, q = p;
-            while (q + step < nb && B[q + step] < x) { q += step; step <<= 1; }
-            uint32_t hi2 = q + step < nb ? q + step + 1 : nb;
-            p = q + lower_bound_u32(B + q, hi2 - q, x);
-            if (p < nb && B[p] == x) ++c;
-        }
-        tab[k] = c;
-    }
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nedges; k += (uint64_t)gridDim.x * blockDim.x)
+        tri_entry_narrow(ce_off, ce_val, arc_item_base, nitems, e_first, k, l0_base, l1_base, tab);
 }
 
-// The edges k_tri_table leaves: one wave per edge, lanes split the shorter list and binary-search
-// the longer one.
+// The edges k_tri_table leaves: one wave per edge.
 __global__ void __launch_bounds__(256) k_tri_table_wide(const uint64_t *__restrict__ ce_off,
                                                         const uint32_t *__restrict__ ce_val, uint64_t arc_item_base,
                                                         uint32_t nitems, uint64_t nedges, uint64_t l0_base,
                                                         uint64_t l1_base, uint32_t *__restrict__ tab) {
-    const int lane = threadIdx.x & 63;
     const uint64_t e_first = ce_off[arc_item_base];
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nedges; k += nwaves) {
-        const TriLists L = tri_lists(ce_off, ce_val, arc_item_base, nitems, e_first + k, l0_base, l1_base);
-        if (L.na <= kTriWide) continue;  // wave-uniform
-        uint64_t c = 0;
-        for (uint32_t t = lane; t < L.na; t += 64) c += contains_u32(L.B, L.nb, L.A[t]);
-        c = group_sum<64>(c);
-        if (lane == 0) tab[k] = (uint32_t)c;
+    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nedges; k += nwaves)
+        tri_entry_wide(ce_off, ce_val, arc_item_base, nitems, e_first, k, l0_base, l1_base, tab);
+}
+
+// The triangle tables of many queries in one launch pair (batch engine): job j covers the items
+// [pref[j], pref[j+1]) (its candidate edges).
+struct TriJob {
+    const uint64_t *ce_off;
+    const uint32_t *ce_val;
+    uint64_t arc_item_base, nedges, l0_base, l1_base;
+    uint32_t nitems;
+    uint32_t *tab;
+};
+
+__device__ inline uint32_t job_of(const uint64_t *__restrict__ pref, uint32_t njobs, uint64_t item) {
+    uint32_t lo = 0, hi = njobs;  // last job with pref[j] <= item
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pref[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_tri_table_multi(const TriJob *__restrict__ jobs, const uint64_t *__restrict__ pref, uint32_t njobs) {
+    const uint64_t total = pref[njobs];
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+        const TriJob &J = jobs[job_of(pref, njobs, g)];
+        const uint64_t k = g - pref[&J - jobs];
+        tri_entry_narrow(J.ce_off, J.ce_val, J.arc_item_base, J.nitems, J.ce_off[J.arc_item_base], k, J.l0_base,
+                         J.l1_base, J.tab);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tri_table_wide_multi(const TriJob *__restrict__ jobs,
+                                                              const uint64_t *__restrict__ pref, uint32_t njobs) {
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t total = pref[njobs];
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < total; w += nwaves) {
+        const uint32_t j = job_of(pref, njobs, w);  // wave-uniform
+        const TriJob &J = jobs[j];
+        tri_entry_wide(J.ce_off, J.ce_val, J.arc_item_base, J.nitems, J.ce_off[J.arc_item_base], w - pref[j], J.l0_base,
+                       J.l1_base, J.tab);
     }
 }
 
@@ -1494,13 +1547,28 @@ bool plan_fusable(const asdm_plan *plan);
 
 constexpr uint64_t kEdgeTableMax = 1ull << 27;  // candidate edges of an edge table (8 B each: 1 GiB)
 
+// Table builds deferred by the batch engine: the triangle tables of every query in one launch pair,
+// then the edge tables (one launch each: they read the triangle tables), then the row-sum tables in
+// one launch per table level.
+struct EdgeJob {
+    EdgeTailSpec T;
+    const uint64_t *ce_off;
+    const uint32_t *ce_val;
+    uint64_t *tab;
+};
+struct TableJobs {
+    std::vector<TriJob> tri;
+    std::vector<EdgeJob> edge;
+    std::vector<TailJob> first, second;
+};
+
 // Edge table of a fused plan (DPlan::etail) when depth d = D-1 has two backward neighbours b1 < b2
 // that are adjacent in the query: the row's candidates are L1 ∩ L2, a function of the candidate
 // edge (row[b1], row[b2]).  Every leaf group must be a pendant leaf of d or a triangle leaf on
 // (b1, d) or (b2, d) with no same-label matched vertex to avoid, or a hoisted leaf (at most
 // kRowLeaves) whose label is not d's; same-label depths of d are corrected per row.
 int build_edge_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp,
-                          std::vector<uint32_t *> &tabs) {
+                          std::vector<uint32_t *> &tabs, TableJobs *jobs) {
     const DPlan &P = plan->h;
     const uint32_t d = P.D - 1, b1 = P.bdep[d][0], b2 = P.bdep[d][1];
     uint32_t jb = UINT32_MAX;
@@ -1544,7 +1612,9 @@ int build_edge_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *c
     int rc = dev_alloc(ctx, (void **)&tab, 8ull * std::max<uint64_t>(T.nedges, 1));
     if (rc) return rc;
     tabs.push_back(reinterpret_cast<uint32_t *>(tab));
-    if (T.nedges) {
+    if (T.nedges && jobs) {
+        jobs->edge.push_back({T, cs->d_ce_off, cs->d_ce_val, tab});
+    } else if (T.nedges) {
         KTimer kt(ctx, "edge_tail_sums");
         const unsigned grid = (unsigned)std::min<uint64_t>((T.nedges + 3) / 4, 1u << 16);
         hipLaunchKernelGGL(k_edge_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, T, cs->d_ce_off, cs->d_ce_val, tab);
@@ -1564,10 +1634,6 @@ int build_edge_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *c
 // candidates: tail_row corrects injectivity against the same-label depths of d, and
 // tail_fix_tile the correction factor over the intersection of the row's list with one other
 // list per source.  Needs hp.tri_tab.
-// Row-sum table builds deferred by the batch engine, to run as one launch per table level.
-struct TableJobs {
-    std::vector<TailJob> first, second;
-};
 
 int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DPlan &hp, std::vector<uint32_t *> &tabs,
                      TableJobs *jobs) {
@@ -1582,7 +1648,7 @@ int build_tail_table(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     hp.tail_rbase = 0;
     if (!plan_fusable(plan)) return ASDM_OK;
     const uint32_t d = P.D - 1;
-    if (P.nb[d] == 2) return build_edge_tail_table(ctx, plan, cs, hp, tabs);
+    if (P.nb[d] == 2) return build_edge_tail_table(ctx, plan, cs, hp, tabs, jobs);
     if (P.nb[d] != 1) return ASDM_OK;
     const uint32_t b = P.bdep[d][0];
     TailSpec T{};
@@ -1703,7 +1769,10 @@ int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
         if (rc) return rc;
         tabs.push_back(tab);
         hp.tri_tab[e] = tab;
-        if (nedges) {
+        if (nedges && jobs) {
+            jobs->tri.push_back({cs->d_ce_off, cs->d_ce_val, cs->ce_base[a], nedges, P.bbase[e][0], P.bbase[e][1],
+                                 cs->csize[p0], tab});
+        } else if (nedges) {
             KTimer kt(ctx, "tri_table");
             const unsigned grid = (unsigned)std::min<uint64_t>((nedges + 255) / 256, 1u << 16);
             hipLaunchKernelGGL(k_tri_table, dim3(grid), dim3(256), 0, ctx->stream, cs->d_ce_off, cs->d_ce_val,
@@ -1718,6 +1787,50 @@ int build_tri_tables(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, DP
     return build_tail_table(ctx, plan, cs, hp, tabs, jobs);
 }
 
+// Device copies of deferred jobs and their item prefix (freed with tabs).
+template <typename Job>
+int upload_jobs(asdm_ctx *ctx, const std::vector<Job> &jobs, const std::vector<uint64_t> &pref, std::vector<uint32_t *> &tabs,
+                Job **jd, uint64_t **pd) {
+    int rc = dev_alloc(ctx, (void **)jd, sizeof(Job) * jobs.size());
+    if (rc) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(*jd));
+    if ((rc = dev_alloc(ctx, (void **)pd, 8 * pref.size()))) return rc;
+    tabs.push_back(reinterpret_cast<uint32_t *>(*pd));
+    if ((rc = check_hip(ctx, hipMemcpy(*jd, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice), "H2D jobs")))
+        return rc;
+    return check_hip(ctx, hipMemcpy(*pd, pref.data(), 8 * pref.size(), hipMemcpyHostToDevice), "H2D jobs");
+}
+
+// The deferred triangle tables in one launch pair.
+int run_tri_jobs(asdm_ctx *ctx, const std::vector<TriJob> &jobs, std::vector<uint32_t *> &tabs) {
+    if (jobs.empty()) return ASDM_OK;
+    std::vector<uint64_t> pref(jobs.size() + 1, 0);
+    for (size_t j = 0; j < jobs.size(); ++j) pref[j + 1] = pref[j] + jobs[j].nedges;
+    TriJob *jd = nullptr;
+    uint64_t *pd = nullptr;
+    if (int rc = upload_jobs(ctx, jobs, pref, tabs, &jd, &pd)) return rc;
+    KTimer kt(ctx, "tri_table");
+    const uint64_t total = pref.back();
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 1u << 16));
+    hipLaunchKernelGGL(k_tri_table_multi, dim3(grid), dim3(256), 0, ctx->stream, jd, pd, (uint32_t)jobs.size());
+    const unsigned wgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((total + 3) / 4, 1u << 14));
+    hipLaunchKernelGGL(k_tri_table_wide_multi, dim3(wgrid), dim3(256), 0, ctx->stream, jd, pd, (uint32_t)jobs.size());
+    kt.stop(8ull * total);
+    return check_hip(ctx, hipGetLastError(), "tri tables");
+}
+
+// The deferred edge tables, one launch each.
+int run_edge_jobs(asdm_ctx *ctx, const std::vector<EdgeJob> &jobs) {
+    for (const EdgeJob &J : jobs) {
+        KTimer kt(ctx, "edge_tail_sums");
+        const unsigned grid = (unsigned)std::min<uint64_t>((J.T.nedges + 3) / 4, 1u << 16);
+        hipLaunchKernelGGL(k_edge_tail_sums, dim3(grid), dim3(256), 0, ctx->stream, J.T, J.ce_off, J.ce_val, J.tab);
+        kt.stop(40ull * J.T.nedges);
+        if (int rc = check_hip(ctx, hipGetLastError(), "edge tail table")) return rc;
+    }
+    return ASDM_OK;
+}
+
 // One launch for the deferred tables of a level (job device copies are freed with tabs).
 int run_tail_jobs(asdm_ctx *ctx, const std::vector<TailJob> &jobs, std::vector<uint32_t *> &tabs) {
     if (jobs.empty()) return ASDM_OK;
@@ -1725,14 +1838,7 @@ int run_tail_jobs(asdm_ctx *ctx, const std::vector<TailJob> &jobs, std::vector<u
     for (size_t j = 0; j < jobs.size(); ++j) pref[j + 1] = pref[j] + jobs[j].T.np;
     TailJob *jd = nullptr;
     uint64_t *pd = nullptr;
-    int rc = dev_alloc(ctx, (void **)&jd, sizeof(TailJob) * jobs.size());
-    if (rc) return rc;
-    tabs.push_back(reinterpret_cast<uint32_t *>(jd));
-    if ((rc = dev_alloc(ctx, (void **)&pd, 8 * pref.size()))) return rc;
-    tabs.push_back(reinterpret_cast<uint32_t *>(pd));
-    if ((rc = check_hip(ctx, hipMemcpy(jd, jobs.data(), sizeof(TailJob) * jobs.size(), hipMemcpyHostToDevice), "H2D jobs")) ||
-        (rc = check_hip(ctx, hipMemcpy(pd, pref.data(), 8 * pref.size(), hipMemcpyHostToDevice), "H2D jobs")))
-        return rc;
+    if (int rc = upload_jobs(ctx, jobs, pref, tabs, &jd, &pd)) return rc;
     KTimer kt(ctx, "tail_sums");
     const unsigned grid = (unsigned)std::min<uint64_t>((pref.back() + 3) / 4, 1u << 16);
     hipLaunchKernelGGL(k_tail_sums_multi, dim3(std::max(1u, grid)), dim3(256), 0, ctx->stream, jd, pd, (uint32_t)jobs.size());
@@ -2251,7 +2357,7 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     int rc = ASDM_OK;
     uint32_t maxD = 0;
     std::vector<uint64_t> seg_start_h(Q + 1, 0);
-    TableJobs jobs;  // every query's row-sum tables, built in two launches below
+    TableJobs jobs;  // every query's triangle and row-sum tables, built in a few launches below
     for (uint32_t i = 0; i < Q; ++i) {
         const asdm_cs *cs = css[i];
         if (cs->store != st) return ctx->fail(ASDM_E_INVALID, "batch candidate spaces must come from one filter call");
@@ -2271,7 +2377,8 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
         maxD = std::max(maxD, hp[i].D);
         seg_start_h[i + 1] = seg_start_h[i] + cs->csize[plans[i]->order[0]];
     }
-    if ((rc = run_tail_jobs(ctx, jobs.first, tabs)) || (rc = run_tail_jobs(ctx, jobs.second, tabs))) {
+    if ((rc = run_tri_jobs(ctx, jobs.tri, tabs)) || (rc = run_edge_jobs(ctx, jobs.edge)) ||
+        (rc = run_tail_jobs(ctx, jobs.first, tabs)) || (rc = run_tail_jobs(ctx, jobs.second, tabs))) {
         cleanup(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
         return rc;
     }
