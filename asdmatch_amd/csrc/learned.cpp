@@ -11,6 +11,7 @@
 #include <cmath>
 #include <memory>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "engine.h"
 
@@ -87,12 +88,48 @@ inline double card_from_log(float y) { return std::exp2(std::min(64.f, std::max(
 }  // namespace
 
 // Cost-based order search over a batch of candidate spaces.  The leaf set is the plain planner's.
-// For every query a greedy order is grown from the plain planner's first vertex (the smallest
-// candidate set) or, with multi_start, from each vertex of the enumerated part: each step adds the
-// frontier vertex whose extended prefix has the smallest estimated cardinality.  The order whose
-// summed prefix cardinalities -- the estimated partial embeddings the engine materializes -- is
-// smallest wins (the subsetToCard-driven choice of plan/optimizer.cpp:7-200, over greedy orders).  All greedy runs of all queries advance in lock step, with one estimator call per step
-// for every subset they ask about.  Singletons use the exact |C(u)|.
+// The cost of an order is its summed prefix cardinalities: the estimated partial embeddings the
+// engine materializes.  With multi_start, a query whose enumerated part has at most kDpVertices
+// vertices gets the optimal connected order by a dynamic program over its connected subsets (the
+// subset DP of plan/optimizer.cpp:380-548, over the subsets the engine can enumerate): one
+// estimator call holds every such subset of every such query, then
+//   cost(S) = card(S) + min over v in S, S \ {v} connected and adjacent to v, of cost(S \ {v}).
+// Other queries grow greedy orders from the plain planner's first vertex (the smallest candidate
+// set) or, with multi_start, from each vertex of the enumerated part: each step adds the frontier
+// vertex whose extended prefix has the smallest estimated cardinality, and the cheapest order wins.
+// All greedy runs of all queries advance in lock step, with one estimator call per step for every
+// subset they ask about.  Singletons use the exact |C(u)|.
+constexpr uint32_t kDpVertices = 12;
+
+// The optimal order of rest by the subset DP above; cards holds every connected subset's estimate.
+std::vector<uint32_t> dp_order(const asdm_cs *cs, uint64_t rest, const std::unordered_map<uint64_t, double> &cards,
+                               const std::vector<uint64_t> &subsets) {
+    std::unordered_map<uint64_t, std::pair<double, uint32_t>> best;  // cost, last vertex
+    for (uint64_t r = rest; r; r &= r - 1) {
+        const uint32_t u = (uint32_t)__builtin_ctzll(r);
+        best[1ull << u] = {(double)cs->csize[u], u};
+    }
+    for (uint64_t S : subsets) {  // by increasing size
+        std::pair<double, uint32_t> b{0.0, UINT32_MAX};
+        for (uint64_t r = S; r; r &= r - 1) {
+            const uint32_t v = (uint32_t)__builtin_ctzll(r);
+            const uint64_t P = S & ~(1ull << v);
+            const auto it = best.find(P);
+            if (it == best.end() || !(frontier_of(cs, P, rest) >> v & 1)) continue;
+            const double c = it->second.first + cards.at(S);
+            if (b.second == UINT32_MAX || c < b.first) b = {c, v};
+        }
+        best[S] = b;
+    }
+    std::vector<uint32_t> order;
+    for (uint64_t S = rest; S;) {
+        const uint32_t v = best.at(S).second;
+        order.push_back(v);
+        S &= ~(1ull << v);
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
 int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, bool multi_start, const BatchEstimator &est,
                 std::vector<std::vector<uint32_t>> &orders, std::vector<uint32_t> &Ds,
                 std::vector<std::unordered_map<uint64_t, double>> &cards) {
@@ -108,11 +145,35 @@ int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, bool mu
     };
     std::vector<Run> runs;
     std::vector<uint64_t> rest(Q), leaves(Q);
+    std::vector<std::vector<uint64_t>> dp_sets(Q);  // connected subsets (size >= 2) by size, DP queries
+    std::vector<char> dp(Q, 0);
+    std::vector<std::pair<uint32_t, uint64_t>> dp_reqs;
     for (size_t i = 0; i < Q; ++i) {
         const asdm_cs *cs = css[i];
         const uint64_t all = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
         leaves[i] = choose_leaves(cs, flags);
         rest[i] = all & ~leaves[i];
+        for (uint64_t r = rest[i]; r; r &= r - 1) cards[i][r & -r] = cs->csize[__builtin_ctzll(r)];
+        if (multi_start && __builtin_popcountll(rest[i]) <= kDpVertices) {
+            dp[i] = 1;
+            std::unordered_set<uint64_t> seen;
+            std::vector<uint64_t> cur;
+            for (uint64_t r = rest[i]; r; r &= r - 1) cur.push_back(r & -r);
+            while (!cur.empty()) {
+                std::vector<uint64_t> nxt;
+                for (uint64_t S : cur)
+                    for (uint64_t f = frontier_of(cs, S, rest[i]); f; f &= f - 1) {
+                        const uint64_t T = S | (f & -f);
+                        if (seen.insert(T).second) nxt.push_back(T);
+                    }
+                for (uint64_t T : nxt) {
+                    dp_sets[i].push_back(T);
+                    dp_reqs.emplace_back((uint32_t)i, T);
+                }
+                cur.swap(nxt);
+            }
+            continue;
+        }
         const uint64_t starts = multi_start ? rest[i] : (1ull << smallest_candidates(cs, rest[i]));
         for (uint64_t r = starts; r; r &= r - 1) {
             const uint32_t u = (uint32_t)__builtin_ctzll(r);
@@ -123,6 +184,12 @@ int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, bool mu
     std::vector<std::pair<uint32_t, uint64_t>> reqs;  // (query, subset) asked this step
     std::vector<std::pair<size_t, uint32_t>> who;     // (run, vertex) of each request
     std::vector<double> y;
+    if (!dp_reqs.empty()) {
+        if (int rc = est(dp_reqs, y)) return rc;
+        for (size_t k = 0; k < dp_reqs.size(); ++k) cards[dp_reqs[k].first][dp_reqs[k].second] = y[k];
+    }
+    for (size_t i = 0; i < Q; ++i)
+        if (dp[i]) orders[i] = dp_order(css[i], rest[i], cards[i], dp_sets[i]);
     for (;;) {
         reqs.clear();
         who.clear();
@@ -158,7 +225,7 @@ int cost_orders(const std::vector<const asdm_cs *> &css, uint32_t flags, bool mu
     std::vector<double> bestcost(Q, 0);
     std::vector<char> have(Q, 0);
     for (const Run &r : runs)  // first run (smallest start vertex) on ties
-        if (!have[r.q] || r.cost < bestcost[r.q]) {
+        if (!dp[r.q] && (!have[r.q] || r.cost < bestcost[r.q])) {
             have[r.q] = 1;
             bestcost[r.q] = r.cost;
             orders[r.q] = r.order;

@@ -190,6 +190,45 @@ def test_learned_plans_keep_reference_counts(ctx):
         assert [int(c) for c in c2] == [r["count"] for r in recs], case["name"]
 
 
+def test_cost_planner_finds_the_cheapest_connected_order(ctx):
+    """With ASDM_PLAN_COST the order of the enumerated part minimizes the summed prefix estimates
+    (first vertex: |C(u)|; then the independence model) by a subset DP; checked against every
+    connected order of the same vertices, and the counts stay the reference's."""
+    import itertools
+
+    yeast = A.Graph.load(T.yeast_data_path())
+    gold = T.yeast_golden()["queries"]
+    checked = 0
+    for name in gold:
+        q = A.Graph.load(T.yeast_query_path(name))
+        if q.n > 8:
+            continue
+        cs = ctx.filter(q, yeast)
+        if cs is None:
+            continue
+        p = cs.plan(q, cost=True)
+        order, D = p.order()
+        enum = order[:D]
+        off, nbr, _ = q.csr()
+        adj = [set(int(w) for w in nbr[off[u]:off[u + 1]]) for u in range(q.n)]
+        sizes = cs.sizes()
+
+        def cost(o):
+            masks, m = [], 0
+            for u in o:
+                m |= 1 << u
+                masks.append(m)
+            return float(sizes[o[0]]) + float(np.sum(A.plain_estimate(cs, masks[1:]))) if len(o) > 1 else float(sizes[o[0]])
+
+        connected = [o for o in itertools.permutations(enum)
+                     if all(adj[o[i]] & set(o[:i]) for i in range(1, len(o)))]
+        best = min(cost(o) for o in connected)
+        assert cost(enum) <= best * (1 + 1e-9), (name, enum, cost(enum), best)
+        assert ctx.count(p, cs) == gold[name]["count"], name
+        checked += 1
+    assert checked >= 10
+
+
 def test_row_limit_drops_only_the_query_over_it(ctx):
     case = [c for c in T.synth_golden() if c["name"] == "pl_small_l4"][0]
     g = T.make_graph(case)
