@@ -140,7 +140,11 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
 /* Count only the last vertex in closed form (no multi-leaf factorization): a plain enumeration
  * used to cross-check the leaf factorization. */
 #define ASDM_PLAN_NO_FACTOR 2u
-#define ASDM_PLAN_COST 4u      /* multi-start cost-based order search (independence-model cardinalities) */
+/* Cost-based order search (independence-model cardinalities): the order of the enumerated vertices
+ * with the smallest summed prefix estimates -- exact over all connected orders (subset DP,
+ * plan/optimizer.cpp:380-548) for at most 12 enumerated vertices, else the best greedy order grown
+ * from each of them. */
+#define ASDM_PLAN_COST 4u
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
 /* Plan with an explicit order whose first num_enumerated vertices are enumerated and the rest are
@@ -263,9 +267,9 @@ int asdm_mpnn_estimate(asdm_mpnn *m, const asdm_graph *g, const asdm_cs *cs, con
  * prod_u |C(u)| * prod over sub-query edges (u, w) of W(u->w) / (|C(u)| |C(w)|). */
 int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, double *cards);
 /* Plan whose enumeration order is chosen greedily by the learned estimates (the plain planner's
- * leaf set and first vertex; with ASDM_PLAN_COST in flags, the best of the greedy orders grown from
- * every enumerated vertex by summed estimated prefix cardinalities).  The plain planner is used when
- * the network cannot run. */
+ * leaf set and first vertex; with ASDM_PLAN_COST in flags, the order with the smallest summed
+ * estimated prefix cardinalities, as for asdm_plan_create).  The plain planner is used when the
+ * network cannot run. */
 int asdm_plan_create_learned(asdm_mpnn *m, const asdm_graph *g, const asdm_graph *q, const asdm_cs *cs,
                              uint32_t flags, asdm_plan **out);
 
