@@ -83,6 +83,17 @@ def test_parallel_parse_of_large_files_matches_sequential_and_oracle():
         open(q, "w").write("\n".join(moved))
         with pytest.raises(A.AsdmError):  # the reference's loader needs offsets before edges
             A.Graph.load(q)
+        # vertex and edge lines interleaved, each edge after both endpoints' vertex lines: valid for
+        # the reference's loader, so the sequential scan (taken when an edge precedes a vertex)
+        # must load it
+        off, nbr, lab = g.csr()
+        out = [lines[0]]
+        for v in range(g.n):
+            out.append(lines[1 + v])
+            out.extend(f"e {int(w)} {v}" for w in nbr[off[v]:off[v + 1]] if int(w) < v)
+        r2 = os.path.join(d, "interleaved.graph")
+        open(r2, "w").write("\n".join(out) + "\n")
+        assert _same_csr(g, A.Graph.load(r2))
         # a malformed record deep inside a chunk fails as the sequential scan does
         bad = lines[:]
         bad[len(bad) // 2] = "e 1"
