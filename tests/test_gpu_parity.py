@@ -378,6 +378,38 @@ def test_enumerate_matches_bruteforce(ctx):
                 assert all(ok), (name, u, int(w))
 
 
+def test_tiny_queries_in_every_engine(ctx):
+    """Corner shapes -- one vertex, one edge (distinct and equal labels), a same-label path, a
+    triangle and a star of one label -- through the single-query engine, the batch engine and
+    enumeration, against brute force."""
+    rng = random.Random(77)
+    queries = (
+        ([0], []),
+        ([0, 1], [(0, 1)]),
+        ([0, 0], [(0, 1)]),
+        ([0, 0, 0], [(0, 1), (1, 2)]),
+        ([0, 0, 0], [(0, 1), (1, 2), (0, 2)]),
+        ([1, 0, 0, 0], [(0, 1), (0, 2), (0, 3)]),
+    )
+    for trial in range(3):
+        gl, ge = _rand_graph(rng, 40, 0.2, 2)
+        g, og = T.graph_from_lists(gl, ge)
+        qs, want = [], []
+        for ql, qe in queries:
+            q, oq = T.graph_from_lists(ql, qe)
+            qs.append(q)
+            want.append(len(_embeddings_bruteforce(ql, qe, gl, ge)))
+        assert [A.count_embeddings(ctx, q, g) for q in qs] == want, trial
+        batch, _ = ctx.batch_count(g, qs)
+        assert [int(c) for c in batch] == want, trial
+        for q, w in zip(qs, want):
+            cs = ctx.filter(q, g)
+            if cs is None:
+                assert w == 0
+                continue
+            assert len(ctx.enumerate(cs.plan(q, factor=False), cs, max_rows=w + 1)) == w
+
+
 def test_local_counts_match_reference(ctx):
     for case in T.synth_golden():
         if not case["local"]:
