@@ -8,7 +8,7 @@
 #include <cstring>
 #include <vector>
 
-#include "asdm.h"
+#include "host.h"
 
 namespace {
 
@@ -32,7 +32,7 @@ bool read_file(const char *path, std::vector<unsigned char> &buf) {
 
 extern "C" {
 
-int asdm_card_read(const char *path, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n, double *dp_time) {
+int asdm_card_read(const char *path, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n, double *dp_time) try {
     if (!path || !n) return ASDM_E_INVALID;
     std::vector<unsigned char> b;
     if (!read_file(path, b)) return ASDM_E_IO;
@@ -70,9 +70,9 @@ int asdm_card_read(const char *path, uint64_t *masks, double *cards, uint32_t ca
         std::memcpy(&cards[i], b.data() + p + 16 + 16 * i, 8);
     }
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_card_write(const char *path, const uint64_t *masks, const double *cards, uint32_t n, double dp_time) {
+int asdm_card_write(const char *path, const uint64_t *masks, const double *cards, uint32_t n, double dp_time) try {
     if (!path || (n && (!masks || !cards))) return ASDM_E_INVALID;
     FILE *f = std::fopen(path, "wb");
     if (!f) return ASDM_E_IO;
@@ -85,6 +85,6 @@ int asdm_card_write(const char *path, const uint64_t *masks, const double *cards
     int32_t t = (int32_t)(dp_time * 10000);
     ok = ok && std::fwrite(&t, 4, 1, f) == 1;
     return std::fclose(f) == 0 && ok ? ASDM_OK : ASDM_E_IO;
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

@@ -257,7 +257,7 @@ using namespace asdm;
 
 extern "C" {
 
-int asdm_ctx_create(int device, asdm_ctx **out) {
+int asdm_ctx_create(int device, asdm_ctx **out) try {
     if (!out) return ASDM_E_INVALID;
     *out = nullptr;
     int ndev = 0;
@@ -276,7 +276,7 @@ int asdm_ctx_create(int device, asdm_ctx **out) {
     if (hipMalloc((void **)&ctx->d_counters, sizeof(uint64_t) * kNumCounters) != hipSuccess) return ASDM_E_OOM;
     *out = ctx.release();
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 void asdm_ctx_destroy(asdm_ctx *ctx) {
     if (!ctx) return;
@@ -291,35 +291,35 @@ void asdm_ctx_destroy(asdm_ctx *ctx) {
 
 const char *asdm_last_error(const asdm_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
-int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out) {
+int asdm_last_stats(const asdm_ctx *ctx, asdm_stats *out) try {
     if (!ctx || !out) return ASDM_E_INVALID;
     *out = ctx->stats;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_ctx_set_time_limit(asdm_ctx *ctx, double ms) {
+int asdm_ctx_set_time_limit(asdm_ctx *ctx, double ms) try {
     if (!ctx || ms < 0) return ASDM_E_INVALID;
     ctx->time_limit_ms = ms;
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
-int asdm_ctx_set_row_limit(asdm_ctx *ctx, uint64_t rows) {
+int asdm_ctx_set_row_limit(asdm_ctx *ctx, uint64_t rows) try {
     if (!ctx) return ASDM_E_INVALID;
     ctx->row_limit = rows;
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
 // Per-kernel device timing, collected while profiling is on (HIP events on the context stream).
-int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) {
+int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) try {
     if (!ctx) return ASDM_E_INVALID;
     resolve_timers(ctx);
     ctx->profiling = on != 0;
     ctx->kstats.clear();
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
 int asdm_kernel_stat(const asdm_ctx *ctx, uint32_t i, char *name, uint32_t name_cap, double *ms, uint64_t *launches,
-                     uint64_t *bytes) {
+                     uint64_t *bytes) try {
     if (!ctx) return ASDM_E_INVALID;
     resolve_timers(const_cast<asdm_ctx *>(ctx));
     if (i >= ctx->kstats.size()) return ASDM_E_INVALID;
@@ -332,9 +332,9 @@ int asdm_kernel_stat(const asdm_ctx *ctx, uint32_t i, char *name, uint32_t name_
     if (launches) *launches = k.launches;
     if (bytes) *bytes = k.bytes;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_graph_upload(asdm_ctx *ctx, asdm_graph *g) {
+int asdm_graph_upload(asdm_ctx *ctx, asdm_graph *g) try {
     if (!ctx || !g) return ASDM_E_INVALID;
     if (g->dev.device == ctx->device) return ASDM_OK;
     if (g->dev.device >= 0) release_device_graph(*g);
@@ -378,10 +378,10 @@ int asdm_graph_upload(asdm_ctx *ctx, asdm_graph *g) {
         return ctx->fail(rc, "graph upload failed");
     }
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
 int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k, uint32_t *out,
-                     uint32_t cap, uint32_t *n) {
+                     uint32_t cap, uint32_t *n) try {
     if (!ctx || !lists || !sizes || !n || k == 0) return ASDM_E_INVALID;
     (void)hipSetDevice(ctx->device);
     std::vector<uint64_t> offs(k + 1, 0);
@@ -418,6 +418,6 @@ int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t
     dev_free(ctx, dn);
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
-}
+} ASDM_CTX_CATCH(ctx)
 
 }  // extern "C"

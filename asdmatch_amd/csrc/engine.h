@@ -125,6 +125,33 @@ struct asdm_ctx {
     }
 };
 
+namespace asdm {
+
+// Records code and message as the context's last error where that is possible, for a handler
+// that must not throw.
+inline int ctx_failure(asdm_ctx *c, int code, const char *msg) noexcept {
+    try {
+        return c ? c->fail(code, msg) : code;
+    } catch (...) {
+        return code;
+    }
+}
+
+}  // namespace asdm
+
+// Handler of the function-try-block of an entry point with a context `c`: as ASDM_HOST_CATCH, and
+// the failure is also the context's last error.
+#define ASDM_CTX_CATCH(c)                                                                  \
+    catch (const std::bad_alloc &) {                                                      \
+        return asdm::ctx_failure((c), ASDM_E_OOM, "host allocation failed");             \
+    }                                                                                     \
+    catch (const std::exception &e) {                                                     \
+        return asdm::ctx_failure((c), ASDM_E_INVALID, e.what());                          \
+    }                                                                                     \
+    catch (...) {                                                                         \
+        return asdm::ctx_failure((c), ASDM_E_INVALID, "unexpected host exception");      \
+    }
+
 struct asdm_cs {
     // Device buffers shared by the candidate spaces of one filter call (a batch of queries); the
     // last view freed releases them.

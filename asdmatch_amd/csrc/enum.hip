@@ -2600,7 +2600,7 @@ using namespace asdm;
 
 extern "C" {
 
-int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_scale, uint32_t tile_iters) {
+int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_scale, uint32_t tile_iters) try {
     if (!ctx || !(leaf_lanes == 1 || leaf_lanes == 4 || leaf_lanes == 16 || leaf_lanes == 64) || !(class_scale > 0) ||
         tile_iters < 1 || tile_iters > 64)
         return ASDM_E_INVALID;
@@ -2608,22 +2608,22 @@ int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_s
     ctx->tune_class_scale = class_scale;
     ctx->tune_tile_iters = tile_iters;
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
-int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale, uint32_t *tile_iters) {
+int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale, uint32_t *tile_iters) try {
     if (!ctx) return ASDM_E_INVALID;
     if (leaf_lanes) *leaf_lanes = ctx->tune_leaf_lanes;
     if (class_scale) *class_scale = ctx->tune_class_scale;
     if (tile_iters) *tile_iters = ctx->tune_tile_iters;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 // Autotuner of the batch launch shape: after one untimed run of the caller's batch, every shape of
 // a small grid runs it `reps` times; the shape with the smallest median time is kept on the context.
 // Every run's counts must equal the first run's (they are plan- and shape-independent).
 int asdm_batch_autotune(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
                         uint64_t budget_bytes, asdm_mpnn *m, uint32_t reps, uint32_t *leaf_lanes, double *class_scale,
-                        uint32_t *tile_iters, float *ms_best) {
+                        uint32_t *tile_iters, float *ms_best) try {
     if (!ctx || !g || (!qs && nq)) return ASDM_E_INVALID;
     reps = std::max<uint32_t>(1, reps);
     const uint32_t save_l = ctx->tune_leaf_lanes, save_t = ctx->tune_tile_iters;
@@ -2666,22 +2666,22 @@ int asdm_batch_autotune(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *co
     if (tile_iters) *tile_iters = bt;
     if (ms_best) *ms_best = (float)best_ms;
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
-int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint64_t *count) {
+int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint64_t *count) try {
     return run_engine(ctx, plan, cs, budget_bytes, nullptr, count);
-}
+} ASDM_CTX_CATCH(ctx)
 
 int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint64_t *mat,
-                     uint64_t *count) {
+                     uint64_t *count) try {
     if (!mat) return ASDM_E_INVALID;
     return run_engine(ctx, plan, cs, budget_bytes, mat, count);
-}
+} ASDM_CTX_CATCH(ctx)
 
 int asdm_enumerate(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes, uint32_t *out,
-                   uint64_t cap_rows, uint64_t *count) {
+                   uint64_t cap_rows, uint64_t *count) try {
     return run_engine(ctx, plan, cs, budget_bytes, nullptr, count, true, out, out ? cap_rows : 0);
-}
+} ASDM_CTX_CATCH(ctx)
 
 
 // Batched multi-query count: the filter of all queries runs as one batched pass (filter_many);
@@ -2690,15 +2690,15 @@ int asdm_enumerate(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint
 // counted with the single-query engine instead.  lat_ms[i] is the time at which query i's count
 // was known, from the call start.
 int asdm_batch_count(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq, uint64_t budget_bytes,
-                     uint64_t *counts, float *lat_ms) {
+                     uint64_t *counts, float *lat_ms) try {
     return asdm_batch_count_learned(ctx, g, qs, nq, budget_bytes, nullptr, counts, nullptr, lat_ms);
-}
+} ASDM_CTX_CATCH(ctx)
 
 // The same with the plans of all queries ordered by the learned estimator (one lock-step greedy
 // over the batch); m == NULL gives the plain planner.  With qstatus, a query that fails on its own
 // (row limit, unsupported shape) gets its status there and the others are still counted.
 int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq,
-                             uint64_t budget_bytes, asdm_mpnn *m, uint64_t *counts, int *qstatus, float *lat_ms) {
+                             uint64_t budget_bytes, asdm_mpnn *m, uint64_t *counts, int *qstatus, float *lat_ms) try {
     if (!ctx || !g || (!qs && nq) || (!counts && nq)) return ASDM_E_INVALID;
     if (g->dev.device != ctx->device) return ctx->fail(ASDM_E_INVALID, "data graph not uploaded to this device");
     (void)hipSetDevice(ctx->device);
@@ -2779,5 +2779,5 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
     for (auto p : bplans) asdm_plan_free(p);
     for (auto c : cs) asdm_cs_free(c);
     return rc;
-}
+} ASDM_CTX_CATCH(ctx)
 }  // extern "C"

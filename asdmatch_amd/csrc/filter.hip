@@ -578,44 +578,44 @@ int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs,
 
 extern "C" {
 
-int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asdm_cs **out) {
+int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asdm_cs **out) try {
     if (!ctx || !q || !g || !out) return ASDM_E_INVALID;
     *out = nullptr;
     int status = ASDM_E_INVALID;
     int rc = filter_many(ctx, g, &q, 1, out, &status);
     if (rc) return rc;
     return status;
-}
+} ASDM_CTX_CATCH(ctx)
 
 int asdm_filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nq, asdm_cs **cs,
-                     int *status) {
+                     int *status) try {
     if (!ctx || !g || (nq && (!qs || !cs || !status))) return ASDM_E_INVALID;
     for (uint32_t i = 0; i < nq; ++i)
         if (!qs[i]) return ASDM_E_INVALID;
     return filter_many(ctx, g, qs, nq, cs, status);
-}
+} ASDM_CTX_CATCH(ctx)
 
 uint32_t asdm_cs_root(const asdm_cs *cs) { return cs ? cs->root : 0; }
 
-int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes) {
+int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes) try {
     if (!cs || !sizes) return ASDM_E_INVALID;
     std::memcpy(sizes, cs->csize.data(), sizeof(uint32_t) * cs->nq);
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_cs_candidates(const asdm_cs *cs, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *n) {
+int asdm_cs_candidates(const asdm_cs *cs, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *n) try {
     if (!cs || u >= cs->nq || !n) return ASDM_E_INVALID;
     *n = cs->csize[u];
     if (!out) return ASDM_OK;
     if (cap < cs->csize[u]) return ASDM_E_INVALID;
     (void)hipSetDevice(cs->ctx->device);
     return download(cs->ctx, out, cs->d_cand + cs->cand_off[u], sizeof(uint32_t) * cs->csize[u]);
-}
+} ASDM_HOST_CATCH
 
-int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w) {
+int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w) try {
     if (!cs || !w || a >= cs->narcs) return ASDM_E_INVALID;
     *w = cs->arc_weight[a];
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

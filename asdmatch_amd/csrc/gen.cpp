@@ -143,7 +143,7 @@ using asdm::Rng;
 extern "C" {
 
 int asdm_gen_chung_lu(uint32_t n, uint64_t ne, double beta, uint32_t nlabels, double zipf_s, uint64_t seed,
-                      asdm_graph **out) {
+                      asdm_graph **out) try {
     if (!out || n < 2 || beta <= 1.0 || nlabels == 0) return ASDM_E_INVALID;
     *out = nullptr;
     if (ne > (uint64_t)n * (n - 1) / 4 || 2 * ne >= (1ull << 32)) return ASDM_E_INVALID;
@@ -167,10 +167,10 @@ int asdm_gen_chung_lu(uint32_t n, uint64_t ne, double beta, uint32_t nlabels, do
     }
     *out = asdm::finish_random(n, edges, nlabels, zipf_s, r);
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 int asdm_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint32_t nlabels,
-                  double zipf_s, uint64_t seed, asdm_graph **out) {
+                  double zipf_s, uint64_t seed, asdm_graph **out) try {
     if (!out || scale == 0 || scale > 30 || nlabels == 0 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
         return ASDM_E_INVALID;
     *out = nullptr;
@@ -207,10 +207,10 @@ int asdm_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, doub
     }
     *out = asdm::finish_random(n, edges, nlabels, zipf_s, r);
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 int asdm_gen_query(const asdm_graph *g, uint32_t k, double density, uint32_t max_extra, uint64_t seed,
-                   asdm_graph **out) {
+                   asdm_graph **out) try {
     if (!g || !out || k == 0 || k > 64 || density < 0 || density > 1) return ASDM_E_INVALID;
     *out = nullptr;
     Rng r(seed);
@@ -270,6 +270,6 @@ int asdm_gen_query(const asdm_graph *g, uint32_t k, double density, uint32_t max
     for (uint32_t i = 0; i < k; ++i) lab[i] = g->lab[verts[i]];
     *out = asdm::graph_from_edges(k, lab.data(), edges.size() / 2, edges.data());
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

@@ -205,12 +205,17 @@ bool parse_parallel(Scanner &s, asdm_graph &g, uint32_t n, uint32_t &maxlab, int
         for (size_t i = 0; i < P.v.size(); i += 3) {
             const uint32_t id = P.v[i], l = P.v[i + 1], d = P.v[i + 2];
             if (id >= n) { rc = ASDM_E_IO; return true; }
+            if (l > asdm::kMaxLabel) { rc = ASDM_E_UNSUPPORTED; return true; }
             g.lab[id] = l;
             g.off[id + 1] = g.off[id] + d;
             maxlab = std::max(maxlab, l);
         }
         if (P.bad) { rc = ASDM_E_IO; return true; }
     }
+    // the offsets are final here: non-decreasing up to m keeps every placement inside nbr
+    for (uint32_t v = 0; v < n; ++v)
+        if (g.off[v] > g.off[v + 1]) { rc = ASDM_E_IO; return true; }
+    if (g.off[n] != g.m) { rc = ASDM_E_IO; return true; }
     // thread k owns the adjacency lists of vertices [v0, v1): it reads every edge record and places
     // the endpoints in its range, so no cursor is shared
     std::vector<uint32_t> fill(n, 0);
@@ -281,6 +286,7 @@ static int load_binary(const std::vector<char> &buf, asdm_graph **out) {
     p += 4ull * (n + 1);
     if (m) std::memcpy(g->nbr.data(), p, 4ull * m);
     // the indexes trust the layout: check it (labels in range, offsets, ids, sorted adjacency)
+    if (nlab > asdm::kMaxLabel + 1) return ASDM_E_UNSUPPORTED;
     if (g->off[0] != 0 || g->off[n] != m) return ASDM_E_IO;
     for (uint32_t v = 0; v < n; ++v)
         if (g->lab[v] >= nlab || g->off[v] > g->off[v + 1]) return ASDM_E_IO;
@@ -308,7 +314,7 @@ static int load_binary(const std::vector<char> &buf, asdm_graph **out) {
     return ASDM_OK;
 }
 
-int asdm_graph_save_binary(const asdm_graph *g, const char *path) {
+int asdm_graph_save_binary(const asdm_graph *g, const char *path) try {
     if (!g || !path) return ASDM_E_INVALID;
     FILE *f = std::fopen(path, "wb");
     if (!f) return ASDM_E_IO;
@@ -320,9 +326,9 @@ int asdm_graph_save_binary(const asdm_graph *g, const char *path) {
     ok = ok && (m == 0 || std::fwrite(g->nbr.data(), 4, m, f) == m);
     ok = (std::fclose(f) == 0) && ok;
     return ok ? ASDM_OK : ASDM_E_IO;
-}
+} ASDM_HOST_CATCH
 
-int asdm_graph_load_text(const char *path, asdm_graph **out) {
+int asdm_graph_load_text(const char *path, asdm_graph **out) try {
     if (!path || !out) return ASDM_E_INVALID;
     *out = nullptr;
     Scanner s;
@@ -334,6 +340,7 @@ int asdm_graph_load_text(const char *path, asdm_graph **out) {
     g->n = n;
     g->m = 2ull * me;
     if (g->m >= (1ull << 32)) return ASDM_E_UNSUPPORTED;
+    if ((uint64_t)me > s.buf.size() / 5) return ASDM_E_IO;  // each edge needs a record of at least 5 bytes
     g->lab.assign(n, 0);
     g->off.assign((size_t)n + 1, 0);
     g->nbr.assign(g->m, 0);
@@ -357,12 +364,15 @@ int asdm_graph_load_text(const char *path, asdm_graph **out) {
         if (c == 'v') {
             uint32_t id, l, d;
             if (!s.next_u32(id) || !s.next_u32(l) || !s.next_u32(d) || id >= n) return ASDM_E_IO;
+            if (l > asdm::kMaxLabel) return ASDM_E_UNSUPPORTED;
             g->lab[id] = l;
             g->off[id + 1] = g->off[id] + d;
             maxlab = std::max(maxlab, l);
         } else if (c == 'e') {
             uint32_t a, b;
             if (!s.next_u32(a) || !s.next_u32(b) || a >= n || b >= n) return ASDM_E_IO;
+            // every write stays below off[v + 1] <= m, whatever the degrees declared so far
+            if (g->off[a + 1] > g->m || g->off[b + 1] > g->m) return ASDM_E_IO;
             if (g->off[a] + fill[a] >= g->off[a + 1] || g->off[b] + fill[b] >= g->off[b + 1]) return ASDM_E_IO;
             g->nbr[g->off[a] + fill[a]++] = b;
             g->nbr[g->off[b] + fill[b]++] = a;
@@ -375,19 +385,21 @@ int asdm_graph_load_text(const char *path, asdm_graph **out) {
     asdm::finish_graph(*g);
     *out = g.release();
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges, asdm_graph **out) {
+int asdm_graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges, asdm_graph **out) try {
     if (!out || (n && !labels) || (ne && !edges)) return ASDM_E_INVALID;
     *out = nullptr;
     if (2 * ne >= (1ull << 32)) return ASDM_E_UNSUPPORTED;
     for (uint64_t i = 0; i < 2 * ne; ++i)
         if (edges[i] >= n) return ASDM_E_INVALID;
+    for (uint32_t v = 0; v < n; ++v)
+        if (labels[v] > asdm::kMaxLabel) return ASDM_E_UNSUPPORTED;
     *out = asdm::graph_from_edges(n, labels, ne, edges);
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_graph_write_text(const asdm_graph *g, const char *path) {
+int asdm_graph_write_text(const asdm_graph *g, const char *path) try {
     if (!g || !path) return ASDM_E_INVALID;
     FILE *f = std::fopen(path, "w");
     if (!f) return ASDM_E_IO;
@@ -400,7 +412,7 @@ int asdm_graph_write_text(const asdm_graph *g, const char *path) {
             if (v < g->nbr[i]) std::fprintf(f, "e %u %u\n", v, g->nbr[i]);
     int rc = std::fclose(f) == 0 ? ASDM_OK : ASDM_E_IO;
     return rc;
-}
+} ASDM_HOST_CATCH
 
 void asdm_graph_free(asdm_graph *g) {
     if (!g) return;
@@ -408,23 +420,23 @@ void asdm_graph_free(asdm_graph *g) {
     delete g;
 }
 
-int asdm_graph_info(const asdm_graph *g, uint32_t *n, uint64_t *m, uint32_t *nlabels) {
+int asdm_graph_info(const asdm_graph *g, uint32_t *n, uint64_t *m, uint32_t *nlabels) try {
     if (!g) return ASDM_E_INVALID;
     if (n) *n = g->n;
     if (m) *m = g->m;
     if (nlabels) *nlabels = g->nlab;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_graph_csr(const asdm_graph *g, uint32_t *off, uint32_t *nbr, uint32_t *lab) {
+int asdm_graph_csr(const asdm_graph *g, uint32_t *off, uint32_t *nbr, uint32_t *lab) try {
     if (!g) return ASDM_E_INVALID;
     if (off) std::memcpy(off, g->off.data(), sizeof(uint32_t) * ((size_t)g->n + 1));
     if (nbr && g->m) std::memcpy(nbr, g->nbr.data(), sizeof(uint32_t) * g->m);
     if (lab && g->n) std::memcpy(lab, g->lab.data(), sizeof(uint32_t) * g->n);
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_graph_edges(const asdm_graph *g, uint32_t *edges, uint64_t cap, uint64_t *ne) {
+int asdm_graph_edges(const asdm_graph *g, uint32_t *edges, uint64_t cap, uint64_t *ne) try {
     if (!g || !ne) return ASDM_E_INVALID;
     std::vector<std::pair<uint32_t, uint32_t>> es;
     es.reserve(g->m / 2);
@@ -440,6 +452,6 @@ int asdm_graph_edges(const asdm_graph *g, uint32_t *edges, uint64_t cap, uint64_
         edges[2 * i + 1] = es[i].second;
     }
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

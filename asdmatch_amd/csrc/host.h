@@ -2,12 +2,28 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
 #include "asdm.h"
 
+// Handler of the function-try-block around every int-returning entry point: no C++ exception
+// crosses the C ABI.  A failed host allocation returns ASDM_E_OOM, any other exception
+// ASDM_E_INVALID.
+#define ASDM_HOST_CATCH                     \
+    catch (const std::bad_alloc &) {        \
+        return ASDM_E_OOM;                  \
+    }                                       \
+    catch (...) {                           \
+        return ASDM_E_INVALID;              \
+    }
+
 namespace asdm {
+
+// Largest label value a graph may hold: labels index per-label tables of (max label + 1) entries.
+constexpr uint32_t kMaxLabel = (1u << 24) - 1;
 
 // Label-sorted CSR as the reference's Graph holds it (relation/graph.h:34-59): neighbours of v
 // are nbr[off[v] .. off[v+1]) sorted by (label(nbr), nbr).

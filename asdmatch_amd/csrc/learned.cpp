@@ -312,7 +312,7 @@ using namespace asdm;
 extern "C" {
 
 int asdm_mpnn_features(const asdm_graph *g, const asdm_cs *cs, const uint64_t *masks, uint32_t n, uint32_t *node_off,
-                       float *x, uint32_t *adj_off, uint32_t *adj, uint32_t *num_nodes, uint32_t *num_adj) {
+                       float *x, uint32_t *adj_off, uint32_t *adj, uint32_t *num_nodes, uint32_t *num_adj) try {
     if (!g || !cs || (!masks && n) || !num_nodes || !num_adj) return ASDM_E_INVALID;
     FeatBatch fb;
     for (uint32_t i = 0; i < n; ++i) {
@@ -331,10 +331,10 @@ int asdm_mpnn_features(const asdm_graph *g, const asdm_cs *cs, const uint64_t *m
     *num_nodes = N;
     *num_adj = E;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 int asdm_mpnn_estimate(asdm_mpnn *m, const asdm_graph *g, const asdm_cs *cs, const uint64_t *masks, uint32_t n,
-                       double *cards) {
+                       double *cards) try {
     if (!m || !g || !cs || (!masks && n) || (!cards && n)) return ASDM_E_INVALID;
     FeatBatch fb;
     const uint64_t valid = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
@@ -346,9 +346,9 @@ int asdm_mpnn_estimate(asdm_mpnn *m, const asdm_graph *g, const asdm_cs *cs, con
     if (int rc = forward(m, fb, y)) return rc;
     for (uint32_t i = 0; i < n; ++i) cards[i] = card_from_log(y[i]);
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, double *cards) {
+int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, double *cards) try {
     if (!cs || (n && (!masks || !cards))) return ASDM_E_INVALID;
     const uint64_t valid = cs->nq == 64 ? ~0ull : ((1ull << cs->nq) - 1);
     for (uint32_t i = 0; i < n; ++i) {
@@ -363,15 +363,15 @@ int asdm_plain_estimate(const asdm_cs *cs, const uint64_t *masks, uint32_t n, do
         cards[i] = est;
     }
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 int asdm_plan_create_learned(asdm_mpnn *m, const asdm_graph *g, const asdm_graph *q, const asdm_cs *cs,
-                             uint32_t flags, asdm_plan **out) {
+                             uint32_t flags, asdm_plan **out) try {
     if (!m || !g || !q || !cs || !out || q->n != cs->nq) return ASDM_E_INVALID;
     std::vector<asdm_plan *> plans;
     int rc = learned_plans(m, g, {cs}, flags, plans);
     *out = rc ? nullptr : plans[0];
     return rc;
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

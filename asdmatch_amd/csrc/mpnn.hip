@@ -295,7 +295,7 @@ int upload_batch(asdm_ctx *ctx, DevBatch &d, uint32_t B, const uint32_t *node_of
 
 extern "C" {
 
-int asdm_mpnn_create(asdm_ctx *ctx, uint64_t seed, asdm_mpnn **out) {
+int asdm_mpnn_create(asdm_ctx *ctx, uint64_t seed, asdm_mpnn **out) try {
     if (!ctx || !out) return ASDM_E_INVALID;
     (void)hipSetDevice(ctx->device);
     auto m = std::make_unique<asdm_mpnn>();
@@ -327,7 +327,7 @@ int asdm_mpnn_create(asdm_ctx *ctx, uint64_t seed, asdm_mpnn **out) {
     if (rc) return rc;
     *out = m.release();
     return ASDM_OK;
-}
+} ASDM_CTX_CATCH(ctx)
 
 void asdm_mpnn_free(asdm_mpnn *m) {
     if (!m) return;
@@ -338,23 +338,23 @@ void asdm_mpnn_free(asdm_mpnn *m) {
 uint32_t asdm_mpnn_num_params(void) { return (uint32_t)MpnnLayout::total; }
 
 // fp32 master parameters (n = asdm_mpnn_num_params()) in and out; set() refreshes the bf16 copy.
-int asdm_mpnn_get_params(asdm_mpnn *m, float *p) {
+int asdm_mpnn_get_params(asdm_mpnn *m, float *p) try {
     if (!m || !p) return ASDM_E_INVALID;
     int rc = check_hip(m->ctx, hipMemcpyAsync(p, m->master, 4 * MpnnLayout::total, hipMemcpyDeviceToHost, m->ctx->stream), "D2H");
     return rc ? rc : check_hip(m->ctx, hipStreamSynchronize(m->ctx->stream), "sync");
-}
+} ASDM_HOST_CATCH
 
-int asdm_mpnn_set_params(asdm_mpnn *m, const float *p) {
+int asdm_mpnn_set_params(asdm_mpnn *m, const float *p) try {
     if (!m || !p) return ASDM_E_INVALID;
     const int n = MpnnLayout::total;
     int rc = check_hip(m->ctx, hipMemcpyAsync(m->master, p, 4 * n, hipMemcpyHostToDevice, m->ctx->stream), "H2D");
     if (rc) return rc;
     hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, m->ctx->stream, m->master, m->wb, n);
     return check_hip(m->ctx, hipStreamSynchronize(m->ctx->stream), "sync");
-}
+} ASDM_HOST_CATCH
 
 int asdm_mpnn_forward(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const float *x, const uint32_t *adj_off,
-                      const uint32_t *adj, float *y) {
+                      const uint32_t *adj, float *y) try {
     if (!m || !node_off || !x || !adj_off || !y) return ASDM_E_INVALID;
     if (B == 0) return ASDM_OK;
     asdm_ctx *ctx = m->ctx;
@@ -369,11 +369,11 @@ int asdm_mpnn_forward(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const 
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(y, d.y, 4 * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return rc;
-}
+} ASDM_HOST_CATCH
 
 // Gradient of 0.5 * mean (y - t)^2 w.r.t. the bf16 weights (fp32), without an update.
 int asdm_mpnn_grad(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const float *x, const uint32_t *adj_off,
-                   const uint32_t *adj, const float *targets, float *grad, float *y) {
+                   const uint32_t *adj, const float *targets, float *grad, float *y) try {
     if (!m || !node_off || !x || !adj_off || !targets || !grad) return ASDM_E_INVALID;
     asdm_ctx *ctx = m->ctx;
     const int n = MpnnLayout::total;
@@ -389,11 +389,11 @@ int asdm_mpnn_grad(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const flo
     if (!rc && y) rc = check_hip(ctx, hipMemcpyAsync(y, d.y, 4 * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return rc;
-}
+} ASDM_HOST_CATCH
 
 // One Adam step on a batch; *loss = 0.5 * mean (y - t)^2 before the step.
 int asdm_mpnn_train_step(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const float *x, const uint32_t *adj_off,
-                         const uint32_t *adj, const float *targets, float lr, float *loss) {
+                         const uint32_t *adj, const float *targets, float lr, float *loss) try {
     if (!m || !node_off || !x || !adj_off || !targets || B == 0) return ASDM_E_INVALID;
     asdm_ctx *ctx = m->ctx;
     const int n = MpnnLayout::total;
@@ -424,9 +424,9 @@ int asdm_mpnn_train_step(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, con
         *loss = (float)(s / B);
     }
     return rc;
-}
+} ASDM_HOST_CATCH
 
-int asdm_mpnn_save(asdm_mpnn *m, const char *path) {
+int asdm_mpnn_save(asdm_mpnn *m, const char *path) try {
     if (!m || !path) return ASDM_E_INVALID;
     std::vector<float> p(MpnnLayout::total);
     int rc = asdm_mpnn_get_params(m, p.data());
@@ -436,9 +436,9 @@ int asdm_mpnn_save(asdm_mpnn *m, const char *path) {
     const uint32_t hdr[4] = {0x4d504e4eu, (uint32_t)kF, (uint32_t)kH, (uint32_t)MpnnLayout::total};
     bool ok = std::fwrite(hdr, 4, 4, f) == 4 && std::fwrite(p.data(), 4, p.size(), f) == p.size();
     return std::fclose(f) == 0 && ok ? ASDM_OK : ASDM_E_IO;
-}
+} ASDM_HOST_CATCH
 
-int asdm_mpnn_load(asdm_mpnn *m, const char *path) {
+int asdm_mpnn_load(asdm_mpnn *m, const char *path) try {
     if (!m || !path) return ASDM_E_INVALID;
     FILE *f = std::fopen(path, "rb");
     if (!f) return ASDM_E_IO;
@@ -449,6 +449,6 @@ int asdm_mpnn_load(asdm_mpnn *m, const char *path) {
     std::fclose(f);
     if (!ok) return ASDM_E_IO;
     return asdm_mpnn_set_params(m, p.data());
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

@@ -399,7 +399,7 @@ using namespace asdm;
 extern "C" {
 
 int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const uint64_t *masks, const double *cards,
-                                uint32_t ncards, uint32_t flags, asdm_plan **out) {
+                                uint32_t ncards, uint32_t flags, asdm_plan **out) try {
     if (!q || !cs || !out || q->n != cs->nq) return ASDM_E_INVALID;
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -419,9 +419,9 @@ int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const ui
     p->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = p.release();
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, asdm_plan **out) {
+int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags, asdm_plan **out) try {
     if (!q || !cs || !out || q->n != cs->nq) return ASDM_E_INVALID;
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
@@ -435,10 +435,10 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
     p->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = p.release();
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t num_enumerated,
-                             uint32_t flags, asdm_plan **out) {
+                             uint32_t flags, asdm_plan **out) try {
     if (!q || !cs || !order || !out || q->n != cs->nq || num_enumerated < 1 || num_enumerated > cs->nq)
         return ASDM_E_INVALID;
     *out = nullptr;
@@ -472,11 +472,11 @@ int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint3
     p->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = p.release();
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 void asdm_plan_free(asdm_plan *p) { delete p; }
 
-int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n) {
+int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n) try {
     if (!p || !n) return ASDM_E_INVALID;
     *n = (uint32_t)p->prefix_cards.size();
     if (!masks || !cards) return ASDM_OK;
@@ -486,20 +486,20 @@ int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t
         cards[i] = p->prefix_cards[i].second;
     }
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_plan_estimate(const asdm_plan *p, double *rows_leaf, double *rows_total) {
+int asdm_plan_estimate(const asdm_plan *p, double *rows_leaf, double *rows_total) try {
     if (!p) return ASDM_E_INVALID;
     if (rows_leaf) *rows_leaf = p->est_leaf;
     if (rows_total) *rows_total = p->est_total;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
-int asdm_plan_order(const asdm_plan *p, uint32_t *order, uint32_t *num_enumerated) {
+int asdm_plan_order(const asdm_plan *p, uint32_t *order, uint32_t *num_enumerated) try {
     if (!p) return ASDM_E_INVALID;
     if (order) std::memcpy(order, p->order.data(), sizeof(uint32_t) * p->order.size());
     if (num_enumerated) *num_enumerated = p->D;
     return ASDM_OK;
-}
+} ASDM_HOST_CATCH
 
 }  // extern "C"

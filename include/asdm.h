@@ -53,6 +53,8 @@ extern "C" {
 #define ASDM_E_BUDGET (-6)        /* frontier budget too small for one expansion */
 #define ASDM_E_UNSUPPORTED (-7)   /* query outside the supported envelope (e.g. > 32 vertices) */
 #define ASDM_E_TIMEOUT (-8)       /* the per-query time limit elapsed (count unknown) */
+/* No C++ exception leaves the library: an int-returning entry point whose host allocation fails
+ * returns ASDM_E_OOM (with a context, also its last error). */
 
 typedef struct asdm_ctx asdm_ctx;     /* device, stream, scratch arena */
 typedef struct asdm_graph asdm_graph; /* host graph (label-sorted CSR), optionally on device */
@@ -80,14 +82,18 @@ const char *asdm_version(void);
  * adjacency sorted by (label(nbr), nbr).  Declared degrees must equal the edge records.
  * Replaces Graph::loadGraphFromTextFile (relation/graph.cpp:68-148).  Files of 32 MB and more are
  * parsed by host threads, with the same result.  A file written by asdm_graph_save_binary (it
- * starts with "ASDMCSR1") is read as that binary CSR instead. */
+ * starts with "ASDMCSR1") is read as that binary CSR instead.  Labels above 2^24 - 1 return
+ * ASDM_E_UNSUPPORTED (per-label tables hold max label + 1 entries); a header whose edge count the
+ * file cannot hold, or degrees that do not add up to it, return ASDM_E_IO before any adjacency
+ * entry is placed out of range. */
 int asdm_graph_load_text(const char *path, asdm_graph **out);
 /* Binary CSR cache of a loaded graph (labels, offsets, sorted adjacency); asdm_graph_load_text
  * reads it back without parsing or sorting.  The reference has a binary graph format
  * (relation/graph.cpp:293-332) that its main never uses; this one is for callers that load the
  * same large graph many times. */
 int asdm_graph_save_binary(const asdm_graph *g, const char *path);
-/* Graph from n labels and ne undirected edges (edges[2*i], edges[2*i+1]), in record order. */
+/* Graph from n labels and ne undirected edges (edges[2*i], edges[2*i+1]), in record order.
+ * Endpoints >= n return ASDM_E_INVALID, labels above 2^24 - 1 ASDM_E_UNSUPPORTED. */
 int asdm_graph_from_edges(uint32_t n, const uint32_t *labels, uint64_t ne, const uint32_t *edges,
                           asdm_graph **out);
 int asdm_graph_write_text(const asdm_graph *g, const char *path);

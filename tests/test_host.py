@@ -101,6 +101,50 @@ def test_parallel_parse_of_large_files_matches_sequential_and_oracle():
         open(r, "w").write("\n".join(bad))
         with pytest.raises(A.AsdmError):
             A.Graph.load(r)
+        # declared degrees beyond the header's edge count, and an out-of-range label, in the
+        # parallel path: rejected before any adjacency entry is placed
+        for v, line in ((3, f"v 3 {int(lab[3])} 1000000"), (7, f"v 7 4294967295 {int(off[8] - off[7])}")):
+            bad = lines[:]
+            bad[1 + v] = line
+            open(r, "w").write("\n".join(bad))
+            with pytest.raises(A.AsdmError) as e:
+                A.Graph.load(r)
+            assert e.value.code == (-2 if v == 3 else -7)
+
+
+def test_malformed_degrees_labels_and_headers_are_rejected():
+    cases = [
+        # degrees declared beyond the header's edge count, then more edge records than fit
+        ("t 2 1\nv 0 0 0\nv 1 0 5\ne 1 1\ne 1 1\n", -2),
+        # a label whose per-label tables cannot exist (max label + 1 overflows)
+        ("t 2 1\nv 0 4294967295 1\nv 1 0 1\ne 0 1\n", -7),
+        ("t 2 1\nv 0 16777216 1\nv 1 0 1\ne 0 1\n", -7),
+        # a header edge count that the file cannot hold (each edge needs a record)
+        ("t 2 1000000\nv 0 0 1\nv 1 0 1\ne 0 1\n", -2),
+    ]
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "m.graph")
+        for text, code in cases:
+            open(p, "w").write(text)
+            with pytest.raises(A.AsdmError) as e:
+                A.Graph.load(p)
+            assert e.value.code == code, text
+        # the largest label value is accepted
+        open(p, "w").write("t 2 1\nv 0 16777215 1\nv 1 0 1\ne 0 1\n")
+        assert A.Graph.load(p).info()[2] == 1 << 24
+        with pytest.raises(A.AsdmError) as e:
+            A.Graph.from_edges([0, 1 << 24], [(0, 1)])
+        assert e.value.code == -7
+        # a binary cache whose label count field is out of range
+        g = A.Graph.from_edges([0, 1, 1], [(0, 1), (1, 2)])
+        b = os.path.join(d, "g.csr")
+        g.save_binary(b)
+        raw = bytearray(open(b, "rb").read())
+        raw[12:16] = np.uint32(0xFFFFFFFF).tobytes()
+        open(b, "wb").write(bytes(raw))
+        with pytest.raises(A.AsdmError) as e:
+            A.Graph.load(b)
+        assert e.value.code == -7
 
 
 def test_binary_cache_round_trip():
