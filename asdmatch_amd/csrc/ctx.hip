@@ -384,21 +384,21 @@ int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t
                      uint32_t cap, uint32_t *n) try {
     if (!ctx || !lists || !sizes || !n || k == 0) return ASDM_E_INVALID;
     (void)hipSetDevice(ctx->device);
-    std::vector<uint64_t> offs(k + 1, 0);
+    std::vector<uint64_t> offs(k + 1ull, 0);
     for (uint32_t j = 0; j < k; ++j) offs[j + 1] = offs[j] + sizes[j];
     uint64_t minsz = UINT64_MAX;
     for (uint32_t j = 0; j < k; ++j) minsz = std::min<uint64_t>(minsz, sizes[j]);
     uint32_t *dflat = nullptr, *dout = nullptr, *dn = nullptr;
     uint64_t *doffs = nullptr;
     int rc = dev_alloc(ctx, (void **)&dflat, sizeof(uint32_t) * std::max<uint64_t>(offs[k], 1));
-    if (!rc) rc = dev_alloc(ctx, (void **)&doffs, sizeof(uint64_t) * (k + 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&doffs, sizeof(uint64_t) * (k + 1ull));
     if (!rc) rc = dev_alloc(ctx, (void **)&dout, sizeof(uint32_t) * std::max<uint64_t>(minsz, 1));
     if (!rc) rc = dev_alloc(ctx, (void **)&dn, sizeof(uint32_t));
     for (uint32_t j = 0; j < k && !rc; ++j)
         if (sizes[j])
             rc = check_hip(ctx, hipMemcpyAsync(dflat + offs[j], lists[j], sizeof(uint32_t) * sizes[j], hipMemcpyHostToDevice,
                                                ctx->stream), "copy");
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(doffs, offs.data(), sizeof(uint64_t) * (k + 1), hipMemcpyHostToDevice, ctx->stream), "copy");
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(doffs, offs.data(), sizeof(uint64_t) * (k + 1ull), hipMemcpyHostToDevice, ctx->stream), "copy");
     if (!rc) {
         hipLaunchKernelGGL(k_intersect_k, dim3(1), dim3(64), 0, ctx->stream, dflat, doffs, k, dout, dn);
         rc = check_hip(ctx, hipGetLastError(), "intersect launch");

@@ -274,20 +274,20 @@ int upload_batch(asdm_ctx *ctx, DevBatch &d, uint32_t B, const uint32_t *node_of
             for (uint32_t e = adj_off[n]; e < adj_off[n + 1]; ++e)
                 if (adj[e] < node_off[g] || adj[e] >= node_off[g + 1])
                     return ctx->fail(ASDM_E_INVALID, "neighbour outside its graph");
-    int rc = dev_alloc(ctx, (void **)&d.node_off, 4 * (B + 1));
+    int rc = dev_alloc(ctx, (void **)&d.node_off, 4ull * (B + 1ull));
     if (!rc) rc = dev_alloc(ctx, (void **)&d.x, 4 * (size_t)std::max<uint32_t>(N, 1) * kF);
-    if (!rc) rc = dev_alloc(ctx, (void **)&d.adj_off, 4 * (N + 1));
-    if (!rc) rc = dev_alloc(ctx, (void **)&d.adj, 4 * std::max<uint32_t>(E, 1));
-    if (!rc) rc = dev_alloc(ctx, (void **)&d.y, 4 * std::max<uint32_t>(B, 1));
-    if (!rc && targets) rc = dev_alloc(ctx, (void **)&d.t, 4 * std::max<uint32_t>(B, 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&d.adj_off, 4ull * (N + 1ull));
+    if (!rc) rc = dev_alloc(ctx, (void **)&d.adj, 4ull * std::max<uint32_t>(E, 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&d.y, 4ull * std::max<uint32_t>(B, 1));
+    if (!rc && targets) rc = dev_alloc(ctx, (void **)&d.t, 4ull * std::max<uint32_t>(B, 1));
     auto up = [&](void *dst, const void *src, size_t bytes) {
         return bytes ? check_hip(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D") : 0;
     };
-    if (!rc) rc = up(d.node_off, node_off, 4 * (B + 1));
+    if (!rc) rc = up(d.node_off, node_off, 4ull * (B + 1ull));
     if (!rc) rc = up(d.x, x, 4 * (size_t)N * kF);
-    if (!rc) rc = up(d.adj_off, adj_off, 4 * (N + 1));
+    if (!rc) rc = up(d.adj_off, adj_off, 4ull * (N + 1ull));
     if (!rc) rc = up(d.adj, adj, 4 * (size_t)E);
-    if (!rc && targets) rc = up(d.t, targets, 4 * B);
+    if (!rc && targets) rc = up(d.t, targets, 4ull * B);
     return rc;
 }
 
@@ -366,7 +366,7 @@ int asdm_mpnn_forward(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const 
     hipLaunchKernelGGL(k_mpnn_forward, dim3(std::min<uint32_t>(B, 4096)), dim3(256), 0, ctx->stream, m->wb, gb, B, d.y);
     kt.stop();
     rc = check_hip(ctx, hipGetLastError(), "mpnn forward");
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(y, d.y, 4 * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(y, d.y, 4ull * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return rc;
 } ASDM_HOST_CATCH
@@ -386,7 +386,7 @@ int asdm_mpnn_grad(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, const flo
                        ctx->stream, m->wb, gb, B, d.t, d.y, m->grad);
     rc = check_hip(ctx, hipGetLastError(), "mpnn backward");
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(grad, m->grad, 4 * n, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-    if (!rc && y) rc = check_hip(ctx, hipMemcpyAsync(y, d.y, 4 * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc && y) rc = check_hip(ctx, hipMemcpyAsync(y, d.y, 4ull * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return rc;
 } ASDM_HOST_CATCH
@@ -417,7 +417,7 @@ int asdm_mpnn_train_step(asdm_mpnn *m, uint32_t B, const uint32_t *node_off, con
     if ((rc = check_hip(ctx, hipGetLastError(), "adam"))) return rc;
     if (loss) {
         std::vector<float> y(B), t(targets, targets + B);
-        rc = check_hip(ctx, hipMemcpyAsync(y.data(), d.y, 4 * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        rc = check_hip(ctx, hipMemcpyAsync(y.data(), d.y, 4ull * B, hipMemcpyDeviceToHost, ctx->stream), "D2H");
         if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
         double s = 0;
         for (uint32_t i = 0; i < B; ++i) s += 0.5 * (y[i] - t[i]) * (y[i] - t[i]);
