@@ -27,6 +27,10 @@ int main(int argc, char **argv) {
     Args a(argc, argv);
     const std::string qp = a.str("-q"), dp = a.str("-d"), rp = a.str("-r"), sp = a.str("-s"), cp = a.str("-c");
     int memKB = a.num("-m");
+    if (memKB < 0) {
+        std::fprintf(stderr, "ASDMatch.out: -m takes a positive size in KB\n");
+        return 1;
+    }
     if (memKB == 0) memKB = 1600000;
     const uint64_t budget = (uint64_t)1024 * (uint64_t)memKB;
 

@@ -57,6 +57,11 @@ def test_cli_reports_unopenable_files_like_the_reference():
     assert r.returncode == 255
 
 
+def test_cli_rejects_a_negative_memory_budget():
+    r = run(["ASDMatch.out", "-q", "/nonexistent.graph", "-d", T.yeast_data_path(), "-m", "-5"])
+    assert r.returncode == 1 and "-m takes a positive size" in r.stderr
+
+
 @pytest.mark.gpu
 def test_cli_counts_and_lines(tmp_path):
     gold = T.yeast_golden()["queries"]
