@@ -2708,11 +2708,20 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
     };
     auto per_query = [](int code) { return code == ASDM_E_TIMEOUT || code == ASDM_E_UNSUPPORTED; };
     std::vector<asdm_cs *> cs(nq, nullptr);
+    std::vector<asdm_plan *> bplans;
+    // frees the plans, then the candidate spaces, on every return (an exception's included)
+    struct Release {
+        std::vector<asdm_cs *> &cs;
+        std::vector<asdm_plan *> &plans;
+        ~Release() {
+            for (auto p : plans) asdm_plan_free(p);
+            for (auto c : cs) asdm_cs_free(c);
+        }
+    } release{cs, bplans};
     std::vector<int> status(nq, ASDM_E_INVALID);
     int rc = filter_many(ctx, g, qs, nq, cs.data(), status.data());
     if (rc) return rc;
     std::vector<asdm_cs *> bcs;
-    std::vector<asdm_plan *> bplans;
     std::vector<uint32_t> bidx;
     for (uint32_t i = 0; i < nq; ++i) {
         counts[i] = 0;
@@ -2776,8 +2785,6 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
             if (qstatus) qstatus[bidx[k]] = bst[k];
         }
     }
-    for (auto p : bplans) asdm_plan_free(p);
-    for (auto c : cs) asdm_cs_free(c);
     return rc;
 } ASDM_CTX_CATCH(ctx)
 }  // extern "C"
