@@ -1478,9 +1478,24 @@ __global__ void __launch_bounds__(256) k_bfill(const DPlan *__restrict__ plans, 
     if (lane == 0 && bytes_ctr) atomicAdd(bytes_ctr, (unsigned long long)bytes);
 }
 
+// Logical block index that keeps runs of C consecutive logical blocks on one XCD.  The hardware
+// places block b on XCD b % 8, so plain indexing spreads consecutive tiles (rows that share their
+// prefixes, and so their list and table entries) over the eight L2 caches.  Here chunks of C blocks
+// rotate over the XCDs: neighbouring tiles share an L2, and every XCD still gets a fine-grained share
+// of the launch.  Blocks past the last whole round of 8 * C keep their own index, so the map is a
+// bijection on [0, gridDim.x).
+__device__ inline uint32_t xcd_chunked_block(uint32_t C) {
+    const uint32_t b = blockIdx.x, span = 8 * C, full = gridDim.x / span * span;
+    if (b >= full) return b;
+    const uint32_t x = b % 8, k = b / 8;  // the k-th block placed on XCD x
+    return (k / C) * span + x * C + (k % C);
+}
+
 // Fused rows of the batch in flattened tiles of 64 rows (fused_tile); tiles as in k_bcount with
-// tile_rows = 64.  Five waves per SIMD (at most 96 VGPRs, about 100 B per lane spilled) hide more
-// of the lookups' latency than four without spills (116 VGPRs).
+// tile_rows = 64, taken in XCD-chunked block order (chunks of 64 blocks: 2-3 % less time per launch
+// than plain block order; one contiguous range per XCD is 10 % slower, as heavy tiles cluster).
+// Five waves per SIMD (at most 96 VGPRs, about 100 B per lane spilled) hide more of the lookups'
+// latency than four without spills (116 VGPRs).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_bfused_flat(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                      const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
                                                      uint32_t d, const uint32_t *__restrict__ rows,
@@ -1490,7 +1505,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
                                                      unsigned long long *__restrict__ qcount,
                                                      unsigned long long *__restrict__ bytes_ctr) {
     const int lane = threadIdx.x & 63;
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t wave = ((uint64_t)xcd_chunked_block(64) * blockDim.x + threadIdx.x) >> 6;
     const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t t0 = tile_pref[qa], t1 = tile_pref[qb];
     uint64_t bytes = 0, acc = 0;
