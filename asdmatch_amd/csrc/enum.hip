@@ -476,9 +476,12 @@ __device__ inline uint64_t leaf_row(const DPlan *__restrict__ P, const uint32_t 
     return prod;
 }
 
-// Rows of width D -> sum over rows of the product of the leaf-group counts (mod 2^64).
+// Rows of width D -> sum over rows of the product of the leaf-group counts (mod 2^64).  The
+// lookups are dependent binary searches, so the kernel runs at eight waves per SIMD (64 VGPRs,
+// about 200 B per lane spilled): config-2 queries/s 1.23 at four waves (105 VGPRs, no spills),
+// 1.49 at six, 1.61 at seven, 1.64 at eight.
 template <int G>
-__global__ void __launch_bounds__(256) k_leaf(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_leaf(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
                                               uint64_t nrows, const uint64_t *__restrict__ ce_off,
                                               const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand,
                                               unsigned long long *__restrict__ total,
@@ -1397,9 +1400,12 @@ __device__ inline uint32_t tile_query(const uint64_t *__restrict__ tile_pref, ui
 
 // Per row: finish it (leaf depth) into qcount[query] and count 0 children, or count its children.
 // Every row of a launch is of the given kind (kLeafRows or kExpandRows; fused rows run in
-// k_bfused_flat).
+// k_bfused_flat).  At least seven waves per SIMD: the leaf-depth instantiations need 109 VGPRs
+// for four waves; at seven (72 VGPRs, about 180 B per lane spilled) the config-4 batch takes
+// 76.5 ms against 79.3 (77.3 at six waves, 77.8 at eight).  The expansion kind fits seven
+// waves unforced.
 template <int G, int KIND>
-__global__ void __launch_bounds__(256) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                 const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
                                                 uint32_t tile_rows, uint32_t d, const uint32_t *__restrict__ rows,
                                                 const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
