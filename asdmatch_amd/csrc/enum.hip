@@ -698,27 +698,6 @@ __device__ inline uint64_t leaf_set_count_lane(const DPlan *__restrict__ P, uint
     return c;
 }
 
-// One lane: completions of the leaf group g (inclusion-exclusion over set partitions for k >= 2).
-__device__ inline uint64_t group_count_lane(const DPlan *__restrict__ P, uint32_t g, const RowView &R,
-                                            const uint64_t *__restrict__ ce_off, const uint32_t *__restrict__ ce_val,
-                                            const uint32_t *__restrict__ cand) {
-    const uint32_t e0 = P->gleaf[g][0];
-    const uint32_t ns = P->ns[e0];
-    uint32_t y[kMaxQ];
-    for (uint32_t s = 0; s < ns; ++s) y[s] = matched_id(P, cand, R, P->sdep[e0][s]);
-    const uint32_t k = P->gsize[g];
-    if (k == 1) return leaf_set_count_lane(P, g, 1u, R, ce_off, ce_val, cand, y, ns);
-    uint64_t cs[1u << kMaxGroup];
-    for (uint32_t m = 1; m < (1u << k); ++m) cs[m] = leaf_set_count_lane(P, g, m, R, ce_off, ce_val, cand, y, ns);
-    uint64_t tot = 0;
-    for (uint32_t p = 0; p < P->nparts[k]; ++p) {
-        uint64_t term = (uint64_t)P->pcoef[k][p];
-        for (uint32_t b = 0; b < P->pnblk[k][p]; ++b) term *= cs[P->pblk[k][p][b]];
-        tot += term;
-    }
-    return tot;
-}
-
 // One lane: completions of a single-leaf group of a fusable plan (its leaf has one list, or two
 // forming a triangle, and at most kFusedSame same-label matched vertices).
 constexpr uint32_t kFusedSame = 8;
