@@ -229,3 +229,26 @@ def test_context_fails_loudly_without_gpu():
     with pytest.raises(A.AsdmError) as e:
         A.Context(0)
     assert e.value.code == -5
+
+
+def test_failed_host_allocation_returns_oom_instead_of_aborting(tmp_path):
+    # A header of 4e9 vertices asks for tens of GB of host arrays.  Under an 8 GiB address-space
+    # limit the allocation throws inside the library, and the entry point must turn that into
+    # ASDM_E_OOM rather than let the exception cross the C ABI (which would abort the process).
+    import subprocess
+    import sys
+
+    p = tmp_path / "huge.graph"
+    p.write_text("t 4000000000 0\n")
+    code = (
+        "import resource, ctypes\n"
+        "resource.setrlimit(resource.RLIMIT_AS, (8 << 30, 8 << 30))\n"
+        "import asdmatch_amd as A\n"
+        "lib = ctypes.CDLL(A.LIB_PATH)\n"
+        "g = ctypes.c_void_p()\n"
+        f"print('rc', lib.asdm_graph_load_text({str(p)!r}.encode(), ctypes.byref(g)), flush=True)\n"
+    )
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240,
+                       cwd=os.path.join(os.path.dirname(__file__), ".."))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "rc -3" in r.stdout, r.stdout + r.stderr[-2000:]
