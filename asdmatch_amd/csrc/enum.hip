@@ -449,9 +449,22 @@ __device__ inline uint64_t group_count(const DPlan *__restrict__ P, uint32_t g, 
     load_same(S, P, cand, row, P->gleaf[g][0]);
     const uint32_t k = P->gsize[g];
     if (k == 1) return leaf_set_count<G>(P, g, 1u, row, ce_off, ce_val, cand, S, bytes);
-    // intersection sizes of every nonempty subset of the group, then the partition sum
-    uint64_t cs[1u << kMaxGroup];
-    for (uint32_t m = 1; m < (1u << k); ++m) cs[m] = leaf_set_count<G>(P, g, m, row, ce_off, ce_val, cand, S, bytes);
+    // intersection sizes of every nonempty subset of the group, then the partition sum.  The sizes
+    // sit in LDS, one row per lane group of the 256-thread block: the partition blocks index them at
+    // run time, which in registers would hold 2^kMaxGroup 64-bit values per lane.  Each size is at
+    // most one list's length (32 bits).  The group leader's value is the group's (a single-list
+    // subset is counted by the leader alone), so the leader writes and the group reads.
+    __shared__ uint32_t sizes_lds[256 / G][1u << kMaxGroup];
+    uint32_t *cs = sizes_lds[threadIdx.x / G];
+    const bool leader = (threadIdx.x & (G - 1)) == 0;
+    for (uint32_t m = 1; m < (1u << k); ++m) {
+        const uint64_t v = leaf_set_count<G>(P, g, m, row, ce_off, ce_val, cand, S, bytes);
+        if (leader) cs[m] = (uint32_t)v;
+    }
+    // the group's lanes are in one wave: order the leader's writes before their reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     uint64_t tot = 0;
     for (uint32_t p = 0; p < P->nparts[k]; ++p) {
         uint64_t term = (uint64_t)P->pcoef[k][p];
@@ -478,8 +491,9 @@ __device__ inline uint64_t leaf_row(const DPlan *__restrict__ P, const uint32_t 
 
 // Rows of width D -> sum over rows of the product of the leaf-group counts (mod 2^64).  The
 // lookups are dependent binary searches, so the kernel runs at eight waves per SIMD (64 VGPRs,
-// about 200 B per lane spilled): config-2 queries/s 1.23 at four waves (105 VGPRs, no spills),
-// 1.49 at six, 1.61 at seven, 1.64 at eight.
+// 48 B per lane spilled): config-2 queries/s 1.23 at four waves (105 VGPRs, no spills), 1.49 at
+// six, 1.61 at seven, 1.64 at eight with the subset sizes of group_count in registers (about
+// 200 B spilled), 1.87 with them in LDS.
 template <int G>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_leaf(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
                                               uint64_t nrows, const uint64_t *__restrict__ ce_off,
@@ -1380,9 +1394,9 @@ __device__ inline uint32_t tile_query(const uint64_t *__restrict__ tile_pref, ui
 // Per row: finish it (leaf depth) into qcount[query] and count 0 children, or count its children.
 // Every row of a launch is of the given kind (kLeafRows or kExpandRows; fused rows run in
 // k_bfused_flat).  At least seven waves per SIMD: the leaf-depth instantiations need 109 VGPRs
-// for four waves; at seven (72 VGPRs, about 180 B per lane spilled) the config-4 batch takes
-// 76.5 ms against 79.3 (77.3 at six waves, 77.8 at eight).  The expansion kind fits seven
-// waves unforced.
+// for four waves; at seven (72 VGPRs, then about 180 B per lane spilled, up to 20 B since
+// group_count keeps its subset sizes in LDS) the config-4 batch took 76.5 ms against 79.3 (77.3
+// at six waves, 77.8 at eight).  The expansion kind fits seven waves unforced.
 template <int G, int KIND>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) k_bcount(const DPlan *__restrict__ plans, const uint64_t *__restrict__ seg_start,
                                                 const uint64_t *__restrict__ tile_pref, uint32_t qa, uint32_t qb,
