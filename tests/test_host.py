@@ -252,3 +252,15 @@ def test_failed_host_allocation_returns_oom_instead_of_aborting(tmp_path):
                        cwd=os.path.join(os.path.dirname(__file__), ".."))
     assert r.returncode == 0, r.stderr[-2000:]
     assert "rc -3" in r.stdout, r.stdout + r.stderr[-2000:]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # The bench measures one GPU; asked for several it says so and exits before touching a device.
+    import subprocess
+    import sys
+
+    root = os.path.join(os.path.dirname(__file__), "..")
+    r = subprocess.run([sys.executable, "bench.py", "--gpus", "2"], capture_output=True, text=True, timeout=300,
+                       cwd=root)
+    assert r.returncode == 2
+    assert "measures one GPU" in r.stderr
