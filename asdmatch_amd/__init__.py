@@ -28,6 +28,8 @@ ASDM_E_EMPTY_CANDIDATES = 1
 ASDM_PLAN_LOCAL = 1
 ASDM_PLAN_NO_FACTOR = 2
 ASDM_PLAN_COST = 4
+ASDM_PLAN_BAGS = 8
+ASDM_PLAN_FRONTIER = 16
 
 _ERRNAMES = {
     -1: "ASDM_E_INVALID", -2: "ASDM_E_IO", -3: "ASDM_E_OOM", -4: "ASDM_E_HIP",
@@ -90,6 +92,8 @@ _sig("asdm_filter_cfl", C.c_int, _P, _P, _P, C.POINTER(_P))
 _sig("asdm_cs_free", None, _P)
 _sig("asdm_cs_root", C.c_uint32, _P)
 _sig("asdm_cs_sizes", C.c_int, _P, _u32p)
+_sig("asdm_cs_info", C.c_int, _P, _u32p, _u32p)
+_sig("asdm_plan_uses_bags", C.c_int, _P)
 _sig("asdm_cs_candidates", C.c_int, _P, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_cs_arc_weight", C.c_int, _P, C.c_uint32, _u64p)
 _sig("asdm_plan_create", C.c_int, _P, _P, _u32p, C.c_uint32, C.POINTER(_P))
@@ -264,8 +268,13 @@ class Context:
         self._check(_lib.asdm_count(self._h, plan._h, cs._h, budget_bytes, C.byref(out)), "count")
         return out.value
 
-    def local_count(self, plan: "Plan", cs: "CandSpace", nv: int, budget_bytes: int = 0):
-        mat = np.zeros((nv, cs.nq), np.uint64)
+    def local_count(self, plan: "Plan", cs: "CandSpace", nv: Optional[int] = None, budget_bytes: int = 0):
+        """(matrix uint64[|Vg|, |Vq|], total count); the matrix holds one row per data vertex of the
+        graph cs was filtered against (nv, when given, must equal that vertex count)."""
+        rows = cs.data_vertices
+        if nv is not None and nv != rows:
+            raise ValueError(f"nv = {nv}, but the candidate space is over a graph of {rows} vertices")
+        mat = np.zeros((rows, cs.nq), np.uint64)
         out = C.c_uint64()
         self._check(_lib.asdm_local_count(self._h, plan._h, cs._h, budget_bytes, _ptr(mat, C.c_uint64),
                                           C.byref(out)), "local_count")
@@ -403,6 +412,15 @@ class CandSpace:
     def root(self) -> int:
         return int(_lib.asdm_cs_root(self._h))
 
+    @property
+    def data_vertices(self) -> int:
+        """Vertex count of the data graph this candidate space was filtered against."""
+        nv = C.c_uint32()
+        rc = _lib.asdm_cs_info(self._h, None, C.byref(nv))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "cs_info")
+        return nv.value
+
     def sizes(self) -> np.ndarray:
         s = np.zeros(self.nq, np.uint32)
         _lib.asdm_cs_sizes(self._h, _ptr(s))
@@ -423,11 +441,14 @@ class CandSpace:
         return w.value
 
     def plan(self, q: Graph, order=None, local: bool = False, cards=None, factor: bool = True,
-             num_enumerated: Optional[int] = None, cost: bool = False) -> "Plan":
-        """cost: multi-start cost-based order search with the independence model's cardinalities."""
+             num_enumerated: Optional[int] = None, cost: bool = False, bags: Optional[bool] = None) -> "Plan":
+        """cost: multi-start cost-based order search with the independence model's cardinalities.
+        bags: True counts through bag tables (tree decomposition), False enumerates partial
+        embeddings, None lets the planner choose."""
         h = _P()
         flags = ((ASDM_PLAN_LOCAL if local else 0) | (0 if factor else ASDM_PLAN_NO_FACTOR) |
-                 (ASDM_PLAN_COST if cost else 0))
+                 (ASDM_PLAN_COST if cost else 0) | (ASDM_PLAN_BAGS if bags else 0) |
+                 (ASDM_PLAN_FRONTIER if bags is False else 0))
         if num_enumerated is not None:
             rc = _lib.asdm_plan_create_ordered(q._h, self._h, _ptr(_u32(order)), num_enumerated, flags, C.byref(h))
             if rc != ASDM_OK:
@@ -443,7 +464,7 @@ class CandSpace:
             o = None if order is None else _ptr(_u32(order))
             rc = _lib.asdm_plan_create(q._h, self._h, o, flags, C.byref(h))
         if rc != ASDM_OK:
-            raise AsdmError(rc, "plan_create")
+            raise AsdmError(rc, f"plan_create: {_lib.asdm_last_error(self._ctx._h).decode()}")
         return Plan(h, self.nq)
 
 
@@ -607,6 +628,11 @@ class Plan:
         a, b = C.c_double(), C.c_double()
         _lib.asdm_plan_estimate(self._h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    @property
+    def uses_bags(self) -> bool:
+        """True when the plan counts through bag tables rather than partial embeddings."""
+        return bool(_lib.asdm_plan_uses_bags(self._h))
 
     def order(self):
         o = np.zeros(self.nq, np.uint32)

@@ -1,0 +1,196 @@
+// Host planner of the bag-table engine: partition terms, quotient graphs and width-two
+// elimination orders.
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <map>
+#include <set>
+
+#include "bags.h"
+#include "engine.h"
+
+namespace asdm {
+
+namespace {
+
+// Partitions of `elems` into blocks of pairwise non-adjacent query vertices (restricted growth).
+void independent_partitions(const asdm_cs *cs, const std::vector<uint32_t> &elems,
+                            std::vector<std::vector<std::vector<uint32_t>>> &out, uint64_t cap, bool &overflow) {
+    std::vector<std::vector<uint32_t>> cur;
+    std::function<void(size_t)> rec = [&](size_t i) {
+        if (overflow) return;
+        if (i == elems.size()) {
+            out.push_back(cur);
+            if (out.size() > cap) overflow = true;
+            return;
+        }
+        const uint32_t u = elems[i];
+        for (size_t b = 0; b < cur.size(); ++b) {
+            bool ok = true;
+            for (uint32_t w : cur[b]) ok &= cs->arc_index(u, w) == UINT32_MAX;
+            if (!ok) continue;
+            cur[b].push_back(u);
+            rec(i + 1);
+            cur[b].pop_back();
+        }
+        cur.push_back({u});
+        rec(i + 1);
+        cur.pop_back();
+    };
+    rec(0);
+}
+
+int64_t mobius(const std::vector<std::vector<uint32_t>> &blocks) {
+    int64_t c = 1;
+    for (auto &b : blocks) {
+        const int64_t s = (int64_t)b.size();
+        int64_t f = 1;
+        for (int64_t t = 2; t < s; ++t) f *= t;
+        c *= (s % 2 == 1 ? 1 : -1) * f;
+    }
+    return c;
+}
+
+// Greedy elimination of the quotient graph with tables of at most two variables.  Each step
+// eliminates a live block (never `keep`) of live degree <= 2, the cheapest first: degree 0 and 1
+// cost a pass over a vector or a table, degree 2 writes a |D(a)| x |D(b)| table (and multiplies
+// two dense tables when both incident factors are dense).
+bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &dom,
+               uint32_t keep, std::vector<BagStep> &steps, double &cost, double &peak) {
+    std::vector<std::vector<int>> kind(nb, std::vector<int>(nb, 0));  // 0 none, 1 edge relation, 2 dense table
+    for (auto &e : edges) kind[e.first][e.second] = kind[e.second][e.first] = 1;
+    std::vector<char> alive(nb, 1);
+    uint32_t left = nb - (keep < nb ? 1 : 0);
+    steps.clear();
+    while (left) {
+        double best = 0;
+        int bv = -1;
+        std::vector<uint32_t> bn;
+        for (uint32_t v = 0; v < nb; ++v) {
+            if (!alive[v] || v == keep) continue;
+            std::vector<uint32_t> n;
+            for (uint32_t w = 0; w < nb; ++w)
+                if (w != v && alive[w] && kind[v][w]) n.push_back(w);
+            if (n.size() > 2) continue;
+            double c;
+            if (n.empty()) c = dom[v];
+            else if (n.size() == 1) c = kind[v][n[0]] == 2 ? dom[v] * dom[n[0]] : dom[v] + dom[n[0]];
+            else {
+                const double out = dom[n[0]] * dom[n[1]];
+                c = out;
+                if (kind[v][n[0]] == 2 && kind[v][n[1]] == 2) c += out * dom[v];
+                else if (kind[v][n[0]] == 2 || kind[v][n[1]] == 2) c += out * 4;
+                c += 1e-3 * out;  // prefer steps that write no table
+            }
+            // degree 0 and 1 steps first: they shrink the graph without writing a table (a graph of
+            // treewidth <= 2 reduces to nothing by such steps in any order, and always has two
+            // vertices of degree <= 2, so keeping one block live never blocks the reduction)
+            if (bv < 0 || n.size() < bn.size() || (n.size() == bn.size() && c < best)) {
+                bv = (int)v;
+                bn = n;
+                best = c;
+            }
+        }
+        if (bv < 0) return false;
+        BagStep s{};
+        s.v = (uint32_t)bv;
+        s.nn = (uint32_t)bn.size();
+        for (uint32_t i = 0; i < s.nn; ++i) s.nb[i] = bn[i];
+        steps.push_back(s);
+        cost += best;
+        if (s.nn == 2) {
+            peak = std::max(peak, dom[bn[0]] * dom[bn[1]]);
+            kind[bn[0]][bn[1]] = kind[bn[1]][bn[0]] = 2;
+        }
+        alive[bv] = 0;
+        for (uint32_t w = 0; w < nb; ++w) kind[bv][w] = kind[w][bv] = 0;
+        --left;
+    }
+    return true;
+}
+
+}  // namespace
+
+bool build_bag_plan(const asdm_cs *cs, bool local, BagPlan &out, std::string &why) {
+    const uint32_t nq = cs->nq;
+    out = BagPlan{};
+    out.local = local;
+    if (!cs->loop_free) {
+        why = "the data graph has self-loops";
+        return false;
+    }
+    // label classes with more than one vertex
+    std::map<uint32_t, std::vector<uint32_t>> classes;
+    for (uint32_t u = 0; u < nq; ++u) classes[cs->qlab[u]].push_back(u);
+    std::vector<std::vector<std::vector<std::vector<uint32_t>>>> per;  // per class: its partitions
+    uint64_t nterms = 1;
+    for (auto &c : classes) {
+        std::vector<std::vector<std::vector<uint32_t>>> parts;
+        bool overflow = false;
+        independent_partitions(cs, c.second, parts, kMaxBagTerms, overflow);
+        nterms *= parts.size();
+        if (overflow || nterms > kMaxBagTerms) {
+            why = "too many partition terms";
+            return false;
+        }
+        per.push_back(std::move(parts));
+    }
+    std::vector<size_t> idx(per.size(), 0);
+    for (uint64_t t = 0; t < nterms; ++t) {
+        BagTerm T;
+        std::vector<std::vector<uint32_t>> blocks;
+        for (size_t c = 0; c < per.size(); ++c)
+            for (auto &b : per[c][idx[c]]) blocks.push_back(b);
+        T.mu = mobius(blocks);
+        const uint32_t nb = (uint32_t)blocks.size();
+        std::vector<uint32_t> bof(nq);
+        std::vector<double> dom(nb);
+        for (uint32_t b = 0; b < nb; ++b) {
+            uint32_t rep = blocks[b][0];
+            for (uint32_t u : blocks[b]) {
+                bof[u] = b;
+                if (cs->csize[u] < cs->csize[rep]) rep = u;
+            }
+            T.rep.push_back(rep);
+            dom[b] = std::max(1.0, (double)cs->csize[rep]);
+        }
+        std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> em;
+        for (uint32_t a = 0; a < cs->narcs; ++a) {
+            const uint32_t A = bof[cs->arc_u[a]], B = bof[cs->arc_u2[a]];
+            if (A == B) continue;  // cannot happen: blocks are independent sets
+            if (A < B && !em.count({A, B})) em[{A, B}] = {a, cs->arc_index(cs->arc_u2[a], cs->arc_u[a])};
+        }
+        for (auto &e : em) {
+            T.edges.push_back(e.first);
+            T.arc_ab.push_back(e.second.first);
+            T.arc_ba.push_back(e.second.second);
+        }
+        T.blocks = std::move(blocks);
+        double peak = 0;
+        if (!eliminate(nb, T.edges, dom, UINT32_MAX, T.count_steps, out.cost, peak)) {
+            why = "a quotient of the query needs tables over three variables";
+            return false;
+        }
+        if (local) {
+            T.local_steps.resize(nb);
+            for (uint32_t k = 0; k < nb; ++k)
+                if (!eliminate(nb, T.edges, dom, k, T.local_steps[k], out.cost, peak)) {
+                    why = "a quotient of the query needs tables over three variables for a marginal";
+                    return false;
+                }
+        }
+        out.peak_table = std::max(out.peak_table, peak);
+        out.terms.push_back(std::move(T));
+        for (size_t c = 0; c < per.size(); ++c) {
+            if (++idx[c] < per[c].size()) break;
+            idx[c] = 0;
+        }
+    }
+    if (out.peak_table > kMaxBagTable) {
+        why = "a bag table would exceed " + std::to_string((uint64_t)kMaxBagTable) + " entries";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace asdm

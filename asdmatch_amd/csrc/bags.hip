@@ -1,0 +1,734 @@
+// Bag-table engine (bags.h): merged-block domains, relation CSRs, width-two variable elimination
+// over dense bag tables, and the local-count scatter.  All values are uint64 mod 2^64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
+
+#include "bags.h"
+#include "engine.h"
+#include "kernels_common.h"
+
+namespace asdm {
+
+namespace {
+
+constexpr int kThreads = 256;
+// Columns of one table row accumulated in LDS (80 KiB: two workgroups per CU).
+constexpr uint32_t kLdsCols = 10240;
+
+unsigned grid_for(uint64_t items, uint64_t per_block) {
+    const uint64_t g = (items + per_block - 1) / per_block;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 20));
+}
+
+// One atomic per wave: the wave's summed algorithmic bytes into *acc.
+__device__ inline void wave_add_bytes(uint64_t b, unsigned long long *acc) {
+    b = group_sum<64>(b);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(acc, (unsigned long long)b);
+}
+
+__device__ inline uint32_t bit_rank(const uint64_t *__restrict__ bits, const uint32_t *__restrict__ rank, uint32_t v) {
+    const uint64_t w = bits[v >> 6];
+    return rank[v >> 6] + (uint32_t)__popcll(w & ((1ull << (v & 63)) - 1));
+}
+__device__ inline bool bit_test(const uint64_t *__restrict__ bits, uint32_t v) { return (bits[v >> 6] >> (v & 63)) & 1ull; }
+
+struct Members {
+    uint32_t n;
+    uint32_t u[kMaxQ];
+};
+
+// Domain of a merged block: AND of its members' candidate bitmaps, popcount per word.
+__global__ void k_dom_words(const uint64_t *__restrict__ bits, uint32_t W, Members m, uint64_t *__restrict__ out,
+                            uint32_t *__restrict__ cnt) {
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += gridDim.x * blockDim.x) {
+        uint64_t x = ~0ull;
+        for (uint32_t k = 0; k < m.n; ++k) x &= bits[(size_t)m.u[k] * W + w];
+        out[w] = x;
+        cnt[w] = (uint32_t)__popcll(x);
+    }
+}
+
+// Rank prefix per word and the ascending data ids of the domain.
+__global__ void k_dom_fill(const uint64_t *__restrict__ bits, uint32_t W, const uint64_t *__restrict__ off,
+                           uint32_t *__restrict__ rank, uint32_t *__restrict__ ids) {
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += gridDim.x * blockDim.x) {
+        uint64_t o = off[w];
+        rank[w] = (uint32_t)o;
+        for (uint64_t x = bits[w]; x; x &= x - 1) ids[o++] = w * 64u + (uint32_t)__builtin_ctzll(x);
+    }
+}
+
+// Relation CSR of a quotient edge A -> B from the candidate-edge lists of a query arc a -> b
+// (a in A, b in B): row i (data vertex ids_a[i]) holds the D(B) positions of its neighbours in
+// D(B), ascending.  Count pass (FILL = false), then fill pass at the scanned offsets.
+template <bool FILL>
+__global__ void k_rel(const uint32_t *__restrict__ ids_a, uint32_t na, const uint64_t *__restrict__ bits_a,
+                      const uint32_t *__restrict__ rank_a, const uint64_t *__restrict__ ce_off,
+                      const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand_b,
+                      const uint64_t *__restrict__ bits_B, const uint32_t *__restrict__ rank_B, uint32_t *__restrict__ cnt,
+                      const uint64_t *__restrict__ off, uint32_t *__restrict__ val, unsigned long long *__restrict__ nbytes) {
+    uint64_t bytes = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+        const uint32_t v = ids_a[i];
+        const uint32_t p = bit_rank(bits_a, rank_a, v);
+        const uint64_t lb = ce_off[p], le = ce_off[p + 1];
+        uint64_t o = FILL ? off[i] : 0;
+        uint32_t c = 0;
+        for (uint64_t e = lb; e < le; ++e) {
+            const uint32_t w = cand_b[ce_val[e]];
+            if (!bit_test(bits_B, w)) continue;
+            if constexpr (FILL) val[o++] = bit_rank(bits_B, rank_B, w);
+            else ++c;
+        }
+        if constexpr (FILL) bytes += 4 * (o - off[i]);
+        else cnt[i] = c;
+        bytes += 28 + 16 * (le - lb);
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+__global__ void k_vec_mul(uint64_t *__restrict__ u, const uint64_t *__restrict__ t, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) u[i] *= t[i];
+}
+
+// t[i] = sum over row i of the relation of x[col]
+__global__ void k_spmv(const uint64_t *__restrict__ off, const uint32_t *__restrict__ val, uint32_t n,
+                       const uint64_t *__restrict__ x, uint64_t *__restrict__ t, unsigned long long *__restrict__ nbytes) {
+    uint64_t bytes = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t s = 0;
+        const uint64_t lb = off[i], le = off[i + 1];
+        for (uint64_t e = lb; e < le; ++e) s += x[val[e]];
+        t[i] = s;
+        bytes += 24 + 12 * (le - lb);
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// t[i] = sum_j F[i][j] x[j]  (F row-major rows x cols): one wave per row.
+__global__ void k_dense_mv(const uint64_t *__restrict__ F, uint32_t rows, uint32_t cols, const uint64_t *__restrict__ x,
+                           uint64_t *__restrict__ t, unsigned long long *__restrict__ nbytes) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / 64);
+    uint64_t bytes = 0;
+    for (uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; r < rows; r += nw) {
+        uint64_t s = 0;
+        for (uint32_t j = lane; j < cols; j += 64) s += F[r * cols + j] * x[j];
+        s = group_sum<64>(s);
+        if (lane == 0) t[r] = s;
+        bytes += lane == 0 ? 16ull * cols + 8 : 0;
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// t[c] = sum_r F[r][c] x[r]  (the transposed product): one thread per column, coalesced rows.
+__global__ void k_dense_mv_t(const uint64_t *__restrict__ F, uint32_t rows, uint32_t cols, const uint64_t *__restrict__ x,
+                             uint64_t *__restrict__ t, unsigned long long *__restrict__ nbytes) {
+    uint64_t bytes = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
+        uint64_t s = 0;
+        for (uint32_t r = 0; r < rows; ++r) s += F[(uint64_t)r * cols + c] * x[r];
+        t[c] = s;
+        bytes += 8ull * rows + 8;
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// acc[0] += sum of x
+__global__ void k_sum(const uint64_t *__restrict__ x, uint32_t n, unsigned long long *__restrict__ acc) {
+    uint64_t s = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += x[i];
+    s = group_sum<64>(s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, (unsigned long long)s);
+}
+
+// scal *= acc; acc = 0
+__global__ void k_scal_fold(uint64_t *__restrict__ scal, uint64_t *__restrict__ acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        scal[0] *= acc[0];
+        acc[0] = 0;
+    }
+}
+
+// total += mu * scal
+__global__ void k_total_add(unsigned long long *__restrict__ total, const uint64_t *__restrict__ scal, int64_t mu) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) total[0] += (unsigned long long)((uint64_t)mu * scal[0]);
+}
+
+// G[x][y] = sum_{j in R(X->v)(x)} u[j] * [y in R(v->Y)(j)]: both factors are edge relations.  One
+// workgroup per row x; the row is accumulated in LDS when it fits (ny <= kLdsCols), else in the
+// zeroed output with global atomics.
+__global__ void __launch_bounds__(kThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
+                                                 const uint64_t *__restrict__ voff, const uint32_t *__restrict__ vval,
+                                                 const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny,
+                                                 uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
+    __shared__ unsigned long long acc[kLdsCols];
+    const bool in_lds = ny <= kLdsCols;
+    uint64_t bytes = 0;
+    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
+        uint64_t *row = G + (uint64_t)x * ny;
+        if (in_lds) {
+            for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) acc[y] = 0;
+            __syncthreads();
+        }
+        const uint64_t lb = xoff[x], le = xoff[x + 1];
+        for (uint64_t e = lb + threadIdx.x; e < le; e += blockDim.x) {
+            const uint32_t j = xval[e];
+            const uint64_t w = u[j];
+            const uint64_t b2 = voff[j], e2 = voff[j + 1];
+            bytes += 28 + 4 * (e2 - b2);
+            if (!w) continue;
+            for (uint64_t f = b2; f < e2; ++f) {
+                if (in_lds) atomicAdd(&acc[vval[f]], (unsigned long long)w);
+                else atomicAdd((unsigned long long *)&row[vval[f]], (unsigned long long)w);
+            }
+        }
+        if (in_lds) {
+            __syncthreads();
+            for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) row[y] = acc[y];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) bytes += 16 + 8ull * ny;
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// G[x][y] = sum_{j in R(Y->v)(y)} F[x][j] u[j]: a dense table on (X, v) rows X, and the edge
+// relation of (v, Y) read from Y's side.  One workgroup per row x; F[x][.] * u staged in LDS.
+__global__ void __launch_bounds__(kThreads) k_de(const uint64_t *__restrict__ F, const uint64_t *__restrict__ u,
+                                                 const uint64_t *__restrict__ yoff, const uint32_t *__restrict__ yval,
+                                                 uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *__restrict__ G,
+                                                 unsigned long long *__restrict__ nbytes) {
+    __shared__ unsigned long long s[kLdsCols];
+    const bool in_lds = nv <= kLdsCols;
+    uint64_t bytes = 0;
+    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
+        const uint64_t *fr = F + (uint64_t)x * nv;
+        if (in_lds) {
+            for (uint32_t j = threadIdx.x; j < nv; j += blockDim.x) s[j] = fr[j] * u[j];
+            __syncthreads();
+        }
+        for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
+            const uint64_t lb = yoff[y], le = yoff[y + 1];
+            uint64_t a = 0;
+            for (uint64_t e = lb; e < le; ++e) {
+                const uint32_t j = yval[e];
+                a += in_lds ? (uint64_t)s[j] : fr[j] * u[j];
+            }
+            G[(uint64_t)x * ny + y] = a;
+            bytes += 24 + 4 * (le - lb);
+        }
+        if (in_lds) __syncthreads();
+        if (threadIdx.x == 0) bytes += 16ull * nv;
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// G[x][y] = sum_{j in R(X->v)(x)} u[j] F[j][y]: the edge relation of (X, v) and a dense table on
+// (v, Y) rows v.  One workgroup per row x, threads across y (coalesced rows of F).
+__global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
+                                                 const uint64_t *__restrict__ u, const uint64_t *__restrict__ F,
+                                                 uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
+                                                 unsigned long long *__restrict__ nbytes) {
+    uint64_t bytes = 0;
+    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
+        const uint64_t lb = xoff[x], le = xoff[x + 1];
+        for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
+            uint64_t a = 0;
+            for (uint64_t e = lb; e < le; ++e) {
+                const uint32_t j = xval[e];
+                a += u[j] * F[(uint64_t)j * ny + y];
+            }
+            G[(uint64_t)x * ny + y] = a;
+            bytes += 16 + 8 * (le - lb);
+        }
+        if (threadIdx.x == 0) bytes += 16 + 12 * (le - lb);
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// G = A diag(u) B with A (nx x nv) and B (nv x ny) dense: 64 x 64 output tiles, 16 x 16 threads
+// with 4 x 4 outputs each, K-chunks of 16 staged in LDS.
+constexpr int kT = 64, kK = 16;
+__global__ void __launch_bounds__(256) k_dd(const uint64_t *__restrict__ A, const uint64_t *__restrict__ u,
+                                            const uint64_t *__restrict__ B, uint32_t nx, uint32_t nv, uint32_t ny,
+                                            uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
+    __shared__ uint64_t As[kK][kT];
+    __shared__ uint64_t Bs[kK][kT];
+    const uint32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const uint64_t tiles_y = (ny + kT - 1) / kT;
+    const uint64_t ntiles = ((uint64_t)(nx + kT - 1) / kT) * tiles_y;
+    uint64_t bytes = 0;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t x0 = (uint32_t)(t / tiles_y) * kT, y0 = (uint32_t)(t % tiles_y) * kT;
+        uint64_t c[4][4] = {};
+        for (uint32_t k0 = 0; k0 < nv; k0 += kK) {
+            for (int i = threadIdx.x; i < kK * kT; i += 256) {
+                const int kk = i / kT, xx = i % kT;
+                const uint32_t gx = x0 + xx, gk = k0 + kk;
+                As[kk][xx] = (gx < nx && gk < nv) ? A[(uint64_t)gx * nv + gk] * u[gk] : 0;
+                const uint32_t gy = y0 + xx;
+                Bs[kk][xx] = (gk < nv && gy < ny) ? B[(uint64_t)gk * ny + gy] : 0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < kK; ++kk) {
+                uint64_t a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) b[i] = Bs[kk][tx * 4 + i];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) c[i][j] += a[i] * b[j];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t gx = x0 + ty * 4 + i, gy = y0 + tx * 4 + j;
+                if (gx < nx && gy < ny) G[(uint64_t)gx * ny + gy] = c[i][j];
+            }
+        if (threadIdx.x == 0) bytes += 8ull * kT * (2ull * nv + kT);
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// out (cols x rows) = transpose of in (rows x cols), 32 x 32 tiles through LDS.
+__global__ void k_transpose(const uint64_t *__restrict__ in, uint32_t rows, uint32_t cols, uint64_t *__restrict__ out) {
+    __shared__ uint64_t tile[32][33];
+    const uint64_t tiles_c = (cols + 31) / 32;
+    const uint64_t ntiles = ((uint64_t)(rows + 31) / 32) * tiles_c;
+    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t r0 = (uint32_t)(t / tiles_c) * 32, c0 = (uint32_t)(t % tiles_c) * 32;
+        for (uint32_t k = ty; k < 32; k += 8)
+            if (r0 + k < rows && c0 + tx < cols) tile[k][tx] = in[(uint64_t)(r0 + k) * cols + c0 + tx];
+        __syncthreads();
+        for (uint32_t k = ty; k < 32; k += 8)
+            if (c0 + k < cols && r0 + tx < rows) out[(uint64_t)(c0 + k) * rows + r0 + tx] = tile[tx][k];
+        __syncthreads();
+    }
+}
+
+// out (zeroed) keeps G's entries on the edges of the relation R(X->Y); the rest stay zero.
+__global__ void k_mask(const uint64_t *__restrict__ G, const uint64_t *__restrict__ off, const uint32_t *__restrict__ val,
+                       uint32_t nx, uint32_t ny, uint64_t *__restrict__ out) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += gridDim.x * blockDim.x)
+        for (uint64_t e = off[x]; e < off[x + 1]; ++e) out[(uint64_t)x * ny + val[e]] = G[(uint64_t)x * ny + val[e]];
+}
+
+__global__ void k_hadamard(uint64_t *__restrict__ a, const uint64_t *__restrict__ b, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] *= b[i];
+}
+
+// mat[ids[i]][u] += mu * scal * marg[i] for every query vertex u of the block.  Domain ids are
+// distinct, so every cell has one writer per launch and no atomics are needed.
+__global__ void k_scatter_local(const uint32_t *__restrict__ ids, uint32_t n, const uint64_t *__restrict__ marg,
+                                const uint64_t *__restrict__ scal, int64_t mu, Members m, uint32_t nq,
+                                uint64_t *__restrict__ mat) {
+    const uint64_t c = (uint64_t)mu * scal[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t add = c * marg[i];
+        for (uint32_t k = 0; k < m.n; ++k) mat[(uint64_t)ids[i] * nq + m.u[k]] += add;
+    }
+}
+
+// ------------------------------------------------------------------------------ executor
+struct Dom {
+    const uint32_t *ids = nullptr;
+    const uint64_t *bits = nullptr;
+    const uint32_t *rank = nullptr;
+    uint32_t n = 0;
+};
+struct Rel {
+    const uint64_t *off = nullptr;
+    const uint32_t *val = nullptr;
+};
+struct Table {  // dense factor of a block pair (lo, hi): row-major, rows D(lo)
+    uint64_t *p = nullptr;
+};
+
+struct BagRun {
+    asdm_ctx *ctx;
+    const asdm_cs *cs;
+    const BagPlan *plan;
+    unsigned long long *d_acc = nullptr;    // ctx counters: [0] total, [4] scratch sum, [5] term scalar, [6..] bytes
+    uint64_t *d_mat = nullptr;
+    std::vector<void *> owned;              // per-term device buffers
+    std::chrono::steady_clock::time_point t0;
+
+    unsigned long long *bytes_slot(int k) { return d_acc + 6 + k; }
+
+    int alloc(void **p, size_t bytes) {
+        int rc = dev_alloc(ctx, p, std::max<size_t>(bytes, 8));
+        if (!rc) owned.push_back(*p);
+        return rc;
+    }
+    void release(void *p) {
+        auto it = std::find(owned.begin(), owned.end(), p);
+        if (it != owned.end()) owned.erase(it);
+        dev_free(ctx, p);
+    }
+    void release_all() {
+        for (void *p : owned) dev_free(ctx, p);
+        owned.clear();
+    }
+    int launched(const char *what) { return check_hip(ctx, hipGetLastError(), what); }
+    bool timed_out() const {
+        return ctx->time_limit_ms > 0 &&
+               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() > ctx->time_limit_ms;
+    }
+
+    int build_dom(const std::vector<uint32_t> &members, Dom &d) {
+        if (members.size() == 1) {
+            const uint32_t u = members[0];
+            d.ids = cs->d_cand + cs->cand_off[u];
+            d.bits = cs->d_bits + (size_t)u * cs->W;
+            d.rank = cs->d_rank + (size_t)u * cs->W;
+            d.n = cs->csize[u];
+            return ASDM_OK;
+        }
+        const uint32_t W = cs->W;
+        Members m{};
+        m.n = (uint32_t)members.size();
+        for (uint32_t k = 0; k < m.n; ++k) m.u[k] = members[k];
+        uint64_t *bits = nullptr, *off = nullptr;
+        uint32_t *cnt = nullptr, *rank = nullptr, *ids = nullptr;
+        int rc = alloc((void **)&bits, 8ull * W);
+        if (!rc) rc = alloc((void **)&cnt, 4ull * W);
+        if (!rc) rc = alloc((void **)&off, 8ull * (W + 1));
+        if (!rc) rc = alloc((void **)&rank, 4ull * W);
+        if (rc) return rc;
+        KTimer kt(ctx, "bag_domain");
+        hipLaunchKernelGGL(k_dom_words, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, ctx->stream, cs->d_bits, W, m, bits, cnt);
+        if ((rc = launched("bag domain words"))) return rc;
+        if ((rc = scan_u32_to_u64(ctx, cnt, off, W))) return rc;
+        uint64_t n = 0;
+        rc = check_hip(ctx, hipMemcpyAsync(&n, off + W, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (!rc) rc = alloc((void **)&ids, 4ull * std::max<uint64_t>(n, 1));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_dom_fill, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, ctx->stream, bits, W, off, rank, ids);
+        kt.stop(12ull * W + 4 * n);
+        if ((rc = launched("bag domain fill"))) return rc;
+        release(cnt);
+        release(off);
+        d.ids = ids;
+        d.bits = bits;
+        d.rank = rank;
+        d.n = (uint32_t)n;
+        return ASDM_OK;
+    }
+
+    // Relation A -> B through query arc `arc` (u_a in A, u_b in B).
+    int build_rel(uint32_t arc, const Dom &A, bool a_single, const Dom &B, bool b_single, Rel &r) {
+        const uint32_t ua = cs->arc_u[arc], ub = cs->arc_u2[arc];
+        if (a_single && b_single) {  // the candidate-edge lists are the relation
+            r.off = cs->d_ce_off + cs->ce_base[arc];
+            r.val = cs->d_ce_val;
+            return ASDM_OK;
+        }
+        uint32_t *cnt = nullptr, *val = nullptr;
+        uint64_t *off = nullptr;
+        int rc = alloc((void **)&cnt, 4ull * std::max<uint32_t>(A.n, 1));
+        if (!rc) rc = alloc((void **)&off, 8ull * (A.n + 1));
+        if (rc) return rc;
+        const uint64_t *bits_a = cs->d_bits + (size_t)ua * cs->W;
+        const uint32_t *rank_a = cs->d_rank + (size_t)ua * cs->W;
+        const uint32_t *cand_b = cs->d_cand + cs->cand_off[ub];
+        {
+            KTimer kt(ctx, "bag_relation");
+            hipLaunchKernelGGL(k_rel<false>, dim3(grid_for(A.n, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
+                               bits_a, rank_a, cs->d_ce_off + cs->ce_base[arc], cs->d_ce_val, cand_b, B.bits, B.rank, cnt,
+                               nullptr, nullptr, bytes_slot(0));
+            kt.stop();
+        }
+        if ((rc = launched("bag relation count"))) return rc;
+        if ((rc = scan_u32_to_u64(ctx, cnt, off, A.n))) return rc;
+        uint64_t nnz = 0;
+        rc = check_hip(ctx, hipMemcpyAsync(&nnz, off + A.n, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (!rc) rc = alloc((void **)&val, 4ull * std::max<uint64_t>(nnz, 1));
+        if (rc) return rc;
+        {
+            KTimer kt(ctx, "bag_relation");
+            hipLaunchKernelGGL(k_rel<true>, dim3(grid_for(A.n, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
+                               bits_a, rank_a, cs->d_ce_off + cs->ce_base[arc], cs->d_ce_val, cand_b, B.bits, B.rank,
+                               nullptr, off, val, bytes_slot(0));
+            kt.stop();
+        }
+        if ((rc = launched("bag relation fill"))) return rc;
+        release(cnt);
+        r.off = off;
+        r.val = val;
+        return ASDM_OK;
+    }
+
+    // Dense table of the pair (lo, hi) with rows D(want_rows): the stored table or its transpose.
+    int oriented(uint32_t lo, uint32_t hi, const Table &t, uint32_t want_rows, const std::vector<Dom> &dom,
+                 uint64_t **out, bool &temp) {
+        temp = false;
+        if (want_rows == lo) {
+            *out = t.p;
+            return ASDM_OK;
+        }
+        int rc = alloc((void **)out, 8ull * dom[lo].n * dom[hi].n);
+        if (rc) return rc;
+        temp = true;
+        const uint64_t tiles = ((uint64_t)(dom[lo].n + 31) / 32) * ((dom[hi].n + 31) / 32);
+        KTimer kt(ctx, "bag_transpose");
+        hipLaunchKernelGGL(k_transpose, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, t.p, dom[lo].n, dom[hi].n, *out);
+        kt.stop(16ull * dom[lo].n * dom[hi].n);
+        return launched("bag transpose");
+    }
+
+    // Runs one elimination sequence.  On return *scal (device) holds the product of the sums of
+    // the blocks eliminated with no live neighbour, and u[keep] the marginal of `keep`.
+    int eliminate(const BagTerm &T, const std::vector<BagStep> &steps, const std::vector<Dom> &dom,
+                  const std::map<std::pair<uint32_t, uint32_t>, std::pair<Rel, Rel>> &rels, std::vector<uint64_t *> &u,
+                  uint64_t *scal) {
+        const uint32_t nb = (uint32_t)T.blocks.size();
+        u.assign(nb, nullptr);
+        int rc = ASDM_OK;
+        for (uint32_t b = 0; b < nb && !rc; ++b) {
+            rc = alloc((void **)&u[b], 8ull * std::max<uint32_t>(dom[b].n, 1));
+            if (!rc) rc = fill_u64(ctx, u[b], dom[b].n, 1);
+        }
+        if (!rc) rc = fill_u64(ctx, scal, 1, 1);
+        if (rc) return rc;
+        // live factors: kind 1 = edge relation, 2 = dense table
+        std::map<std::pair<uint32_t, uint32_t>, std::pair<int, Table>> fac;
+        for (auto &e : T.edges) fac[e] = {1, Table{}};
+        auto rel = [&](uint32_t a, uint32_t b) -> Rel {  // rows D(a)
+            auto it = rels.find({std::min(a, b), std::max(a, b)});
+            return a < b ? it->second.first : it->second.second;
+        };
+        for (const BagStep &s : steps) {
+            if (timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
+            const uint32_t v = s.v;
+            const Dom &dv = dom[v];
+            if (s.nn == 0) {
+                hipLaunchKernelGGL(k_sum, dim3(grid_for(dv.n, kThreads)), dim3(kThreads), 0, ctx->stream, u[v], dv.n, d_acc + 4);
+                hipLaunchKernelGGL(k_scal_fold, dim3(1), dim3(64), 0, ctx->stream, scal, (uint64_t *)(d_acc + 4));
+                if ((rc = launched("bag sum"))) return rc;
+            } else if (s.nn == 1) {
+                const uint32_t a = s.nb[0];
+                auto key = std::make_pair(std::min(a, v), std::max(a, v));
+                auto &f = fac[key];
+                uint64_t *t = nullptr;
+                if ((rc = alloc((void **)&t, 8ull * std::max<uint32_t>(dom[a].n, 1)))) return rc;
+                KTimer kt(ctx, "bag_vector");
+                if (f.first == 1) {
+                    const Rel r = rel(a, v);
+                    hipLaunchKernelGGL(k_spmv, dim3(grid_for(dom[a].n, kThreads)), dim3(kThreads), 0, ctx->stream, r.off,
+                                       r.val, dom[a].n, u[v], t, bytes_slot(1));
+                } else if (a < v) {  // table rows D(a)
+                    hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(dom[a].n, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
+                                       f.second.p, dom[a].n, dv.n, u[v], t, bytes_slot(1));
+                } else {  // table rows D(v)
+                    hipLaunchKernelGGL(k_dense_mv_t, dim3(grid_for(dom[a].n, kThreads)), dim3(kThreads), 0, ctx->stream,
+                                       f.second.p, dv.n, dom[a].n, u[v], t, bytes_slot(1));
+                }
+                kt.stop();
+                hipLaunchKernelGGL(k_vec_mul, dim3(grid_for(dom[a].n, kThreads)), dim3(kThreads), 0, ctx->stream, u[a], t,
+                                   dom[a].n);
+                if ((rc = launched("bag vector step"))) return rc;
+                release(t);
+            } else {
+                const uint32_t X = std::min(s.nb[0], s.nb[1]), Y = std::max(s.nb[0], s.nb[1]);
+                const uint32_t nx = dom[X].n, ny = dom[Y].n, nv = dv.n;
+                auto kxv = std::make_pair(std::min(X, v), std::max(X, v));
+                auto kvy = std::make_pair(std::min(v, Y), std::max(v, Y));
+                auto &fxv = fac[kxv];
+                auto &fvy = fac[kvy];
+                uint64_t *G = nullptr;
+                if ((rc = alloc((void **)&G, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
+                const char *kname = fxv.first == 1 ? (fvy.first == 1 ? "bag_table_ee" : "bag_table_ed")
+                                                   : (fvy.first == 1 ? "bag_table_de" : "bag_table_dd");
+                const unsigned rows_grid = grid_for(nx, 1);
+                if (fxv.first == 1 && fvy.first == 1) {
+                    const Rel rx = rel(X, v), rv = rel(v, Y);
+                    if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
+                    KTimer kt(ctx, kname);
+                    hipLaunchKernelGGL(k_ee, dim3(std::min(rows_grid, 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
+                                       rx.val, rv.off, rv.val, u[v], nx, ny, G, bytes_slot(2));
+                    kt.stop();
+                } else if (fvy.first == 1) {  // dense (X, v), edge (v, Y)
+                    uint64_t *F = nullptr;
+                    bool tmp = false;
+                    if ((rc = oriented(kxv.first, kxv.second, fxv.second, X, dom, &F, tmp))) return rc;
+                    const Rel ry = rel(Y, v);
+                    KTimer kt(ctx, kname);
+                    hipLaunchKernelGGL(k_de, dim3(std::min(rows_grid, 1u << 16)), dim3(kThreads), 0, ctx->stream, F, u[v],
+                                       ry.off, ry.val, nx, nv, ny, G, bytes_slot(2));
+                    kt.stop();
+                    if (tmp) release(F);
+                } else if (fxv.first == 1) {  // edge (X, v), dense (v, Y)
+                    uint64_t *F = nullptr;
+                    bool tmp = false;
+                    if ((rc = oriented(kvy.first, kvy.second, fvy.second, v, dom, &F, tmp))) return rc;
+                    const Rel rx = rel(X, v);
+                    KTimer kt(ctx, kname);
+                    hipLaunchKernelGGL(k_ed, dim3(std::min(rows_grid, 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
+                                       rx.val, u[v], F, nx, ny, G, bytes_slot(2));
+                    kt.stop();
+                    if (tmp) release(F);
+                } else {
+                    uint64_t *Fa = nullptr, *Fb = nullptr;
+                    bool ta = false, tb = false;
+                    if ((rc = oriented(kxv.first, kxv.second, fxv.second, X, dom, &Fa, ta))) return rc;
+                    if ((rc = oriented(kvy.first, kvy.second, fvy.second, v, dom, &Fb, tb))) return rc;
+                    const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
+                    KTimer kt(ctx, kname);
+                    hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, Fa, u[v], Fb, nx, nv, ny, G,
+                                       bytes_slot(2));
+                    kt.stop();
+                    if (ta) release(Fa);
+                    if (tb) release(Fb);
+                }
+                if ((rc = launched("bag table"))) return rc;
+                // join with a factor already on (X, Y)
+                auto kxy = std::make_pair(X, Y);
+                auto it = fac.find(kxy);
+                if (it == fac.end()) {
+                    fac[kxy] = {2, Table{G}};
+                } else if (it->second.first == 1) {
+                    uint64_t *M = nullptr;
+                    if ((rc = alloc((void **)&M, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
+                    if ((rc = fill_u64(ctx, M, (uint64_t)nx * ny, 0))) return rc;
+                    const Rel r = rel(X, Y);
+                    hipLaunchKernelGGL(k_mask, dim3(grid_for(nx, kThreads)), dim3(kThreads), 0, ctx->stream, G, r.off, r.val,
+                                       nx, ny, M);
+                    if ((rc = launched("bag mask"))) return rc;
+                    release(G);
+                    it->second = {2, Table{M}};
+                } else {
+                    const uint64_t n = (uint64_t)nx * ny;
+                    hipLaunchKernelGGL(k_hadamard, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
+                                       it->second.second.p, G, n);
+                    if ((rc = launched("bag join"))) return rc;
+                    release(G);
+                }
+            }
+            // drop v's factors
+            for (auto it = fac.begin(); it != fac.end();) {
+                if (it->first.first == v || it->first.second == v) {
+                    if (it->second.first == 2) release(it->second.second.p);
+                    it = fac.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        return ASDM_OK;
+    }
+
+    int run_term(const BagTerm &T) {
+        const uint32_t nb = (uint32_t)T.blocks.size();
+        std::vector<Dom> dom(nb);
+        int rc = ASDM_OK;
+        for (uint32_t b = 0; b < nb && !rc; ++b) rc = build_dom(T.blocks[b], dom[b]);
+        if (rc) return rc;
+        for (uint32_t b = 0; b < nb; ++b)
+            if (dom[b].n == 0) return ASDM_OK;  // no data vertex fits the merged block: the term is 0
+        std::map<std::pair<uint32_t, uint32_t>, std::pair<Rel, Rel>> rels;
+        for (size_t e = 0; e < T.edges.size() && !rc; ++e) {
+            const uint32_t A = T.edges[e].first, B = T.edges[e].second;
+            // the arcs run from the member of one block to the member of the other: orient them
+            uint32_t ab = T.arc_ab[e], ba = T.arc_ba[e];
+            bool a_from_A = false;
+            for (uint32_t u : T.blocks[A]) a_from_A |= cs->arc_u[ab] == u;
+            if (!a_from_A) std::swap(ab, ba);
+            Rel r1, r2;
+            rc = build_rel(ab, dom[A], T.blocks[A].size() == 1, dom[B], T.blocks[B].size() == 1, r1);
+            if (!rc) rc = build_rel(ba, dom[B], T.blocks[B].size() == 1, dom[A], T.blocks[A].size() == 1, r2);
+            rels[{A, B}] = {r1, r2};
+        }
+        if (rc) return rc;
+        uint64_t *scal = (uint64_t *)(d_acc + 5);
+        std::vector<uint64_t *> u;
+        if ((rc = eliminate(T, T.count_steps, dom, rels, u, scal))) return rc;
+        hipLaunchKernelGGL(k_total_add, dim3(1), dim3(64), 0, ctx->stream, d_acc, scal, T.mu);
+        if ((rc = launched("bag total"))) return rc;
+        for (uint64_t *p : u) release(p);
+        if (d_mat) {
+            for (uint32_t t = 0; t < nb; ++t) {
+                if ((rc = eliminate(T, T.local_steps[t], dom, rels, u, scal))) return rc;
+                Members m{};
+                m.n = (uint32_t)T.blocks[t].size();
+                for (uint32_t k = 0; k < m.n; ++k) m.u[k] = T.blocks[t][k];
+                KTimer kt(ctx, "bag_local_scatter");
+                hipLaunchKernelGGL(k_scatter_local, dim3(grid_for(dom[t].n, kThreads)), dim3(kThreads), 0, ctx->stream,
+                                   dom[t].ids, dom[t].n, u[t], scal, T.mu, m, cs->nq, d_mat);
+                kt.stop(12ull * dom[t].n + 16ull * dom[t].n * m.n);
+                if ((rc = launched("bag local scatter"))) return rc;
+                for (uint64_t *p : u) release(p);
+            }
+        }
+        return ASDM_OK;
+    }
+};
+
+}  // namespace
+
+int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *mat_host, uint64_t *count) {
+    const BagPlan &B = *plan->bags;
+    if (mat_host && !B.local) return ctx->fail(ASDM_E_INVALID, "local counting needs a plan made with ASDM_PLAN_LOCAL");
+    (void)hipSetDevice(ctx->device);
+    auto t0 = std::chrono::steady_clock::now();
+    BagRun R{ctx, cs, &B};
+    R.t0 = t0;
+    R.d_acc = (unsigned long long *)ctx->d_counters;
+    int rc = fill_u64(ctx, ctx->d_counters, kNumCounters, 0);
+    const uint64_t matn = (uint64_t)cs->nv * cs->nq;
+    if (!rc && mat_host) {
+        rc = dev_alloc(ctx, (void **)&R.d_mat, 8ull * std::max<uint64_t>(matn, 1));
+        if (!rc) rc = fill_u64(ctx, R.d_mat, matn, 0);
+    }
+    hipEvent_t ea, eb;
+    (void)hipEventCreate(&ea);
+    (void)hipEventCreate(&eb);
+    (void)hipEventRecord(ea, ctx->stream);
+    for (size_t t = 0; t < B.terms.size() && !rc; ++t) {
+        rc = R.run_term(B.terms[t]);
+        R.release_all();
+    }
+    (void)hipEventRecord(eb, ctx->stream);
+    uint64_t res[kNumCounters] = {};
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(res, ctx->d_counters, 8 * kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (!rc && mat_host)
+        rc = check_hip(ctx, hipMemcpyAsync(mat_host, R.d_mat, 8 * matn, hipMemcpyDeviceToHost, ctx->stream), "D2H matrix");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    float kms = 0;
+    (void)hipEventElapsedTime(&kms, ea, eb);
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    R.release_all();
+    if (R.d_mat) dev_free(ctx, R.d_mat);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (rc) return rc;
+    *count = res[0];
+    resolve_timers(ctx);
+    add_kernel_bytes(ctx, "bag_relation", res[6]);
+    add_kernel_bytes(ctx, "bag_vector", res[7]);
+    add_kernel_bytes(ctx, "bag_tables", res[8]);
+    ctx->stats.exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats.kernel_ms = kms;
+    ctx->stats.alg_bytes = res[6] + res[7] + res[8];
+    ctx->stats.rows_total = 0;
+    ctx->stats.launches = 0;
+    ctx->stats.plan_ms = plan->plan_ms;
+    ctx->stats.filter_ms = cs->filter_ms;
+    return ASDM_OK;
+}
+
+}  // namespace asdm

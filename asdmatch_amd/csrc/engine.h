@@ -5,11 +5,13 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <string>
 #include <unordered_map>
 #include <utility>
 #include <vector>
 
+#include "bags.h"
 #include "host.h"
 
 namespace asdm {
@@ -198,6 +200,9 @@ struct asdm_plan {
     double plan_ms = 0;
     double est_leaf = 0, est_total = 0;  // estimated rows at depth D and summed over depths 1..D
     std::vector<std::pair<uint64_t, double>> prefix_cards;  // (prefix subset mask, estimate)
+    // Set when the query is counted by the bag-table engine (bags.h) instead of the frontier
+    // engine: partition terms and their elimination orders.
+    std::shared_ptr<asdm::BagPlan> bags;
 };
 
 namespace asdm {
@@ -249,6 +254,9 @@ int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asd
 // CFL root choice and BFS order (plan.cpp): base_count[u] = |{v : label, degree, NLF ok}|.
 uint32_t cfl_root(const asdm_graph *q, const asdm_graph *g, const std::vector<uint32_t> &base_count);
 std::vector<uint32_t> bfs_order(const asdm_graph *q, uint32_t root);
+
+// Bag-table engine (bags.hip): count (mat_host == NULL) or local counts of a plan with bags.
+int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *mat_host, uint64_t *count);
 
 // Kernel timing helpers (ctx.hip)
 struct KTimer {

@@ -2214,6 +2214,10 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
                uint64_t *count, bool enumerate = false, uint32_t *enum_out = nullptr, uint64_t enum_cap = 0) {
     if (!ctx || !plan || !cs || !count) return ASDM_E_INVALID;
     if (plan->h.nq != cs->nq) return ctx->fail(ASDM_E_INVALID, "plan and candidate space disagree");
+    if (plan->bags) {
+        if (enumerate) return ctx->fail(ASDM_E_INVALID, "enumeration needs a plan made with ASDM_PLAN_NO_FACTOR");
+        return run_bags(ctx, plan, cs, mat_host, count);
+    }
     if (mat_host && !plan->local) return ctx->fail(ASDM_E_INVALID, "local counting needs a plan made with ASDM_PLAN_LOCAL");
     // a plan without factorization has at most one leaf (the last vertex, counted in closed form)
     if (enumerate && plan->D + 1 != plan->h.nq && plan->D != plan->h.nq)

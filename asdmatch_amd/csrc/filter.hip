@@ -597,6 +597,13 @@ int asdm_filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const
 
 uint32_t asdm_cs_root(const asdm_cs *cs) { return cs ? cs->root : 0; }
 
+int asdm_cs_info(const asdm_cs *cs, uint32_t *nq, uint32_t *nv) try {
+    if (!cs) return ASDM_E_INVALID;
+    if (nq) *nq = cs->nq;
+    if (nv) *nv = cs->nv;
+    return ASDM_OK;
+} ASDM_HOST_CATCH
+
 int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes) try {
     if (!cs || !sizes) return ASDM_E_INVALID;
     std::memcpy(sizes, cs->csize.data(), sizeof(uint32_t) * cs->nq);

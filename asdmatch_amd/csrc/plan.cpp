@@ -392,6 +392,33 @@ int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint
     return ASDM_OK;
 }
 
+// Rows above which the automatic planner considers the bag-table engine, and the largest bag
+// plan cost (element operations) it accepts without being asked.
+constexpr double kAutoBagRows = 16777216.0;
+constexpr double kAutoBagCost = 4.0e10;
+
+// Attaches a bag plan when flags ask for one (ASDM_PLAN_BAGS), or when the frontier plan is large
+// and the bag plan is cheap (automatic choice, unless ASDM_PLAN_FRONTIER or an explicit order).
+int attach_bags(const asdm_cs *cs, uint32_t flags, bool explicit_order, asdm_plan *p, std::string &err) {
+    const bool local = flags & ASDM_PLAN_LOCAL;
+    if (flags & ASDM_PLAN_BAGS) {
+        auto b = std::make_shared<BagPlan>();
+        std::string why;
+        if (!build_bag_plan(cs, local, *b, why)) {
+            err = "bag-table plan: " + why;
+            return ASDM_E_UNSUPPORTED;
+        }
+        p->bags = b;
+        return ASDM_OK;
+    }
+    if (explicit_order || (flags & (ASDM_PLAN_FRONTIER | ASDM_PLAN_NO_FACTOR)) || p->est_total <= kAutoBagRows)
+        return ASDM_OK;
+    auto b = std::make_shared<BagPlan>();
+    std::string why;
+    if (build_bag_plan(cs, local, *b, why) && b->cost <= kAutoBagCost && b->cost < 64.0 * p->est_total) p->bags = b;
+    return ASDM_OK;
+}
+
 }  // namespace asdm
 
 using namespace asdm;
@@ -412,6 +439,7 @@ int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const ui
         return fallback;
     };
     int rc = build_plan(q, cs, nullptr, flags, card_of, p.get(), err);
+    if (!rc) rc = attach_bags(cs, flags, false, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;
@@ -428,6 +456,7 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
     auto p = std::make_unique<asdm_plan>();
     std::string err;
     int rc = build_plan(q, cs, order, flags, [](uint64_t, double fallback) { return fallback; }, p.get(), err);
+    if (!rc) rc = attach_bags(cs, flags, order != nullptr, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;
@@ -475,6 +504,8 @@ int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint3
 } ASDM_HOST_CATCH
 
 void asdm_plan_free(asdm_plan *p) { delete p; }
+
+int asdm_plan_uses_bags(const asdm_plan *p) { return p && p->bags ? 1 : 0; }
 
 int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n) try {
     if (!p || !n) return ASDM_E_INVALID;

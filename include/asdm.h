@@ -132,6 +132,9 @@ int asdm_filter_cfl(asdm_ctx *ctx, const asdm_graph *q, const asdm_graph *g, asd
 /* Must be called before the context that built cs is destroyed. */
 void asdm_cs_free(asdm_cs *cs);
 uint32_t asdm_cs_root(const asdm_cs *cs);
+/* Query vertices and data-graph vertices of the candidate space: a local-count matrix for cs
+ * holds nv rows of nq values. */
+int asdm_cs_info(const asdm_cs *cs, uint32_t *nq, uint32_t *nv);
 /* |C(u)| for every u (sizes[nq]). */
 int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes);
 /* Copies C(u) (ascending data ids) into out[cap]; *n = |C(u)|. */
@@ -151,6 +154,20 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
  * plan/optimizer.cpp:380-548) for at most 12 enumerated vertices, else the best greedy order grown
  * from each of them. */
 #define ASDM_PLAN_COST 4u
+/* Count through a tree decomposition of the query (bag tables of at most two query vertices,
+ * joined by variable elimination; injectivity by Moebius inversion over the partitions of each
+ * label class into non-adjacent blocks) instead of enumerating partial embeddings: the
+ * reference's bag materialization and global join (matching/adaptive.cpp:10-330,
+ * matching/join.cpp:56-103,910-1014,1109-1124) with the bags' tuples replaced by per-bag count
+ * tables.  Fails with ASDM_E_UNSUPPORTED when some quotient of the query has treewidth above two
+ * or a table would be too large.  Without this flag or ASDM_PLAN_FRONTIER, asdm_plan_create picks
+ * this engine by itself when the frontier plan's estimated rows exceed 2^24 and the bag plan is
+ * cheaper. */
+#define ASDM_PLAN_BAGS 8u
+/* Never use the bag-table engine (enumerate partial embeddings). */
+#define ASDM_PLAN_FRONTIER 16u
+/* Which engine a plan uses: 1 = bag tables, 0 = frontier enumeration. */
+int asdm_plan_uses_bags(const asdm_plan *p);
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
 /* Plan with an explicit order whose first num_enumerated vertices are enumerated and the rest are

@@ -1,0 +1,157 @@
+"""GPU parity of the bag-table engine (counting through a tree decomposition of the query with
+Moebius inversion over same-label partitions): counts and full local-count matrices against brute
+force, the oracle's local counts, the reference's golden counts and matrices, and the frontier
+engine.
+Every test here runs kernels on the MI355X."""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import asdmatch_amd as A
+from oracle import oracle as O
+from tests import common as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def bag_plan(cs, q, local=False):
+    """The bag plan of q, or None when some quotient needs a table over three variables."""
+    try:
+        p = cs.plan(q, local=local, bags=True)
+    except A.AsdmError as e:
+        if e.code == -7:  # ASDM_E_UNSUPPORTED
+            return None
+        raise
+    assert p.uses_bags
+    return p
+
+
+def random_case(rng, n_lo, n_hi, k_lo, k_hi, nlab):
+    n = rng.randint(n_lo, n_hi)
+    p = rng.uniform(0.15, 0.5)
+    edges = [(a, b) for a in range(n) for b in range(a + 1, n) if rng.random() < p]
+    if not edges:
+        edges = [(0, 1)]
+    glab = [rng.randrange(nlab) for _ in range(n)]
+    k = rng.randint(k_lo, k_hi)
+    qe = set((rng.randrange(i), i) for i in range(1, k))
+    for a in range(k):
+        for b in range(a + 1, k):
+            if rng.random() < 0.15:
+                qe.add((a, b))
+    qlab = [rng.randrange(nlab) for _ in range(k)]
+    return glab, edges, qlab, sorted(qe)
+
+
+def test_bag_counts_and_local_counts_against_bruteforce(ctx):
+    """Random small graphs with few labels (many same-label pairs, hence many partition terms):
+    the count equals brute force and the local-count matrix equals the oracle's enumeration."""
+    rng = random.Random(5)
+    done = 0
+    for trial in range(200):
+        glab, ge, qlab, qe = random_case(rng, 6, 14, 2, 7, rng.randint(1, 3))
+        g, og = T.graph_from_lists(glab, ge)
+        q, oq = T.graph_from_lists(qlab, qe)
+        cs = ctx.filter(q, g)
+        want = O.bruteforce_count(oq, og)
+        if cs is None:
+            assert want == 0
+            continue
+        p = bag_plan(cs, q, local=True)
+        if p is None:
+            continue
+        say(f"bags trial {trial}: k={len(qlab)} edges={qe} labels={qlab}")
+        mat, c = ctx.local_count(p, cs)
+        assert c == want, (trial, qlab, qe)
+        ocs = O.OCandSpace(oq, og)
+        omat, oc = ocs.local_count()
+        assert oc == want
+        assert np.array_equal(mat, omat), (trial, qlab, qe)
+        assert ctx.count(bag_plan(cs, q), cs) == want
+        done += 1
+    assert done >= 80
+
+
+def test_bag_counts_and_local_counts_match_reference(ctx):
+    """The synthetic golden cases (counts and LOCAL_COUNT matrices of the reference's ALL_LEVEL
+    build) whose quotients all have treewidth <= 2: equal counts, bit-exact matrices."""
+    n = nloc = 0
+    for case in T.synth_golden():
+        g = T.make_graph(case)
+        for rec in case["queries"]:
+            if rec["count"] is None:
+                continue
+            q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+            cs = ctx.filter(q, g)
+            local = bool(rec.get("local_npz"))
+            p = bag_plan(cs, q, local=local)
+            if p is None:
+                continue
+            say(f"bags reference {case['name']} {rec['seed']} local={local}")
+            if local:
+                mat, c = ctx.local_count(p, cs)
+                ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
+                assert np.array_equal(mat, ref), (case["name"], rec["seed"])
+                nloc += 1
+            else:
+                c = ctx.count(p, cs)
+            assert c == rec["count"], (case["name"], rec["seed"])
+            n += 1
+    say(f"bags reference: {n} counts, {nloc} matrices")
+    assert n >= 20 and nloc >= 12
+
+
+def test_bag_engine_matches_frontier_on_power_law_graphs(ctx):
+    """Chung-Lu graphs with 4-16 labels and random-walk queries of 5-9 vertices: the bag engine's
+    count and local counts equal the frontier engine's (both exact; independent methods)."""
+    checked = 0
+    for seed, nl in ((11, 4), (12, 8), (13, 16)):
+        g = A.Graph.chung_lu(3000, 15000, 2.5, nl, 0.0, seed)
+        ctx.upload(g)
+        for j in range(12):
+            k = 5 + j % 5
+            try:
+                q = g.random_walk_query(k, 0.5, 0xFFFFFFFF, seed * 100 + j)
+            except A.AsdmError:
+                continue
+            cs = ctx.filter(q, g)
+            if cs is None:
+                continue
+            p = bag_plan(cs, q, local=True)
+            if p is None:
+                continue
+            say(f"bags vs frontier: graph {seed} query {j} k={k}")
+            pf = cs.plan(q, local=True, bags=False)
+            assert not pf.uses_bags
+            mf, cf = ctx.local_count(pf, cs)
+            mb, cb = ctx.local_count(p, cs)
+            assert cb == cf, (seed, j)
+            assert np.array_equal(mb, mf), (seed, j)
+            assert ctx.count(bag_plan(cs, q), cs) == ctx.count(cs.plan(q, bags=False), cs)
+            checked += 1
+    assert checked >= 15
+
+
+def test_bag_count_wraps_like_size_t(ctx):
+    """A star with many same-label leaves on a complete graph has more than 2^64 embeddings: the
+    bag engine's count is the true count mod 2^64 (falling factorial of the hub degree)."""
+    n = 300
+    glab = [0] * n
+    ge = [(a, b) for a in range(n) for b in range(a + 1, n)]
+    g, _ = T.graph_from_lists(glab, ge)
+    k = 8  # hub + 7 leaves (Bell(7) = 877 partition terms): n (n-1) ... (n-7) > 2^64
+    q, _ = T.graph_from_lists([0] * k, [(0, i) for i in range(1, k)])
+    cs = ctx.filter(q, g)
+    p = bag_plan(cs, q)
+    assert p is not None
+    want = 1
+    for i in range(k):
+        want *= n - i
+    assert want > 2 ** 64
+    assert ctx.count(p, cs) == want % 2 ** 64
