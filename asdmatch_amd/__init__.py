@@ -94,6 +94,7 @@ _sig("asdm_cs_root", C.c_uint32, _P)
 _sig("asdm_cs_sizes", C.c_int, _P, _u32p)
 _sig("asdm_cs_info", C.c_int, _P, _u32p, _u32p)
 _sig("asdm_plan_uses_bags", C.c_int, _P)
+_sig("asdm_plan_bag_info", C.c_int, _P, _u32p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("asdm_cs_candidates", C.c_int, _P, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_cs_arc_weight", C.c_int, _P, C.c_uint32, _u64p)
 _sig("asdm_plan_create", C.c_int, _P, _P, _u32p, C.c_uint32, C.POINTER(_P))
@@ -634,6 +635,12 @@ class Plan:
         """True when the plan counts through bag tables rather than partial embeddings."""
         return bool(_lib.asdm_plan_uses_bags(self._h))
 
+    def bag_info(self):
+        """(partition terms, estimated table-element operations, entries of the largest table)."""
+        t, c, pk = C.c_uint32(), C.c_double(), C.c_double()
+        _lib.asdm_plan_bag_info(self._h, C.byref(t), C.byref(c), C.byref(pk))
+        return t.value, c.value, pk.value
+
     def order(self):
         o = np.zeros(self.nq, np.uint32)
         d = C.c_uint32()
@@ -641,18 +648,20 @@ class Plan:
         return o.tolist(), d.value
 
 
-def count_embeddings(ctx: Context, q: Graph, g: Graph, budget_bytes: int = 0, order=None) -> int:
-    """Filter, plan and count: the reference's `Number of matches` (0 when a candidate set is empty)."""
+def count_embeddings(ctx: Context, q: Graph, g: Graph, budget_bytes: int = 0, order=None,
+                     bags: Optional[bool] = None) -> int:
+    """Filter, plan and count: the reference's `Number of matches` (0 when a candidate set is empty).
+    bags: None lets the planner pick the engine, True/False forces bag tables / enumeration."""
     cs = ctx.filter(q, g)
     if cs is None:
         return 0
-    plan = cs.plan(q, order=order)
+    plan = cs.plan(q, order=order, bags=bags)
     return ctx.count(plan, cs, budget_bytes)
 
 
-def local_count(ctx: Context, q: Graph, g: Graph, budget_bytes: int = 0):
+def local_count(ctx: Context, q: Graph, g: Graph, budget_bytes: int = 0, bags: Optional[bool] = None):
     cs = ctx.filter(q, g)
     if cs is None:
         return np.zeros((g.n, q.n), np.uint64), 0
-    plan = cs.plan(q, local=True)
+    plan = cs.plan(q, local=True, bags=bags)
     return ctx.local_count(plan, cs, g.n, budget_bytes)

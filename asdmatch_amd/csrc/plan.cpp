@@ -392,10 +392,14 @@ int finish_plan(const asdm_cs *cs, std::vector<uint32_t> order, uint32_t D, uint
     return ASDM_OK;
 }
 
-// Rows above which the automatic planner considers the bag-table engine, and the largest bag
-// plan cost (element operations) it accepts without being asked.
+// The automatic planner considers the bag-table engine when the frontier plan's estimated rows
+// exceed kAutoBagRows or it enumerates at least kAutoBagDepth vertices (the independence model
+// underestimates deep plans on power-law graphs by orders of magnitude, while the bag plan's cost
+// is known exactly from its table sizes), and takes it when its cost (element operations) is at
+// most kAutoBagCost.
 constexpr double kAutoBagRows = 16777216.0;
-constexpr double kAutoBagCost = 4.0e10;
+constexpr uint32_t kAutoBagDepth = 8;
+constexpr double kAutoBagCost = 1.0e14;
 
 // Attaches a bag plan when flags ask for one (ASDM_PLAN_BAGS), or when the frontier plan is large
 // and the bag plan is cheap (automatic choice, unless ASDM_PLAN_FRONTIER or an explicit order).
@@ -411,11 +415,12 @@ int attach_bags(const asdm_cs *cs, uint32_t flags, bool explicit_order, asdm_pla
         p->bags = b;
         return ASDM_OK;
     }
-    if (explicit_order || (flags & (ASDM_PLAN_FRONTIER | ASDM_PLAN_NO_FACTOR)) || p->est_total <= kAutoBagRows)
+    if (explicit_order || (flags & (ASDM_PLAN_FRONTIER | ASDM_PLAN_NO_FACTOR)) ||
+        (p->est_total <= kAutoBagRows && p->D < kAutoBagDepth))
         return ASDM_OK;
     auto b = std::make_shared<BagPlan>();
     std::string why;
-    if (build_bag_plan(cs, local, *b, why) && b->cost <= kAutoBagCost && b->cost < 64.0 * p->est_total) p->bags = b;
+    if (build_bag_plan(cs, local, *b, why) && b->cost <= kAutoBagCost) p->bags = b;
     return ASDM_OK;
 }
 
@@ -506,6 +511,14 @@ int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint3
 void asdm_plan_free(asdm_plan *p) { delete p; }
 
 int asdm_plan_uses_bags(const asdm_plan *p) { return p && p->bags ? 1 : 0; }
+
+int asdm_plan_bag_info(const asdm_plan *p, uint32_t *terms, double *cost, double *peak_entries) try {
+    if (!p) return ASDM_E_INVALID;
+    if (terms) *terms = p->bags ? (uint32_t)p->bags->terms.size() : 0;
+    if (cost) *cost = p->bags ? p->bags->cost : 0;
+    if (peak_entries) *peak_entries = p->bags ? p->bags->peak_table : 0;
+    return ASDM_OK;
+} ASDM_HOST_CATCH
 
 int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t cap, uint32_t *n) try {
     if (!p || !n) return ASDM_E_INVALID;

@@ -82,10 +82,11 @@ def query(g: Graph, s: Slot, seed: int) -> Graph:
 
 def admit(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: int = 40, local: bool = False,
           guard_ms: float = 10000, log=print) -> list:
-    """Walks each slot's seed sequence and returns [(slot index, seed, count)] of admitted queries.
+    """Walks each slot's seed sequence and returns [(slot index, seed, count, engine)] of admitted
+    queries, engine being the one the planner chose ("frontier" or "bags").
 
-    A draw is rejected when it exceeds row_limit; guard_ms only protects the run against a draw
-    whose per-row work is extreme (such draws are logged as 'guard')."""
+    A frontier draw is rejected when it exceeds row_limit; guard_ms only protects the run against a
+    draw whose work is extreme (such draws are logged as 'guard')."""
     out = []
     ctx.set_row_limit(row_limit)
     ctx.set_time_limit(guard_ms)
@@ -104,17 +105,19 @@ def admit(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: int = 40,
                     log(f"  {cfg} slot {i} draw {j}: empty candidate set")
                     continue
                 plan = cs.plan(q, local=local)
+                engine = "bags" if plan.uses_bags else "frontier"
                 try:
                     t0 = time.perf_counter()
-                    c = (ctx.local_count(plan, cs, g.n)[1] if local else ctx.count(plan, cs))
+                    c = (ctx.local_count(plan, cs)[1] if local else ctx.count(plan, cs))
                     dt = time.perf_counter() - t0
                 except AsdmError as e:
                     if e.code != -8:
                         raise
-                    log(f"  {cfg} slot {i} draw {j}: rejected ({'guard' if 'time' in str(e) else 'rows'})")
+                    log(f"  {cfg} slot {i} draw {j}: rejected by the {engine} engine "
+                        f"({'guard' if 'time' in str(e) else 'rows'})")
                     continue
-                got = (i, seed, c)
-                log(f"  {cfg} slot {i} draw {j}: admitted, count {c}, {dt * 1e3:.2f} ms")
+                got = (i, seed, c, engine)
+                log(f"  {cfg} slot {i} draw {j}: admitted ({engine} engine), count {c}, {dt * 1e3:.2f} ms")
                 break
             log(f"{cfg} slot {i} k={s.k} -> {got}")
             if got:
@@ -185,10 +188,13 @@ def build_manifest(cfgs=("c1", "c2", "c3", "c4"), log=print) -> dict:
         local = cfg == "c3"
         if cfg in BATCHED_RULE:
             q = admit_batched(ctx, g, cfg, ROW_LIMITS[cfg], MAX_DRAWS[cfg], log=log)
+            engines = ["frontier"] * len(q)
         else:
-            q = admit(ctx, g, cfg, ROW_LIMITS[cfg], max_draws=MAX_DRAWS[cfg], local=local, log=log)
-        m[cfg] = {"row_limit": ROW_LIMITS[cfg], "local_plan": local, "queries": q,
-                  "rule": "batched" if cfg in BATCHED_RULE else "single"}
+            got = admit(ctx, g, cfg, ROW_LIMITS[cfg], max_draws=MAX_DRAWS[cfg], local=local, log=log)
+            q = [x[:3] for x in got]
+            engines = [x[3] for x in got]
+        m[cfg] = {"row_limit": ROW_LIMITS[cfg], "local_plan": local, "queries": q, "engines": engines,
+                  "slots": len(slots(cfg)), "rule": "batched" if cfg in BATCHED_RULE else "single"}
         with open(MANIFEST, "w") as f:
             json.dump(m, f, indent=1)
         del g

@@ -161,13 +161,17 @@ int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
  * matching/join.cpp:56-103,910-1014,1109-1124) with the bags' tuples replaced by per-bag count
  * tables.  Fails with ASDM_E_UNSUPPORTED when some quotient of the query has treewidth above two
  * or a table would be too large.  Without this flag or ASDM_PLAN_FRONTIER, asdm_plan_create picks
- * this engine by itself when the frontier plan's estimated rows exceed 2^24 and the bag plan is
- * cheaper. */
+ * this engine by itself when the frontier plan's estimated rows exceed 2^24 or it enumerates eight
+ * or more vertices, and the bag plan costs at most 1e14 table-element operations (about ten
+ * seconds on one MI355X). */
 #define ASDM_PLAN_BAGS 8u
 /* Never use the bag-table engine (enumerate partial embeddings). */
 #define ASDM_PLAN_FRONTIER 16u
 /* Which engine a plan uses: 1 = bag tables, 0 = frontier enumeration. */
 int asdm_plan_uses_bags(const asdm_plan *p);
+/* Size of a bag-table plan: partition terms, estimated table-element operations and entries of
+ * its largest table (all 0 for a frontier plan). */
+int asdm_plan_bag_info(const asdm_plan *p, uint32_t *terms, double *cost, double *peak_entries);
 int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *order, uint32_t flags,
                      asdm_plan **out);
 /* Plan with an explicit order whose first num_enumerated vertices are enumerated and the rest are

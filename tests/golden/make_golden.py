@@ -4,6 +4,8 @@ sources (oracle/ref.mk -> oracle/_ref/).  Run in the build container, where /roo
 
     make -f oracle/ref.mk -j8 && python3 tests/golden/make_golden.py
 
+    python3 tests/golden/make_golden.py c3_small12 c4_sample c2_more   (selected steps)
+
 Fixtures (data only: inputs and the reference's outputs):
   yeast/yeast.graph            the data graph shipped in the reference's dataset/yeast.tar.gz
   yeast/queries/*.graph        a subset of its shipped query graphs
@@ -12,6 +14,10 @@ Fixtures (data only: inputs and the reference's outputs):
                                count-build output (oracle/_ref/ref_driver) where it finishes, and
                                the reference's CFL root-independent candidate sets
                                (oracle/_ref/ref_cs_dump) for a subset
+  c3_small12.json + *_local.npz  12-vertex queries on a small 16-label power-law graph with the
+                               reference's LOCAL_COUNT matrices
+  c4_reference.json,           reference counts of the admitted benchmark queries that finish
+  c2_reference_all.json        within the time limit (and the count build's output beside them)
   synth.json                   seeded synthetic cases (generator parameters + sha256 of the graph
                                text the generator writes) with the reference's counts and candidate
                                set sizes; local-count matrices as .npz
@@ -224,8 +230,111 @@ def c2_sample():
                    "count": cnt, "count_build_output": wrong}, f, indent=1)
 
 
+# 12-vertex local counts (config 3's query size) on a graph small enough for the reference's
+# ALL_LEVEL + LOCAL_COUNT build to enumerate every embedding: power law, 16 uniform labels.
+C3_SMALL = [dict(name="c3_small12", kind="chung_lu", n=1200, ne=3600, beta=2.5, nl=16, zipf=0.0, seed=0xA5D00033),
+            dict(name="c3_small12_dense", kind="chung_lu", n=1500, ne=12000, beta=2.5, nl=16, zipf=0.0,
+                 seed=0xA5D00034)]
+
+
+def c3_small12(want=4, draws=60):
+    cases = []
+    for s in C3_SMALL:
+        cases.append(_c3_small_case(s, want, draws, min_count=100 if s["name"] == "c3_small12" else 10000))
+    with open(os.path.join(OUT, "c3_small12.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+
+
+def _c3_small_case(s, want, draws, min_count):
+    g = make_graph(s)
+    dpath = f"/tmp/golden_{s['name']}.graph"
+    g.write(dpath)
+    case = dict(s, data_sha256=sha256_file(dpath), local=True, queries=[])
+    for j in range(draws):
+        qseed = s["seed"] + 1000 + j
+        try:
+            q = g.random_walk_query(12, 1.0, 0xFFFFFFFF, qseed)
+        except Exception:
+            continue
+        qpath = f"/tmp/golden_{s['name']}_q{j}.graph"
+        q.write(qpath)
+        csv = f"/tmp/golden_{s['name']}_q{j}.csv"
+        cnt, st = run_ref("ref_driver_local", qpath, dpath, extra=("-c", csv), timeout=300)
+        print(s["name"], j, cnt, st, flush=True)
+        if cnt is None or cnt < min_count:
+            continue
+        mat = np.loadtxt(csv, delimiter=",", dtype=np.uint64, ndmin=2)
+        npz = f"{s['name']}_q{j}_local.npz"
+        np.savez_compressed(os.path.join(OUT, npz), local=mat)
+        cs = cs_dump(qpath, dpath)
+        case["queries"].append({"k": 12, "density": 1.0, "max_extra": 0xFFFFFFFF, "seed": qseed,
+                                "query_sha256": sha256_file(qpath), "count": cnt, "status": st, "local_npz": npz,
+                                "cand_sizes": [len(c) for c in cs["cand"]]})
+        os.remove(csv)
+        if len(case["queries"]) >= want:
+            break
+    os.remove(dpath)
+    return case
+
+
+def _ref_many(jobs, dpath, timeout, procs=8, binary="ref_driver_local"):
+    """Runs the reference on (key, query path) jobs in parallel; returns {key: (count, status, s)}."""
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(job):
+        key, qpath = job
+        t0 = time.perf_counter()
+        cnt, st = run_ref(binary, qpath, dpath, timeout=timeout)
+        return key, (cnt, st, round(time.perf_counter() - t0, 2))
+
+    out = {}
+    with ThreadPoolExecutor(procs) as ex:
+        for key, res in ex.map(one, jobs):
+            out[key] = res
+            print(key, res, flush=True)
+    return out
+
+
+def workload_sample(cfg, timeout, procs=8):
+    """Reference ALL_LEVEL counts of the admitted queries of a benchmark configuration
+    (asdmatch_amd/workload_manifest.json) that finish within `timeout` seconds each; the count
+    build's output on the same queries is recorded beside them."""
+    from asdmatch_amd import workloads as W
+
+    g = W.graph(cfg)
+    dpath = f"/tmp/golden_{cfg}.graph"
+    g.write(dpath)
+    sl = W.slots(cfg)
+    jobs = []
+    for i, seed, _ in W.load_manifest()[cfg]["queries"]:
+        qpath = f"/tmp/golden_{cfg}_s{i}.graph"
+        W.query(g, sl[i], seed).write(qpath)
+        jobs.append(((i, seed), qpath))
+    exact = _ref_many(jobs, dpath, timeout, procs)
+    cbuild = _ref_many(jobs, dpath, timeout, procs, binary="ref_driver")
+    recs = []
+    for (i, seed), qpath in jobs:
+        cnt, st, sec = exact[(i, seed)]
+        cb, cst, csec = cbuild[(i, seed)]
+        recs.append({"slot": i, "seed": seed, "k": sl[i].k, "count": cnt, "status": st, "seconds": sec,
+                     "count_build_output": cb, "count_build_status": cst, "count_build_seconds": csec})
+        os.remove(qpath)
+    with open(os.path.join(OUT, f"{cfg}_reference{'_all' if cfg == 'c2' else ''}.json"), "w") as f:
+        json.dump({"source": "oracle/_ref/ref_driver_local (reference ALL_LEVEL build, exact enumeration), one "
+                             f"process per query, {procs} in parallel, {timeout} s limit each; count build: "
+                             "oracle/_ref/ref_driver",
+                   "graph": f"workloads.graph('{cfg}')", "data_sha256": sha256_file(dpath), "queries": recs}, f,
+                  indent=1)
+    os.remove(dpath)
+
+
 if __name__ == "__main__":
-    yeast()
-    synth()
-    cards()
-    c2_sample()
+    steps = sys.argv[1:] or ["yeast", "synth", "cards", "c2_sample"]
+    for step in steps:
+        if step == "c4_sample":
+            workload_sample("c4", timeout=150)
+        elif step == "c2_more":
+            workload_sample("c2", timeout=600)
+        else:
+            globals()[step]()

@@ -155,3 +155,68 @@ def test_bag_count_wraps_like_size_t(ctx):
         want *= n - i
     assert want > 2 ** 64
     assert ctx.count(p, cs) == want % 2 ** 64
+
+
+def _c3_small_cases():
+    import json
+    with open(f"{T.GOLDEN}/c3_small12.json") as f:
+        return json.load(f)
+
+
+def test_twelve_vertex_local_counts_match_reference(ctx):
+    """12-vertex random-walk queries (config 3's size) on small 16-label power-law graphs: the
+    LOCAL_COUNT matrices of the reference's ALL_LEVEL build, bit-exact, through the bag engine and
+    through the frontier engine."""
+    nb = 0
+    for case in _c3_small_cases():
+        g = T.make_graph(case)
+        ctx.upload(g)
+        for rec in case["queries"]:
+            q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+            assert T.text_sha256(q) == rec["query_sha256"], (case["name"], rec["seed"])
+            cs = ctx.filter(q, g)
+            ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
+            say(f"12-vertex local counts: {case['name']} seed {rec['seed']} count {rec['count']}")
+            mf, cf = ctx.local_count(cs.plan(q, local=True, bags=False), cs)
+            assert cf == rec["count"] and np.array_equal(mf, ref), (case["name"], rec["seed"])
+            p = bag_plan(cs, q, local=True)
+            if p is not None:
+                mb, cb = ctx.local_count(p, cs)
+                assert cb == rec["count"] and np.array_equal(mb, ref), (case["name"], rec["seed"])
+                nb += 1
+    assert nb >= 4
+
+
+def test_config3_query_at_full_size(ctx):
+    """The admitted 12-vertex query of config 3 (200,000 vertices, 4,000,000 edges, 16 labels),
+    checked through properties that hold at any size: the local-count matrix's column sums equal
+    the count; a cell is zero when its vertex is outside the query vertex's candidate set; the count
+    is unchanged, and the columns are permuted, when the query's vertices are renumbered."""
+    from asdmatch_amd import workloads as W
+
+    (slot, seed, count), = [tuple(x) for x in W.load_manifest()["c3"]["queries"]]
+    g = W.graph("c3")
+    ctx.upload(g)
+    q = W.query(g, W.slots("c3")[slot], seed)
+    cs = ctx.filter(q, g)
+    p = cs.plan(q, local=True)
+    say(f"config 3: slot {slot} seed {seed}, bag engine: {p.uses_bags}")
+    mat, c = ctx.local_count(p, cs)
+    assert c == count
+    assert all(int(s) == c for s in mat.sum(axis=0, dtype=np.uint64))
+    for u in range(q.n):
+        inside = np.zeros(g.n, bool)
+        inside[cs.candidates(u)] = True
+        assert not mat[~inside, u].any(), u
+    assert ctx.count(cs.plan(q), cs) == c
+    # renumber the query vertices
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(q.n)  # new id of old vertex i is perm[i]
+    _, _, qlab = q.csr()
+    lab2 = np.zeros(q.n, np.uint32)
+    lab2[perm] = qlab
+    q2 = A.Graph.from_edges(lab2, perm[q.edges().astype(np.int64)].astype(np.uint32))
+    cs2 = ctx.filter(q2, g)
+    mat2, c2 = ctx.local_count(cs2.plan(q2, local=True), cs2)
+    assert c2 == c
+    assert np.array_equal(mat2[:, perm], mat)

@@ -98,7 +98,7 @@ def test_budget_does_not_change_counts(ctx):
     for rec in case["queries"]:
         q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
         cs = ctx.filter(q, g)
-        plan = cs.plan(q)
+        plan = cs.plan(q, bags=False)
         for budget in (1 << 12, 1 << 16, 1 << 20, 0):
             assert ctx.count(plan, cs, budget) == rec["count"], budget
 
@@ -419,11 +419,12 @@ def test_local_counts_match_reference(ctx):
             if not rec.get("local_npz"):
                 continue
             q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
-            mat, c = A.local_count(ctx, q, g)
             ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
-            assert c == rec["count"]
-            assert np.array_equal(mat, ref), (case["name"], rec["seed"])
-            assert all(int(s) == c for s in mat.sum(axis=0))
+            for bags in (None, False):  # the planner's choice, and the frontier engine
+                mat, c = A.local_count(ctx, q, g, bags=bags)
+                assert c == rec["count"]
+                assert np.array_equal(mat, ref), (case["name"], rec["seed"], bags)
+                assert all(int(s) == c for s in mat.sum(axis=0))
 
 
 def test_local_counts_same_under_many_pair_batches(ctx):
@@ -439,7 +440,7 @@ def test_local_counts_same_under_many_pair_batches(ctx):
             if not rec.get("local_npz") or done >= 6:
                 continue
             q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
-            mat, c = A.local_count(ctx, q, g, budget_bytes=1 << 16)
+            mat, c = A.local_count(ctx, q, g, budget_bytes=1 << 16, bags=False)
             ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
             assert c == rec["count"]
             assert np.array_equal(mat, ref), (case["name"], rec["seed"])
