@@ -108,6 +108,7 @@ _sig("asdm_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p)
 _sig("asdm_local_count", C.c_int, _P, _P, _P, C.c_uint64, _u64p, _u64p)
 _sig("asdm_enumerate", C.c_int, _P, _P, _P, C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64, _u64p)
 _sig("asdm_intersect_k", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
+_sig("asdm_intersect_k_noalloc", C.c_int, _P, C.POINTER(_u32p), _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_batch_count", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _u64p, C.POINTER(C.c_float))
 _f32p = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int)
@@ -392,6 +393,20 @@ class Context:
         out = np.zeros(max(cap, 1), np.uint32)
         n = C.c_uint32()
         self._check(_lib.asdm_intersect_k(self._h, ptrs, _ptr(sizes), k, _ptr(out), cap, C.byref(n)), "intersect")
+        return out[: n.value]
+
+    def intersect_noalloc(self, lists) -> np.ndarray:
+        """intersect() through the allocation-free entry point (fixed per-GPU scratch; at most 64
+        lists and 2^21 elements in total, else AsdmError ASDM_E_BUDGET)."""
+        arrs = [_u32(a) for a in lists]
+        k = len(arrs)
+        ptrs = (_u32p * max(k, 1))(*[_ptr(a) for a in arrs])
+        sizes = _u32([len(a) for a in arrs])
+        cap = int(min(len(a) for a in arrs)) if arrs else 0
+        out = np.zeros(max(cap, 1), np.uint32)
+        n = C.c_uint32()
+        self._check(_lib.asdm_intersect_k_noalloc(self._h, ptrs, _ptr(sizes), k, _ptr(out), cap, C.byref(n)),
+                    "intersect_noalloc")
         return out[: n.value]
 
 

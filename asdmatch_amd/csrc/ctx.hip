@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "engine.h"
 #include "kernels_common.h"
@@ -251,6 +252,39 @@ __global__ void k_intersect_k(const uint32_t *__restrict__ flat, const uint64_t 
     if (lane == 0) *nout = run;
 }
 
+// ---------------------------------------------------------------- allocation-free scratch
+// Device memory of asdm_intersect_k_noalloc: the staged lists, their offsets, the result and its
+// size, in one file-scope array of a fixed size.  Every device holds its own copy for the life of
+// the process; host code reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in, not a device address).
+constexpr uint64_t kNoAllocListWords = 1ull << 21;  // list elements staged per call
+constexpr uint32_t kNoAllocMaxLists = 64;
+constexpr size_t kNoAllocFlatOff = 0;
+constexpr size_t kNoAllocOutOff = kNoAllocFlatOff + 4 * kNoAllocListWords;
+constexpr size_t kNoAllocOffsOff = kNoAllocOutOff + 4 * kNoAllocListWords;
+constexpr size_t kNoAllocCountOff = kNoAllocOffsOff + 8 * (kNoAllocMaxLists + 1);
+constexpr size_t kNoAllocBytes = (kNoAllocCountOff + 8 + 255) & ~(size_t)255;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices] = {};
+
+// Looks up the current device's scratch address once per device (the device must be current).
+int noalloc_scratch_for(asdm_ctx *ctx, int device, uint8_t **out) {
+    if (device < 0 || device >= kMaxDevices) return ctx->fail(ASDM_E_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        int rc = check_hip(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+        if (rc) return rc;
+        g_scratch_ptr[device] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return ASDM_OK;
+}
+
 }  // namespace asdm
 
 using namespace asdm;
@@ -271,9 +305,15 @@ int asdm_ctx_create(int device, asdm_ctx **out) try {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ASDM_E_NODEVICE;
     auto ctx = std::make_unique<asdm_ctx>();
     ctx->device = device;
+    ctx->err.reserve(512);  // room for any fixed message without allocating when it is set
     if (hipSetDevice(device) != hipSuccess) return ASDM_E_HIP;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return ASDM_E_HIP;
     if (hipMalloc((void **)&ctx->d_counters, sizeof(uint64_t) * kNumCounters) != hipSuccess) return ASDM_E_OOM;
+    if (int rc = noalloc_scratch_for(ctx.get(), device, &ctx->noalloc_scratch)) {
+        (void)hipFree(ctx->d_counters);
+        (void)hipStreamDestroy(ctx->stream);
+        return rc;
+    }
     *out = ctx.release();
     return ASDM_OK;
 } ASDM_HOST_CATCH
@@ -417,6 +457,70 @@ int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t
     dev_free(ctx, dout);
     dev_free(ctx, dn);
     (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+} ASDM_CTX_CATCH(ctx)
+
+// asdm_intersect_k with every device buffer in this device's static scratch: no allocation on
+// the device or the host.  Calls that overlap on one device share the scratch and race.
+int asdm_intersect_k_noalloc(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k,
+                             uint32_t *out, uint32_t cap, uint32_t *n) try {
+    if (!ctx || !lists || !sizes || !n || k == 0) return ASDM_E_INVALID;
+    // error messages are literals assigned into the context's reserved error string: no allocation
+    if (k > kNoAllocMaxLists) {
+        ctx->err.assign("more than 64 lists for the allocation-free scratch");
+        return ASDM_E_BUDGET;
+    }
+    uint64_t offs[kNoAllocMaxLists + 1];
+    offs[0] = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        if (sizes[j] && !lists[j]) return ASDM_E_INVALID;
+        offs[j + 1] = offs[j] + sizes[j];
+    }
+    static_assert(kNoAllocListWords == 2097152, "the message below states the limit");
+    if (offs[k] > kNoAllocListWords) {
+        ctx->err.assign("lists hold more than 2097152 elements in total, the allocation-free scratch's limit");
+        return ASDM_E_BUDGET;
+    }
+    uint8_t *base = ctx->noalloc_scratch;
+    if (!base) {
+        ctx->err.assign("context has no scratch address");
+        return ASDM_E_INVALID;
+    }
+    (void)hipSetDevice(ctx->device);
+    // HIP failures record a literal message (check_hip would build a string)
+    auto hip_ok = [ctx](hipError_t e, const char *what) {
+        if (e == hipSuccess) return (int)ASDM_OK;
+        ctx->err.assign(what);
+        return (int)(e == hipErrorOutOfMemory ? ASDM_E_OOM : ASDM_E_HIP);
+    };
+    uint32_t *dflat = (uint32_t *)(base + kNoAllocFlatOff);
+    uint32_t *dout = (uint32_t *)(base + kNoAllocOutOff);
+    uint64_t *doffs = (uint64_t *)(base + kNoAllocOffsOff);
+    uint32_t *dn = (uint32_t *)(base + kNoAllocCountOff);
+    int rc = ASDM_OK;
+    for (uint32_t j = 0; j < k && !rc; ++j)
+        if (sizes[j])
+            rc = hip_ok(hipMemcpyAsync(dflat + offs[j], lists[j], sizeof(uint32_t) * sizes[j], hipMemcpyHostToDevice,
+                                               ctx->stream), "copy");
+    if (!rc) rc = hip_ok(hipMemcpyAsync(doffs, offs, sizeof(uint64_t) * (k + 1ull), hipMemcpyHostToDevice, ctx->stream), "copy");
+    if (!rc) {
+        // the kernel writes the result size unconditionally, so nothing needs clearing first
+        hipLaunchKernelGGL(k_intersect_k, dim3(1), dim3(64), 0, ctx->stream, dflat, doffs, k, dout, dn);
+        rc = hip_ok(hipGetLastError(), "intersect launch");
+    }
+    uint32_t hn = 0;
+    if (!rc) rc = hip_ok(hipMemcpyAsync(&hn, dn, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    if (!rc) rc = hip_ok(hipStreamSynchronize(ctx->stream), "sync");
+    if (rc) return rc;
+    *n = hn;
+    if (out && hn > cap) {
+        ctx->err.assign("output capacity too small");
+        return ASDM_E_INVALID;
+    }
+    if (out && hn) {
+        rc = hip_ok(hipMemcpyAsync(out, dout, sizeof(uint32_t) * hn, hipMemcpyDeviceToHost, ctx->stream), "copy");
+        if (!rc) rc = hip_ok(hipStreamSynchronize(ctx->stream), "sync");
+    }
     return rc;
 } ASDM_CTX_CATCH(ctx)
 

@@ -110,6 +110,9 @@ struct asdm_ctx {
         hipEvent_t a, b;
     };
     std::vector<PendingTimer> pending;  // recorded, resolved after the stream has drained
+    // device address of this device's allocation-free scratch (asdm_intersect_k_noalloc), looked
+    // up once per device when a context is created
+    uint8_t *noalloc_scratch = nullptr;
     uint64_t *d_counters = nullptr;  // [kNumCounters] device counters: 0 count, then algorithmic
                                      // bytes per kernel (single engine: 1..3; batch engine: 1..16)
     // caching allocator: freed blocks are kept for reuse (all work is ordered on `stream`, so a

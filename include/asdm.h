@@ -318,6 +318,14 @@ int asdm_plan_cards(const asdm_plan *p, uint64_t *masks, double *cards, uint32_t
  * out[cap] receives the ascending intersection, *n its size. */
 int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k,
                      uint32_t *out, uint32_t cap, uint32_t *n);
+/* asdm_intersect_k without any allocation, device or host: the lists are staged, and the result
+ * built, in a fixed scratch array that each GPU holds for the life of the process.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Limits: at most 64 lists and
+ * 2^21 list elements in total per call, else ASDM_E_BUDGET.  Every call on one GPU shares that
+ * scratch, so calls that overlap on one GPU (from different host threads or contexts) race. */
+int asdm_intersect_k_noalloc(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k,
+                             uint32_t *out, uint32_t cap, uint32_t *n);
 
 #ifdef __cplusplus
 }
