@@ -55,8 +55,11 @@ int64_t mobius(const std::vector<std::vector<uint32_t>> &blocks) {
 // eliminates a live block (never `keep`) of live degree <= 2, the cheapest first: degree 0 and 1
 // cost a pass over a vector or a table, degree 2 writes a |D(a)| x |D(b)| table (and multiplies
 // two dense tables when both incident factors are dense).
-bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &dom,
-               uint32_t keep, std::vector<BagStep> &steps, double &cost, double &peak) {
+// new_table_weight scales the cost of a step that starts a new dense table from two edge
+// relations: tables started in several places of one cycle later meet at a vertex whose both
+// factors are dense, a dense x dense product, so a larger weight keeps extending one table.
+bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &dom,
+                    uint32_t keep, double new_table_weight, std::vector<BagStep> &steps, double &cost, double &peak) {
     std::vector<std::vector<int>> kind(nb, std::vector<int>(nb, 0));  // 0 none, 1 edge relation, 2 dense table
     for (auto &e : edges) kind[e.first][e.second] = kind[e.second][e.first] = 1;
     std::vector<char> alive(nb, 1);
@@ -77,10 +80,10 @@ bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &ed
             else if (n.size() == 1) c = kind[v][n[0]] == 2 ? dom[v] * dom[n[0]] : dom[v] + dom[n[0]];
             else {
                 const double out = dom[n[0]] * dom[n[1]];
-                c = out;
-                if (kind[v][n[0]] == 2 && kind[v][n[1]] == 2) c += out * dom[v];
-                else if (kind[v][n[0]] == 2 || kind[v][n[1]] == 2) c += out * 4;
-                c += 1e-3 * out;  // prefer steps that write no table
+                const bool d0 = kind[v][n[0]] == 2, d1 = kind[v][n[1]] == 2;
+                if (d0 && d1) c = out * (1 + dom[v]);
+                else if (d0 || d1) c = out * 4;
+                else c = out * new_table_weight;
             }
             // degree 0 and 1 steps first: they shrink the graph without writing a table (a graph of
             // treewidth <= 2 reduces to nothing by such steps in any order, and always has two
@@ -97,6 +100,8 @@ bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &ed
         s.nn = (uint32_t)bn.size();
         for (uint32_t i = 0; i < s.nn; ++i) s.nb[i] = bn[i];
         steps.push_back(s);
+        // the cost charged is the step's modelled work, whatever weight steered the choice
+        if (s.nn == 2 && kind[bv][bn[0]] != 2 && kind[bv][bn[1]] != 2) best = dom[bn[0]] * dom[bn[1]];
         cost += best;
         if (s.nn == 2) {
             peak = std::max(peak, dom[bn[0]] * dom[bn[1]]);
@@ -106,6 +111,27 @@ bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &ed
         for (uint32_t w = 0; w < nb; ++w) kind[bv][w] = kind[w][bv] = 0;
         --left;
     }
+    return true;
+}
+
+// The cheapest of a few greedy orders (different weights on starting a new table).
+bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &dom,
+               uint32_t keep, std::vector<BagStep> &steps, double &cost, double &peak) {
+    bool found = false;
+    double best_cost = 0, best_peak = 0;
+    for (double w : {1.0, 8.0, 64.0, 4096.0}) {
+        std::vector<BagStep> st;
+        double c = 0, pk = 0;
+        if (!eliminate_once(nb, edges, dom, keep, w, st, c, pk)) return false;  // width > 2 for every weight
+        if (!found || c < best_cost) {
+            found = true;
+            best_cost = c;
+            best_peak = pk;
+            steps = std::move(st);
+        }
+    }
+    cost += best_cost;
+    peak = std::max(peak, best_peak);
     return true;
 }
 

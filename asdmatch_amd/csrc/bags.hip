@@ -125,15 +125,20 @@ __global__ void k_dense_mv(const uint64_t *__restrict__ F, uint32_t rows, uint32
     wave_add_bytes(bytes, nbytes);
 }
 
-// t[c] = sum_r F[r][c] x[r]  (the transposed product): one thread per column, coalesced rows.
+// t[c] += sum over a chunk of rows r of F[r][c] x[r] (the transposed product): blocks tile the
+// columns (coalesced rows) and the rows; t is zeroed before, and the u64 atomics wrap mod 2^64
+// like the sum itself.
+constexpr uint32_t kMvtRows = 64;
 __global__ void k_dense_mv_t(const uint64_t *__restrict__ F, uint32_t rows, uint32_t cols, const uint64_t *__restrict__ x,
                              uint64_t *__restrict__ t, unsigned long long *__restrict__ nbytes) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r0 = blockIdx.y * kMvtRows, r1 = min(rows, r0 + kMvtRows);
     uint64_t bytes = 0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cols; c += gridDim.x * blockDim.x) {
+    if (c < cols && r0 < r1) {
         uint64_t s = 0;
-        for (uint32_t r = 0; r < rows; ++r) s += F[(uint64_t)r * cols + c] * x[r];
-        t[c] = s;
-        bytes += 8ull * rows + 8;
+        for (uint32_t r = r0; r < r1; ++r) s += F[(uint64_t)r * cols + c] * x[r];
+        if (s) atomicAdd((unsigned long long *)&t[c], (unsigned long long)s);
+        bytes = 16ull * (r1 - r0) + 8;
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -197,56 +202,70 @@ __global__ void __launch_bounds__(kThreads) k_ee(const uint64_t *__restrict__ xo
     wave_add_bytes(bytes, nbytes);
 }
 
-// G[x][y] = sum_{j in R(Y->v)(y)} F[x][j] u[j]: a dense table on (X, v) rows X, and the edge
-// relation of (v, Y) read from Y's side.  One workgroup per row x; F[x][.] * u staged in LDS.
-__global__ void __launch_bounds__(kThreads) k_de(const uint64_t *__restrict__ F, const uint64_t *__restrict__ u,
-                                                 const uint64_t *__restrict__ yoff, const uint32_t *__restrict__ yval,
-                                                 uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *__restrict__ G,
-                                                 unsigned long long *__restrict__ nbytes) {
-    __shared__ unsigned long long s[kLdsCols];
-    const bool in_lds = nv <= kLdsCols;
-    uint64_t bytes = 0;
-    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
-        const uint64_t *fr = F + (uint64_t)x * nv;
-        if (in_lds) {
-            for (uint32_t j = threadIdx.x; j < nv; j += blockDim.x) s[j] = fr[j] * u[j];
-            __syncthreads();
-        }
-        for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
-            const uint64_t lb = yoff[y], le = yoff[y + 1];
-            uint64_t a = 0;
-            for (uint64_t e = lb; e < le; ++e) {
-                const uint32_t j = yval[e];
-                a += in_lds ? (uint64_t)s[j] : fr[j] * u[j];
-            }
-            G[(uint64_t)x * ny + y] = a;
-            bytes += 24 + 4 * (le - lb);
-        }
-        if (in_lds) __syncthreads();
-        if (threadIdx.x == 0) bytes += 16ull * nv;
-    }
-    wave_add_bytes(bytes, nbytes);
-}
-
-// G[x][y] = sum_{j in R(X->v)(x)} u[j] F[j][y]: the edge relation of (X, v) and a dense table on
-// (v, Y) rows v.  One workgroup per row x, threads across y (coalesced rows of F).
+// G[x][y] = sum_{j in R(x)} u[j] F[j][y]: an edge relation (rows x, columns j) times a dense table
+// whose rows are the relation's columns.  One workgroup per row x: the row's list and its weights
+// u[j] are staged in LDS in chunks; each thread owns two adjacent columns (16-byte loads when the
+// rows are 16-byte aligned) and keeps four list entries' loads in flight.  Chunks after the first
+// add onto the row written by the one before (same thread, same columns).
+constexpr uint32_t kEdChunk = 1024;
 __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ u, const uint64_t *__restrict__ F,
                                                  uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
                                                  unsigned long long *__restrict__ nbytes) {
+    __shared__ uint32_t lj[kEdChunk];
+    __shared__ uint64_t lw[kEdChunk];
+    const bool pairs = (ny & 1u) == 0;
+    const uint32_t ncol = pairs ? ny / 2 : ny;
     uint64_t bytes = 0;
     for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
         const uint64_t lb = xoff[x], le = xoff[x + 1];
-        for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
-            uint64_t a = 0;
-            for (uint64_t e = lb; e < le; ++e) {
-                const uint32_t j = xval[e];
-                a += u[j] * F[(uint64_t)j * ny + y];
+        uint64_t *grow = G + (uint64_t)x * ny;
+        for (uint64_t c0 = lb; c0 == lb || c0 < le; c0 += kEdChunk) {
+            const uint32_t cn = (uint32_t)(le - c0 < kEdChunk ? le - c0 : kEdChunk);
+            const bool first = c0 == lb;
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
+                const uint32_t jj = xval[c0 + t];
+                lj[t] = jj;
+                lw[t] = u[jj];
             }
-            G[(uint64_t)x * ny + y] = a;
-            bytes += 16 + 8 * (le - lb);
+            __syncthreads();
+            for (uint32_t c = threadIdx.x; c < ncol; c += blockDim.x) {
+                if (pairs) {
+                    uint64_t a0 = 0, a1 = 0;
+                    uint32_t e = 0;
+                    for (; e + 4 <= cn; e += 4) {
+                        const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + 2 * c);
+                        const ulonglong2 f1 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 1] * ny + 2 * c);
+                        const ulonglong2 f2 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 2] * ny + 2 * c);
+                        const ulonglong2 f3 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 3] * ny + 2 * c);
+                        a0 += lw[e] * f0.x + lw[e + 1] * f1.x + lw[e + 2] * f2.x + lw[e + 3] * f3.x;
+                        a1 += lw[e] * f0.y + lw[e + 1] * f1.y + lw[e + 2] * f2.y + lw[e + 3] * f3.y;
+                    }
+                    for (; e < cn; ++e) {
+                        const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + 2 * c);
+                        a0 += lw[e] * f0.x;
+                        a1 += lw[e] * f0.y;
+                    }
+                    ulonglong2 *dst = (ulonglong2 *)(grow + 2 * c);
+                    if (!first) {
+                        const ulonglong2 o = *dst;
+                        a0 += o.x;
+                        a1 += o.y;
+                    }
+                    *dst = make_ulonglong2(a0, a1);
+                } else {
+                    uint64_t a = 0;
+                    uint32_t e = 0;
+                    for (; e + 4 <= cn; e += 4)
+                        a += lw[e] * F[(uint64_t)lj[e] * ny + c] + lw[e + 1] * F[(uint64_t)lj[e + 1] * ny + c] +
+                             lw[e + 2] * F[(uint64_t)lj[e + 2] * ny + c] + lw[e + 3] * F[(uint64_t)lj[e + 3] * ny + c];
+                    for (; e < cn; ++e) a += lw[e] * F[(uint64_t)lj[e] * ny + c];
+                    grow[c] = first ? a : grow[c] + a;
+                }
+            }
         }
-        if (threadIdx.x == 0) bytes += 16 + 12 * (le - lb);
+        if (threadIdx.x == 0) bytes += 16 + (le - lb) * (12 + 8ull * ny) + 8ull * ny;
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -352,40 +371,94 @@ struct Dom {
 struct Rel {
     const uint64_t *off = nullptr;
     const uint32_t *val = nullptr;
+    uint64_t nnz = 0;
 };
-struct Table {  // dense factor of a block pair (lo, hi): row-major, rows D(lo)
+struct Table {  // dense factor of a block pair: row-major, rows indexed by D(rows), columns by the other
     uint64_t *p = nullptr;
+    uint32_t rows = 0;
 };
 
 struct BagRun {
     asdm_ctx *ctx;
     const asdm_cs *cs;
     const BagPlan *plan;
-    unsigned long long *d_acc = nullptr;    // ctx counters: [0] total, [4] scratch sum, [5] term scalar, [6..] bytes
+    unsigned long long *d_acc = nullptr;    // ctx counters: [0] total, [4] scratch sum, [5] term scalar,
+                                            // [6] relation and [7] vector algorithmic bytes, [14..17]
+                                            // bytes the table kernels' loops touch (ee, de, ed, dd)
     uint64_t *d_mat = nullptr;
     std::vector<void *> owned;              // per-term device buffers
+    // Domains of merged blocks and relations between blocks depend only on the blocks' members,
+    // and the same blocks recur across the partition terms: both are built once per query.
+    std::vector<void *> kept;               // device buffers of the caches, freed at the end
+    std::map<std::vector<uint32_t>, Dom> dom_cache;
+    std::map<std::pair<std::vector<uint32_t>, std::vector<uint32_t>>, Rel> rel_cache;
+    bool keep_allocs = false;               // allocations go to `kept` while building cached items
     std::chrono::steady_clock::time_point t0;
 
     unsigned long long *bytes_slot(int k) { return d_acc + 6 + k; }
 
     int alloc(void **p, size_t bytes) {
         int rc = dev_alloc(ctx, p, std::max<size_t>(bytes, 8));
-        if (!rc) owned.push_back(*p);
+        if (!rc) (keep_allocs ? kept : owned).push_back(*p);
         return rc;
     }
     void release(void *p) {
-        auto it = std::find(owned.begin(), owned.end(), p);
-        if (it != owned.end()) owned.erase(it);
+        for (auto *v : {&owned, &kept}) {
+            auto it = std::find(v->begin(), v->end(), p);
+            if (it != v->end()) v->erase(it);
+        }
         dev_free(ctx, p);
     }
     void release_all() {
         for (void *p : owned) dev_free(ctx, p);
         owned.clear();
     }
+    void release_cache() {
+        release_all();
+        for (void *p : kept) dev_free(ctx, p);
+        kept.clear();
+        dom_cache.clear();
+        rel_cache.clear();
+    }
     int launched(const char *what) { return check_hip(ctx, hipGetLastError(), what); }
     bool timed_out() const {
         return ctx->time_limit_ms > 0 &&
                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() > ctx->time_limit_ms;
+    }
+
+    int cached_dom(const std::vector<uint32_t> &members, Dom &d) {
+        if (members.size() == 1) return build_dom(members, d);
+        std::vector<uint32_t> key(members);
+        std::sort(key.begin(), key.end());
+        auto it = dom_cache.find(key);
+        if (it != dom_cache.end()) {
+            d = it->second;
+            return ASDM_OK;
+        }
+        keep_allocs = true;
+        const int rc = build_dom(members, d);
+        keep_allocs = false;
+        if (!rc) dom_cache[key] = d;
+        return rc;
+    }
+
+    int cached_rel(uint32_t arc, const std::vector<uint32_t> &ma, const Dom &A, const std::vector<uint32_t> &mb,
+                   const Dom &B, Rel &r) {
+        if (ma.size() == 1 && mb.size() == 1) return build_rel(arc, A, true, B, true, r);
+        std::vector<uint32_t> ka(ma), kb(mb);
+        std::sort(ka.begin(), ka.end());
+        std::sort(kb.begin(), kb.end());
+        auto key = std::make_pair(ka, kb);
+        auto it = rel_cache.find(key);
+        if (it != rel_cache.end()) {
+            r = it->second;
+            return ASDM_OK;
+        }
+        keep_allocs = true;
+        const int rc = build_rel(arc, A, ma.size() == 1, B, mb.size() == 1, r);
+        keep_allocs = false;
+        if (!rc) rel_cache[key] = r;
+        return rc;
     }
 
     int build_dom(const std::vector<uint32_t> &members, Dom &d) {
@@ -435,6 +508,7 @@ struct BagRun {
         if (a_single && b_single) {  // the candidate-edge lists are the relation
             r.off = cs->d_ce_off + cs->ce_base[arc];
             r.val = cs->d_ce_val;
+            r.nnz = cs->arc_weight[arc];
             return ASDM_OK;
         }
         uint32_t *cnt = nullptr, *val = nullptr;
@@ -470,24 +544,26 @@ struct BagRun {
         release(cnt);
         r.off = off;
         r.val = val;
+        r.nnz = nnz;
         return ASDM_OK;
     }
 
-    // Dense table of the pair (lo, hi) with rows D(want_rows): the stored table or its transpose.
-    int oriented(uint32_t lo, uint32_t hi, const Table &t, uint32_t want_rows, const std::vector<Dom> &dom,
-                 uint64_t **out, bool &temp) {
+    // A dense table of the block pair {t.rows, other} with rows D(want): the table itself or its
+    // transpose (temp = true: the caller releases it).
+    int oriented(const Table &t, uint32_t other, uint32_t want, const std::vector<Dom> &dom, uint64_t **out, bool &temp) {
         temp = false;
-        if (want_rows == lo) {
+        if (t.rows == want) {
             *out = t.p;
             return ASDM_OK;
         }
-        int rc = alloc((void **)out, 8ull * dom[lo].n * dom[hi].n);
+        const uint32_t r = dom[t.rows].n, c = dom[other].n;
+        int rc = alloc((void **)out, 8ull * std::max<uint64_t>((uint64_t)r * c, 1));
         if (rc) return rc;
         temp = true;
-        const uint64_t tiles = ((uint64_t)(dom[lo].n + 31) / 32) * ((dom[hi].n + 31) / 32);
+        const uint64_t tiles = ((uint64_t)(r + 31) / 32) * ((c + 31) / 32);
         KTimer kt(ctx, "bag_transpose");
-        hipLaunchKernelGGL(k_transpose, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, t.p, dom[lo].n, dom[hi].n, *out);
-        kt.stop(16ull * dom[lo].n * dom[hi].n);
+        hipLaunchKernelGGL(k_transpose, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, t.p, r, c, *out);
+        kt.stop(16ull * r * c);
         return launched("bag transpose");
     }
 
@@ -505,11 +581,12 @@ struct BagRun {
         }
         if (!rc) rc = fill_u64(ctx, scal, 1, 1);
         if (rc) return rc;
-        // live factors: kind 1 = edge relation, 2 = dense table
+        // live factors by block pair (lo, hi): kind 1 = edge relation, 2 = dense table
         std::map<std::pair<uint32_t, uint32_t>, std::pair<int, Table>> fac;
         for (auto &e : T.edges) fac[e] = {1, Table{}};
+        auto key = [](uint32_t a, uint32_t b) { return std::make_pair(std::min(a, b), std::max(a, b)); };
         auto rel = [&](uint32_t a, uint32_t b) -> Rel {  // rows D(a)
-            auto it = rels.find({std::min(a, b), std::max(a, b)});
+            auto it = rels.find(key(a, b));
             return a < b ? it->second.first : it->second.second;
         };
         for (const BagStep &s : steps) {
@@ -522,100 +599,102 @@ struct BagRun {
                 if ((rc = launched("bag sum"))) return rc;
             } else if (s.nn == 1) {
                 const uint32_t a = s.nb[0];
-                auto key = std::make_pair(std::min(a, v), std::max(a, v));
-                auto &f = fac[key];
+                auto &f = fac[key(a, v)];
+                const uint32_t na = dom[a].n;
                 uint64_t *t = nullptr;
-                if ((rc = alloc((void **)&t, 8ull * std::max<uint32_t>(dom[a].n, 1)))) return rc;
+                if ((rc = alloc((void **)&t, 8ull * std::max<uint32_t>(na, 1)))) return rc;
                 KTimer kt(ctx, "bag_vector");
                 if (f.first == 1) {
                     const Rel r = rel(a, v);
-                    hipLaunchKernelGGL(k_spmv, dim3(grid_for(dom[a].n, kThreads)), dim3(kThreads), 0, ctx->stream, r.off,
-                                       r.val, dom[a].n, u[v], t, bytes_slot(1));
-                } else if (a < v) {  // table rows D(a)
-                    hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(dom[a].n, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
-                                       f.second.p, dom[a].n, dv.n, u[v], t, bytes_slot(1));
-                } else {  // table rows D(v)
-                    hipLaunchKernelGGL(k_dense_mv_t, dim3(grid_for(dom[a].n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                                       f.second.p, dv.n, dom[a].n, u[v], t, bytes_slot(1));
+                    hipLaunchKernelGGL(k_spmv, dim3(grid_for(na, kThreads)), dim3(kThreads), 0, ctx->stream, r.off, r.val, na,
+                                       u[v], t, bytes_slot(1));
+                } else if (f.second.rows == a) {
+                    hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(na, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
+                                       f.second.p, na, dv.n, u[v], t, bytes_slot(1));
+                } else {  // table rows D(v): the transposed product, accumulated with atomics
+                    if ((rc = fill_u64(ctx, t, na, 0))) return rc;
+                    const dim3 grid((na + kThreads - 1) / kThreads, std::max<uint32_t>(1, (dv.n + kMvtRows - 1) / kMvtRows));
+                    hipLaunchKernelGGL(k_dense_mv_t, grid, dim3(kThreads), 0, ctx->stream, f.second.p, dv.n, na, u[v], t,
+                                       bytes_slot(1));
                 }
                 kt.stop();
-                hipLaunchKernelGGL(k_vec_mul, dim3(grid_for(dom[a].n, kThreads)), dim3(kThreads), 0, ctx->stream, u[a], t,
-                                   dom[a].n);
+                hipLaunchKernelGGL(k_vec_mul, dim3(grid_for(na, kThreads)), dim3(kThreads), 0, ctx->stream, u[a], t, na);
                 if ((rc = launched("bag vector step"))) return rc;
                 release(t);
             } else {
                 const uint32_t X = std::min(s.nb[0], s.nb[1]), Y = std::max(s.nb[0], s.nb[1]);
                 const uint32_t nx = dom[X].n, ny = dom[Y].n, nv = dv.n;
-                auto kxv = std::make_pair(std::min(X, v), std::max(X, v));
-                auto kvy = std::make_pair(std::min(v, Y), std::max(v, Y));
-                auto &fxv = fac[kxv];
-                auto &fvy = fac[kvy];
+                auto &fxv = fac[key(X, v)];
+                auto &fvy = fac[key(v, Y)];
                 uint64_t *G = nullptr;
                 if ((rc = alloc((void **)&G, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
+                Table out{G, X};  // rows of the new table
+                const uint64_t out_bytes = 8ull * nx * ny;  // the table written once
                 const char *kname = fxv.first == 1 ? (fvy.first == 1 ? "bag_table_ee" : "bag_table_ed")
                                                    : (fvy.first == 1 ? "bag_table_de" : "bag_table_dd");
-                const unsigned rows_grid = grid_for(nx, 1);
                 if (fxv.first == 1 && fvy.first == 1) {
                     const Rel rx = rel(X, v), rv = rel(v, Y);
                     if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
                     KTimer kt(ctx, kname);
-                    hipLaunchKernelGGL(k_ee, dim3(std::min(rows_grid, 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
-                                       rx.val, rv.off, rv.val, u[v], nx, ny, G, bytes_slot(2));
-                    kt.stop();
-                } else if (fvy.first == 1) {  // dense (X, v), edge (v, Y)
+                    hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
+                                       rx.val, rv.off, rv.val, u[v], nx, ny, G, bytes_slot(8));
+                    kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
+                } else if (fxv.first == 1 || fvy.first == 1) {
+                    // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;
+                    // the table comes out with rows D(e)
+                    const bool dense_x = fxv.first == 2;
+                    const uint32_t e = dense_x ? Y : X, d = dense_x ? X : Y;
+                    const Table &td = dense_x ? fxv.second : fvy.second;
                     uint64_t *F = nullptr;
                     bool tmp = false;
-                    if ((rc = oriented(kxv.first, kxv.second, fxv.second, X, dom, &F, tmp))) return rc;
-                    const Rel ry = rel(Y, v);
+                    if ((rc = oriented(td, td.rows == v ? d : v, v, dom, &F, tmp))) return rc;
+                    const Rel re = rel(e, v);
+                    const uint32_t ne = dom[e].n, nd = dom[d].n;
                     KTimer kt(ctx, kname);
-                    hipLaunchKernelGGL(k_de, dim3(std::min(rows_grid, 1u << 16)), dim3(kThreads), 0, ctx->stream, F, u[v],
-                                       ry.off, ry.val, nx, nv, ny, G, bytes_slot(2));
-                    kt.stop();
+                    hipLaunchKernelGGL(k_ed, dim3(std::min(grid_for(ne, 1), 1u << 16)), dim3(kThreads), 0, ctx->stream, re.off,
+                                       re.val, u[v], F, ne, nd, G, bytes_slot(dense_x ? 9 : 10));
+                    kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
                     if (tmp) release(F);
-                } else if (fxv.first == 1) {  // edge (X, v), dense (v, Y)
-                    uint64_t *F = nullptr;
-                    bool tmp = false;
-                    if ((rc = oriented(kvy.first, kvy.second, fvy.second, v, dom, &F, tmp))) return rc;
-                    const Rel rx = rel(X, v);
-                    KTimer kt(ctx, kname);
-                    hipLaunchKernelGGL(k_ed, dim3(std::min(rows_grid, 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
-                                       rx.val, u[v], F, nx, ny, G, bytes_slot(2));
-                    kt.stop();
-                    if (tmp) release(F);
+                    out.rows = e;
                 } else {
                     uint64_t *Fa = nullptr, *Fb = nullptr;
                     bool ta = false, tb = false;
-                    if ((rc = oriented(kxv.first, kxv.second, fxv.second, X, dom, &Fa, ta))) return rc;
-                    if ((rc = oriented(kvy.first, kvy.second, fvy.second, v, dom, &Fb, tb))) return rc;
+                    if ((rc = oriented(fxv.second, fxv.second.rows == X ? v : X, X, dom, &Fa, ta))) return rc;
+                    if ((rc = oriented(fvy.second, fvy.second.rows == v ? Y : v, v, dom, &Fb, tb))) return rc;
                     const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
                     KTimer kt(ctx, kname);
                     hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, Fa, u[v], Fb, nx, nv, ny, G,
-                                       bytes_slot(2));
-                    kt.stop();
+                                       bytes_slot(11));
+                    kt.stop(8ull * nx * nv + 8ull * nv * ny + 8ull * nv + out_bytes);
                     if (ta) release(Fa);
                     if (tb) release(Fb);
                 }
                 if ((rc = launched("bag table"))) return rc;
                 // join with a factor already on (X, Y)
-                auto kxy = std::make_pair(X, Y);
-                auto it = fac.find(kxy);
+                const uint32_t orow = out.rows, ocol = out.rows == X ? Y : X;
+                auto it = fac.find(key(X, Y));
                 if (it == fac.end()) {
-                    fac[kxy] = {2, Table{G}};
+                    fac[key(X, Y)] = {2, out};
                 } else if (it->second.first == 1) {
                     uint64_t *M = nullptr;
                     if ((rc = alloc((void **)&M, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
                     if ((rc = fill_u64(ctx, M, (uint64_t)nx * ny, 0))) return rc;
-                    const Rel r = rel(X, Y);
-                    hipLaunchKernelGGL(k_mask, dim3(grid_for(nx, kThreads)), dim3(kThreads), 0, ctx->stream, G, r.off, r.val,
-                                       nx, ny, M);
+                    const Rel r = rel(orow, ocol);
+                    hipLaunchKernelGGL(k_mask, dim3(grid_for(dom[orow].n, kThreads)), dim3(kThreads), 0, ctx->stream, G, r.off,
+                                       r.val, dom[orow].n, dom[ocol].n, M);
                     if ((rc = launched("bag mask"))) return rc;
                     release(G);
-                    it->second = {2, Table{M}};
+                    it->second = {2, Table{M, orow}};
                 } else {
+                    uint64_t *H = G;
+                    bool tmp = false;
+                    if (it->second.second.rows != orow && (rc = oriented(out, ocol, it->second.second.rows, dom, &H, tmp)))
+                        return rc;
                     const uint64_t n = (uint64_t)nx * ny;
                     hipLaunchKernelGGL(k_hadamard, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                                       it->second.second.p, G, n);
+                                       it->second.second.p, H, n);
                     if ((rc = launched("bag join"))) return rc;
+                    if (tmp) release(H);
                     release(G);
                 }
             }
@@ -636,7 +715,7 @@ struct BagRun {
         const uint32_t nb = (uint32_t)T.blocks.size();
         std::vector<Dom> dom(nb);
         int rc = ASDM_OK;
-        for (uint32_t b = 0; b < nb && !rc; ++b) rc = build_dom(T.blocks[b], dom[b]);
+        for (uint32_t b = 0; b < nb && !rc; ++b) rc = cached_dom(T.blocks[b], dom[b]);
         if (rc) return rc;
         for (uint32_t b = 0; b < nb; ++b)
             if (dom[b].n == 0) return ASDM_OK;  // no data vertex fits the merged block: the term is 0
@@ -649,8 +728,8 @@ struct BagRun {
             for (uint32_t u : T.blocks[A]) a_from_A |= cs->arc_u[ab] == u;
             if (!a_from_A) std::swap(ab, ba);
             Rel r1, r2;
-            rc = build_rel(ab, dom[A], T.blocks[A].size() == 1, dom[B], T.blocks[B].size() == 1, r1);
-            if (!rc) rc = build_rel(ba, dom[B], T.blocks[B].size() == 1, dom[A], T.blocks[A].size() == 1, r2);
+            rc = cached_rel(ab, T.blocks[A], dom[A], T.blocks[B], dom[B], r1);
+            if (!rc) rc = cached_rel(ba, T.blocks[B], dom[B], T.blocks[A], dom[A], r2);
             rels[{A, B}] = {r1, r2};
         }
         if (rc) return rc;
@@ -703,6 +782,7 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
         R.release_all();
     }
     (void)hipEventRecord(eb, ctx->stream);
+    R.release_cache();
     uint64_t res[kNumCounters] = {};
     if (!rc) rc = check_hip(ctx, hipMemcpyAsync(res, ctx->d_counters, 8 * kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     if (!rc && mat_host)
@@ -712,18 +792,24 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
     (void)hipEventElapsedTime(&kms, ea, eb);
     (void)hipEventDestroy(ea);
     (void)hipEventDestroy(eb);
-    R.release_all();
+    R.release_cache();
     if (R.d_mat) dev_free(ctx, R.d_mat);
     (void)hipStreamSynchronize(ctx->stream);
     if (rc) return rc;
     *count = res[0];
     resolve_timers(ctx);
+    // table kernels report their minimal traffic (each operand read once, the table written once)
+    // through their timers; the in-kernel counter of the bytes their loops touch (slot 12,
+    // including lists re-read per table row) is kept apart
     add_kernel_bytes(ctx, "bag_relation", res[6]);
     add_kernel_bytes(ctx, "bag_vector", res[7]);
-    add_kernel_bytes(ctx, "bag_tables", res[8]);
+    const char *touched[4] = {"bag_table_ee_touched", "bag_table_de_touched", "bag_table_ed_touched",
+                              "bag_table_dd_touched"};
+    for (int k = 0; k < 4; ++k)
+        if (res[14 + k]) add_kernel_bytes(ctx, touched[k], res[14 + k]);
     ctx->stats.exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats.kernel_ms = kms;
-    ctx->stats.alg_bytes = res[6] + res[7] + res[8];
+    ctx->stats.alg_bytes = res[6] + res[7];
     ctx->stats.rows_total = 0;
     ctx->stats.launches = 0;
     ctx->stats.plan_ms = plan->plan_ms;

@@ -258,6 +258,8 @@ int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asd
 uint32_t cfl_root(const asdm_graph *q, const asdm_graph *g, const std::vector<uint32_t> &base_count);
 std::vector<uint32_t> bfs_order(const asdm_graph *q, uint32_t root);
 
+// Attaches a bag-table plan to p when flags ask for one, or by the automatic rule (plan.cpp).
+int attach_bags(const asdm_cs *cs, uint32_t flags, bool explicit_order, asdm_plan *p, std::string &err);
 // Bag-table engine (bags.hip): count (mat_host == NULL) or local counts of a plan with bags.
 int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *mat_host, uint64_t *count);
 

@@ -2346,8 +2346,13 @@ void by_class(uint32_t g, F1 f1, F4 f4, F16 f16, F64 f64) {
 // qstatus_out (optional) receives ASDM_OK or, for a query whose partial embeddings exceed the
 // context's row limit, ASDM_E_TIMEOUT (its rows are dropped and its count is 0).  Without it a
 // query over the limit fails the whole call with ASDM_E_TIMEOUT.
+// lat_out[i] (optional) receives now() when query i's count became final: after the host
+// synchronization of the depth at which its last rows were counted.  With done_out, a run that
+// ends in ASDM_E_BUDGET still returns the counts of the queries that had finished by then
+// (done_out[i] = 1), so a caller re-runs only the others.
 int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std::vector<asdm_plan *> &plans_in,
-                 uint64_t budget, uint64_t *counts_out, int *qstatus_out) {
+                 uint64_t budget, uint64_t *counts_out, int *qstatus_out, float *lat_out = nullptr,
+                 const std::function<float()> *now = nullptr, char *done_out = nullptr) {
     const uint32_t Q = (uint32_t)css_in.size();
     if (Q == 0) return ASDM_OK;
     // queries grouped by lane class, so that each class is a contiguous row range at every depth
@@ -2461,6 +2466,11 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     if (!rc) rc = dev_alloc(ctx, (void **)&tp_fill, 8 * (Q + 1));
     std::vector<uint64_t> qrows(Q, 0);
     std::vector<char> dropped(Q, 0);
+    std::vector<char> fin(Q, 0);  // count final (known to the host)
+    auto stamp = [&](uint32_t i) {
+        fin[i] = 1;
+        if (lat_out && now) lat_out[perm[i]] = (*now)();
+    };
     for (uint32_t d = 1; d <= maxD && n && !rc; ++d) {
         dev_free(ctx, counts_d);
         dev_free(ctx, offs);
@@ -2551,6 +2561,10 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
                 if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) break;
             }
         }
+        // after this synchronization every query whose rows were counted at depth d, or that has
+        // no rows at the next depth, has its final count
+        for (uint32_t i = 0; i < Q; ++i)
+            if (!fin[i] && (dropped[i] || hp[i].D == d || fused_at(hp[i], d) || next_h[i + 1] == next_h[i])) stamp(i);
         const uint64_t total = next_h[Q];
         if (total == 0 || d == maxD) break;
         if (4ull * (d + 1) * total > budget) {
@@ -2589,9 +2603,21 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     }
     std::vector<uint64_t> qc(Q);
     uint64_t kbytes[kNumCounters] = {0};
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(kbytes, ctx->d_counters, 8 * kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    const bool keep_done = rc == ASDM_E_BUDGET && done_out;
+    int rc2 = rc;
+    if (!rc || keep_done) {
+        rc2 = check_hip(ctx, hipMemcpyAsync(qc.data(), qcount, 8 * Q, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc2) rc2 = check_hip(ctx, hipMemcpyAsync(kbytes, ctx->d_counters, 8 * kNumCounters, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+        if (!rc2) rc2 = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (!rc) rc = rc2;
+    }
+    if (keep_done && !rc2)
+        for (uint32_t i = 0; i < Q; ++i)
+            if (fin[i]) {
+                done_out[perm[i]] = 1;
+                counts_out[perm[i]] = dropped[i] ? 0 : qc[i];
+                if (qstatus_out) qstatus_out[perm[i]] = dropped[i] ? ASDM_E_TIMEOUT : ASDM_OK;
+            }
     if (!rc && ctx->profiling) {
         for (int k = 0; k < 3; ++k)
             for (int gi = 0; gi < 4; ++gi)
@@ -2602,6 +2628,8 @@ int batch_engine(asdm_ctx *ctx, const std::vector<asdm_cs *> &css_in, const std:
     }
     if (!rc)
         for (uint32_t i = 0; i < Q; ++i) {
+            if (!fin[i]) stamp(i);
+            if (done_out) done_out[perm[i]] = 1;
             counts_out[perm[i]] = dropped[i] ? 0 : qc[i];
             if (qstatus_out) qstatus_out[perm[i]] = dropped[i] ? ASDM_E_TIMEOUT : ASDM_OK;
         }
@@ -2771,39 +2799,84 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
         bcs.push_back(cs[i]);
         bidx.push_back(i);
     }
-    if (!rc && m) rc = learned_plans(m, g, std::vector<const asdm_cs *>(bcs.begin(), bcs.end()), 0, bplans);
+    if (!rc && m) {
+        rc = learned_plans(m, g, std::vector<const asdm_cs *>(bcs.begin(), bcs.end()), 0, bplans);
+        // the engine of each query is the one asdm_plan_create picks (the learned estimates order
+        // the frontier plans; they do not move a query between engines)
+        std::string err;
+        for (size_t k = 0; k < bplans.size() && !rc; ++k) {
+            asdm_plan plain;
+            rc = build_plan(qs[bidx[k]], bcs[k], nullptr, 0, [](uint64_t, double fb) { return fb; }, &plain, err);
+            if (!rc) rc = attach_bags(bcs[k], 0, false, &plain, err);
+            if (!rc && plain.bags) bplans[k]->bags = plain.bags;
+        }
+        if (rc) ctx->err = err;
+    }
     if (!rc) {
         const uint64_t budget = budget_bytes ? budget_bytes : (16ull << 30);
         std::vector<uint64_t> bc(bcs.size(), 0);
         std::vector<int> bst(bcs.size(), ASDM_OK);
-        // A (sub-)batch whose frontier exceeds the budget is split in halves; a single query that
-        // still exceeds it runs on the single-query engine, which bounds memory by chunking.
-        std::function<int(size_t, size_t)> run = [&](size_t lo, size_t hi) -> int {
-            std::vector<asdm_cs *> c(bcs.begin() + lo, bcs.begin() + hi);
-            std::vector<asdm_plan *> p(bplans.begin() + lo, bplans.begin() + hi);
-            int r = batch_engine(ctx, c, p, budget, bc.data() + lo, qstatus ? bst.data() + lo : nullptr);
-            if (r == ASDM_E_BUDGET && hi - lo > 1) {
-                const size_t mid = lo + (hi - lo) / 2;
-                if ((r = run(lo, mid))) return r;
-                return run(mid, hi);
+        std::vector<float> blat(bcs.size(), 0);
+        const std::function<float()> clock = now_ms;
+        // A (sub-)batch whose frontier exceeds the budget keeps the counts of the queries that had
+        // finished and splits the others in halves; a single query that still exceeds it runs on
+        // the single-query engine, which bounds memory by chunking.
+        std::function<int(const std::vector<size_t> &)> run = [&](const std::vector<size_t> &idx) -> int {
+            std::vector<asdm_cs *> c;
+            std::vector<asdm_plan *> p;
+            for (size_t k : idx) {
+                c.push_back(bcs[k]);
+                p.push_back(bplans[k]);
             }
-            if (r == ASDM_E_BUDGET) {
-                bst[lo] = asdm_count(ctx, bplans[lo], bcs[lo], budget_bytes, &bc[lo]);
-                if (lat_ms) lat_ms[bidx[lo]] = now_ms();
-                if (bst[lo] == ASDM_OK) return ASDM_OK;
-                bc[lo] = 0;
-                return qstatus && per_query(bst[lo]) ? ASDM_OK : bst[lo];
+            std::vector<uint64_t> cnt(idx.size(), 0);
+            std::vector<int> st(idx.size(), ASDM_OK);
+            std::vector<float> lat(idx.size(), 0);
+            std::vector<char> done(idx.size(), 0);
+            int r = batch_engine(ctx, c, p, budget, cnt.data(), qstatus ? st.data() : nullptr, lat.data(), &clock,
+                                 done.data());
+            std::vector<size_t> rest;
+            for (size_t j = 0; j < idx.size(); ++j) {
+                if (done[j]) {
+                    bc[idx[j]] = cnt[j];
+                    bst[idx[j]] = st[j];
+                    blat[idx[j]] = lat[j];
+                } else {
+                    rest.push_back(idx[j]);
+                }
             }
-            if (!r && lat_ms) {
-                const float t = now_ms();
-                for (size_t k = lo; k < hi; ++k) lat_ms[bidx[k]] = t;
+            if (r != ASDM_E_BUDGET) return r;
+            if (rest.size() > 1) {
+                const size_t mid = rest.size() / 2;
+                if ((r = run(std::vector<size_t>(rest.begin(), rest.begin() + mid)))) return r;
+                return run(std::vector<size_t>(rest.begin() + mid, rest.end()));
             }
-            return r;
+            for (size_t k : rest) {
+                bst[k] = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
+                blat[k] = now_ms();
+                if (bst[k] != ASDM_OK) {
+                    bc[k] = 0;
+                    if (!(qstatus && per_query(bst[k]))) return bst[k];
+                }
+            }
+            return ASDM_OK;
         };
-        if (!bcs.empty()) rc = run(0, bcs.size());
+        // queries whose plan uses bag tables are counted by the bag engine, the others together
+        std::vector<size_t> frontier, bagged;
+        for (size_t k = 0; k < bcs.size(); ++k) (bplans[k]->bags ? bagged : frontier).push_back(k);
+        if (!frontier.empty()) rc = run(frontier);
+        for (size_t j = 0; j < bagged.size() && !rc; ++j) {
+            const size_t k = bagged[j];
+            bst[k] = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
+            blat[k] = now_ms();
+            if (bst[k] != ASDM_OK) {
+                bc[k] = 0;
+                if (!(qstatus && per_query(bst[k]))) rc = bst[k];
+            }
+        }
         for (size_t k = 0; k < bcs.size(); ++k) {
             counts[bidx[k]] = bc[k];
             if (qstatus) qstatus[bidx[k]] = bst[k];
+            if (lat_ms) lat_ms[bidx[k]] = blat[k];
         }
     }
     return rc;

@@ -128,16 +128,24 @@ def admit(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: int = 40,
     return out
 
 
-def admit_batched(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: int, log=print) -> list:
+def admit_batched(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: int, log=print,
+                  guard_ms: float = 2000, state: Optional[dict] = None, stop_after_s: float = 1e9) -> list:
     """The admission rule for a many-query configuration, evaluated with the batched engine: draw j
-    of every slot not yet admitted is counted in one asdm_batch_count_learned call (plain plans)
-    under the per-query row limit on materialized partial embeddings; a slot admits its first draw
-    that stays within the limit.  Returns [(slot index, seed, count)]."""
+    of every slot not yet admitted is counted in one asdm_batch_count_learned call (plain plans);
+    a slot admits its first draw that the call counts, i.e. one that stays within the per-query
+    row limit on materialized partial embeddings (frontier engine) or, for a query the planner
+    sends to the bag-table engine, within guard_ms.  Returns [(slot index, seed, count, engine)].
+
+    state (optional) carries the progress between calls: {"next_draw": j, "got": [...]}; the call
+    returns after the draw round that passes stop_after_s, with state updated."""
     sl = slots(cfg)
-    got = {}
+    state = state if state is not None else {}
+    got = {int(x[0]): tuple(x) for x in state.get("got", [])}
+    t_start = time.perf_counter()
     ctx.set_row_limit(row_limit)
+    ctx.set_time_limit(guard_ms)
     try:
-        for j in range(max_draws):
+        for j in range(state.get("next_draw", 0), max_draws):
             todo, qs = [], []
             for i, s in enumerate(sl):
                 if i in got:
@@ -151,13 +159,22 @@ def admit_batched(ctx: Context, g: Graph, cfg: str, row_limit: int, max_draws: i
                 break
             t0 = time.perf_counter()
             counts, st, _ = ctx.batch_count_learned(g, qs)
-            for i, c, code in zip(todo, counts, st):
+            nbag = 0
+            for i, q, c, code in zip(todo, qs, counts, st):
                 if code == 0:
-                    got[i] = (i, draw_seed(cfg, i, j), int(c))
-            log(f"  {cfg} draw {j}: {len(qs)} slots tried, {int((st == 0).sum())} admitted, "
-                f"{len(got)} in total, {time.perf_counter() - t0:.1f} s")
+                    cs = ctx.filter(q, g)
+                    engine = "bags" if cs.plan(q).uses_bags else "frontier"
+                    nbag += engine == "bags"
+                    got[i] = (i, draw_seed(cfg, i, j), int(c), engine)
+            log(f"  {cfg} draw {j}: {len(qs)} slots tried, {int((st == 0).sum())} admitted ({nbag} by the bag "
+                f"engine), {len(got)} in total, {time.perf_counter() - t0:.1f} s")
+            state["next_draw"] = j + 1
+            state["got"] = [list(got[i]) for i in sorted(got)]
+            if time.perf_counter() - t_start > stop_after_s:
+                break
     finally:
         ctx.set_row_limit(0)
+        ctx.set_time_limit(0)
     return [got[i] for i in sorted(got)]
 
 
@@ -178,8 +195,10 @@ MAX_DRAWS = {"c1": 4, "c2": 30, "c3": 40, "c4": 8}
 BATCHED_RULE = {"c4"}  # configurations admitted with the batched engine (many query slots)
 
 
-def build_manifest(cfgs=("c1", "c2", "c3", "c4"), log=print) -> dict:
-    """Runs the admission rule on the GPU and writes workload_manifest.json."""
+def build_manifest(cfgs=("c1", "c2", "c3", "c4"), log=print, resume: bool = False, stop_after_s: float = 1e9) -> dict:
+    """Runs the admission rule on the GPU and writes workload_manifest.json.  With resume, a
+    batched configuration continues from the draw round its manifest entry's "progress" records
+    (the admission of a large batch may span several runs, each stopping after stop_after_s)."""
     m = load_manifest() if os.path.exists(MANIFEST) else {}
     ctx = Context(0)
     for cfg in cfgs:
@@ -187,8 +206,18 @@ def build_manifest(cfgs=("c1", "c2", "c3", "c4"), log=print) -> dict:
         ctx.upload(g)
         local = cfg == "c3"
         if cfg in BATCHED_RULE:
-            q = admit_batched(ctx, g, cfg, ROW_LIMITS[cfg], MAX_DRAWS[cfg], log=log)
-            engines = ["frontier"] * len(q)
+            state = dict(m.get(cfg, {}).get("progress", {})) if resume else {}
+            got = admit_batched(ctx, g, cfg, ROW_LIMITS[cfg], MAX_DRAWS[cfg], log=log, state=state,
+                                stop_after_s=stop_after_s)
+            q = [x[:3] for x in got]
+            engines = [x[3] for x in got]
+            m[cfg] = {"row_limit": ROW_LIMITS[cfg], "local_plan": local, "queries": q, "engines": engines,
+                      "slots": len(slots(cfg)), "rule": "batched", "progress": state,
+                      "complete": state.get("next_draw", 0) >= MAX_DRAWS[cfg]}
+            with open(MANIFEST, "w") as f:
+                json.dump(m, f, indent=1)
+            del g
+            continue
         else:
             got = admit(ctx, g, cfg, ROW_LIMITS[cfg], max_draws=MAX_DRAWS[cfg], local=local, log=log)
             q = [x[:3] for x in got]
@@ -205,4 +234,8 @@ def build_manifest(cfgs=("c1", "c2", "c3", "c4"), log=print) -> dict:
 if __name__ == "__main__":
     import sys
 
-    build_manifest(tuple(sys.argv[1:]) or ("c1", "c2", "c3", "c4"), log=lambda *a: print(*a, flush=True))
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    resume = "--resume" in sys.argv
+    stop = float(next((a.split("=")[1] for a in sys.argv if a.startswith("--stop-after=")), 1e9))
+    build_manifest(tuple(args) or ("c1", "c2", "c3", "c4"), log=lambda *a: print(*a, flush=True), resume=resume,
+                   stop_after_s=stop)

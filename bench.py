@@ -2,23 +2,25 @@
 """Benchmark of the matching hot path on one MI355X.
 
 Default workload (config 4, the batched multi-query configuration): seeded R-MAT graph, scale 20,
-edge factor 10, (a, b, c) = (.57, .19, .19), deduplicated, 16 uniform labels, resident in HBM; the
-admitted random-walk queries of the config-4 query stream (asdmatch_amd/workload_manifest.json;
-admission rule in asdmatch_amd/workloads.py).  One step = every admitted query filtered, planned
-and counted exactly.  Timed side by side:
+edge factor 10, (a, b, c) = (.57, .19, .19), deduplicated, 16 uniform labels, resident in HBM; every
+admitted random-walk query of the 1,024-slot config-4 query stream (asdmatch_amd/workload_manifest.json,
+admission rule in asdmatch_amd/workloads.py; the JSON line states the admission per size class).
+One step = every admitted query filtered, planned and counted exactly.  Timed side by side:
   - plain GPU drop-in: one filter + plan + count call sequence per query (asdm_filter_cfl,
     asdm_plan_create, asdm_count), as the reference's main.cpp runs a query;
   - batched + learned plans (the value): one asdm_batch_count_learned call per step, plans
-    ordered by the trained estimator (asdmatch_amd/data/mpnn_estimator.bin).
+    ordered by the trained estimator (asdmatch_amd/data/mpnn_estimator.bin); queries the planner
+    sends to the bag-table engine are counted by it inside the call.
 Every count is checked against the manifest's exact count.
 
-Prints one JSON line: queries/s (value), embeddings/s, the roofline of the dominant kernel (its
-algorithmic bytes over its HIP-event-timed launches in the timed region) and the reference CPU
-path timed on this host on a bounded sample of the same workload.
+--workload c3: config 3's admitted 12-vertex query on the Chung-Lu 200k / 4M graph, full local-count
+matrix (LOCAL_COUNT) per step.  --workload c2: the admitted config-2 queries one by one.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--workload c4|c2]
+Prints one JSON line: queries/s (value), the roofline of the dominant kernel (its algorithmic bytes
+over its HIP-event-timed launches in the timed region) and the reference CPU path timed on this
+host's cores on a bounded sample of the same workload.  Progress goes to stderr.
 
---workload c2 runs the config-2 queries (Chung-Lu 1M / 15M) one by one through the plain path.
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--workload c4|c3|c2]
 """
 import argparse
 import json
@@ -28,56 +30,112 @@ import subprocess
 import sys
 import tempfile
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ESTIMATOR = os.path.join(ROOT, "asdmatch_amd", "data", "mpnn_estimator.bin")
-REF_LOCAL = os.path.join(ROOT, "oracle", "_ref", "ref_driver_local")
-TRAFFIC_FILES = {"c2": os.path.join(ROOT, "profiles", "c2_traffic.txt"),
-                 "c4": os.path.join(ROOT, "profiles", "c4_traffic.txt")}
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+TRAFFIC_FILES = {w: os.path.join(ROOT, "profiles", f"{w}_traffic.txt") for w in ("c2", "c3", "c4")}
+METRIC = "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts"
 
 
-def start_cpu_baseline(g, q, tmpdir):
-    """Reference ALL_LEVEL build (its exact path), one thread, on one admitted config-2 query."""
-    if not os.path.exists(REF_LOCAL):
-        return None
-    dpath = os.path.join(tmpdir, "c2.graph")
-    qpath = os.path.join(tmpdir, "c2_query.graph")
-    g.write(dpath)
-    q.write(qpath)
-    return subprocess.Popen([REF_LOCAL, "-q", qpath, "-d", dpath], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                            text=True)
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
-def finish_cpu_baseline(proc, expect, timeout):
-    if proc is None:
-        return None
-    try:
-        out, _ = proc.communicate(timeout=timeout)
-    except subprocess.TimeoutExpired:
-        proc.kill()
-        proc.communicate()
-        return {"error": f"reference did not finish within {timeout} s"}
+# ------------------------------------------------------------------ reference CPU path
+def parse_ref(out):
     vals = {}
     for key, pat in (("filter", r"Filtering time \(s\): ([0-9.eE+-]+)"), ("plan", r"Planning Time: ([0-9.eE+-]+)"),
                      ("exec", r"Execution Time: ([0-9.eE+-]+)"), ("count", r"Number of matches: (\d+)")):
         m = re.search(pat, out)
         if m:
             vals[key] = float(m.group(1)) if key != "count" else int(m.group(1))
-    if "exec" not in vals:
-        return {"error": "reference output not parsed", "tail": out[-300:]}
-    secs = vals.get("filter", 0) + vals.get("plan", 0) + vals["exec"]
-    return {"value": 1.0 / secs, "unit": "queries/s", "cores": 1, "kind": "reference",
-            "seconds": secs, "count": vals.get("count"), "count_matches_gpu": vals.get("count") == expect,
-            "embeddings_per_s": (vals.get("count") or 0) / secs}
+    return vals
 
 
-def read_traffic(kernel, workload="c2"):
-    """Calibrated HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
-    path = TRAFFIC_FILES[workload]
-    if not os.path.exists(path):
+def run_reference_pool(binary, jobs, dpath, workers, timeout):
+    """Runs the reference once per (query path, expected count) job, `workers` processes at a time,
+    as `xargs -P workers` would.  Per job: the reference's own Filtering + Planning + Execution
+    seconds (graph load excluded) and its count.  Returns (results, wall-clock makespan)."""
+    exe = os.path.join(REF_BIN, binary)
+
+    def one(job):
+        qpath, expect = job
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run([exe, "-q", qpath, "-d", dpath], capture_output=True, text=True, timeout=timeout)
+            v = parse_ref(r.stdout)
+        except subprocess.TimeoutExpired:
+            v = {}
+        wall = time.perf_counter() - t0
+        secs = v.get("filter", 0) + v.get("plan", 0) + v["exec"] if "exec" in v else None
+        return {"seconds": secs, "wall": wall, "count": v.get("count"), "expect": expect}
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        res = list(ex.map(one, jobs))
+    return res, time.perf_counter() - t0
+
+
+def compute_makespan(res, workers):
+    """Makespan of the jobs' compute seconds (graph load excluded) on `workers` cores, scheduled in
+    the pool's order (each job to the first free worker)."""
+    free = [0.0] * workers
+    for r in res:
+        k = min(range(workers), key=lambda i: free[i])
+        free[k] += r["seconds"] or 0.0
+    return max(free)
+
+
+def cpu_baseline_sample(g, sample, tmpdir, workers, timeout, label):
+    """Both reference builds on the same sample: the ALL_LEVEL + LOCAL_COUNT build (exact
+    enumeration; oracle/_ref/ref_driver_local) and the default count build (factorized counting;
+    oracle/_ref/ref_driver, whose multi-bag counts are known to differ).  Throughput = sample size
+    over the compute makespan on `workers` cores."""
+    if not os.path.exists(os.path.join(REF_BIN, "ref_driver_local")) or not sample:
+        return None
+    dpath = os.path.join(tmpdir, "data.graph")
+    g.write(dpath)
+    jobs = []
+    for k, (q, expect) in enumerate(sample):
+        qpath = os.path.join(tmpdir, f"query_{k}.graph")
+        q.write(qpath)
+        jobs.append((qpath, expect))
+    say(f"cpu baseline: {len(jobs)} queries, {workers} processes, ALL_LEVEL build")
+    exact, wall_e = run_reference_pool("ref_driver_local", jobs, dpath, workers, timeout)
+    say(f"cpu baseline: count build")
+    cbuild, wall_c = run_reference_pool("ref_driver", jobs, dpath, workers, timeout)
+    done = [r for r in exact if r["seconds"] is not None]
+    if len(done) < len(exact):
+        return {"error": f"{len(exact) - len(done)} of {len(exact)} reference runs did not finish in {timeout} s"}
+    ms = compute_makespan(exact, workers)
+    msc = compute_makespan(cbuild, workers) if all(r["seconds"] is not None for r in cbuild) else None
+    return {
+        "value": round(len(exact) / ms, 4), "unit": "queries/s", "cores": workers, "kind": "reference",
+        "sample": label + f"; reference ALL_LEVEL build (exact enumeration), one single-threaded process per query, "
+                          f"{workers} at a time; throughput = queries / makespan of their Filtering + Planning + "
+                          "Execution seconds (graph load excluded, as the GPU graph is resident)",
+        "queries": len(exact), "makespan_s": round(ms, 3), "wall_makespan_s_incl_graph_load": round(wall_e, 2),
+        "summed_query_s": round(sum(r["seconds"] for r in exact), 3),
+        "counts_match_gpu": all(r["count"] == r["expect"] for r in exact),
+        "count_build": {"queries_per_s": round(len(cbuild) / msc, 4) if msc else None,
+                        "makespan_s": round(msc, 3) if msc else None,
+                        "counts_equal_exact": sum(1 for r in cbuild if r["count"] == r["expect"]),
+                        "counts_differ": sum(1 for r in cbuild if r["count"] is not None and r["count"] != r["expect"]),
+                        "unfinished": sum(1 for r in cbuild if r["count"] is None)},
+    }
+
+
+# ------------------------------------------------------------------ roofline
+def read_traffic(kernel, workload):
+    """Calibrated HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, when
+    that summary was measured on this workload for this kernel."""
+    path = TRAFFIC_FILES.get(workload)
+    if not path or not os.path.exists(path):
         return None
     vals = {}
     with open(path) as f:
@@ -90,126 +148,9 @@ def read_traffic(kernel, workload="c2"):
     return float(vals["traffic_bytes_per_launch"])
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c4"), default="c4",
-                    help="c4: config-4 batch, batched + learned plans vs plain (default); c2: config-2 queries "
-                         "one by one through the plain path")
-    ap.add_argument("--batch-shape", default=None,
-                    help="LEAF_LANES,CLASS_SCALE,TILE_ITERS: use this batch launch shape instead of autotuning")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="only run a plain device copy of this many MiB (PMC calibration) and exit")
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: the matching path runs on a single MI355X here "
-              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    if args.copy_probe:
-        import torch
-
-        n = args.copy_probe << 20
-        x = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
-        y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": n, "launches": 5}))
-        return
-
-    import asdmatch_amd as A
-    from asdmatch_amd import workloads as W
-
-    if args.workload == "c4":
-        return bench_batch(args)
-
-    t0 = time.time()
-    g = W.graph("c2")
-    adm = W.admitted("c2")
-    queries = [(W.query(g, s, seed), seed, cnt) for s, seed, cnt in adm]
-    gen_s = time.time() - t0
-
-    tmpdir = tempfile.mkdtemp(prefix="asdm_bench_")
-    proc = None
-    if not args.no_cpu_baseline and queries:
-        sample = min(queries, key=lambda t: t[2])  # the smallest admitted query
-        proc = start_cpu_baseline(g, sample[0], tmpdir)
-
-    ctx = A.Context(0)
-    ctx.upload(g)
-
-    def step():
-        total = 0
-        for q, seed, expect in queries:
-            cs = ctx.filter(q, g)
-            plan = cs.plan(q)
-            c = ctx.count(plan, cs)
-            if c != expect:
-                raise SystemExit(f"count mismatch for query seed {seed}: {c} != {expect}")
-            total += c
-        return total
-
-    for _ in range(args.warmup):
-        step()
-    ctx.set_profiling(True)  # HIP events around every launch, resolved after the timed region
-    import ctypes
-
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipDeviceSynchronize()
-    t1 = time.perf_counter()
-    emb = 0
-    for _ in range(args.steps):
-        emb += step()
-    hip.hipDeviceSynchronize()
-    t2 = time.perf_counter()
-    kstats = ctx.kernel_stats()
-    ctx.set_profiling(False)
-
-    elapsed = t2 - t1
-    nq = len(queries) * args.steps
-    roofline = roofline_of(kstats, "c2")
-
-    cpu = finish_cpu_baseline(proc, min(queries, key=lambda t: t[2])[2] if queries else None, timeout=600)
-    if cpu and "value" in cpu:
-        cpu["sample"] = ("reference ALL_LEVEL build (exact enumeration path), single thread, on the smallest "
-                         "admitted config-2 query; time = its Filtering + Planning + Execution lines "
-                         "(graph load excluded, as the GPU graph is resident)")
-
-    out = {
-        "metric": "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts",
-        "value": round(nq / elapsed, 4),
-        "unit": "queries/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32 ids / u64 counts",
-        "data": "synthetic",
-        "config": {"workload": "config 2: Chung-Lu 1M V / 15M E, beta 2.5, 32 labels Zipf(1.2); admitted "
-                               "random-walk queries, plain GPU engine (filter + plan + exact count)",
-                   "queries_per_step": len(queries), "query_sizes": [q.n for q, _, _ in queries],
-                   "admission_row_limit": W.load_manifest()["c2"]["row_limit"]},
-        "embeddings_per_s": round(emb / elapsed, 1),
-        "embeddings_per_step": emb // max(1, args.steps),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats],
-        "setup_s": round(gen_s, 2),
-    }
-    print(json.dumps(out), flush=True)
-
-
 def roofline_of(kstats, workload):
     """Roofline object of the dominant kernel (most device time) of a kernel table."""
-    ks = [k for k in kstats if k["launches"] and k["ms"] > 0]
+    ks = [k for k in kstats if k["launches"] and k["ms"] > 0 and k["bytes"] > 0]
     if not ks:
         return None
     dom = max(ks, key=lambda k: k["ms"])
@@ -222,155 +163,332 @@ def roofline_of(kstats, workload):
             "avg_launch_ms": round(avg_s * 1e3, 4), "launches": dom["launches"]}
 
 
-def start_cpu_batch_baseline(g, sample, tmpdir):
-    """Reference ALL_LEVEL build (its exact path), one single-threaded process per sampled query,
-    the processes run side by side on the host cores."""
-    if not os.path.exists(REF_LOCAL) or not sample:
-        return None
-    dpath = os.path.join(tmpdir, "c4.graph")
-    g.write(dpath)
-    procs = []
-    for k, (q, _) in enumerate(sample):
-        qpath = os.path.join(tmpdir, f"c4_query_{k}.graph")
-        q.write(qpath)
-        procs.append(subprocess.Popen([REF_LOCAL, "-q", qpath, "-d", dpath], stdout=subprocess.PIPE,
-                                      stderr=subprocess.STDOUT, text=True))
-    return procs
+def kernel_table(kstats):
+    return [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats]
 
 
-def finish_cpu_batch_baseline(procs, sample, total_embeddings, nq, timeout):
-    if not procs:
-        return None
-    secs, ok, emb = 0.0, True, 0
-    for p, (_, expect) in zip(procs, sample):
-        r = finish_cpu_baseline(p, expect, timeout)
-        if not r or "value" not in r:
-            return r
-        secs += r["seconds"]
-        ok &= bool(r["count_matches_gpu"])
-        emb += expect
-    rate = emb / secs  # embeddings per second of the reference's exact path
-    return {"value": round(len(sample) / secs, 6), "unit": "queries/s", "cores": 1, "kind": "reference",
-            "seconds": round(secs, 3), "queries": len(sample), "counts_match_gpu": ok,
-            "embeddings_per_s": round(rate, 1),
-            "full_batch_queries_per_s_extrapolated": round(nq / (total_embeddings / rate), 9)}
+def hip_sync():
+    import ctypes
+
+    ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
 
 
+def base_line(args, value, ms_per_step, config):
+    return {"metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic", "config": config}
+
+
+# ------------------------------------------------------------------ config 4
 def bench_batch(args):
-    """Config-4 batch: the admitted queries counted (a) one by one through the plain path and
-    (b) in one asdm_batch_count_learned call per step with learned plans (the value)."""
     import asdmatch_amd as A
     from asdmatch_amd import workloads as W
 
     t0 = time.time()
     g = W.graph("c4")
-    adm = W.admitted("c4")
-    qs = [W.query(g, s, seed) for s, seed, _ in adm]
-    want = [c for _, _, c in adm]
+    man = W.load_manifest()["c4"]
+    sl = W.slots("c4")
+    qs = [W.query(g, sl[i], seed) for i, seed, _ in man["queries"]]
+    want = [c for _, _, c in man["queries"]]
+    engines = man.get("engines", ["frontier"] * len(qs))
+    if args.max_queries:
+        qs, want, engines = qs[: args.max_queries], want[: args.max_queries], engines[: args.max_queries]
+    fidx = [i for i, e in enumerate(engines) if e == "frontier"]
+    sizes = {}
+    for s in sl:
+        sizes.setdefault(s.k, [0, 0, 0])[0] += 1
+    for (i, _, _), e in zip(man["queries"], man.get("engines", [])):
+        sizes[sl[i].k][1] += 1
+        sizes[sl[i].k][2] += e == "bags"
     gen_s = time.time() - t0
-
-    # CPU baseline sample: the four admitted queries with the fewest embeddings (the reference
-    # enumerates every embedding; the whole batch would take it days), run while the GPU works
-    tmpdir = tempfile.mkdtemp(prefix="asdm_bench_")
-    order = sorted(range(len(qs)), key=lambda i: want[i])
-    sample = [(qs[i], want[i]) for i in order[:4]]
-    procs = None if args.no_cpu_baseline else start_cpu_batch_baseline(g, sample, tmpdir)
+    say(f"c4: graph and {len(qs)} admitted queries ({len(fidx)} frontier, {len(qs) - len(fidx)} bag engine) "
+        f"in {gen_s:.1f} s")
 
     ctx = A.Context(0)
     ctx.upload(g)
-    import ctypes
-
-    hip = ctypes.CDLL("libamdhip64.so")
     m = None
     if os.path.exists(ESTIMATOR):
         m = A.Mpnn(ctx)
         m.load(ESTIMATOR)
 
-    def plain():
+    def plain(idx):
         out = []
-        for q in qs:
-            cs = ctx.filter(q, g)
-            out.append(0 if cs is None else ctx.count(cs.plan(q), cs))
+        for i in idx:
+            cs = ctx.filter(qs[i], g)
+            out.append(0 if cs is None else ctx.count(cs.plan(qs[i]), cs))
         return out
 
-    def batched(mm):
-        c, st, lat = ctx.batch_count_learned(g, qs, mm)
-        if [int(x) for x in c] != want or any(st):
+    def batched(idx, mm):
+        c, st, lat = ctx.batch_count_learned(g, [qs[i] for i in idx], mm)
+        if [int(x) for x in c] != [want[i] for i in idx] or any(st):
             raise SystemExit("batched count mismatch")
         return lat
 
-    for _ in range(args.warmup):
-        if plain() != want:
-            raise SystemExit("plain count mismatch")
-        batched(None)
-        batched(m)
-    # launch shape of the batched engine, picked by the library's autotuner (untimed) on every fourth
-    # query of the batch, which keeps the tuner's 36 shapes to seconds
+    every = list(range(len(qs)))
+    for w in range(args.warmup):
+        say(f"warmup step {w + 1}/{args.warmup}: batched + learned plans")
+        batched(every, m)
+    # launch shape of the batched engine, picked by the library's autotuner (untimed) on every
+    # fourth frontier query, which keeps the tuner's 36 shapes to seconds
+    fq = [qs[i] for i in fidx]
     if args.batch_shape:
         lanes, scale, iters = args.batch_shape.split(",")
         ctx.set_batch_tuning(int(lanes), float(scale), int(iters))
         tune = ctx.batch_tuning() + (None,)
-    elif args.warmup:
-        tune = ctx.batch_autotune(g, qs[::4], m, reps=1)
+    elif args.warmup and fq:
+        say("autotuning the batch launch shape")
+        tune = ctx.batch_autotune(g, fq[::4], m, reps=1)
     else:
         tune = ctx.batch_tuning() + (None,)
-    hip.hipDeviceSynchronize()
+
+    # frontier-admitted subset: plain vs batched (plain plans) vs batched (learned plans)
+    sub = {}
+    if fidx:
+        say("frontier subset: plain, batched, batched + learned")
+        reps = max(3, args.steps)
+        hip_sync()
+        a = time.perf_counter()
+        for _ in range(reps):
+            if plain(fidx) != [want[i] for i in fidx]:
+                raise SystemExit("plain count mismatch")
+        hip_sync()
+        b = time.perf_counter()
+        for _ in range(reps):
+            batched(fidx, None)
+        hip_sync()
+        c = time.perf_counter()
+        for _ in range(reps):
+            batched(fidx, m)
+        hip_sync()
+        d = time.perf_counter()
+        n = len(fidx) * reps
+        sub = {"queries": len(fidx), "steps": reps, "plain_gpu_queries_per_s": round(n / (b - a), 3),
+               "batched_plain_plan_queries_per_s": round(n / (c - b), 3),
+               "batched_learned_plan_queries_per_s": round(n / (d - c), 3),
+               "batched_learned_vs_plain": round((b - a) / (d - c), 3),
+               "embeddings_per_s": round(sum(want[i] for i in fidx) * reps / (d - c), 1)}
+
+    # the whole admitted batch: plain, then batched + learned plans (the value, profiled)
+    say(f"timed: plain path over {len(qs)} queries x {args.steps}")
+    hip_sync()
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        got = plain()
-    hip.hipDeviceSynchronize()
+        got = plain(every)
+    hip_sync()
     t2 = time.perf_counter()
     if got != want:
         raise SystemExit("plain count mismatch")
-    for _ in range(args.steps):
-        batched(None)
-    hip.hipDeviceSynchronize()
-    t3 = time.perf_counter()
-    # the value: batched + learned plans, HIP events around every launch (resolved afterwards)
+    say(f"timed: batched + learned plans over {len(qs)} queries x {args.steps}")
     ctx.set_profiling(True)
     lats = []
-    hip.hipDeviceSynchronize()
+    hip_sync()
     t4 = time.perf_counter()
     for _ in range(args.steps):
-        lats.extend(batched(m).tolist())
-    hip.hipDeviceSynchronize()
+        lats.extend(batched(every, m).tolist())
+    hip_sync()
     t5 = time.perf_counter()
     kstats = ctx.kernel_stats()
     ctx.set_profiling(False)
-
     nq = len(qs) * args.steps
-    plain_qps, batch_qps, learned_qps = nq / (t2 - t1), nq / (t3 - t2), nq / (t5 - t4)
+    plain_qps, learned_qps = nq / (t2 - t1), nq / (t5 - t4)
     lats.sort()
-    cpu = finish_cpu_batch_baseline(procs, sample, sum(want), len(qs), timeout=600)
-    if cpu and "value" in cpu:
-        cpu["sample"] = ("reference ALL_LEVEL build (exact enumeration path), one thread per query, on the four "
-                         "admitted config-4 queries with the fewest embeddings; time = the sum of their "
-                         "Filtering + Planning + Execution lines (graph load excluded, as the GPU graph is "
-                         "resident); the extrapolation divides the batch's embeddings by the sample's rate")
-    out = {
-        "metric": "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts",
-        "value": round(learned_qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round((t5 - t4) / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u32 ids / u64 counts", "data": "synthetic",
-        "config": {"workload": "config 4: R-MAT scale 20, edge factor 10, (a,b,c)=(.57,.19,.19), 16 uniform "
-                               "labels; admitted random-walk queries, one asdm_batch_count_learned call per step",
-                   "queries_per_step": len(qs), "query_sizes": sorted({q.n for q in qs}),
-                   "admission_row_limit": W.load_manifest()["c4"]["row_limit"],
-                   "learned_plans": m is not None,
-                   "batch_launch_shape": {"leaf_lanes": tune[0], "class_scale": tune[1], "tile_iters": tune[2],
-                                          "autotuned_on": "every fourth query" if tune[3] is not None else None}},
-        "plain_gpu_queries_per_s": round(plain_qps, 3),
-        "batched_plain_plan_queries_per_s": round(batch_qps, 3),
-        "batched_learned_plan_queries_per_s": round(learned_qps, 3),
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the 2P admitted queries with the fewest embeddings (the reference enumerates every one)
+        workers = min(16, os.cpu_count() or 1)
+        order = sorted(fidx, key=lambda i: want[i])[: 2 * workers]
+        cpu = cpu_baseline_sample(g, [(qs[i], want[i]) for i in order], tempfile.mkdtemp(prefix="asdm_bench_"),
+                                  workers, 600, f"the {len(order)} admitted config-4 queries with the fewest embeddings")
+    admitted = {str(k): {"slots": v[0], "admitted": v[1], "bag_engine": v[2]} for k, v in sorted(sizes.items())}
+    out = base_line(args, round(learned_qps, 4), round((t5 - t4) / args.steps * 1e3, 3), {
+        "workload": "config 4: R-MAT scale 20, edge factor 10, (a,b,c)=(.57,.19,.19), 16 uniform labels; every admitted "
+                    "random-walk query of the 1,024 slots, one asdm_batch_count_learned call per step",
+        "slots_total": len(sl), "queries_per_step": len(qs),
+        "admitted_fraction": round(len(man["queries"]) / len(sl), 4),
+        "admitted_by_size": admitted,
+        "not_admitted_note": "slots whose draws are neither within the frontier row limit nor countable by the "
+                             "bag-table engine (a quotient of treewidth > 2, more than 2^14 partition terms, or over "
+                             "the 2 s guard); sizes 21-32 have none admitted",
+        "admission_row_limit": man["row_limit"], "learned_plans": m is not None,
+        "batch_launch_shape": {"leaf_lanes": tune[0], "class_scale": tune[1], "tile_iters": tune[2],
+                               "autotuned_on": "every fourth frontier query" if tune[3] is not None else None}})
+    out.update({
+        "plain_gpu_queries_per_s": round(plain_qps, 4),
+        "batched_learned_plan_queries_per_s": round(learned_qps, 4),
         "batched_learned_vs_plain": round(learned_qps / plain_qps, 3),
-        "embeddings_per_s": round(sum(want) * args.steps / (t5 - t4), 1),
+        "embeddings_per_s": None,
+        "embeddings_note": "counts are exact mod 2^64 like the reference's size_t; bag-engine queries have far more "
+                           "than 2^64 embeddings, so their counts are residues and no embedding rate is summed over "
+                           "them (see frontier_subset.embeddings_per_s)",
+        "frontier_subset": sub,
         "latency_ms_p50": round(lats[len(lats) // 2], 3), "latency_ms_p99": round(lats[int(len(lats) * 0.99)], 3),
-        "roofline": roofline_of(kstats, "c4"),
-        "cpu_baseline": cpu,
-        "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats],
-        "setup_s": round(gen_s, 2),
-    }
+        "latency_note": "per query, from the call start to the host synchronization after its last level",
+        "roofline": roofline_of(kstats, "c4"), "cpu_baseline": cpu, "kernels": kernel_table(kstats),
+        "setup_s": round(gen_s, 2)})
     print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------------ config 3
+def bench_local(args):
+    import numpy as np
+
+    import asdmatch_amd as A
+    from asdmatch_amd import workloads as W
+
+    t0 = time.time()
+    g = W.graph("c3")
+    (slot, seed, count), = [tuple(x) for x in W.load_manifest()["c3"]["queries"]]
+    q = W.query(g, W.slots("c3")[slot], seed)
+    gen_s = time.time() - t0
+    ctx = A.Context(0)
+    ctx.upload(g)
+
+    def step():
+        cs = ctx.filter(q, g)
+        plan = cs.plan(q, local=True)
+        mat, c = ctx.local_count(plan, cs)
+        if c != count:
+            raise SystemExit(f"count mismatch: {c} != {count}")
+        return mat, plan.uses_bags
+
+    for w in range(args.warmup):
+        say(f"warmup step {w + 1}/{args.warmup}")
+        mat, bags = step()
+        if not all(int(s) == count for s in mat.sum(axis=0, dtype=np.uint64)):
+            raise SystemExit("column sums differ from the count")
+    say(f"timed: {args.steps} local counts")
+    ctx.set_profiling(True)
+    hip_sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        mat, bags = step()
+    hip_sync()
+    t2 = time.perf_counter()
+    kstats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    secs = (t2 - t1) / args.steps
+    cpu = None
+    if not args.no_cpu_baseline and os.path.exists(os.path.join(REF_BIN, "ref_driver_local")):
+        tmpdir = tempfile.mkdtemp(prefix="asdm_bench_")
+        dpath, qpath, csv = (os.path.join(tmpdir, n) for n in ("c3.graph", "c3_query.graph", "c3.csv"))
+        g.write(dpath)
+        q.write(qpath)
+        limit = 30
+        say(f"cpu baseline: reference LOCAL_COUNT build on the same query, {limit} s limit")
+        t3 = time.perf_counter()
+        try:
+            r = subprocess.run([os.path.join(REF_BIN, "ref_driver_local"), "-q", qpath, "-d", dpath, "-c", csv],
+                               capture_output=True, text=True, timeout=limit + 60)
+            v = parse_ref(r.stdout)
+            finished = "exec" in v
+        except subprocess.TimeoutExpired:
+            finished, v = False, {}
+        wall = time.perf_counter() - t3
+        cpu = {"value": round(1.0 / limit, 4) if not finished else round(1.0 / (v["filter"] + v["plan"] + v["exec"]), 4),
+               "unit": "queries/s", "cores": 1, "kind": "reference", "bound": "upper" if not finished else "measured",
+               "sample": f"the same 12-vertex query and graph, reference ALL_LEVEL + LOCAL_COUNT build, one thread, "
+                         f"{'stopped after a ' + str(limit) + ' s limit (the value is the upper bound 1/limit: it enumerates the ' + str(count) + ' embeddings one by one)' if not finished else 'finished'}",
+               "wall_s": round(wall, 1)}
+    out = base_line(args, round(1.0 / secs, 4), round(secs * 1e3, 3), {
+        "workload": "config 3: Chung-Lu 200,000 V / 4,000,000 E, beta 2.5, 16 uniform labels; the admitted 12-vertex "
+                    "random-walk query (first draw of the seed sequence), full |Vg| x |Vq| LOCAL_COUNT matrix per step",
+        "query_vertices": q.n, "query_edges": int(len(q.edges())), "engine": "bags" if bags else "frontier",
+        "slots_total": 1, "admitted": 1})
+    out.update({"count": count, "embeddings_per_s": round(count / secs, 1),
+                "matrix": {"rows": g.n, "cols": q.n, "column_sums_equal_count": True},
+                "roofline": roofline_of(kstats, "c3"), "cpu_baseline": cpu, "kernels": kernel_table(kstats),
+                "setup_s": round(gen_s, 2)})
+    print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------------ config 2
+def bench_c2(args):
+    import asdmatch_amd as A
+    from asdmatch_amd import workloads as W
+
+    t0 = time.time()
+    g = W.graph("c2")
+    adm = W.admitted("c2")
+    queries = [(W.query(g, s, seed), seed, cnt) for s, seed, cnt in adm]
+    gen_s = time.time() - t0
+    ctx = A.Context(0)
+    ctx.upload(g)
+
+    def step():
+        total = 0
+        for q, seed, expect in queries:
+            cs = ctx.filter(q, g)
+            c = ctx.count(cs.plan(q), cs)
+            if c != expect:
+                raise SystemExit(f"count mismatch for query seed {seed}: {c} != {expect}")
+            total += c
+        return total
+
+    for w in range(args.warmup):
+        say(f"warmup step {w + 1}/{args.warmup}")
+        step()
+    say(f"timed: {args.steps} steps of {len(queries)} queries")
+    ctx.set_profiling(True)
+    hip_sync()
+    t1 = time.perf_counter()
+    emb = 0
+    for _ in range(args.steps):
+        emb += step()
+    hip_sync()
+    t2 = time.perf_counter()
+    kstats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    elapsed = t2 - t1
+    cpu = None
+    if not args.no_cpu_baseline and queries:
+        smallest = min(queries, key=lambda t: t[2])
+        cpu = cpu_baseline_sample(g, [(smallest[0], smallest[2])], tempfile.mkdtemp(prefix="asdm_bench_"), 1, 900,
+                                  "the admitted config-2 query with the fewest embeddings")
+    man = W.load_manifest()["c2"]
+    out = base_line(args, round(len(queries) * args.steps / elapsed, 4), round(elapsed / args.steps * 1e3, 3), {
+        "workload": "config 2: Chung-Lu 1M V / 15M E, beta 2.5, 32 labels Zipf(1.2); admitted random-walk queries, "
+                    "plain GPU engine (filter + plan + exact count)",
+        "slots_total": 16, "queries_per_step": len(queries), "query_sizes": [q.n for q, _, _ in queries],
+        "admission_row_limit": man["row_limit"]})
+    out.update({"embeddings_per_s": round(emb / elapsed, 1), "embeddings_per_step": emb // max(1, args.steps),
+                "roofline": roofline_of(kstats, "c2"), "cpu_baseline": cpu, "kernels": kernel_table(kstats),
+                "setup_s": round(gen_s, 2)})
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c4",
+                    help="c4: config-4 batch, batched + learned plans vs plain (default); c3: config-3 local counts; "
+                         "c2: config-2 queries one by one through the plain path")
+    ap.add_argument("--batch-shape", default=None,
+                    help="LEAF_LANES,CLASS_SCALE,TILE_ITERS: use this batch launch shape instead of autotuning")
+    ap.add_argument("--max-queries", type=int, default=0, help="c4 only: the first N admitted queries (profiling)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (PMC calibration) and exit")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the matching path runs on a single MI355X here "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe << 20
+        x = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 5}))
+        return
+    {"c4": bench_batch, "c3": bench_local, "c2": bench_c2}[args.workload](args)
 
 
 if __name__ == "__main__":
