@@ -203,11 +203,14 @@ __global__ void __launch_bounds__(kThreads) k_ee(const uint64_t *__restrict__ xo
 }
 
 // G[x][y] = sum_{j in R(x)} u[j] F[j][y]: an edge relation (rows x, columns j) times a dense table
-// whose rows are the relation's columns.  One workgroup per row x: the row's list and its weights
-// u[j] are staged in LDS in chunks; each thread owns two adjacent columns (16-byte loads when the
-// rows are 16-byte aligned) and keeps four list entries' loads in flight.  Chunks after the first
-// add onto the row written by the one before (same thread, same columns).
+// whose rows are the relation's columns.  Work items are (column tile, row x) pairs in tile-major
+// order, so the workgroups in flight read the same 2 * kThreads-column strip of F and its re-reads
+// hit L2 and the MALL rather than HBM.  A workgroup stages the row's list and its weights u[j] in
+// LDS (in chunks); each thread owns two columns of the strip (one 16-byte load when rows are
+// 16-byte aligned) and keeps four list entries' loads in flight.  Chunks after the first add onto
+// the values the previous chunk wrote (same thread, same columns).
 constexpr uint32_t kEdChunk = 1024;
+constexpr uint32_t kEdTile = 2 * kThreads;
 __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ u, const uint64_t *__restrict__ F,
                                                  uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
@@ -215,57 +218,58 @@ __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xo
     __shared__ uint32_t lj[kEdChunk];
     __shared__ uint64_t lw[kEdChunk];
     const bool pairs = (ny & 1u) == 0;
-    const uint32_t ncol = pairs ? ny / 2 : ny;
+    const uint64_t tiles = (ny + kEdTile - 1) / kEdTile;
+    const uint64_t items = tiles * nx;
     uint64_t bytes = 0;
-    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
+    for (uint64_t b = blockIdx.x; b < items; b += gridDim.x) {
+        const uint32_t tile = (uint32_t)(b / nx), x = (uint32_t)(b % nx);
+        const uint32_t c0 = tile * kEdTile;
+        // pairs: columns c0 + 2t, c0 + 2t + 1; otherwise columns c0 + t and c0 + t + kThreads
+        const uint32_t ya = pairs ? c0 + 2 * threadIdx.x : c0 + threadIdx.x;
+        const uint32_t yb = pairs ? ya + 1 : ya + kThreads;
+        const bool va = ya < ny, vb = yb < ny;
         const uint64_t lb = xoff[x], le = xoff[x + 1];
         uint64_t *grow = G + (uint64_t)x * ny;
-        for (uint64_t c0 = lb; c0 == lb || c0 < le; c0 += kEdChunk) {
-            const uint32_t cn = (uint32_t)(le - c0 < kEdChunk ? le - c0 : kEdChunk);
-            const bool first = c0 == lb;
+        uint64_t a0 = 0, a1 = 0;
+        for (uint64_t k0 = lb; k0 == lb || k0 < le; k0 += kEdChunk) {
+            const uint32_t cn = (uint32_t)(le - k0 < kEdChunk ? le - k0 : kEdChunk);
             __syncthreads();
             for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
-                const uint32_t jj = xval[c0 + t];
+                const uint32_t jj = xval[k0 + t];
                 lj[t] = jj;
                 lw[t] = u[jj];
             }
             __syncthreads();
-            for (uint32_t c = threadIdx.x; c < ncol; c += blockDim.x) {
-                if (pairs) {
-                    uint64_t a0 = 0, a1 = 0;
-                    uint32_t e = 0;
-                    for (; e + 4 <= cn; e += 4) {
-                        const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + 2 * c);
-                        const ulonglong2 f1 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 1] * ny + 2 * c);
-                        const ulonglong2 f2 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 2] * ny + 2 * c);
-                        const ulonglong2 f3 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 3] * ny + 2 * c);
-                        a0 += lw[e] * f0.x + lw[e + 1] * f1.x + lw[e + 2] * f2.x + lw[e + 3] * f3.x;
-                        a1 += lw[e] * f0.y + lw[e + 1] * f1.y + lw[e + 2] * f2.y + lw[e + 3] * f3.y;
-                    }
-                    for (; e < cn; ++e) {
-                        const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + 2 * c);
-                        a0 += lw[e] * f0.x;
-                        a1 += lw[e] * f0.y;
-                    }
-                    ulonglong2 *dst = (ulonglong2 *)(grow + 2 * c);
-                    if (!first) {
-                        const ulonglong2 o = *dst;
-                        a0 += o.x;
-                        a1 += o.y;
-                    }
-                    *dst = make_ulonglong2(a0, a1);
-                } else {
-                    uint64_t a = 0;
-                    uint32_t e = 0;
-                    for (; e + 4 <= cn; e += 4)
-                        a += lw[e] * F[(uint64_t)lj[e] * ny + c] + lw[e + 1] * F[(uint64_t)lj[e + 1] * ny + c] +
-                             lw[e + 2] * F[(uint64_t)lj[e + 2] * ny + c] + lw[e + 3] * F[(uint64_t)lj[e + 3] * ny + c];
-                    for (; e < cn; ++e) a += lw[e] * F[(uint64_t)lj[e] * ny + c];
-                    grow[c] = first ? a : grow[c] + a;
+            if (pairs && va) {
+                uint32_t e = 0;
+                for (; e + 4 <= cn; e += 4) {
+                    const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + ya);
+                    const ulonglong2 f1 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 1] * ny + ya);
+                    const ulonglong2 f2 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 2] * ny + ya);
+                    const ulonglong2 f3 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 3] * ny + ya);
+                    a0 += lw[e] * f0.x + lw[e + 1] * f1.x + lw[e + 2] * f2.x + lw[e + 3] * f3.x;
+                    a1 += lw[e] * f0.y + lw[e + 1] * f1.y + lw[e + 2] * f2.y + lw[e + 3] * f3.y;
+                }
+                for (; e < cn; ++e) {
+                    const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + ya);
+                    a0 += lw[e] * f0.x;
+                    a1 += lw[e] * f0.y;
+                }
+            } else if (!pairs) {
+                for (uint32_t e = 0; e < cn; ++e) {
+                    const uint64_t *fr = F + (uint64_t)lj[e] * ny;
+                    if (va) a0 += lw[e] * fr[ya];
+                    if (vb) a1 += lw[e] * fr[yb];
                 }
             }
         }
-        if (threadIdx.x == 0) bytes += 16 + (le - lb) * (12 + 8ull * ny) + 8ull * ny;
+        if (pairs && va) {
+            *(ulonglong2 *)(grow + ya) = make_ulonglong2(a0, a1);
+        } else if (!pairs) {
+            if (va) grow[ya] = a0;
+            if (vb) grow[yb] = a1;
+        }
+        if (threadIdx.x == 0) bytes += 16 + (le - lb) * (12 + 8ull * (ny - c0 < kEdTile ? ny - c0 : kEdTile)) + 8ull * kEdTile;
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -634,8 +638,8 @@ struct BagRun {
                                                    : (fvy.first == 1 ? "bag_table_de" : "bag_table_dd");
                 if (fxv.first == 1 && fvy.first == 1) {
                     const Rel rx = rel(X, v), rv = rel(v, Y);
-                    if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
                     KTimer kt(ctx, kname);
+                    if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
                     hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
                                        rx.val, rv.off, rv.val, u[v], nx, ny, G, bytes_slot(8));
                     kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
@@ -651,8 +655,9 @@ struct BagRun {
                     const Rel re = rel(e, v);
                     const uint32_t ne = dom[e].n, nd = dom[d].n;
                     KTimer kt(ctx, kname);
-                    hipLaunchKernelGGL(k_ed, dim3(std::min(grid_for(ne, 1), 1u << 16)), dim3(kThreads), 0, ctx->stream, re.off,
-                                       re.val, u[v], F, ne, nd, G, bytes_slot(dense_x ? 9 : 10));
+                    const uint64_t items = (uint64_t)ne * ((nd + kEdTile - 1) / kEdTile);
+                    hipLaunchKernelGGL(k_ed, dim3(std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 15)), dim3(kThreads), 0,
+                                       ctx->stream, re.off, re.val, u[v], F, ne, nd, G, bytes_slot(dense_x ? 9 : 10));
                     kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
                     if (tmp) release(F);
                     out.rows = e;
