@@ -64,29 +64,49 @@ __global__ void k_dom_fill(const uint64_t *__restrict__ bits, uint32_t W, const 
 
 // Relation CSR of a quotient edge A -> B from the candidate-edge lists of a query arc a -> b
 // (a in A, b in B): row i (data vertex ids_a[i]) holds the D(B) positions of its neighbours in
-// D(B), ascending.  Count pass (FILL = false), then fill pass at the scanned offsets.
+// D(B), ascending.  Count pass (FILL = false), then fill pass at the scanned offsets.  kRelLanes
+// lanes per row stride over the row's candidate-edge list; the fill keeps list order with a
+// ballot prefix inside the lane group.
+constexpr int kRelLanes = 16;
 template <bool FILL>
 __global__ void k_rel(const uint32_t *__restrict__ ids_a, uint32_t na, const uint64_t *__restrict__ bits_a,
                       const uint32_t *__restrict__ rank_a, const uint64_t *__restrict__ ce_off,
                       const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand_b,
                       const uint64_t *__restrict__ bits_B, const uint32_t *__restrict__ rank_B, uint32_t *__restrict__ cnt,
                       const uint64_t *__restrict__ off, uint32_t *__restrict__ val, unsigned long long *__restrict__ nbytes) {
+    constexpr int G = kRelLanes;
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     uint64_t bytes = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-        const uint32_t v = ids_a[i];
-        const uint32_t p = bit_rank(bits_a, rank_a, v);
+    // one row per aligned group of G lanes: the group's lanes share the row and its trip count, so
+    // the ballots see the whole group (other groups of the wave may have left their loops)
+    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; i < na; i += stride) {
+        const uint32_t p = bit_rank(bits_a, rank_a, ids_a[i]);
         const uint64_t lb = ce_off[p], le = ce_off[p + 1];
         uint64_t o = FILL ? off[i] : 0;
         uint32_t c = 0;
-        for (uint64_t e = lb; e < le; ++e) {
-            const uint32_t w = cand_b[ce_val[e]];
-            if (!bit_test(bits_B, w)) continue;
-            if constexpr (FILL) val[o++] = bit_rank(bits_B, rank_B, w);
-            else ++c;
+        for (uint64_t e0 = lb; e0 < le; e0 += G) {
+            const uint64_t e = e0 + lg;
+            bool ok = false;
+            uint32_t pos = 0;
+            if (e < le) {
+                const uint32_t w = cand_b[ce_val[e]];
+                ok = bit_test(bits_B, w);
+                if (FILL && ok) pos = bit_rank(bits_B, rank_B, w);
+            }
+            if constexpr (FILL) {
+                const uint64_t gm = group_bits<G>(__ballot(ok));
+                if (ok) val[o + __popcll(gm & ((1ull << lg) - 1))] = pos;
+                o += __popcll(gm);
+            } else {
+                c += ok;
+            }
         }
-        if constexpr (FILL) bytes += 4 * (o - off[i]);
-        else cnt[i] = c;
-        bytes += 28 + 16 * (le - lb);
+        if constexpr (!FILL) {
+            c = (uint32_t)group_sum<G>(c);
+            if (lg == 0) cnt[i] = c;
+        }
+        if (lg == 0) bytes += 28 + 16 * (le - lb) + (FILL ? 4 * (o - off[i]) : 0);
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -95,16 +115,22 @@ __global__ void k_vec_mul(uint64_t *__restrict__ u, const uint64_t *__restrict__
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) u[i] *= t[i];
 }
 
-// t[i] = sum over row i of the relation of x[col]
+// t[i] = sum over row i of the relation of x[col]: kRelLanes lanes per row.
 __global__ void k_spmv(const uint64_t *__restrict__ off, const uint32_t *__restrict__ val, uint32_t n,
                        const uint64_t *__restrict__ x, uint64_t *__restrict__ t, unsigned long long *__restrict__ nbytes) {
+    constexpr int G = kRelLanes;
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     uint64_t bytes = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; i < n; i += stride) {
         uint64_t s = 0;
         const uint64_t lb = off[i], le = off[i + 1];
-        for (uint64_t e = lb; e < le; ++e) s += x[val[e]];
-        t[i] = s;
-        bytes += 24 + 12 * (le - lb);
+        for (uint64_t e = lb + lg; e < le; e += G) s += x[val[e]];
+        s = group_sum<G>(s);
+        if (lg == 0) {
+            t[i] = s;
+            bytes += 24 + 12 * (le - lb);
+        }
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -167,7 +193,8 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
 // G[x][y] = sum_{j in R(X->v)(x)} u[j] * [y in R(v->Y)(j)]: both factors are edge relations.  One
 // workgroup per row x; the row is accumulated in LDS when it fits (ny <= kLdsCols), else in the
 // zeroed output with global atomics.
-__global__ void __launch_bounds__(kThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
+constexpr int kEeThreads = 512;
+__global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ voff, const uint32_t *__restrict__ vval,
                                                  const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny,
                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
@@ -209,8 +236,12 @@ __global__ void __launch_bounds__(kThreads) k_ee(const uint64_t *__restrict__ xo
 // LDS (in chunks); each thread owns two columns of the strip (one 16-byte load when rows are
 // 16-byte aligned) and keeps four list entries' loads in flight.  Chunks after the first add onto
 // the values the previous chunk wrote (same thread, same columns).
+// SIDE only names the instantiation after the step that launches it (0: edge (X, v) x dense
+// (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed side), so that
+// profilers report the two apart.
 constexpr uint32_t kEdChunk = 1024;
 constexpr uint32_t kEdTile = 2 * kThreads;
+template <int SIDE>
 __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ u, const uint64_t *__restrict__ F,
                                                  uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
@@ -346,6 +377,28 @@ __global__ void k_mask(const uint64_t *__restrict__ G, const uint64_t *__restric
                        uint32_t nx, uint32_t ny, uint64_t *__restrict__ out) {
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += gridDim.x * blockDim.x)
         for (uint64_t e = off[x]; e < off[x + 1]; ++e) out[(uint64_t)x * ny + val[e]] = G[(uint64_t)x * ny + val[e]];
+}
+
+// The same in place: one workgroup per row marks the row's edges in an LDS bitmap, then zeroes the
+// columns that are not marked (ny <= kMaskCols).
+constexpr uint32_t kMaskCols = 1u << 16;
+__global__ void k_mask_inplace(uint64_t *__restrict__ G, const uint64_t *__restrict__ off, const uint32_t *__restrict__ val,
+                               uint32_t nx, uint32_t ny) {
+    __shared__ uint32_t bm[kMaskCols / 32];
+    const uint32_t nw = (ny + 31) / 32;
+    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
+        for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) bm[w] = 0;
+        __syncthreads();
+        for (uint64_t e = off[x] + threadIdx.x; e < off[x + 1]; e += blockDim.x) {
+            const uint32_t y = val[e];
+            atomicOr(&bm[y >> 5], 1u << (y & 31));
+        }
+        __syncthreads();
+        uint64_t *row = G + (uint64_t)x * ny;
+        for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x)
+            if (!((bm[y >> 5] >> (y & 31)) & 1u)) row[y] = 0;
+        __syncthreads();
+    }
 }
 
 __global__ void k_hadamard(uint64_t *__restrict__ a, const uint64_t *__restrict__ b, uint64_t n) {
@@ -525,7 +578,7 @@ struct BagRun {
         const uint32_t *cand_b = cs->d_cand + cs->cand_off[ub];
         {
             KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel<false>, dim3(grid_for(A.n, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
+            hipLaunchKernelGGL(k_rel<false>, dim3(grid_for((uint64_t)A.n * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
                                bits_a, rank_a, cs->d_ce_off + cs->ce_base[arc], cs->d_ce_val, cand_b, B.bits, B.rank, cnt,
                                nullptr, nullptr, bytes_slot(0));
             kt.stop();
@@ -539,7 +592,7 @@ struct BagRun {
         if (rc) return rc;
         {
             KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel<true>, dim3(grid_for(A.n, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
+            hipLaunchKernelGGL(k_rel<true>, dim3(grid_for((uint64_t)A.n * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
                                bits_a, rank_a, cs->d_ce_off + cs->ce_base[arc], cs->d_ce_val, cand_b, B.bits, B.rank,
                                nullptr, off, val, bytes_slot(0));
             kt.stop();
@@ -610,7 +663,7 @@ struct BagRun {
                 KTimer kt(ctx, "bag_vector");
                 if (f.first == 1) {
                     const Rel r = rel(a, v);
-                    hipLaunchKernelGGL(k_spmv, dim3(grid_for(na, kThreads)), dim3(kThreads), 0, ctx->stream, r.off, r.val, na,
+                    hipLaunchKernelGGL(k_spmv, dim3(grid_for((uint64_t)na * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, r.off, r.val, na,
                                        u[v], t, bytes_slot(1));
                 } else if (f.second.rows == a) {
                     hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(na, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
@@ -640,7 +693,7 @@ struct BagRun {
                     const Rel rx = rel(X, v), rv = rel(v, Y);
                     KTimer kt(ctx, kname);
                     if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
-                    hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kThreads), 0, ctx->stream, rx.off,
+                    hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kEeThreads), 0, ctx->stream, rx.off,
                                        rx.val, rv.off, rv.val, u[v], nx, ny, G, bytes_slot(8));
                     kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
                 } else if (fxv.first == 1 || fvy.first == 1) {
@@ -656,8 +709,13 @@ struct BagRun {
                     const uint32_t ne = dom[e].n, nd = dom[d].n;
                     KTimer kt(ctx, kname);
                     const uint64_t items = (uint64_t)ne * ((nd + kEdTile - 1) / kEdTile);
-                    hipLaunchKernelGGL(k_ed, dim3(std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 15)), dim3(kThreads), 0,
-                                       ctx->stream, re.off, re.val, u[v], F, ne, nd, G, bytes_slot(dense_x ? 9 : 10));
+                    const dim3 grid((unsigned)std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 15));
+                    if (dense_x)
+                        hipLaunchKernelGGL(k_ed<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, u[v], F, ne, nd, G,
+                                           bytes_slot(9));
+                    else
+                        hipLaunchKernelGGL(k_ed<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, u[v], F, ne, nd, G,
+                                           bytes_slot(10));
                     kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
                     if (tmp) release(F);
                     out.rows = e;
@@ -681,15 +739,25 @@ struct BagRun {
                 if (it == fac.end()) {
                     fac[key(X, Y)] = {2, out};
                 } else if (it->second.first == 1) {
-                    uint64_t *M = nullptr;
-                    if ((rc = alloc((void **)&M, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
-                    if ((rc = fill_u64(ctx, M, (uint64_t)nx * ny, 0))) return rc;
                     const Rel r = rel(orow, ocol);
-                    hipLaunchKernelGGL(k_mask, dim3(grid_for(dom[orow].n, kThreads)), dim3(kThreads), 0, ctx->stream, G, r.off,
-                                       r.val, dom[orow].n, dom[ocol].n, M);
-                    if ((rc = launched("bag mask"))) return rc;
-                    release(G);
-                    it->second = {2, Table{M, orow}};
+                    KTimer kt(ctx, "bag_mask");
+                    if (dom[ocol].n <= kMaskCols) {
+                        hipLaunchKernelGGL(k_mask_inplace, dim3(std::min(grid_for(dom[orow].n, 1), 1u << 16)), dim3(kThreads), 0,
+                                           ctx->stream, G, r.off, r.val, dom[orow].n, dom[ocol].n);
+                        kt.stop(8ull * dom[orow].n * dom[ocol].n + 8ull * (dom[orow].n + 1) + 4 * r.nnz);
+                        if ((rc = launched("bag mask"))) return rc;
+                        it->second = {2, out};
+                    } else {
+                        uint64_t *M = nullptr;
+                        if ((rc = alloc((void **)&M, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
+                        if ((rc = fill_u64(ctx, M, (uint64_t)nx * ny, 0))) return rc;
+                        hipLaunchKernelGGL(k_mask, dim3(grid_for(dom[orow].n, kThreads)), dim3(kThreads), 0, ctx->stream, G,
+                                           r.off, r.val, dom[orow].n, dom[ocol].n, M);
+                        kt.stop(16ull * dom[orow].n * dom[ocol].n);
+                        if ((rc = launched("bag mask"))) return rc;
+                        release(G);
+                        it->second = {2, Table{M, orow}};
+                    }
                 } else {
                     uint64_t *H = G;
                     bool tmp = false;

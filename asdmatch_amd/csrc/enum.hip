@@ -2733,6 +2733,17 @@ int asdm_enumerate(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint
 } ASDM_CTX_CATCH(ctx)
 
 
+namespace asdm {
+// Learned engine choice (asdm_batch_count_learned): a query the plain rule sends to the bag engine
+// runs on the frontier when its learned row estimate is at most kLearnedFrontierRows and the
+// frontier's modelled time (rows x kFrontierSecondsPerRow) is below the bag plan's (cost x
+// kBagSecondsPerCost, both measured on one MI355X); it falls back to bags past kLearnedRowCap rows.
+constexpr double kLearnedFrontierRows = 16777216.0;
+constexpr double kFrontierSecondsPerRow = 1.0e-9;
+constexpr double kBagSecondsPerCost = 2.0e-13;
+constexpr uint64_t kLearnedRowCap = 1ull << 26;
+}  // namespace asdm
+
 // Batched multi-query count: the filter of all queries runs as one batched pass (filter_many);
 // the queries are then counted together by the level-synchronous batch engine.  A query whose
 // plan is not supported by the batch engine, or a batch whose frontier exceeds the budget, is
@@ -2772,6 +2783,7 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
     if (rc) return rc;
     std::vector<asdm_cs *> bcs;
     std::vector<uint32_t> bidx;
+    std::vector<std::shared_ptr<BagPlan>> fallback(nq);  // bag plans of queries moved to the frontier
     for (uint32_t i = 0; i < nq; ++i) {
         counts[i] = 0;
         if (qstatus) qstatus[i] = status[i];
@@ -2801,14 +2813,22 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
     }
     if (!rc && m) {
         rc = learned_plans(m, g, std::vector<const asdm_cs *>(bcs.begin(), bcs.end()), 0, bplans);
-        // the engine of each query is the one asdm_plan_create picks (the learned estimates order
-        // the frontier plans; they do not move a query between engines)
+        // Engine choice.  The plain rule (asdm_plan_create's) sends deep or large plans to the bag
+        // engine, because the independence estimates of the frontier's rows are unreliable there.
+        // The learned estimates are not: a query whose learned row estimate makes the frontier
+        // cheaper than its bag plan runs on the frontier instead, under a row cap, and falls back
+        // to the bag plan if it reaches the cap.
         std::string err;
         for (size_t k = 0; k < bplans.size() && !rc; ++k) {
             asdm_plan plain;
             rc = build_plan(qs[bidx[k]], bcs[k], nullptr, 0, [](uint64_t, double fb) { return fb; }, &plain, err);
             if (!rc) rc = attach_bags(bcs[k], 0, false, &plain, err);
-            if (!rc && plain.bags) bplans[k]->bags = plain.bags;
+            if (rc || !plain.bags) continue;
+            const double rows = bplans[k]->est_total;
+            if (rows <= kLearnedFrontierRows && rows * kFrontierSecondsPerRow < plain.bags->cost * kBagSecondsPerCost)
+                fallback[k] = plain.bags;
+            else
+                bplans[k]->bags = plain.bags;
         }
         if (rc) ctx->err = err;
     }
@@ -2860,10 +2880,45 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
             }
             return ASDM_OK;
         };
-        // queries whose plan uses bag tables are counted by the bag engine, the others together
-        std::vector<size_t> frontier, bagged;
-        for (size_t k = 0; k < bcs.size(); ++k) (bplans[k]->bags ? bagged : frontier).push_back(k);
+        // queries whose plan uses bag tables are counted by the bag engine, the others together;
+        // queries moved to the frontier by the learned estimates run in a batch of their own under
+        // the row cap, and those that reach it join the bag engine's queue
+        std::vector<size_t> frontier, moved, bagged;
+        for (size_t k = 0; k < bcs.size(); ++k) (bplans[k]->bags ? bagged : fallback[k] ? moved : frontier).push_back(k);
         if (!frontier.empty()) rc = run(frontier);
+        if (!rc && !moved.empty()) {
+            const uint64_t saved_limit = ctx->row_limit;
+            ctx->row_limit = saved_limit ? std::min(saved_limit, kLearnedRowCap) : kLearnedRowCap;
+            std::vector<asdm_cs *> c;
+            std::vector<asdm_plan *> p;
+            for (size_t k : moved) {
+                c.push_back(bcs[k]);
+                p.push_back(bplans[k]);
+            }
+            std::vector<uint64_t> cnt(moved.size(), 0);
+            std::vector<int> st(moved.size(), ASDM_OK);
+            std::vector<float> lat(moved.size(), 0);
+            rc = batch_engine(ctx, c, p, budget, cnt.data(), st.data(), lat.data(), &clock, nullptr);
+            ctx->row_limit = saved_limit;
+            for (size_t j = 0; j < moved.size() && !rc; ++j) {
+                const size_t k = moved[j];
+                if (st[j] == ASDM_E_TIMEOUT) {  // over the cap: count it with its bag plan
+                    bplans[k]->bags = fallback[k];
+                    bagged.push_back(k);
+                } else {
+                    bc[k] = cnt[j];
+                    bst[k] = st[j];
+                    blat[k] = lat[j];
+                }
+            }
+            if (rc == ASDM_E_BUDGET) {  // too large to batch: each moved query on its bag plan
+                rc = ASDM_OK;
+                for (size_t k : moved) {
+                    bplans[k]->bags = fallback[k];
+                    bagged.push_back(k);
+                }
+            }
+        }
         for (size_t j = 0; j < bagged.size() && !rc; ++j) {
             const size_t k = bagged[j];
             bst[k] = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);

@@ -374,12 +374,12 @@ def bench_local(args):
         dpath, qpath, csv = (os.path.join(tmpdir, n) for n in ("c3.graph", "c3_query.graph", "c3.csv"))
         g.write(dpath)
         q.write(qpath)
-        limit = 30
-        say(f"cpu baseline: reference LOCAL_COUNT build on the same query, {limit} s limit")
+        limit = 90
+        say(f"cpu baseline: reference LOCAL_COUNT build on the same query, {limit} s wall-clock limit")
         t3 = time.perf_counter()
         try:
             r = subprocess.run([os.path.join(REF_BIN, "ref_driver_local"), "-q", qpath, "-d", dpath, "-c", csv],
-                               capture_output=True, text=True, timeout=limit + 60)
+                               capture_output=True, text=True, timeout=limit)
             v = parse_ref(r.stdout)
             finished = "exec" in v
         except subprocess.TimeoutExpired:
@@ -388,7 +388,9 @@ def bench_local(args):
         cpu = {"value": round(1.0 / limit, 4) if not finished else round(1.0 / (v["filter"] + v["plan"] + v["exec"]), 4),
                "unit": "queries/s", "cores": 1, "kind": "reference", "bound": "upper" if not finished else "measured",
                "sample": f"the same 12-vertex query and graph, reference ALL_LEVEL + LOCAL_COUNT build, one thread, "
-                         f"{'stopped after a ' + str(limit) + ' s limit (the value is the upper bound 1/limit: it enumerates the ' + str(count) + ' embeddings one by one)' if not finished else 'finished'}",
+                         + (f"stopped after {limit} s of wall time, graph load included (the value is the upper "
+                            f"bound 1/{limit}: the build enumerates the {count} embeddings one by one)"
+                            if not finished else "finished"),
                "wall_s": round(wall, 1)}
     out = base_line(args, round(1.0 / secs, 4), round(secs * 1e3, 3), {
         "workload": "config 3: Chung-Lu 200,000 V / 4,000,000 E, beta 2.5, 16 uniform labels; the admitted 12-vertex "
