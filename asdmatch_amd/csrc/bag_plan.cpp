@@ -53,20 +53,31 @@ int64_t mobius(const std::vector<std::vector<uint32_t>> &blocks) {
 
 // Greedy elimination of the quotient graph with tables of at most two variables.  Each step
 // eliminates a live block (never `keep`) of live degree <= 2, the cheapest first: degree 0 and 1
-// cost a pass over a vector or a table, degree 2 writes a |D(a)| x |D(b)| table (and multiplies
-// two dense tables when both incident factors are dense).
+// cost a pass over a vector or a table.  Degree 2 joins the two factors of v into one factor on
+// its neighbours (a, b): when (a, b) already carry an edge factor, the product is only needed on
+// that factor's edges, so the step costs per edge and the factor stays sparse; otherwise it
+// writes a |D(a)| x |D(b)| table (and multiplies two dense tables when both incident factors are
+// dense).  nnz holds the estimated entries of each edge relation.
 // new_table_weight scales the cost of a step that starts a new dense table from two edge
 // relations: tables started in several places of one cycle later meet at a vertex whose both
 // factors are dense, a dense x dense product, so a larger weight keeps extending one table.
-bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &dom,
-                    uint32_t keep, double new_table_weight, std::vector<BagStep> &steps, double &cost, double &peak) {
+bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &nnz_in,
+                    const std::vector<double> &dom, uint32_t keep, double new_table_weight, std::vector<BagStep> &steps,
+                    double &cost, double &peak) {
     std::vector<std::vector<int>> kind(nb, std::vector<int>(nb, 0));  // 0 none, 1 edge relation, 2 dense table
-    for (auto &e : edges) kind[e.first][e.second] = kind[e.second][e.first] = 1;
+    std::vector<std::vector<double>> nnz(nb, std::vector<double>(nb, 0));
+    for (size_t i = 0; i < edges.size(); ++i) {
+        const uint32_t a = edges[i].first, b = edges[i].second;
+        kind[a][b] = kind[b][a] = 1;
+        nnz[a][b] = nnz[b][a] = nnz_in[i];
+    }
+    // mean list length of a's rows in the relation (a, v)
+    auto list_len = [&](uint32_t a, uint32_t v) { return nnz[a][v] / std::max(1.0, dom[a]); };
     std::vector<char> alive(nb, 1);
     uint32_t left = nb - (keep < nb ? 1 : 0);
     steps.clear();
     while (left) {
-        double best = 0;
+        double best = 0, best_charged = 0;
         int bv = -1;
         std::vector<uint32_t> bn;
         for (uint32_t v = 0; v < nb; ++v) {
@@ -75,16 +86,28 @@ bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>
             for (uint32_t w = 0; w < nb; ++w)
                 if (w != v && alive[w] && kind[v][w]) n.push_back(w);
             if (n.size() > 2) continue;
-            double c;
+            double c, charged;
             if (n.empty()) c = dom[v];
             else if (n.size() == 1) c = kind[v][n[0]] == 2 ? dom[v] * dom[n[0]] : dom[v] + dom[n[0]];
             else {
-                const double out = dom[n[0]] * dom[n[1]];
                 const bool d0 = kind[v][n[0]] == 2, d1 = kind[v][n[1]] == 2;
-                if (d0 && d1) c = out * (1 + dom[v]);
-                else if (d0 || d1) c = out * 4;
-                else c = out * new_table_weight;
+                if (kind[n[0]][n[1]] == 1) {  // sparse join on the edges of (n0, n1)
+                    const double e = nnz[n[0]][n[1]];
+                    if (d0 && d1) c = e * dom[v];
+                    else if (d0) c = e * (1 + list_len(n[1], v));
+                    else if (d1) c = e * (1 + list_len(n[0], v));
+                    else c = e * (1 + std::min(list_len(n[0], v), list_len(n[1], v)));
+                } else {
+                    const double out = dom[n[0]] * dom[n[1]];
+                    if (d0 && d1) c = out * (1 + dom[v]);
+                    else if (d0 || d1) c = out * 4;
+                    else c = out * new_table_weight;
+                }
             }
+            // the cost charged is the step's modelled work, whatever weight steered the choice
+            charged = c;
+            if (n.size() == 2 && kind[n[0]][n[1]] != 1 && kind[v][n[0]] != 2 && kind[v][n[1]] != 2)
+                charged = dom[n[0]] * dom[n[1]];
             // degree 0 and 1 steps first: they shrink the graph without writing a table (a graph of
             // treewidth <= 2 reduces to nothing by such steps in any order, and always has two
             // vertices of degree <= 2, so keeping one block live never blocks the reduction)
@@ -92,6 +115,7 @@ bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>
                 bv = (int)v;
                 bn = n;
                 best = c;
+                best_charged = charged;
             }
         }
         if (bv < 0) return false;
@@ -100,10 +124,8 @@ bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>
         s.nn = (uint32_t)bn.size();
         for (uint32_t i = 0; i < s.nn; ++i) s.nb[i] = bn[i];
         steps.push_back(s);
-        // the cost charged is the step's modelled work, whatever weight steered the choice
-        if (s.nn == 2 && kind[bv][bn[0]] != 2 && kind[bv][bn[1]] != 2) best = dom[bn[0]] * dom[bn[1]];
-        cost += best;
-        if (s.nn == 2) {
+        cost += best_charged;
+        if (s.nn == 2 && kind[bn[0]][bn[1]] != 1) {
             peak = std::max(peak, dom[bn[0]] * dom[bn[1]]);
             kind[bn[0]][bn[1]] = kind[bn[1]][bn[0]] = 2;
         }
@@ -115,14 +137,14 @@ bool eliminate_once(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>
 }
 
 // The cheapest of a few greedy orders (different weights on starting a new table).
-bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &dom,
-               uint32_t keep, std::vector<BagStep> &steps, double &cost, double &peak) {
+bool eliminate(uint32_t nb, const std::vector<std::pair<uint32_t, uint32_t>> &edges, const std::vector<double> &nnz,
+               const std::vector<double> &dom, uint32_t keep, std::vector<BagStep> &steps, double &cost, double &peak) {
     bool found = false;
     double best_cost = 0, best_peak = 0;
     for (double w : {1.0, 8.0, 64.0, 4096.0}) {
         std::vector<BagStep> st;
         double c = 0, pk = 0;
-        if (!eliminate_once(nb, edges, dom, keep, w, st, c, pk)) return false;  // width > 2 for every weight
+        if (!eliminate_once(nb, edges, nnz, dom, keep, w, st, c, pk)) return false;  // width > 2 for every weight
         if (!found || c < best_cost) {
             found = true;
             best_cost = c;
@@ -191,16 +213,20 @@ bool build_bag_plan(const asdm_cs *cs, bool local, BagPlan &out, std::string &wh
             T.arc_ab.push_back(e.second.first);
             T.arc_ba.push_back(e.second.second);
         }
+        // entries of each quotient edge's relation: at most the arc's candidate edges
+        std::vector<double> nnz(T.edges.size());
+        for (size_t i = 0; i < T.edges.size(); ++i)
+            nnz[i] = std::min((double)cs->arc_weight[T.arc_ab[i]], dom[T.edges[i].first] * dom[T.edges[i].second]);
         T.blocks = std::move(blocks);
         double peak = 0;
-        if (!eliminate(nb, T.edges, dom, UINT32_MAX, T.count_steps, out.cost, peak)) {
+        if (!eliminate(nb, T.edges, nnz, dom, UINT32_MAX, T.count_steps, out.cost, peak)) {
             why = "a quotient of the query needs tables over three variables";
             return false;
         }
         if (local) {
             T.local_steps.resize(nb);
             for (uint32_t k = 0; k < nb; ++k)
-                if (!eliminate(nb, T.edges, dom, k, T.local_steps[k], out.cost, peak)) {
+                if (!eliminate(nb, T.edges, nnz, dom, k, T.local_steps[k], out.cost, peak)) {
                     why = "a quotient of the query needs tables over three variables for a marginal";
                     return false;
                 }

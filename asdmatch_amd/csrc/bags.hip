@@ -115,21 +115,27 @@ __global__ void k_vec_mul(uint64_t *__restrict__ u, const uint64_t *__restrict__
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) u[i] *= t[i];
 }
 
-// t[i] = sum over row i of the relation of x[col]: kRelLanes lanes per row.
-__global__ void k_spmv(const uint64_t *__restrict__ off, const uint32_t *__restrict__ val, uint32_t n,
-                       const uint64_t *__restrict__ x, uint64_t *__restrict__ t, unsigned long long *__restrict__ nbytes) {
+// t[i] = sum over row i of the relation of w x[col], w the entry's weight (1 when the weight array
+// is null): kRelLanes lanes per row.
+__global__ void k_spmv(const uint64_t *__restrict__ off, const uint32_t *__restrict__ val, const uint64_t *__restrict__ w,
+                       uint32_t n, const uint64_t *__restrict__ x, uint64_t *__restrict__ t,
+                       unsigned long long *__restrict__ nbytes) {
     constexpr int G = kRelLanes;
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t base = off[0];  // weights are indexed from the relation's first entry
     uint64_t bytes = 0;
     for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; i < n; i += stride) {
         uint64_t s = 0;
         const uint64_t lb = off[i], le = off[i + 1];
-        for (uint64_t e = lb + lg; e < le; e += G) s += x[val[e]];
+        if (w)
+            for (uint64_t e = lb + lg; e < le; e += G) s += w[e - base] * x[val[e]];
+        else
+            for (uint64_t e = lb + lg; e < le; e += G) s += x[val[e]];
         s = group_sum<G>(s);
         if (lg == 0) {
             t[i] = s;
-            bytes += 24 + 12 * (le - lb);
+            bytes += 24 + (w ? 20 : 12) * (le - lb);
         }
     }
     wave_add_bytes(bytes, nbytes);
@@ -190,16 +196,19 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
     if (threadIdx.x == 0 && blockIdx.x == 0) total[0] += (unsigned long long)((uint64_t)mu * scal[0]);
 }
 
-// G[x][y] = sum_{j in R(X->v)(x)} u[j] * [y in R(v->Y)(j)]: both factors are edge relations.  One
-// workgroup per row x; the row is accumulated in LDS when it fits (ny <= kLdsCols), else in the
-// zeroed output with global atomics.
+// G[x][y] = sum_{j in R(X->v)(x)} u[j] wx * wv [y in R(v->Y)(j)]: both factors are edge relations
+// (wx, wv their edge weights, 1 where the weight array is null).  One workgroup per row x; the
+// row is accumulated in LDS when it fits (ny <= kLdsCols), else in the zeroed output with global
+// atomics.
 constexpr int kEeThreads = 512;
 __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
-                                                 const uint64_t *__restrict__ voff, const uint32_t *__restrict__ vval,
+                                                 const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
+                                                 const uint32_t *__restrict__ vval, const uint64_t *__restrict__ vw,
                                                  const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny,
                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
     __shared__ unsigned long long acc[kLdsCols];
     const bool in_lds = ny <= kLdsCols;
+    const uint64_t xbase = xoff[0], vbase = voff[0];  // weights are indexed from each relation's first entry
     uint64_t bytes = 0;
     for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
         uint64_t *row = G + (uint64_t)x * ny;
@@ -210,13 +219,14 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
         const uint64_t lb = xoff[x], le = xoff[x + 1];
         for (uint64_t e = lb + threadIdx.x; e < le; e += blockDim.x) {
             const uint32_t j = xval[e];
-            const uint64_t w = u[j];
+            const uint64_t w = u[j] * (xw ? xw[e - xbase] : 1ull);
             const uint64_t b2 = voff[j], e2 = voff[j + 1];
             bytes += 28 + 4 * (e2 - b2);
             if (!w) continue;
             for (uint64_t f = b2; f < e2; ++f) {
-                if (in_lds) atomicAdd(&acc[vval[f]], (unsigned long long)w);
-                else atomicAdd((unsigned long long *)&row[vval[f]], (unsigned long long)w);
+                const uint64_t wf = vw ? w * vw[f - vbase] : w;
+                if (in_lds) atomicAdd(&acc[vval[f]], (unsigned long long)wf);
+                else atomicAdd((unsigned long long *)&row[vval[f]], (unsigned long long)wf);
             }
         }
         if (in_lds) {
@@ -229,7 +239,8 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
     wave_add_bytes(bytes, nbytes);
 }
 
-// G[x][y] = sum_{j in R(x)} u[j] F[j][y]: an edge relation (rows x, columns j) times a dense table
+// G[x][y] = sum_{j in R(x)} u[j] w[e] F[j][y]: an edge relation (rows x, columns j; w its edge
+// weights, 1 where null) times a dense table
 // whose rows are the relation's columns.  Work items are (column tile, row x) pairs in tile-major
 // order, so the workgroups in flight read the same 2 * kThreads-column strip of F and its re-reads
 // hit L2 and the MALL rather than HBM.  A workgroup stages the row's list and its weights u[j] in
@@ -243,12 +254,14 @@ constexpr uint32_t kEdChunk = 1024;
 constexpr uint32_t kEdTile = 2 * kThreads;
 template <int SIDE>
 __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
-                                                 const uint64_t *__restrict__ u, const uint64_t *__restrict__ F,
+                                                 const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
+                                                 const uint64_t *__restrict__ F,
                                                  uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
                                                  unsigned long long *__restrict__ nbytes) {
     __shared__ uint32_t lj[kEdChunk];
     __shared__ uint64_t lw[kEdChunk];
     const bool pairs = (ny & 1u) == 0;
+    const uint64_t xbase = xoff[0];  // weights are indexed from the relation's first entry
     const uint64_t tiles = (ny + kEdTile - 1) / kEdTile;
     const uint64_t items = tiles * nx;
     uint64_t bytes = 0;
@@ -268,7 +281,7 @@ __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xo
             for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
                 const uint32_t jj = xval[k0 + t];
                 lj[t] = jj;
-                lw[t] = u[jj];
+                lw[t] = xw ? u[jj] * xw[k0 + t - xbase] : u[jj];
             }
             __syncthreads();
             if (pairs && va) {
@@ -355,6 +368,147 @@ __global__ void __launch_bounds__(256) k_dd(const uint64_t *__restrict__ A, cons
     wave_add_bytes(bytes, nbytes);
 }
 
+// ---- sparse joins: the product of v's two factors evaluated only on the edges of the relation
+// R(X->Y) that already joins X and Y, times each edge's own weight (1 where the relation has
+// none), written to wout.  The work items are the relation's edges, kJoinLanes lanes per edge
+// (a power-law hub's row holds many edges, so rows would leave most lanes idle behind a few);
+// an edge finds its row by binary search in the offsets.
+// Weight arrays are indexed by entry - off[0]: a relation over single query vertices is a slice
+// of the candidate-edge store, whose offsets do not start at zero.
+constexpr int kJoinLanes = 8;
+
+// Row r of a CSR with n rows that holds entry e (off[0] <= e < off[n]): the last r with off[r] <= e.
+__device__ inline uint32_t row_of(const uint64_t *__restrict__ off, uint32_t n, uint64_t e) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Both factors edge relations: the product at (x, y) is the sum of u[j] wa wb over the common
+// entries j of the rows R(X->v)(x) and R(Y->v)(y) (ascending D(v) positions).  The lanes split
+// the shorter row and binary-search the longer.
+__global__ void k_join_ee(const uint64_t *__restrict__ roff, const uint32_t *__restrict__ rval,
+                          const uint64_t *__restrict__ rw, uint32_t nx, uint64_t nnz, const uint64_t *__restrict__ aoff,
+                          const uint32_t *__restrict__ aval, const uint64_t *__restrict__ aw,
+                          const uint64_t *__restrict__ boff, const uint32_t *__restrict__ bval,
+                          const uint64_t *__restrict__ bw, const uint64_t *__restrict__ u, uint64_t *__restrict__ wout,
+                          unsigned long long *__restrict__ nbytes) {
+    constexpr int G = kJoinLanes;
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t rbase = roff[0], abase = aoff[0], bbase = boff[0];
+    uint64_t bytes = 0;
+    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; k < nnz; k += stride) {
+        const uint64_t e = rbase + k;
+        const uint32_t x = row_of(roff, nx, e), y = rval[e];
+        const uint64_t a0 = aoff[x], na = aoff[x + 1] - a0;
+        const uint64_t b0 = boff[y], nb = boff[y + 1] - b0;
+        const bool a_short = na <= nb;
+        const uint64_t s0 = a_short ? a0 : b0, ns = a_short ? na : nb;
+        const uint64_t l0 = a_short ? b0 : a0, nl = a_short ? nb : na;
+        const uint32_t *sv = a_short ? aval : bval, *lv = a_short ? bval : aval;
+        const uint64_t *sw = a_short ? aw : bw, *lw = a_short ? bw : aw;
+        const uint64_t sbase = a_short ? abase : bbase, lbase = a_short ? bbase : abase;
+        uint64_t acc = 0;
+        for (uint64_t t = lg; t < ns; t += G) {
+            const uint32_t j = sv[s0 + t];
+            const uint32_t p = lower_bound_u32(lv + l0, (uint32_t)nl, j);
+            if (p < nl && lv[l0 + p] == j) {
+                uint64_t m = u[j];
+                if (sw) m *= sw[s0 + t - sbase];
+                if (lw) m *= lw[l0 + p - lbase];
+                acc += m;
+            }
+        }
+        acc = group_sum<G>(acc);
+        if (lg == 0) {
+            wout[k] = rw ? acc * rw[k] : acc;
+            bytes += 32 + 12 * (ns + nl);
+        }
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// One factor an edge relation L (rows: the list owner o, x when own_x else y) and the other a
+// dense table F over (the other end a, v) read as F[a * sa + j * sj]: the product at (x, y) is
+// the sum over the entries j of L(o) of u[j] wl F(a, j).
+__global__ void k_join_ed(const uint64_t *__restrict__ roff, const uint32_t *__restrict__ rval,
+                          const uint64_t *__restrict__ rw, uint32_t nx, uint64_t nnz, const uint64_t *__restrict__ loff,
+                          const uint32_t *__restrict__ lval, const uint64_t *__restrict__ lw, int own_x,
+                          const uint64_t *__restrict__ F, uint64_t sa, uint64_t sj, const uint64_t *__restrict__ u,
+                          uint64_t *__restrict__ wout, unsigned long long *__restrict__ nbytes) {
+    constexpr int G = kJoinLanes;
+    const int lg = threadIdx.x & (G - 1);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t rbase = roff[0], lbase = loff[0];
+    uint64_t bytes = 0;
+    for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; k < nnz; k += stride) {
+        const uint64_t e = rbase + k;
+        const uint32_t x = row_of(roff, nx, e), y = rval[e];
+        const uint64_t o = own_x ? x : y, a = own_x ? y : x;
+        const uint64_t l0 = loff[o], l1 = loff[o + 1];
+        const uint64_t *fa = F + a * sa;
+        uint64_t acc = 0;
+        for (uint64_t t = l0 + lg; t < l1; t += G) {
+            const uint32_t j = lval[t];
+            uint64_t m = u[j] * fa[(uint64_t)j * sj];
+            if (lw) m *= lw[t - lbase];
+            acc += m;
+        }
+        acc = group_sum<G>(acc);
+        if (lg == 0) {
+            wout[k] = rw ? acc * rw[k] : acc;
+            bytes += 32 + 28 * (l1 - l0);
+        }
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// Both factors dense, A(x, j) = A[x * sax + j * saj] and B(j, y) = B[j * sbj + y * sby]: the
+// product at (x, y) is sum_j A(x, j) u[j] B(j, y), one wave per edge, lanes over j.
+__global__ void k_join_dd(const uint64_t *__restrict__ roff, const uint32_t *__restrict__ rval,
+                          const uint64_t *__restrict__ rw, uint32_t nx, uint64_t nnz, const uint64_t *__restrict__ A,
+                          uint64_t sax, uint64_t saj, const uint64_t *__restrict__ B, uint64_t sbj, uint64_t sby,
+                          uint32_t nv, const uint64_t *__restrict__ u, uint64_t *__restrict__ wout,
+                          unsigned long long *__restrict__ nbytes) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / 64);
+    const uint64_t rbase = roff[0];
+    uint64_t bytes = 0;
+    for (uint64_t k = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; k < nnz; k += nw) {
+        const uint64_t e = rbase + k;
+        const uint64_t x = row_of(roff, nx, e), y = rval[e];
+        uint64_t acc = 0;
+        for (uint32_t j = lane; j < nv; j += 64) acc += A[x * sax + j * saj] * u[j] * B[j * sbj + y * sby];
+        acc = group_sum<64>(acc);
+        if (lane == 0) {
+            wout[k] = rw ? acc * rw[k] : acc;
+            bytes += 32 + 24ull * nv;
+        }
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
+// Weights of the reverse relation R(Y->X) from those of R(X->Y): entry (y, x) takes the weight of
+// entry (x, y), found by binary search in row x.  One lane per entry.
+__global__ void k_join_reverse(const uint64_t *__restrict__ foff, const uint32_t *__restrict__ fval,
+                               const uint64_t *__restrict__ wf, const uint64_t *__restrict__ roff,
+                               const uint32_t *__restrict__ rval, uint32_t ny, uint64_t nnz, uint64_t *__restrict__ wr) {
+    const uint64_t fbase = foff[0], rbase = roff[0];
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = rbase + k;
+        const uint32_t y = row_of(roff, ny, e), x = rval[e];
+        const uint64_t f0 = foff[x];
+        const uint32_t n = (uint32_t)(foff[x + 1] - f0);
+        const uint32_t p = lower_bound_u32(fval + f0, n, y);
+        wr[k] = p < n && fval[f0 + p] == y ? wf[f0 + p - fbase] : 0;  // the entry is always there
+    }
+}
+
 // out (cols x rows) = transpose of in (rows x cols), 32 x 32 tiles through LDS.
 __global__ void k_transpose(const uint64_t *__restrict__ in, uint32_t rows, uint32_t cols, uint64_t *__restrict__ out) {
     __shared__ uint64_t tile[32][33];
@@ -368,35 +522,6 @@ __global__ void k_transpose(const uint64_t *__restrict__ in, uint32_t rows, uint
         __syncthreads();
         for (uint32_t k = ty; k < 32; k += 8)
             if (c0 + k < cols && r0 + tx < rows) out[(uint64_t)(c0 + k) * rows + r0 + tx] = tile[tx][k];
-        __syncthreads();
-    }
-}
-
-// out (zeroed) keeps G's entries on the edges of the relation R(X->Y); the rest stay zero.
-__global__ void k_mask(const uint64_t *__restrict__ G, const uint64_t *__restrict__ off, const uint32_t *__restrict__ val,
-                       uint32_t nx, uint32_t ny, uint64_t *__restrict__ out) {
-    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += gridDim.x * blockDim.x)
-        for (uint64_t e = off[x]; e < off[x + 1]; ++e) out[(uint64_t)x * ny + val[e]] = G[(uint64_t)x * ny + val[e]];
-}
-
-// The same in place: one workgroup per row marks the row's edges in an LDS bitmap, then zeroes the
-// columns that are not marked (ny <= kMaskCols).
-constexpr uint32_t kMaskCols = 1u << 16;
-__global__ void k_mask_inplace(uint64_t *__restrict__ G, const uint64_t *__restrict__ off, const uint32_t *__restrict__ val,
-                               uint32_t nx, uint32_t ny) {
-    __shared__ uint32_t bm[kMaskCols / 32];
-    const uint32_t nw = (ny + 31) / 32;
-    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
-        for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) bm[w] = 0;
-        __syncthreads();
-        for (uint64_t e = off[x] + threadIdx.x; e < off[x + 1]; e += blockDim.x) {
-            const uint32_t y = val[e];
-            atomicOr(&bm[y >> 5], 1u << (y & 31));
-        }
-        __syncthreads();
-        uint64_t *row = G + (uint64_t)x * ny;
-        for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x)
-            if (!((bm[y >> 5] >> (y & 31)) & 1u)) row[y] = 0;
         __syncthreads();
     }
 }
@@ -429,10 +554,19 @@ struct Rel {
     const uint64_t *off = nullptr;
     const uint32_t *val = nullptr;
     uint64_t nnz = 0;
+    const uint64_t *w = nullptr;  // per-entry weights, w[e - off[0]] for entry e (null: every weight is 1)
 };
 struct Table {  // dense factor of a block pair: row-major, rows indexed by D(rows), columns by the other
     uint64_t *p = nullptr;
     uint32_t rows = 0;
+};
+// Live factor of a block pair (lo, hi), lo < hi: an edge relation in both orientations (rows D(lo)
+// and rows D(hi)), possibly weighted, or a dense table.
+struct Fac {
+    int kind = 0;  // 1 edge relation, 2 dense table
+    Table t;
+    Rel lo, hi;
+    uint64_t *w[2] = {nullptr, nullptr};  // owned weight arrays of lo and hi
 };
 
 struct BagRun {
@@ -441,7 +575,8 @@ struct BagRun {
     const BagPlan *plan;
     unsigned long long *d_acc = nullptr;    // ctx counters: [0] total, [4] scratch sum, [5] term scalar,
                                             // [6] relation and [7] vector algorithmic bytes, [14..17]
-                                            // bytes the table kernels' loops touch (ee, de, ed, dd)
+                                            // bytes the table kernels' loops touch (ee, de, ed, dd),
+                                            // [18..19] those of the sparse joins (ee; ed and dd)
     uint64_t *d_mat = nullptr;
     std::vector<void *> owned;              // per-term device buffers
     // Domains of merged blocks and relations between blocks depend only on the blocks' members,
@@ -638,13 +773,23 @@ struct BagRun {
         }
         if (!rc) rc = fill_u64(ctx, scal, 1, 1);
         if (rc) return rc;
-        // live factors by block pair (lo, hi): kind 1 = edge relation, 2 = dense table
-        std::map<std::pair<uint32_t, uint32_t>, std::pair<int, Table>> fac;
-        for (auto &e : T.edges) fac[e] = {1, Table{}};
+        std::map<std::pair<uint32_t, uint32_t>, Fac> fac;
+        for (auto &e : T.edges) {
+            Fac f;
+            f.kind = 1;
+            f.lo = rels.at(e).first;
+            f.hi = rels.at(e).second;
+            fac[e] = f;
+        }
         auto key = [](uint32_t a, uint32_t b) { return std::make_pair(std::min(a, b), std::max(a, b)); };
         auto rel = [&](uint32_t a, uint32_t b) -> Rel {  // rows D(a)
-            auto it = rels.find(key(a, b));
-            return a < b ? it->second.first : it->second.second;
+            const Fac &f = fac.at(key(a, b));
+            return a < b ? f.lo : f.hi;
+        };
+        auto drop = [&](Fac &f) {
+            if (f.kind == 2) release(f.t.p);
+            for (uint64_t *w : f.w)
+                if (w) release(w);
         };
         for (const BagStep &s : steps) {
             if (timed_out()) return ctx->fail(ASDM_E_TIMEOUT, "per-query time limit exceeded");
@@ -656,22 +801,22 @@ struct BagRun {
                 if ((rc = launched("bag sum"))) return rc;
             } else if (s.nn == 1) {
                 const uint32_t a = s.nb[0];
-                auto &f = fac[key(a, v)];
+                const Fac &f = fac.at(key(a, v));
                 const uint32_t na = dom[a].n;
                 uint64_t *t = nullptr;
                 if ((rc = alloc((void **)&t, 8ull * std::max<uint32_t>(na, 1)))) return rc;
                 KTimer kt(ctx, "bag_vector");
-                if (f.first == 1) {
+                if (f.kind == 1) {
                     const Rel r = rel(a, v);
-                    hipLaunchKernelGGL(k_spmv, dim3(grid_for((uint64_t)na * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, r.off, r.val, na,
-                                       u[v], t, bytes_slot(1));
-                } else if (f.second.rows == a) {
+                    hipLaunchKernelGGL(k_spmv, dim3(grid_for((uint64_t)na * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, r.off, r.val,
+                                       r.w, na, u[v], t, bytes_slot(1));
+                } else if (f.t.rows == a) {
                     hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(na, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
-                                       f.second.p, na, dv.n, u[v], t, bytes_slot(1));
+                                       f.t.p, na, dv.n, u[v], t, bytes_slot(1));
                 } else {  // table rows D(v): the transposed product, accumulated with atomics
                     if ((rc = fill_u64(ctx, t, na, 0))) return rc;
                     const dim3 grid((na + kThreads - 1) / kThreads, std::max<uint32_t>(1, (dv.n + kMvtRows - 1) / kMvtRows));
-                    hipLaunchKernelGGL(k_dense_mv_t, grid, dim3(kThreads), 0, ctx->stream, f.second.p, dv.n, na, u[v], t,
+                    hipLaunchKernelGGL(k_dense_mv_t, grid, dim3(kThreads), 0, ctx->stream, f.t.p, dv.n, na, u[v], t,
                                        bytes_slot(1));
                 }
                 kt.stop();
@@ -680,107 +825,154 @@ struct BagRun {
                 release(t);
             } else {
                 const uint32_t X = std::min(s.nb[0], s.nb[1]), Y = std::max(s.nb[0], s.nb[1]);
-                const uint32_t nx = dom[X].n, ny = dom[Y].n, nv = dv.n;
-                auto &fxv = fac[key(X, v)];
-                auto &fvy = fac[key(v, Y)];
-                uint64_t *G = nullptr;
-                if ((rc = alloc((void **)&G, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
-                Table out{G, X};  // rows of the new table
-                const uint64_t out_bytes = 8ull * nx * ny;  // the table written once
-                const char *kname = fxv.first == 1 ? (fvy.first == 1 ? "bag_table_ee" : "bag_table_ed")
-                                                   : (fvy.first == 1 ? "bag_table_de" : "bag_table_dd");
-                if (fxv.first == 1 && fvy.first == 1) {
-                    const Rel rx = rel(X, v), rv = rel(v, Y);
-                    KTimer kt(ctx, kname);
-                    if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
-                    hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kEeThreads), 0, ctx->stream, rx.off,
-                                       rx.val, rv.off, rv.val, u[v], nx, ny, G, bytes_slot(8));
-                    kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
-                } else if (fxv.first == 1 || fvy.first == 1) {
-                    // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;
-                    // the table comes out with rows D(e)
-                    const bool dense_x = fxv.first == 2;
-                    const uint32_t e = dense_x ? Y : X, d = dense_x ? X : Y;
-                    const Table &td = dense_x ? fxv.second : fvy.second;
-                    uint64_t *F = nullptr;
-                    bool tmp = false;
-                    if ((rc = oriented(td, td.rows == v ? d : v, v, dom, &F, tmp))) return rc;
-                    const Rel re = rel(e, v);
-                    const uint32_t ne = dom[e].n, nd = dom[d].n;
-                    KTimer kt(ctx, kname);
-                    const uint64_t items = (uint64_t)ne * ((nd + kEdTile - 1) / kEdTile);
-                    const dim3 grid((unsigned)std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 15));
-                    if (dense_x)
-                        hipLaunchKernelGGL(k_ed<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, u[v], F, ne, nd, G,
-                                           bytes_slot(9));
-                    else
-                        hipLaunchKernelGGL(k_ed<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, u[v], F, ne, nd, G,
-                                           bytes_slot(10));
-                    kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
-                    if (tmp) release(F);
-                    out.rows = e;
-                } else {
-                    uint64_t *Fa = nullptr, *Fb = nullptr;
-                    bool ta = false, tb = false;
-                    if ((rc = oriented(fxv.second, fxv.second.rows == X ? v : X, X, dom, &Fa, ta))) return rc;
-                    if ((rc = oriented(fvy.second, fvy.second.rows == v ? Y : v, v, dom, &Fb, tb))) return rc;
-                    const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
-                    KTimer kt(ctx, kname);
-                    hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, Fa, u[v], Fb, nx, nv, ny, G,
-                                       bytes_slot(11));
-                    kt.stop(8ull * nx * nv + 8ull * nv * ny + 8ull * nv + out_bytes);
-                    if (ta) release(Fa);
-                    if (tb) release(Fb);
-                }
-                if ((rc = launched("bag table"))) return rc;
-                // join with a factor already on (X, Y)
-                const uint32_t orow = out.rows, ocol = out.rows == X ? Y : X;
                 auto it = fac.find(key(X, Y));
-                if (it == fac.end()) {
-                    fac[key(X, Y)] = {2, out};
-                } else if (it->second.first == 1) {
-                    const Rel r = rel(orow, ocol);
-                    KTimer kt(ctx, "bag_mask");
-                    if (dom[ocol].n <= kMaskCols) {
-                        hipLaunchKernelGGL(k_mask_inplace, dim3(std::min(grid_for(dom[orow].n, 1), 1u << 16)), dim3(kThreads), 0,
-                                           ctx->stream, G, r.off, r.val, dom[orow].n, dom[ocol].n);
-                        kt.stop(8ull * dom[orow].n * dom[ocol].n + 8ull * (dom[orow].n + 1) + 4 * r.nnz);
-                        if ((rc = launched("bag mask"))) return rc;
-                        it->second = {2, out};
-                    } else {
-                        uint64_t *M = nullptr;
-                        if ((rc = alloc((void **)&M, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1)))) return rc;
-                        if ((rc = fill_u64(ctx, M, (uint64_t)nx * ny, 0))) return rc;
-                        hipLaunchKernelGGL(k_mask, dim3(grid_for(dom[orow].n, kThreads)), dim3(kThreads), 0, ctx->stream, G,
-                                           r.off, r.val, dom[orow].n, dom[ocol].n, M);
-                        kt.stop(16ull * dom[orow].n * dom[ocol].n);
-                        if ((rc = launched("bag mask"))) return rc;
-                        release(G);
-                        it->second = {2, Table{M, orow}};
-                    }
-                } else {
-                    uint64_t *H = G;
-                    bool tmp = false;
-                    if (it->second.second.rows != orow && (rc = oriented(out, ocol, it->second.second.rows, dom, &H, tmp)))
-                        return rc;
-                    const uint64_t n = (uint64_t)nx * ny;
-                    hipLaunchKernelGGL(k_hadamard, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                                       it->second.second.p, H, n);
-                    if ((rc = launched("bag join"))) return rc;
-                    if (tmp) release(H);
-                    release(G);
-                }
+                if (it != fac.end() && it->second.kind == 1) rc = join_sparse(X, Y, v, dom, fac.at(key(X, v)), fac.at(key(v, Y)), it->second, u[v]);
+                else rc = join_dense(X, Y, v, dom, fac.at(key(X, v)), fac.at(key(v, Y)), fac, u[v]);
+                if (rc) return rc;
             }
             // drop v's factors
             for (auto it = fac.begin(); it != fac.end();) {
                 if (it->first.first == v || it->first.second == v) {
-                    if (it->second.first == 2) release(it->second.second.p);
+                    drop(it->second);
                     it = fac.erase(it);
                 } else {
                     ++it;
                 }
             }
         }
+        return ASDM_OK;
+    }
+
+    // Eliminates v between X < Y when (X, Y) already carry the edge factor fxy: the product of
+    // v's factors is evaluated on fxy's edges only and multiplied into their weights, so the
+    // factor stays an edge relation.
+    int join_sparse(uint32_t X, uint32_t Y, uint32_t v, const std::vector<Dom> &dom, const Fac &fxv, const Fac &fvy,
+                    Fac &fxy, const uint64_t *uv) {
+        const uint32_t nx = dom[X].n, ny = dom[Y].n, nv = dom[v].n;
+        const Rel rxy = fxy.lo, ryx = fxy.hi;  // rows D(X), rows D(Y)
+        uint64_t *wf = nullptr, *wr = nullptr;
+        int rc = alloc((void **)&wf, 8ull * std::max<uint64_t>(rxy.nnz, 1));
+        if (!rc) rc = alloc((void **)&wr, 8ull * std::max<uint64_t>(ryx.nnz, 1));
+        if (rc) return rc;
+        const dim3 grid(grid_for(rxy.nnz * kJoinLanes, kThreads));
+        auto relv = [&](const Fac &f, uint32_t a) { return a < v ? f.lo : f.hi; };  // rows D(a) of (a, v)
+        if (fxv.kind == 1 && fvy.kind == 1) {
+            const Rel ra = relv(fxv, X), rb = relv(fvy, Y);
+            KTimer kt(ctx, "bag_join_ee");
+            hipLaunchKernelGGL(k_join_ee, grid, dim3(kThreads), 0, ctx->stream, rxy.off, rxy.val, rxy.w, nx, rxy.nnz, ra.off, ra.val,
+                               ra.w, rb.off, rb.val, rb.w, uv, wf, bytes_slot(12));
+            kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * (nx + ny + 2) + 12 * (ra.nnz + rb.nnz) + 8ull * nv);
+        } else if (fxv.kind == 1 || fvy.kind == 1) {
+            const bool own_x = fxv.kind == 1;  // the list side is X: F is the (v, Y) table
+            const uint32_t o = own_x ? X : Y, a = own_x ? Y : X;
+            const Rel rl = relv(own_x ? fxv : fvy, o);
+            const Table &td = (own_x ? fvy : fxv).t;
+            // F(a, j): rows D(a) -> a * nv + j; rows D(v) -> j * |D(a)| + a
+            const uint64_t sa = td.rows == a ? nv : 1, sj = td.rows == a ? 1 : dom[a].n;
+            KTimer kt(ctx, "bag_join_ed");
+            hipLaunchKernelGGL(k_join_ed, grid, dim3(kThreads), 0, ctx->stream, rxy.off, rxy.val, rxy.w, nx, rxy.nnz, rl.off, rl.val,
+                               rl.w, (int)own_x, td.p, sa, sj, uv, wf, bytes_slot(13));
+            // a lower bound: the gathered table entries are not counted (at most the whole table)
+            kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * (dom[o].n + 1) + 12 * rl.nnz + 8ull * nv);
+        } else {
+            // A(x, j): rows D(X) -> x * nv + j, rows D(v) -> j * nx + x; B(j, y): rows D(v) -> j * ny + y,
+            // rows D(Y) -> y * nv + j
+            const bool ax = fxv.t.rows == X, bv = fvy.t.rows == v;
+            KTimer kt(ctx, "bag_join_dd");
+            hipLaunchKernelGGL(k_join_dd, dim3(grid_for(rxy.nnz, kThreads / 64)), dim3(kThreads), 0, ctx->stream, rxy.off,
+                               rxy.val, rxy.w, nx, rxy.nnz, fxv.t.p, ax ? (uint64_t)nv : 1, ax ? 1 : (uint64_t)nx, fvy.t.p,
+                               bv ? (uint64_t)ny : 1, bv ? 1 : (uint64_t)nv, nv, uv, wf, bytes_slot(13));
+            kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * nv * (nx + ny + 1));
+        }
+        if ((rc = launched("bag sparse join"))) return rc;
+        hipLaunchKernelGGL(k_join_reverse, dim3(grid_for(ryx.nnz, kThreads)), dim3(kThreads), 0, ctx->stream, rxy.off,
+                           rxy.val, wf, ryx.off, ryx.val, ny, ryx.nnz, wr);
+        if ((rc = launched("bag sparse join reverse"))) return rc;
+        for (uint64_t *w : fxy.w)
+            if (w) release(w);
+        fxy.w[0] = wf;
+        fxy.w[1] = wr;
+        fxy.lo.w = wf;
+        fxy.hi.w = wr;
+        return ASDM_OK;
+    }
+
+    // Eliminates v between X < Y into a new dense table on (X, Y), multiplied into the dense table
+    // already there if any.
+    int join_dense(uint32_t X, uint32_t Y, uint32_t v, const std::vector<Dom> &dom, const Fac &fxv, const Fac &fvy,
+                   std::map<std::pair<uint32_t, uint32_t>, Fac> &fac, const uint64_t *uv) {
+        const uint32_t nx = dom[X].n, ny = dom[Y].n, nv = dom[v].n;
+        uint64_t *G = nullptr;
+        int rc = alloc((void **)&G, 8ull * std::max<uint64_t>((uint64_t)nx * ny, 1));
+        if (rc) return rc;
+        Table out{G, X};  // rows of the new table
+        const uint64_t out_bytes = 8ull * nx * ny;  // the table written once
+        const char *kname = fxv.kind == 1 ? (fvy.kind == 1 ? "bag_table_ee" : "bag_table_ed")
+                                          : (fvy.kind == 1 ? "bag_table_de" : "bag_table_dd");
+        auto relv = [&](const Fac &f, uint32_t a) { return a < v ? f.lo : f.hi; };  // rows D(a) of (a, v)
+        auto relfrom_v = [&](const Fac &f, uint32_t a) { return a < v ? f.hi : f.lo; };  // rows D(v) of (a, v)
+        if (fxv.kind == 1 && fvy.kind == 1) {
+            const Rel rx = relv(fxv, X), rv = relfrom_v(fvy, Y);
+            KTimer kt(ctx, kname);
+            if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
+            hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kEeThreads), 0, ctx->stream, rx.off,
+                               rx.val, rx.w, rv.off, rv.val, rv.w, uv, nx, ny, G, bytes_slot(8));
+            kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
+        } else if (fxv.kind == 1 || fvy.kind == 1) {
+            // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;
+            // the table comes out with rows D(e)
+            const bool dense_x = fxv.kind == 2;
+            const uint32_t e = dense_x ? Y : X, d = dense_x ? X : Y;
+            const Table &td = dense_x ? fxv.t : fvy.t;
+            uint64_t *F = nullptr;
+            bool tmp = false;
+            if ((rc = oriented(td, td.rows == v ? d : v, v, dom, &F, tmp))) return rc;
+            const Rel re = relv(dense_x ? fvy : fxv, e);
+            const uint32_t ne = dom[e].n, nd = dom[d].n;
+            KTimer kt(ctx, kname);
+            const uint64_t items = (uint64_t)ne * ((nd + kEdTile - 1) / kEdTile);
+            const dim3 grid((unsigned)std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 15));
+            if (dense_x)
+                hipLaunchKernelGGL(k_ed<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
+                                   bytes_slot(9));
+            else
+                hipLaunchKernelGGL(k_ed<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
+                                   bytes_slot(10));
+            kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
+            if (tmp) release(F);
+            out.rows = e;
+        } else {
+            uint64_t *Fa = nullptr, *Fb = nullptr;
+            bool ta = false, tb = false;
+            if ((rc = oriented(fxv.t, fxv.t.rows == X ? v : X, X, dom, &Fa, ta))) return rc;
+            if ((rc = oriented(fvy.t, fvy.t.rows == v ? Y : v, v, dom, &Fb, tb))) return rc;
+            const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
+            KTimer kt(ctx, kname);
+            hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, Fa, uv, Fb, nx, nv, ny, G,
+                               bytes_slot(11));
+            kt.stop(8ull * nx * nv + 8ull * nv * ny + 8ull * nv + out_bytes);
+            if (ta) release(Fa);
+            if (tb) release(Fb);
+        }
+        if ((rc = launched("bag table"))) return rc;
+        // join with a dense table already on (X, Y)
+        auto it = fac.find(std::make_pair(X, Y));
+        if (it == fac.end()) {
+            Fac f;
+            f.kind = 2;
+            f.t = out;
+            fac[std::make_pair(X, Y)] = f;
+            return ASDM_OK;
+        }
+        const uint32_t ocol = out.rows == X ? Y : X;
+        uint64_t *H = G;
+        bool tmp = false;
+        if (it->second.t.rows != out.rows && (rc = oriented(out, ocol, it->second.t.rows, dom, &H, tmp))) return rc;
+        const uint64_t n = (uint64_t)nx * ny;
+        hipLaunchKernelGGL(k_hadamard, dim3(grid_for(n, kThreads)), dim3(kThreads), 0, ctx->stream, it->second.t.p, H, n);
+        if ((rc = launched("bag join"))) return rc;
+        if (tmp) release(H);
+        release(G);
         return ASDM_OK;
     }
 
@@ -876,9 +1068,9 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
     // including lists re-read per table row) is kept apart
     add_kernel_bytes(ctx, "bag_relation", res[6]);
     add_kernel_bytes(ctx, "bag_vector", res[7]);
-    const char *touched[4] = {"bag_table_ee_touched", "bag_table_de_touched", "bag_table_ed_touched",
-                              "bag_table_dd_touched"};
-    for (int k = 0; k < 4; ++k)
+    const char *touched[6] = {"bag_table_ee_touched", "bag_table_de_touched", "bag_table_ed_touched",
+                              "bag_table_dd_touched", "bag_join_ee_touched", "bag_join_ed_touched"};
+    for (int k = 0; k < 6; ++k)
         if (res[14 + k]) add_kernel_bytes(ctx, touched[k], res[14 + k]);
     ctx->stats.exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats.kernel_ms = kms;
