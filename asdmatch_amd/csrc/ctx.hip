@@ -203,6 +203,13 @@ void add_kernel_bytes(asdm_ctx *ctx, const char *name, uint64_t bytes) {
     ctx->kstats[kstat_slot(ctx, name)].bytes += bytes;
 }
 
+void add_kernel_count(asdm_ctx *ctx, const char *name, uint64_t count, uint64_t bytes) {
+    if (!ctx->profiling) return;
+    auto &k = ctx->kstats[kstat_slot(ctx, name)];
+    k.launches += count;
+    k.bytes += bytes;
+}
+
 void release_device_graph(asdm_graph &g) {
     auto &d = g.dev;
     if (d.device < 0) return;

@@ -275,5 +275,7 @@ struct KTimer {
 void resolve_timers(asdm_ctx *ctx);
 // Adds algorithmic bytes to the named kernel's entry (created if missing) while profiling.
 void add_kernel_bytes(asdm_ctx *ctx, const char *name, uint64_t bytes);
+// Adds count to an entry's launches and bytes to its bytes (entries that count events, not kernels).
+void add_kernel_count(asdm_ctx *ctx, const char *name, uint64_t count, uint64_t bytes);
 
 }  // namespace asdm
