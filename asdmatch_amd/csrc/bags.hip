@@ -201,7 +201,8 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
 // G[x][y] = sum_{j in R(X->v)(x)} u[j] wx * wv [y in R(v->Y)(j)]: both factors are edge relations
 // (wx, wv their edge weights, 1 where the weight array is null).  One workgroup per row x; the
 // row is accumulated in LDS when it fits (ny <= kLdsCols), else in the zeroed output with global
-// atomics.
+// atomics.  The waves take the row's entries j in turn and the lanes of a wave split j's list,
+// so a hub's long list is read coalesced and added by 64 lanes at once.
 constexpr int kEeThreads = 512;
 __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
@@ -219,13 +220,14 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
             __syncthreads();
         }
         const uint64_t lb = xoff[x], le = xoff[x + 1];
-        for (uint64_t e = lb + threadIdx.x; e < le; e += blockDim.x) {
+        const uint32_t lane = threadIdx.x & 63;
+        for (uint64_t e = lb + threadIdx.x / 64; e < le; e += blockDim.x / 64) {
             const uint32_t j = xval[e];
             const uint64_t w = u[j] * (xw ? xw[e - xbase] : 1ull);
             const uint64_t b2 = voff[j], e2 = voff[j + 1];
-            bytes += 28 + 4 * (e2 - b2);
+            if (lane == 0) bytes += 28 + 4 * (e2 - b2);
             if (!w) continue;
-            for (uint64_t f = b2; f < e2; ++f) {
+            for (uint64_t f = b2 + lane; f < e2; f += 64) {
                 const uint64_t wf = vw ? w * vw[f - vbase] : w;
                 if (in_lds) atomicAdd(&acc[vval[f]], (unsigned long long)wf);
                 else atomicAdd((unsigned long long *)&row[vval[f]], (unsigned long long)wf);
@@ -242,83 +244,88 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
 }
 
 // G[x][y] = sum_{j in R(x)} u[j] w[e] F[j][y]: an edge relation (rows x, columns j; w its edge
-// weights, 1 where null) times a dense table
-// whose rows are the relation's columns.  Work items are (column tile, row x) pairs in tile-major
-// order, so the workgroups in flight read the same 2 * kThreads-column strip of F and its re-reads
-// hit L2 and the MALL rather than HBM.  A workgroup stages the row's list and its weights u[j] in
-// LDS (in chunks); each thread owns two columns of the strip (one 16-byte load when rows are
-// 16-byte aligned) and keeps four list entries' loads in flight.  Chunks after the first add onto
-// the values the previous chunk wrote (same thread, same columns).
+// weights, 1 where null) times a dense table whose rows are the relation's columns.
+// The columns are cut into strips of GL * CPL columns and the work items are (strip, row x)
+// pairs, one per group of GL lanes; lane l of a group owns the CPL columns l, l + GL, ... of the
+// strip, so every load of a group reads GL consecutive u64 of one row of F.  The strips are dealt
+// to the eight XCDs (workgroup b runs on XCD b % 8; the grid is a multiple of 8 and fits the GPU
+// at once), and each XCD walks its strips in turn, so the rows of an F strip, re-read once per
+// relation entry that names them, are re-read within one XCD's L2.  A group loads GL list
+// entries (column j and weight) into its lanes and broadcasts them by shuffles; the table is
+// written with non-temporal stores, which keep it out of L2.  Measured on the config-4 batch
+// (tables of ~7.5k x 7k): 64 lanes x 1 column beat 32 x 1 by 23 %, 16 x 4, 32 x 2, 32 x 4 and
+// 64 x 2 were slower, and a wave-per-item form with the list in scalar registers was 7 % slower.
 // SIDE only names the instantiation after the step that launches it (0: edge (X, v) x dense
 // (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed side), so that
 // profilers report the two apart.
-constexpr uint32_t kEdChunk = 1024;
-constexpr uint32_t kEdTile = 2 * kThreads;
-template <int SIDE>
+template <int SIDE, int GL, int CPL>
 __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
-                                                 const uint64_t *__restrict__ F,
-                                                 uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
-                                                 unsigned long long *__restrict__ nbytes) {
-    __shared__ uint32_t lj[kEdChunk];
-    __shared__ uint64_t lw[kEdChunk];
-    const bool pairs = (ny & 1u) == 0;
-    const uint64_t xbase = xoff[0];  // weights are indexed from the relation's first entry
-    const uint64_t tiles = (ny + kEdTile - 1) / kEdTile;
-    const uint64_t items = tiles * nx;
+                                                 const uint64_t *__restrict__ F, uint32_t nx, uint32_t ny,
+                                                 uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
+    constexpr uint32_t W = GL * CPL;  // columns per strip
+    const uint32_t lg = threadIdx.x & (GL - 1);
+    const uint32_t nstrips = (ny + W - 1) / W;
+    const uint32_t nxcd = nstrips >= 8 && gridDim.x % 8 == 0 ? 8 : 1;
+    const uint32_t xcd = blockIdx.x % nxcd, lblk = blockIdx.x / nxcd, nblk = gridDim.x / nxcd;
+    const uint64_t my_strips = xcd < nstrips ? (nstrips - xcd + nxcd - 1) / nxcd : 0;
+    const uint64_t items = my_strips * nx;
+    constexpr uint32_t gpb = kThreads / GL;  // lane groups per workgroup
+    const uint64_t xbase = xoff[0];          // weights are indexed from the relation's first entry
     uint64_t bytes = 0;
-    for (uint64_t b = blockIdx.x; b < items; b += gridDim.x) {
-        const uint32_t tile = (uint32_t)(b / nx), x = (uint32_t)(b % nx);
-        const uint32_t c0 = tile * kEdTile;
-        // pairs: columns c0 + 2t, c0 + 2t + 1; otherwise columns c0 + t and c0 + t + kThreads
-        const uint32_t ya = pairs ? c0 + 2 * threadIdx.x : c0 + threadIdx.x;
-        const uint32_t yb = pairs ? ya + 1 : ya + kThreads;
-        const bool va = ya < ny, vb = yb < ny;
+    for (uint64_t it = (uint64_t)lblk * gpb + threadIdx.x / GL; it < items; it += (uint64_t)nblk * gpb) {
+        const uint32_t strip = xcd + nxcd * (uint32_t)(it / nx), x = (uint32_t)(it % nx);
+        const uint32_t c0 = strip * W + lg;
+        bool ok[CPL];
+        const uint64_t *fc[CPL];
+        uint64_t acc[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            ok[c] = c0 + c * GL < ny;
+            fc[c] = F + (ok[c] ? c0 + c * GL : 0);
+            acc[c] = 0;
+        }
         const uint64_t lb = xoff[x], le = xoff[x + 1];
-        uint64_t *grow = G + (uint64_t)x * ny;
-        uint64_t a0 = 0, a1 = 0;
-        for (uint64_t k0 = lb; k0 == lb || k0 < le; k0 += kEdChunk) {
-            const uint32_t cn = (uint32_t)(le - k0 < kEdChunk ? le - k0 : kEdChunk);
-            __syncthreads();
-            for (uint32_t t = threadIdx.x; t < cn; t += blockDim.x) {
-                const uint32_t jj = xval[k0 + t];
-                lj[t] = jj;
-                lw[t] = xw ? u[jj] * xw[k0 + t - xbase] : u[jj];
+        for (uint64_t k0 = lb; k0 < le; k0 += GL) {
+            const uint64_t ke = k0 + lg;
+            uint32_t jl = 0;
+            uint64_t wl = 0;
+            if (ke < le) {
+                jl = xval[ke];
+                wl = xw ? u[jl] * xw[ke - xbase] : u[jl];
             }
-            __syncthreads();
-            if (pairs && va) {
-                uint32_t e = 0;
-                for (; e + 4 <= cn; e += 4) {
-                    const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + ya);
-                    const ulonglong2 f1 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 1] * ny + ya);
-                    const ulonglong2 f2 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 2] * ny + ya);
-                    const ulonglong2 f3 = *(const ulonglong2 *)(F + (uint64_t)lj[e + 3] * ny + ya);
-                    a0 += lw[e] * f0.x + lw[e + 1] * f1.x + lw[e + 2] * f2.x + lw[e + 3] * f3.x;
-                    a1 += lw[e] * f0.y + lw[e + 1] * f1.y + lw[e + 2] * f2.y + lw[e + 3] * f3.y;
+            const uint32_t cnt = (uint32_t)(le - k0 < GL ? le - k0 : GL);
+            uint32_t t = 0;
+            for (; t + 4 <= cnt; t += 4) {
+                uint64_t jr[4], wr[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    jr[q] = (uint64_t)__shfl(jl, (int)t + q, GL) * ny;
+                    wr[q] = __shfl(wl, (int)t + q, GL);
                 }
-                for (; e < cn; ++e) {
-                    const ulonglong2 f0 = *(const ulonglong2 *)(F + (uint64_t)lj[e] * ny + ya);
-                    a0 += lw[e] * f0.x;
-                    a1 += lw[e] * f0.y;
-                }
-            } else if (!pairs) {
-                for (uint32_t e = 0; e < cn; ++e) {
-                    const uint64_t *fr = F + (uint64_t)lj[e] * ny;
-                    if (va) a0 += lw[e] * fr[ya];
-                    if (vb) a1 += lw[e] * fr[yb];
-                }
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (ok[c]) acc[c] += wr[0] * fc[c][jr[0]] + wr[1] * fc[c][jr[1]] + wr[2] * fc[c][jr[2]] + wr[3] * fc[c][jr[3]];
+            }
+            for (; t < cnt; ++t) {
+                const uint64_t j0 = (uint64_t)__shfl(jl, (int)t, GL) * ny;
+                const uint64_t w0 = __shfl(wl, (int)t, GL);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (ok[c]) acc[c] += w0 * fc[c][j0];
             }
         }
-        if (pairs && va) {
-            *(ulonglong2 *)(grow + ya) = make_ulonglong2(a0, a1);
-        } else if (!pairs) {
-            if (va) grow[ya] = a0;
-            if (vb) grow[yb] = a1;
-        }
-        if (threadIdx.x == 0) bytes += 16 + (le - lb) * (12 + 8ull * (ny - c0 < kEdTile ? ny - c0 : kEdTile)) + 8ull * kEdTile;
+        uint64_t *gr = G + (uint64_t)x * ny + c0;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+            if (ok[c]) __builtin_nontemporal_store(acc[c], gr + c * GL);
+        if (lg == 0) bytes += 16 + (le - lb) * (12 + 8ull * W) + 8ull * W;
     }
     wave_add_bytes(bytes, nbytes);
 }
+
+// Strip shape of k_ed: lanes per group and columns per lane.
+constexpr int kEdLanes = 64, kEdCols = 1;
 
 // G = A diag(u) B with A (nx x nv) and B (nv x ny) dense: 64 x 64 output tiles, 16 x 16 threads
 // with 4 x 4 outputs each, K-chunks of 16 staged in LDS.
@@ -403,6 +410,7 @@ __global__ void k_join_ee(const uint64_t *__restrict__ roff, const uint32_t *__r
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t rbase = roff[0], abase = aoff[0], bbase = boff[0];
+    nnz = min(nnz, roff[nx] - rbase);  // nnz: the host's bound on the entries
     uint64_t bytes = 0;
     for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; k < nnz; k += stride) {
         const uint64_t e = rbase + k;
@@ -447,6 +455,7 @@ __global__ void k_join_ed(const uint64_t *__restrict__ roff, const uint32_t *__r
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t rbase = roff[0], lbase = loff[0];
+    nnz = min(nnz, roff[nx] - rbase);  // nnz: the host's bound on the entries
     uint64_t bytes = 0;
     for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; k < nnz; k += stride) {
         const uint64_t e = rbase + k;
@@ -480,6 +489,7 @@ __global__ void k_join_dd(const uint64_t *__restrict__ roff, const uint32_t *__r
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x / 64);
     const uint64_t rbase = roff[0];
+    nnz = min(nnz, roff[nx] - rbase);  // nnz: the host's bound on the entries
     uint64_t bytes = 0;
     for (uint64_t k = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; k < nnz; k += nw) {
         const uint64_t e = rbase + k;
@@ -501,6 +511,7 @@ __global__ void k_join_reverse(const uint64_t *__restrict__ foff, const uint32_t
                                const uint64_t *__restrict__ wf, const uint64_t *__restrict__ roff,
                                const uint32_t *__restrict__ rval, uint32_t ny, uint64_t nnz, uint64_t *__restrict__ wr) {
     const uint64_t fbase = foff[0], rbase = roff[0];
+    nnz = min(nnz, roff[ny] - rbase);  // nnz: the host's bound on the entries
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = rbase + k;
         const uint32_t y = row_of(roff, ny, e), x = rval[e];
@@ -557,7 +568,7 @@ struct Dom {
 struct Rel {
     const uint64_t *off = nullptr;
     const uint32_t *val = nullptr;
-    uint64_t nnz = 0;
+    uint64_t nnz = 0;  // entries, or an upper bound for a relation built on the device (exact: off[rows] - off[0])
     const uint64_t *w = nullptr;  // per-entry weights, w[e - off[0]] for entry e (null: every weight is 1)
 };
 struct Table {  // dense factor of a block pair: row-major, rows indexed by D(rows), columns by the other
@@ -770,11 +781,10 @@ struct BagRun {
         }
         if ((rc = launched("bag relation count"))) return rc;
         if ((rc = scan_u32_to_u64(ctx, cnt, off, A.n))) return rc;
-        uint64_t nnz = 0;
-        rc = check_hip(ctx, hipMemcpyAsync(&nnz, off + A.n, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
-        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
-        if (!rc) rc = alloc((void **)&val, 4ull * std::max<uint64_t>(nnz, 1));
-        if (rc) return rc;
+        // the relation is a subset of the arc's candidate edges: allocate for all of them rather
+        // than wait for the count (no host synchronization); its exact size is off[A.n]
+        const uint64_t nnz = cs->arc_weight[arc];
+        if ((rc = alloc((void **)&val, 4ull * std::max<uint64_t>(nnz, 1)))) return rc;
         {
             KTimer kt(ctx, "bag_relation");
             hipLaunchKernelGGL(k_rel<true>, dim3(grid_for((uint64_t)A.n * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
@@ -1085,13 +1095,16 @@ struct BagRun {
             const Rel re = relv(dense_x ? fvy : fxv, e);
             const uint32_t ne = dom[e].n, nd = dom[d].n;
             KTimer kt(ctx, kname);
-            const uint64_t items = (uint64_t)ne * ((nd + kEdTile - 1) / kEdTile);
-            const dim3 grid((unsigned)std::min<uint64_t>(std::max<uint64_t>(items, 1), 1u << 15));
+            // a resident grid, a multiple of 8 (one share per XCD): at most 8 workgroups per CU
+            constexpr int gl = kEdLanes, cpl = kEdCols;
+            const uint64_t items = (uint64_t)ne * ((nd + gl * cpl - 1) / (gl * cpl));
+            const uint64_t want = (items + kThreads / gl - 1) / (kThreads / gl);
+            const dim3 grid((unsigned)(8 * std::max<uint64_t>(1, std::min<uint64_t>((want + 7) / 8, ctx->num_cus))));
             if (dense_x)
-                hipLaunchKernelGGL(k_ed<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
+                hipLaunchKernelGGL((k_ed<1, gl, cpl>), grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
                                    bytes_slot(9));
             else
-                hipLaunchKernelGGL(k_ed<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
+                hipLaunchKernelGGL((k_ed<0, gl, cpl>), grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
                                    bytes_slot(10));
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);

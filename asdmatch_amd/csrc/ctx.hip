@@ -312,6 +312,7 @@ int asdm_ctx_create(int device, asdm_ctx **out) try {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ASDM_E_NODEVICE;
     auto ctx = std::make_unique<asdm_ctx>();
     ctx->device = device;
+    ctx->num_cus = (uint32_t)std::max(8, prop.multiProcessorCount);
     ctx->err.reserve(512);  // room for any fixed message without allocating when it is set
     if (hipSetDevice(device) != hipSuccess) return ASDM_E_HIP;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return ASDM_E_HIP;

@@ -85,6 +85,7 @@ struct DPlan {
 
 struct asdm_ctx {
     int device = 0;
+    uint32_t num_cus = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     std::string err;
     asdm_stats stats{};
