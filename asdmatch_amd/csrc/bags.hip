@@ -68,7 +68,7 @@ __global__ void k_dom_fill(const uint64_t *__restrict__ bits, uint32_t W, const 
 // (a in A, b in B): row i (data vertex ids_a[i]) holds the D(B) positions of its neighbours in
 // D(B), ascending.  Count pass (FILL = false), then fill pass at the scanned offsets.  kRelLanes
 // lanes per row stride over the row's candidate-edge list; the fill keeps list order with a
-// ballot prefix inside the lane group.
+// ballot prefix inside the lane group.  (A wave per row was twice as slow on config 4.)
 constexpr int kRelLanes = 16;
 template <bool FILL>
 __global__ void k_rel(const uint32_t *__restrict__ ids_a, uint32_t na, const uint64_t *__restrict__ bits_a,
@@ -820,7 +820,8 @@ struct BagRun {
     }
 
     // Runs one elimination sequence.  On return *scal (device) holds the product of the sums of
-    // the blocks eliminated with no live neighbour, and u[keep] the marginal of `keep`.
+    // the blocks eliminated with no live neighbour, and u[keep] the marginal of `keep`; the
+    // vectors u[b] share one allocation, u[0], which the caller releases.
     int eliminate(const BagTerm &T, const std::vector<BagStep> &steps, const std::vector<Dom> &dom,
                   const std::map<std::pair<uint32_t, uint32_t>, std::pair<Rel, Rel>> &rels, std::vector<uint64_t *> &u,
                   uint64_t *scal) {
@@ -829,12 +830,14 @@ struct BagRun {
         for (uint32_t b = 0; b < nb; ++b)
             for (uint32_t q : T.blocks[b]) mask[b] |= 1u << q;
         std::vector<std::vector<uint32_t>> usig(nb);  // blocks eliminated into u[b]
+        // the vectors of all blocks in one buffer (u[0] is its start), set to 1 by one fill
         u.assign(nb, nullptr);
-        int rc = ASDM_OK;
-        for (uint32_t b = 0; b < nb && !rc; ++b) {
-            rc = alloc((void **)&u[b], 8ull * std::max<uint32_t>(dom[b].n, 1));
-            if (!rc) rc = fill_u64(ctx, u[b], dom[b].n, 1);
-        }
+        uint64_t total = 0;
+        for (uint32_t b = 0; b < nb; ++b) total += std::max<uint32_t>(dom[b].n, 1);
+        int rc = alloc((void **)&u[0], 8 * total);
+        if (rc) return rc;
+        for (uint32_t b = 1; b < nb; ++b) u[b] = u[b - 1] + std::max<uint32_t>(dom[b - 1].n, 1);
+        if (!rc) rc = fill_u64(ctx, u[0], total, 1);
         if (!rc) rc = fill_u64(ctx, scal, 1, 1);
         if (rc) return rc;
         std::map<std::pair<uint32_t, uint32_t>, Fac> fac;
@@ -1152,7 +1155,7 @@ struct BagRun {
         if ((rc = eliminate(T, T.count_steps, dom, rels, u, scal))) return rc;
         hipLaunchKernelGGL(k_total_add, dim3(1), dim3(64), 0, ctx->stream, d_acc, scal, T.mu);
         if ((rc = launched("bag total"))) return rc;
-        for (uint64_t *p : u) release(p);
+        release(u[0]);
         if (d_mat) {
             for (uint32_t t = 0; t < nb; ++t) {
                 if ((rc = eliminate(T, T.local_steps[t], dom, rels, u, scal))) return rc;
@@ -1164,7 +1167,7 @@ struct BagRun {
                                    dom[t].ids, dom[t].n, u[t], scal, T.mu, m, cs->nq, d_mat);
                 kt.stop(12ull * dom[t].n + 16ull * dom[t].n * m.n);
                 if ((rc = launched("bag local scatter"))) return rc;
-                for (uint64_t *p : u) release(p);
+                release(u[0]);
             }
         }
         return ASDM_OK;

@@ -167,11 +167,22 @@ static int kstat_slot(asdm_ctx *ctx, const char *name) {
     return (int)ctx->kstats.size() - 1;
 }
 
+static hipEvent_t take_event(asdm_ctx *ctx) {
+    hipEvent_t e = nullptr;
+    if (!ctx->event_pool.empty()) {
+        e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+    } else {
+        (void)hipEventCreate(&e);
+    }
+    return e;
+}
+
 KTimer::KTimer(asdm_ctx *c, const char *name) : ctx(c), slot(-1) {
     if (!ctx->profiling) return;
     slot = kstat_slot(ctx, name);
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
+    a = take_event(ctx);
+    b = take_event(ctx);
     (void)hipEventRecord(a, ctx->stream);
 }
 
@@ -192,8 +203,8 @@ void resolve_timers(asdm_ctx *ctx) {
         (void)hipEventElapsedTime(&ms, t.a, t.b);
         ctx->kstats[t.slot].ms += ms;
         ctx->kstats[t.slot].launches += 1;
-        (void)hipEventDestroy(t.a);
-        (void)hipEventDestroy(t.b);
+        ctx->event_pool.push_back(t.a);
+        ctx->event_pool.push_back(t.b);
     }
     ctx->pending.clear();
 }
@@ -332,6 +343,11 @@ void asdm_ctx_destroy(asdm_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->cache) (void)hipFree(b.p);
     for (auto &b : ctx->live) (void)hipFree(b.p);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto &t : ctx->pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
     (void)hipFree(ctx->d_counters);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -111,6 +111,7 @@ struct asdm_ctx {
         hipEvent_t a, b;
     };
     std::vector<PendingTimer> pending;  // recorded, resolved after the stream has drained
+    std::vector<hipEvent_t> event_pool;  // timer events reused across launches
     // device address of this device's allocation-free scratch (asdm_intersect_k_noalloc), looked
     // up once per device when a context is created
     uint8_t *noalloc_scratch = nullptr;
