@@ -6,6 +6,7 @@
 #include <chrono>
 #include <map>
 #include <memory>
+#include <set>
 
 #include "bags.h"
 #include "engine.h"
@@ -64,37 +65,55 @@ __global__ void k_dom_fill(const uint64_t *__restrict__ bits, uint32_t W, const 
     }
 }
 
-// Relation CSR of a quotient edge A -> B from the candidate-edge lists of a query arc a -> b
-// (a in A, b in B): row i (data vertex ids_a[i]) holds the D(B) positions of its neighbours in
-// D(B), ascending.  Count pass (FILL = false), then fill pass at the scanned offsets.  kRelLanes
-// lanes per row stride over the row's candidate-edge list; the fill keeps list order with a
-// ballot prefix inside the lane group.  (A wave per row was twice as slow on config 4.)
+// Relation CSRs of quotient edges A -> B from the candidate-edge lists of query arcs a -> b
+// (a in A, b in B): row i of a relation (data vertex ids_a[i]) holds the D(B) positions of its
+// neighbours in D(B), ascending.  One launch builds the relations of many jobs: their rows are
+// numbered one after the other (job j: rows row0[j] .. row0[j + 1]), and one scan over all rows'
+// counts gives every job's offsets into one shared value array.  Count pass (FILL = false), then
+// fill pass at the scanned offsets.  kRelLanes lanes per row stride over the row's candidate-edge
+// list; the fill keeps list order with a ballot prefix inside the lane group.  (A wave per row
+// was twice as slow on config 4.)
 constexpr int kRelLanes = 16;
+struct RelJob {
+    const uint32_t *ids_a;   // D(A), ascending data ids
+    const uint64_t *bits_a;  // membership bitmap and rank prefix of C(a)
+    const uint32_t *rank_a;
+    const uint64_t *ce_off;  // candidate-edge offsets of arc a -> b (indexed by position in C(a))
+    const uint32_t *cand_b;  // C(b)
+    const uint64_t *bits_B;  // membership bitmap and rank prefix of D(B)
+    const uint32_t *rank_B;
+};
 template <bool FILL>
-__global__ void k_rel(const uint32_t *__restrict__ ids_a, uint32_t na, const uint64_t *__restrict__ bits_a,
-                      const uint32_t *__restrict__ rank_a, const uint64_t *__restrict__ ce_off,
-                      const uint32_t *__restrict__ ce_val, const uint32_t *__restrict__ cand_b,
-                      const uint64_t *__restrict__ bits_B, const uint32_t *__restrict__ rank_B, uint32_t *__restrict__ cnt,
-                      const uint64_t *__restrict__ off, uint32_t *__restrict__ val, unsigned long long *__restrict__ nbytes) {
+__global__ void k_rel(const RelJob *__restrict__ jobs, const uint64_t *__restrict__ row0, uint32_t nj,
+                      const uint32_t *__restrict__ ce_val, uint32_t *__restrict__ cnt, const uint64_t *__restrict__ off,
+                      uint32_t *__restrict__ val, unsigned long long *__restrict__ nbytes) {
     constexpr int G = kRelLanes;
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+    const uint64_t rows = row0[nj];
     uint64_t bytes = 0;
     // one row per aligned group of G lanes: the group's lanes share the row and its trip count, so
     // the ballots see the whole group (other groups of the wave may have left their loops)
-    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; i < na; i += stride) {
-        const uint32_t p = bit_rank(bits_a, rank_a, ids_a[i]);
-        const uint64_t lb = ce_off[p], le = ce_off[p + 1];
-        uint64_t o = FILL ? off[i] : 0;
+    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; r < rows; r += stride) {
+        uint32_t lo = 0, hi = nj - 1;  // the job holding row r: the last j with row0[j] <= r
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (row0[mid] <= r) lo = mid;
+            else hi = mid - 1;
+        }
+        const RelJob J = jobs[lo];
+        const uint32_t p = bit_rank(J.bits_a, J.rank_a, J.ids_a[r - row0[lo]]);
+        const uint64_t lb = J.ce_off[p], le = J.ce_off[p + 1];
+        uint64_t o = FILL ? off[r] : 0;
         uint32_t c = 0;
         for (uint64_t e0 = lb; e0 < le; e0 += G) {
             const uint64_t e = e0 + lg;
             bool ok = false;
             uint32_t pos = 0;
             if (e < le) {
-                const uint32_t w = cand_b[ce_val[e]];
-                ok = bit_test(bits_B, w);
-                if (FILL && ok) pos = bit_rank(bits_B, rank_B, w);
+                const uint32_t w = J.cand_b[ce_val[e]];
+                ok = bit_test(J.bits_B, w);
+                if (FILL && ok) pos = bit_rank(J.bits_B, J.rank_B, w);
             }
             if constexpr (FILL) {
                 const uint64_t gm = group_bits<G>(__ballot(ok));
@@ -106,9 +125,9 @@ __global__ void k_rel(const uint32_t *__restrict__ ids_a, uint32_t na, const uin
         }
         if constexpr (!FILL) {
             c = (uint32_t)group_sum<G>(c);
-            if (lg == 0) cnt[i] = c;
+            if (lg == 0) cnt[r] = c;
         }
-        if (lg == 0) bytes += 28 + 16 * (le - lb) + (FILL ? 4 * (o - off[i]) : 0);
+        if (lg == 0) bytes += 28 + 16 * (le - lb) + (FILL ? 4 * (o - off[r]) : 0);
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -615,6 +634,8 @@ struct BagRun {
     // Factor results by signature (kind, boundary masks, eliminated masks), kept for the query;
     // cache_cap bounds their bytes (beyond it results are computed and dropped as before).
     std::map<std::vector<uint32_t>, CacheEnt> fcache;
+    std::vector<std::vector<RelJob>> host_jobs;     // host sources of queued job uploads
+    std::vector<std::vector<uint64_t>> host_row0;
     uint64_t cache_bytes = 0, cache_cap = 0, reuse_n = 0, reuse_bytes = 0;
     std::chrono::steady_clock::time_point t0;
 
@@ -695,23 +716,145 @@ struct BagRun {
         return rc;
     }
 
-    int cached_rel(uint32_t arc, const std::vector<uint32_t> &ma, const Dom &A, const std::vector<uint32_t> &mb,
-                   const Dom &B, Rel &r) {
-        if (ma.size() == 1 && mb.size() == 1) return build_rel(arc, A, true, B, true, r);
+    // One relation the executor needs: rows D(A) through query arc `arc` (u_a in A, u_b in B).
+    struct RelNeed {
+        uint32_t arc;
+        std::vector<uint32_t> ma, mb;  // members of A and B
+        Dom A, B;
+    };
+    static std::pair<std::vector<uint32_t>, std::vector<uint32_t>> rel_key(const std::vector<uint32_t> &ma,
+                                                                        const std::vector<uint32_t> &mb) {
         std::vector<uint32_t> ka(ma), kb(mb);
         std::sort(ka.begin(), ka.end());
         std::sort(kb.begin(), kb.end());
-        auto key = std::make_pair(ka, kb);
-        auto it = rel_cache.find(key);
-        if (it != rel_cache.end()) {
-            r = it->second;
+        return {ka, kb};
+    }
+
+    int cached_rel(uint32_t arc, const std::vector<uint32_t> &ma, const Dom &A, const std::vector<uint32_t> &mb,
+                   const Dom &B, Rel &r) {
+        if (ma.size() == 1 && mb.size() == 1) {  // the candidate-edge lists are the relation
+            r.off = cs->d_ce_off + cs->ce_base[arc];
+            r.val = cs->d_ce_val;
+            r.nnz = cs->arc_weight[arc];
+            r.w = nullptr;
             return ASDM_OK;
         }
-        keep_allocs = true;
-        const int rc = build_rel(arc, A, ma.size() == 1, B, mb.size() == 1, r);
+        const auto key = rel_key(ma, mb);
+        auto it = rel_cache.find(key);
+        if (it == rel_cache.end()) {
+            int rc = build_rels({RelNeed{arc, ma, mb, A, B}});
+            if (rc) return rc;
+            it = rel_cache.find(key);
+        }
+        r = it->second;
+        return ASDM_OK;
+    }
+
+    // Builds the relations of `needs` (none of them over two single vertices) in one count launch,
+    // one scan and one fill launch, without a host synchronization; they go to the cache.
+    int build_rels(const std::vector<RelNeed> &needs) {
+        const uint32_t nj = (uint32_t)needs.size();
+        if (!nj) return ASDM_OK;
+        std::vector<RelJob> jobs(nj);
+        std::vector<uint64_t> row0(nj + 1, 0);
+        uint64_t ub = 0;  // each relation is a subset of its arc's candidate edges
+        for (uint32_t j = 0; j < nj; ++j) {
+            const RelNeed &n = needs[j];
+            const uint32_t ua = cs->arc_u[n.arc], ub_ = cs->arc_u2[n.arc];
+            jobs[j] = RelJob{n.A.ids, cs->d_bits + (size_t)ua * cs->W, cs->d_rank + (size_t)ua * cs->W,
+                             cs->d_ce_off + cs->ce_base[n.arc], cs->d_cand + cs->cand_off[ub_], n.B.bits, n.B.rank};
+            row0[j + 1] = row0[j] + n.A.n;
+            ub += cs->arc_weight[n.arc];
+        }
+        const uint64_t rows = row0[nj];
+        keep_allocs = true;  // offsets and values live as long as the cache
+        uint64_t *off = nullptr;
+        uint32_t *val = nullptr;
+        int rc = alloc((void **)&off, 8ull * (rows + 1));
+        if (!rc) rc = alloc((void **)&val, 4ull * std::max<uint64_t>(ub, 1));
         keep_allocs = false;
-        if (!rc) rel_cache[key] = r;
-        return rc;
+        RelJob *d_jobs = nullptr;
+        uint64_t *d_row0 = nullptr;
+        uint32_t *cnt = nullptr;
+        if (!rc) rc = alloc((void **)&d_jobs, sizeof(RelJob) * nj);
+        if (!rc) rc = alloc((void **)&d_row0, 8ull * (nj + 1));
+        if (!rc) rc = alloc((void **)&cnt, 4ull * std::max<uint64_t>(rows, 1));
+        if (rc) return rc;
+        // the host copies stay alive until the queued copies have run (the run's end)
+        host_jobs.push_back(std::move(jobs));
+        host_row0.push_back(std::move(row0));
+        rc = check_hip(ctx, hipMemcpyAsync(d_jobs, host_jobs.back().data(), sizeof(RelJob) * nj, hipMemcpyHostToDevice, ctx->stream),
+                       "H2D relation jobs");
+        if (!rc)
+            rc = check_hip(ctx, hipMemcpyAsync(d_row0, host_row0.back().data(), 8ull * (nj + 1), hipMemcpyHostToDevice, ctx->stream),
+                           "H2D relation rows");
+        if (rc) return rc;
+        const dim3 grid(grid_for(std::max<uint64_t>(rows, 1) * kRelLanes, kThreads));
+        {
+            KTimer kt(ctx, "bag_relation");
+            hipLaunchKernelGGL(k_rel<false>, grid, dim3(kThreads), 0, ctx->stream, d_jobs, d_row0, nj, cs->d_ce_val, cnt, nullptr,
+                               nullptr, bytes_slot(0));
+            kt.stop();
+        }
+        if ((rc = launched("bag relation count"))) return rc;
+        if ((rc = scan_u32_to_u64(ctx, cnt, off, rows))) return rc;
+        {
+            KTimer kt(ctx, "bag_relation");
+            hipLaunchKernelGGL(k_rel<true>, grid, dim3(kThreads), 0, ctx->stream, d_jobs, d_row0, nj, cs->d_ce_val, nullptr, off,
+                               val, bytes_slot(0));
+            kt.stop();
+        }
+        if ((rc = launched("bag relation fill"))) return rc;
+        release(cnt);
+        release(d_jobs);
+        release(d_row0);
+        const std::vector<uint64_t> &r0 = host_row0.back();
+        for (uint32_t j = 0; j < nj; ++j) {
+            Rel r;
+            r.off = off + r0[j];
+            r.val = val;
+            r.nnz = cs->arc_weight[needs[j].arc];
+            rel_cache[rel_key(needs[j].ma, needs[j].mb)] = r;
+        }
+        return ASDM_OK;
+    }
+
+    // The arcs of quotient edge e of term T, oriented: ab from a member of T.edges[e].first.
+    std::pair<uint32_t, uint32_t> edge_arcs(const BagTerm &T, size_t e) const {
+        uint32_t ab = T.arc_ab[e], ba = T.arc_ba[e];
+        bool a_from_A = false;
+        for (uint32_t u : T.blocks[T.edges[e].first]) a_from_A |= cs->arc_u[ab] == u;
+        if (!a_from_A) std::swap(ab, ba);
+        return {ab, ba};
+    }
+
+    // Builds, before the first term, every merged-block relation that some term will use, in one
+    // batched build (the terms of a query share most of them).
+    int prebuild() {
+        std::vector<RelNeed> needs;
+        std::set<std::pair<std::vector<uint32_t>, std::vector<uint32_t>>> seen;
+        for (const BagTerm &T : plan->terms) {
+            const uint32_t nb = (uint32_t)T.blocks.size();
+            std::vector<Dom> dom(nb);
+            int rc = ASDM_OK;
+            for (uint32_t b = 0; b < nb && !rc; ++b) rc = cached_dom(T.blocks[b], dom[b]);
+            if (rc) return rc;
+            bool empty = false;
+            for (uint32_t b = 0; b < nb; ++b) empty |= dom[b].n == 0;
+            if (empty) continue;  // the term is 0 and builds nothing
+            for (size_t e = 0; e < T.edges.size(); ++e) {
+                const uint32_t A = T.edges[e].first, B = T.edges[e].second;
+                if (T.blocks[A].size() == 1 && T.blocks[B].size() == 1) continue;
+                const auto arcs = edge_arcs(T, e);
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t P = side ? B : A, Q = side ? A : B;
+                    const auto key = rel_key(T.blocks[P], T.blocks[Q]);
+                    if (rel_cache.count(key) || !seen.insert(key).second) continue;
+                    needs.push_back(RelNeed{side ? arcs.second : arcs.first, T.blocks[P], T.blocks[Q], dom[P], dom[Q]});
+                }
+            }
+        }
+        return build_rels(needs);
     }
 
     int build_dom(const std::vector<uint32_t> &members, Dom &d) {
@@ -752,51 +895,6 @@ struct BagRun {
         d.bits = bits;
         d.rank = rank;
         d.n = (uint32_t)n;
-        return ASDM_OK;
-    }
-
-    // Relation A -> B through query arc `arc` (u_a in A, u_b in B).
-    int build_rel(uint32_t arc, const Dom &A, bool a_single, const Dom &B, bool b_single, Rel &r) {
-        const uint32_t ua = cs->arc_u[arc], ub = cs->arc_u2[arc];
-        if (a_single && b_single) {  // the candidate-edge lists are the relation
-            r.off = cs->d_ce_off + cs->ce_base[arc];
-            r.val = cs->d_ce_val;
-            r.nnz = cs->arc_weight[arc];
-            return ASDM_OK;
-        }
-        uint32_t *cnt = nullptr, *val = nullptr;
-        uint64_t *off = nullptr;
-        int rc = alloc((void **)&cnt, 4ull * std::max<uint32_t>(A.n, 1));
-        if (!rc) rc = alloc((void **)&off, 8ull * (A.n + 1));
-        if (rc) return rc;
-        const uint64_t *bits_a = cs->d_bits + (size_t)ua * cs->W;
-        const uint32_t *rank_a = cs->d_rank + (size_t)ua * cs->W;
-        const uint32_t *cand_b = cs->d_cand + cs->cand_off[ub];
-        {
-            KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel<false>, dim3(grid_for((uint64_t)A.n * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
-                               bits_a, rank_a, cs->d_ce_off + cs->ce_base[arc], cs->d_ce_val, cand_b, B.bits, B.rank, cnt,
-                               nullptr, nullptr, bytes_slot(0));
-            kt.stop();
-        }
-        if ((rc = launched("bag relation count"))) return rc;
-        if ((rc = scan_u32_to_u64(ctx, cnt, off, A.n))) return rc;
-        // the relation is a subset of the arc's candidate edges: allocate for all of them rather
-        // than wait for the count (no host synchronization); its exact size is off[A.n]
-        const uint64_t nnz = cs->arc_weight[arc];
-        if ((rc = alloc((void **)&val, 4ull * std::max<uint64_t>(nnz, 1)))) return rc;
-        {
-            KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel<true>, dim3(grid_for((uint64_t)A.n * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream, A.ids, A.n,
-                               bits_a, rank_a, cs->d_ce_off + cs->ce_base[arc], cs->d_ce_val, cand_b, B.bits, B.rank,
-                               nullptr, off, val, bytes_slot(0));
-            kt.stop();
-        }
-        if ((rc = launched("bag relation fill"))) return rc;
-        release(cnt);
-        r.off = off;
-        r.val = val;
-        r.nnz = nnz;
         return ASDM_OK;
     }
 
@@ -1140,10 +1238,7 @@ struct BagRun {
         for (size_t e = 0; e < T.edges.size() && !rc; ++e) {
             const uint32_t A = T.edges[e].first, B = T.edges[e].second;
             // the arcs run from the member of one block to the member of the other: orient them
-            uint32_t ab = T.arc_ab[e], ba = T.arc_ba[e];
-            bool a_from_A = false;
-            for (uint32_t u : T.blocks[A]) a_from_A |= cs->arc_u[ab] == u;
-            if (!a_from_A) std::swap(ab, ba);
+            const auto [ab, ba] = edge_arcs(T, e);
             Rel r1, r2;
             rc = cached_rel(ab, T.blocks[A], dom[A], T.blocks[B], dom[B], r1);
             if (!rc) rc = cached_rel(ba, T.blocks[B], dom[B], T.blocks[A], dom[A], r2);
@@ -1199,6 +1294,7 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
     (void)hipEventCreate(&ea);
     (void)hipEventCreate(&eb);
     (void)hipEventRecord(ea, ctx->stream);
+    if (!rc) rc = R.prebuild();
     for (size_t t = 0; t < B.terms.size() && !rc; ++t) {
         rc = R.run_term(B.terms[t]);
         R.release_all();
