@@ -106,21 +106,30 @@ __global__ void k_rel(const RelJob *__restrict__ jobs, const uint64_t *__restric
         const uint64_t lb = J.ce_off[p], le = J.ce_off[p + 1];
         uint64_t o = FILL ? off[r] : 0;
         uint32_t c = 0;
-        for (uint64_t e0 = lb; e0 < le; e0 += G) {
-            const uint64_t e = e0 + lg;
-            bool ok = false;
-            uint32_t pos = 0;
-            if (e < le) {
-                const uint32_t w = J.cand_b[ce_val[e]];
-                ok = bit_test(J.bits_B, w);
-                if (FILL && ok) pos = bit_rank(J.bits_B, J.rank_B, w);
+        // four entries per lane per pass, their lookup chains in flight together (a hub's list
+        // holds thousands of entries); the passes and the ballots keep list order
+        constexpr int Q = 4;
+        for (uint64_t e0 = lb; e0 < le; e0 += Q * G) {
+            uint32_t w[Q];
+            bool ok[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const uint64_t e = e0 + q * G + lg;
+                ok[q] = e < le;
+                w[q] = J.cand_b[ce_val[ok[q] ? e : lb]];
             }
-            if constexpr (FILL) {
-                const uint64_t gm = group_bits<G>(__ballot(ok));
-                if (ok) val[o + __popcll(gm & ((1ull << lg) - 1))] = pos;
-                o += __popcll(gm);
-            } else {
-                c += ok;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) ok[q] = ok[q] && bit_test(J.bits_B, w[q]);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                if constexpr (FILL) {
+                    const uint32_t pos = ok[q] ? bit_rank(J.bits_B, J.rank_B, w[q]) : 0;
+                    const uint64_t gm = group_bits<G>(__ballot(ok[q]));
+                    if (ok[q]) val[o + __popcll(gm & ((1ull << lg) - 1))] = pos;
+                    o += __popcll(gm);
+                } else {
+                    c += ok[q];
+                }
             }
         }
         if constexpr (!FILL) {
@@ -274,6 +283,8 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
 // written with non-temporal stores, which keep it out of L2.  Measured on the config-4 batch
 // (tables of ~7.5k x 7k): 64 lanes x 1 column beat 32 x 1 by 23 %, 16 x 4, 32 x 2, 32 x 4 and
 // 64 x 2 were slower, and a wave-per-item form with the list in scalar registers was 7 % slower.
+// The push form (each dense row read once, its entries added into an LDS row over the relation's
+// lists with 64-bit LDS atomics) was four times slower: ~450M random LDS atomics per table.
 // SIDE only names the instantiation after the step that launches it (0: edge (X, v) x dense
 // (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed side), so that
 // profilers report the two apart.
@@ -483,11 +494,23 @@ __global__ void k_join_ed(const uint64_t *__restrict__ roff, const uint32_t *__r
         const uint64_t l0 = loff[o], l1 = loff[o + 1];
         const uint64_t *fa = F + a * sa;
         uint64_t acc = 0;
-        for (uint64_t t = l0 + lg; t < l1; t += G) {
-            const uint32_t j = lval[t];
-            uint64_t m = u[j] * fa[(uint64_t)j * sj];
-            if (lw) m *= lw[t - lbase];
-            acc += m;
+        // four entries per lane per pass (a hub's list is long and every one of its edges walks it)
+        constexpr int Q = 4;
+        for (uint64_t t0 = l0 + lg; t0 < l1; t0 += Q * G) {
+            uint32_t j[Q];
+            bool in[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const uint64_t t = t0 + q * G;
+                in[q] = t < l1;
+                j[q] = lval[in[q] ? t : l0];
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                uint64_t m = u[j[q]] * fa[(uint64_t)j[q] * sj];
+                if (lw) m *= lw[(in[q] ? t0 + q * G : l0) - lbase];
+                acc += in[q] ? m : 0;
+            }
         }
         acc = group_sum<G>(acc);
         if (lg == 0) {
