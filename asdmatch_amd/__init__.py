@@ -130,6 +130,8 @@ _sig("asdm_plan_create_learned", C.c_int, _P, _P, _P, _P, C.c_uint32, C.POINTER(
 _sig("asdm_plain_estimate", C.c_int, _P, _u64p, C.c_uint32, C.POINTER(C.c_double))
 _sig("asdm_ctx_set_batch_tuning", C.c_int, _P, C.c_uint32, C.c_double, C.c_uint32)
 _sig("asdm_ctx_get_batch_tuning", C.c_int, _P, _u32p, C.POINTER(C.c_double), _u32p)
+_sig("asdm_ctx_set_bag_workers", C.c_int, _P, C.c_uint32)
+_sig("asdm_ctx_get_bag_workers", C.c_int, _P, _u32p)
 _sig("asdm_batch_autotune", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _P, C.c_uint32, _u32p,
      C.POINTER(C.c_double), _u32p, _f32p)
 
@@ -338,6 +340,15 @@ class Context:
         sc = C.c_double()
         self._check(_lib.asdm_ctx_get_batch_tuning(self._h, C.byref(l), C.byref(sc), C.byref(t)), "get_batch_tuning")
         return l.value, sc.value, t.value
+
+    def bag_workers(self) -> int:
+        """Worker contexts (streams) that count a batch's bag-engine queries concurrently."""
+        n = C.c_uint32()
+        self._check(_lib.asdm_ctx_get_bag_workers(self._h, C.byref(n)), "get_bag_workers")
+        return n.value
+
+    def set_bag_workers(self, n: int):
+        self._check(_lib.asdm_ctx_set_bag_workers(self._h, n), "set_bag_workers")
 
     def set_batch_tuning(self, leaf_lanes: int, class_scale: float, tile_iters: int):
         self._check(_lib.asdm_ctx_set_batch_tuning(self._h, leaf_lanes, float(class_scale), tile_iters),

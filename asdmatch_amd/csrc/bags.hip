@@ -900,17 +900,21 @@ struct BagRun {
         if (!rc) rc = alloc((void **)&off, 8ull * (W + 1));
         if (!rc) rc = alloc((void **)&rank, 4ull * W);
         if (rc) return rc;
+        // two timed spans, so that the host synchronization between them is not counted as
+        // kernel time
         KTimer kt(ctx, "bag_domain");
         hipLaunchKernelGGL(k_dom_words, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, ctx->stream, cs->d_bits, W, m, bits, cnt);
         if ((rc = launched("bag domain words"))) return rc;
         if ((rc = scan_u32_to_u64(ctx, cnt, off, W))) return rc;
+        kt.stop(8ull * W * m.n + 12ull * W + 8ull * (W + 1));
         uint64_t n = 0;
         rc = check_hip(ctx, hipMemcpyAsync(&n, off + W, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
         if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (!rc) rc = alloc((void **)&ids, 4ull * std::max<uint64_t>(n, 1));
         if (rc) return rc;
+        KTimer kf(ctx, "bag_domain");
         hipLaunchKernelGGL(k_dom_fill, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, ctx->stream, bits, W, off, rank, ids);
-        kt.stop(12ull * W + 4 * n);
+        kf.stop(8ull * W + 8ull * W + 4ull * W + 4 * n);
         if ((rc = launched("bag domain fill"))) return rc;
         release(cnt);
         release(off);
@@ -1304,7 +1308,7 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
     {
         size_t freeb = 0, totalb = 0;
         (void)hipMemGetInfo(&freeb, &totalb);
-        R.cache_cap = std::min<uint64_t>(freeb / 3, kBagCacheBytes);
+        R.cache_cap = (uint64_t)((double)std::min<uint64_t>(freeb / 3, kBagCacheBytes) * ctx->mem_share);
     }
     R.d_acc = (unsigned long long *)ctx->d_counters;
     int rc = fill_u64(ctx, ctx->d_counters, kNumCounters, 0);

@@ -17,6 +17,8 @@ int check_hip(asdm_ctx *ctx, hipError_t e, const char *what) {
     return e == hipErrorOutOfMemory ? ASDM_E_OOM : ASDM_E_HIP;
 }
 
+// Frees every cached block (after the context's stream has drained: a cached block's last use was
+// on that stream).
 static void trim_cache(asdm_ctx *ctx) {
     if (ctx->cache.empty()) return;
     (void)hipStreamSynchronize(ctx->stream);
@@ -25,13 +27,45 @@ static void trim_cache(asdm_ctx *ctx) {
     ctx->cached_bytes = 0;
 }
 
+// Frees the least recently cached blocks until at most `keep` bytes stay cached.  hipFree waits
+// for the whole device, so a trim is rare and keeps the blocks most likely to be asked for again.
+static void trim_cache_to(asdm_ctx *ctx, size_t keep) {
+    if (ctx->cached_bytes <= keep) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    size_t i = 0;  // the cache is in the order the blocks were freed
+    while (i < ctx->cache.size() && ctx->cached_bytes > keep) {
+        (void)hipFree(ctx->cache[i].p);
+        ctx->cached_bytes -= ctx->cache[i].bytes;
+        ++i;
+    }
+    ctx->cache.erase(ctx->cache.begin(), ctx->cache.begin() + i);
+    ++ctx->trims;
+}
+
+// Size class of a request: 256-byte granules below 64 KiB, above that eight classes per power of
+// two (at most 12.5 % over the request), so blocks freed by one query fit the next one's requests.
+static size_t size_class(size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
+    if (bytes <= (64u << 10)) return bytes;
+    int top = 63 - __builtin_clzll((unsigned long long)bytes);
+    const size_t step = (size_t)1 << (top - 3);
+    return (bytes + step - 1) & ~(step - 1);
+}
+
+// Bytes of cached (free) blocks the context keeps: its share of 96 GiB.
+static size_t cache_limit(const asdm_ctx *ctx) { return (size_t)((double)(96ull << 30) * ctx->mem_share); }
+
 int dev_alloc(asdm_ctx *ctx, void **p, size_t bytes) {
     *p = nullptr;
-    bytes = (std::max<size_t>(bytes, 16) + 255) & ~(size_t)255;
-    // best fit among cached blocks no more than 2x (+1 MiB) larger
+    bytes = size_class(bytes);
+    // the same class, else the best fit among cached blocks no more than 2x (+1 MiB) larger
     int best = -1;
     for (size_t i = 0; i < ctx->cache.size(); ++i) {
         const size_t b = ctx->cache[i].bytes;
+        if (b == bytes) {
+            best = (int)i;
+            break;
+        }
         if (b >= bytes && b <= 2 * bytes + (1u << 20) && (best < 0 || b < ctx->cache[best].bytes)) best = (int)i;
     }
     if (best >= 0) {
@@ -68,7 +102,8 @@ void dev_free(asdm_ctx *ctx, void *p) {
             ctx->live.pop_back();
             break;
         }
-    if (ctx->cached_bytes > (64ull << 30)) trim_cache(ctx);
+    const size_t lim = cache_limit(ctx);
+    if (ctx->cached_bytes > lim) trim_cache_to(ctx, lim / 2);
 }
 
 // ---------------------------------------------------------------- scan
@@ -221,6 +256,26 @@ void add_kernel_count(asdm_ctx *ctx, const char *name, uint64_t count, uint64_t 
     k.bytes += bytes;
 }
 
+void add_host_ms(asdm_ctx *ctx, const char *name, double ms) {
+    if (!ctx->profiling) return;
+    auto &k = ctx->kstats[kstat_slot(ctx, name)];
+    k.ms += ms;
+    k.launches += 1;
+}
+
+void merge_kernel_stats(asdm_ctx *into, asdm_ctx *from) {
+    resolve_timers(from);
+    for (const auto &k : from->kstats) {
+        auto &e = into->kstats[kstat_slot(into, k.name.c_str())];
+        e.ms += k.ms;
+        e.launches += k.launches;
+        e.bytes += k.bytes;
+    }
+    from->kstats.clear();
+}
+
+void release_cached_blocks(asdm_ctx *ctx) { trim_cache(ctx); }
+
 void release_device_graph(asdm_graph &g) {
     auto &d = g.dev;
     if (d.device < 0) return;
@@ -339,6 +394,7 @@ int asdm_ctx_create(int device, asdm_ctx **out) try {
 
 void asdm_ctx_destroy(asdm_ctx *ctx) {
     if (!ctx) return;
+    for (asdm_ctx *w : ctx->workers) asdm_ctx_destroy(w);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &b : ctx->cache) (void)hipFree(b.p);
@@ -372,6 +428,18 @@ int asdm_ctx_set_row_limit(asdm_ctx *ctx, uint64_t rows) try {
     ctx->row_limit = rows;
     return ASDM_OK;
 } ASDM_CTX_CATCH(ctx)
+
+int asdm_ctx_set_bag_workers(asdm_ctx *ctx, uint32_t workers) try {
+    if (!ctx || workers < 1 || workers > 16) return ASDM_E_INVALID;
+    ctx->bag_workers = workers;
+    return ASDM_OK;
+} ASDM_CTX_CATCH(ctx)
+
+int asdm_ctx_get_bag_workers(const asdm_ctx *ctx, uint32_t *workers) try {
+    if (!ctx || !workers) return ASDM_E_INVALID;
+    *workers = ctx->bag_workers;
+    return ASDM_OK;
+} ASDM_HOST_CATCH
 
 // Per-kernel device timing, collected while profiling is on (HIP events on the context stream).
 int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) try {

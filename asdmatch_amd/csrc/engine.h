@@ -115,6 +115,13 @@ struct asdm_ctx {
     // device address of this device's allocation-free scratch (asdm_intersect_k_noalloc), looked
     // up once per device when a context is created
     uint8_t *noalloc_scratch = nullptr;
+    // Worker contexts of the same device (own stream, allocator and counters) that count the
+    // bag-engine queries of a batched call concurrently, created on first use; bag_workers of
+    // them (1: the calling context counts them one after the other).  mem_share is the share of
+    // the device memory this context's caches may hold (1 / workers for a worker).
+    uint32_t bag_workers = 4;
+    std::vector<asdm_ctx *> workers;
+    double mem_share = 1.0;
     uint64_t *d_counters = nullptr;  // [kNumCounters] device counters: 0 count, then algorithmic
                                      // bytes per kernel (single engine: 1..3; batch engine: 1..16)
     // caching allocator: freed blocks are kept for reuse (all work is ordered on `stream`, so a
@@ -126,6 +133,7 @@ struct asdm_ctx {
     std::vector<Block> cache;
     std::vector<Block> live;
     size_t cached_bytes = 0;
+    uint64_t trims = 0;  // partial trims of the cache (each waits for the device)
     int fail(int code, const std::string &msg) {
         err = msg;
         return code;
@@ -279,5 +287,36 @@ void resolve_timers(asdm_ctx *ctx);
 void add_kernel_bytes(asdm_ctx *ctx, const char *name, uint64_t bytes);
 // Adds count to an entry's launches and bytes to its bytes (entries that count events, not kernels).
 void add_kernel_count(asdm_ctx *ctx, const char *name, uint64_t count, uint64_t bytes);
+// Adds host wall time of a phase (entries named "host: ...", no bytes) while profiling.
+void add_host_ms(asdm_ctx *ctx, const char *name, double ms);
+// Adds the kernel table of `from` (timers resolved) into that of `into` and clears it.
+void merge_kernel_stats(asdm_ctx *into, asdm_ctx *from);
+// Frees the context's cached device blocks (after its stream has drained).
+void release_cached_blocks(asdm_ctx *ctx);
+
+// Concurrent bag-engine counting for the batched entry points (bag_pool.cpp).  Jobs are counted
+// on the context's worker contexts, the largest estimated cost first; a job the workers cannot
+// fit in memory is counted again on the calling context alone once they are done.
+class BagPool {
+  public:
+    struct Job {
+        size_t k;                 // index into the result arrays
+        const asdm_plan *plan;    // a plan with bags
+        const asdm_cs *cs;
+        double cost;              // estimated element operations (BagPlan::cost)
+    };
+    // counts / status / lat are indexed by Job::k; per_query says whether a failure of one query
+    // (ASDM_E_TIMEOUT, ASDM_E_UNSUPPORTED) is recorded in status rather than failing the call.
+    BagPool(asdm_ctx *ctx, uint64_t budget, std::function<float()> clock, uint64_t *counts, int *status, float *lat,
+            bool per_query);
+    ~BagPool();
+    int start();                  // spawns the worker threads (none with one worker)
+    void push(const Job &j);      // thread-safe; before finish()
+    int finish();                 // no more jobs: joins, retries, merges the kernel tables
+
+  private:
+    struct State;
+    std::unique_ptr<State> s_;
+};
 
 }  // namespace asdm

@@ -2779,8 +2779,15 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
         }
     } release{cs, bplans};
     std::vector<int> status(nq, ASDM_E_INVALID);
+    float phase = now_ms();
+    auto host_phase = [&](const char *name) {
+        const float t = now_ms();
+        add_host_ms(ctx, name, t - phase);
+        phase = t;
+    };
     int rc = filter_many(ctx, g, qs, nq, cs.data(), status.data());
     if (rc) return rc;
+    host_phase("host: batch filter");
     std::vector<asdm_cs *> bcs;
     std::vector<uint32_t> bidx;
     std::vector<std::shared_ptr<BagPlan>> fallback(nq);  // bag plans of queries moved to the frontier
@@ -2812,7 +2819,9 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
         bidx.push_back(i);
     }
     if (!rc && m) {
+        host_phase("host: plans");
         rc = learned_plans(m, g, std::vector<const asdm_cs *>(bcs.begin(), bcs.end()), 0, bplans);
+        host_phase("host: learned plans");
         // Engine choice.  The plain rule (asdm_plan_create's) sends deep or large plans to the bag
         // engine, because the independence estimates of the frontier's rows are unreliable there.
         // The learned estimates are not: a query whose learned row estimate makes the frontier
@@ -2831,6 +2840,9 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
                 bplans[k]->bags = plain.bags;
         }
         if (rc) ctx->err = err;
+        host_phase("host: engine choice");
+    } else {
+        host_phase("host: plans");
     }
     if (!rc) {
         const uint64_t budget = budget_bytes ? budget_bytes : (16ull << 30);
@@ -2885,7 +2897,13 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
         // the row cap, and those that reach it join the bag engine's queue
         std::vector<size_t> frontier, moved, bagged;
         for (size_t k = 0; k < bcs.size(); ++k) (bplans[k]->bags ? bagged : fallback[k] ? moved : frontier).push_back(k);
-        if (!frontier.empty()) rc = run(frontier);
+        // the bag-engine queries go to the worker pool first, and are counted while this thread
+        // runs the frontier batches on the context's own stream
+        BagPool pool(ctx, budget_bytes, clock, bc.data(), bst.data(), blat.data(), qstatus != nullptr);
+        if (!bagged.empty() || !moved.empty()) rc = pool.start();
+        for (size_t k : bagged) pool.push({k, bplans[k], bcs[k], bplans[k]->bags->cost});
+        if (!rc && !frontier.empty()) rc = run(frontier);
+        host_phase("host: frontier batch");
         if (!rc && !moved.empty()) {
             const uint64_t saved_limit = ctx->row_limit;
             ctx->row_limit = saved_limit ? std::min(saved_limit, kLearnedRowCap) : kLearnedRowCap;
@@ -2904,7 +2922,7 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
                 const size_t k = moved[j];
                 if (st[j] == ASDM_E_TIMEOUT) {  // over the cap: count it with its bag plan
                     bplans[k]->bags = fallback[k];
-                    bagged.push_back(k);
+                    pool.push({k, bplans[k], bcs[k], fallback[k]->cost});
                 } else {
                     bc[k] = cnt[j];
                     bst[k] = st[j];
@@ -2915,19 +2933,14 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
                 rc = ASDM_OK;
                 for (size_t k : moved) {
                     bplans[k]->bags = fallback[k];
-                    bagged.push_back(k);
+                    pool.push({k, bplans[k], bcs[k], fallback[k]->cost});
                 }
             }
         }
-        for (size_t j = 0; j < bagged.size() && !rc; ++j) {
-            const size_t k = bagged[j];
-            bst[k] = asdm_count(ctx, bplans[k], bcs[k], budget_bytes, &bc[k]);
-            blat[k] = now_ms();
-            if (bst[k] != ASDM_OK) {
-                bc[k] = 0;
-                if (!(qstatus && per_query(bst[k]))) rc = bst[k];
-            }
-        }
+        host_phase("host: moved queries");
+        const int prc = pool.finish();
+        if (!rc) rc = prc;
+        host_phase("host: bag pool wait");
         for (size_t k = 0; k < bcs.size(); ++k) {
             counts[bidx[k]] = bc[k];
             if (qstatus) qstatus[bidx[k]] = bst[k];

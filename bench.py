@@ -38,7 +38,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ESTIMATOR = os.path.join(ROOT, "asdmatch_amd", "data", "mpnn_estimator.bin")
 REF_BIN = os.path.join(ROOT, "oracle", "_ref")
-TRAFFIC_FILES = {w: os.path.join(ROOT, "profiles", f"{w}_traffic.txt") for w in ("c2", "c3", "c4")}
+TRAFFIC_FILES = {w: os.path.join(ROOT, "profiles", f"{w}_traffic.txt") for w in ("c2", "c3", "c4", "c4_isolated")}
 METRIC = "embeddings/s + queries/s, batched+learned-plan vs plain GPU, exact uint64 counts"
 
 
@@ -131,12 +131,14 @@ def cpu_baseline_sample(g, sample, tmpdir, workers, timeout, label):
 
 
 # ------------------------------------------------------------------ roofline
-def read_traffic(kernel, workload):
+def read_traffic(kernel, workload, alg_bytes_per_launch):
     """Calibrated HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, when
-    that summary was measured on this workload for this kernel."""
+    that summary was measured on this workload, for this kernel, on launches of the same average
+    algorithmic size (within 5 %).  Otherwise (None, the reason): a figure measured on other
+    launches says nothing about these."""
     path = TRAFFIC_FILES.get(workload)
     if not path or not os.path.exists(path):
-        return None
+        return None, "no PMC summary for this workload"
     vals = {}
     with open(path) as f:
         for line in f:
@@ -144,8 +146,12 @@ def read_traffic(kernel, workload):
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     if vals.get("kernel") != kernel or "traffic_bytes_per_launch" not in vals:
-        return None
-    return float(vals["traffic_bytes_per_launch"])
+        return None, f"the PMC summary covers {vals.get('kernel')}, not {kernel}"
+    ref = float(vals.get("alg_bytes_per_launch", 0))
+    if not ref or abs(alg_bytes_per_launch - ref) > 0.05 * ref:
+        return None, (f"the PMC summary was measured on launches of {ref:.4g} algorithmic bytes, these average "
+                      f"{alg_bytes_per_launch:.4g}")
+    return float(vals["traffic_bytes_per_launch"]), vals.get("source")
 
 
 def roofline_of(kstats, workload):
@@ -157,8 +163,9 @@ def roofline_of(kstats, workload):
     bytes_per_launch = dom["bytes"] / dom["launches"]
     avg_s = dom["ms"] / 1e3 / dom["launches"]
     achieved = bytes_per_launch / avg_s / 1e9
+    traffic, tnote = read_traffic(dom["name"], workload, bytes_per_launch)
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(dom["name"], workload),
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": tnote,
             "kernel": dom["name"], "alg_bytes_per_launch": round(bytes_per_launch),
             "avg_launch_ms": round(avg_s * 1e3, 4), "launches": dom["launches"]}
 
@@ -206,6 +213,8 @@ def bench_batch(args):
 
     ctx = A.Context(0)
     ctx.upload(g)
+    if args.bag_workers:
+        ctx.set_bag_workers(args.bag_workers)
     m = None
     if os.path.exists(ESTIMATOR):
         m = A.Mpnn(ctx)
@@ -231,7 +240,9 @@ def bench_batch(args):
     # launch shape of the batched engine, picked by the library's autotuner (untimed) on every
     # fourth frontier query, which keeps the tuner's 36 shapes to seconds
     fq = [qs[i] for i in fidx]
-    if args.batch_shape:
+    if args.only_timed:
+        tune = ctx.batch_tuning() + (None,)
+    elif args.batch_shape:
         lanes, scale, iters = args.batch_shape.split(",")
         ctx.set_batch_tuning(int(lanes), float(scale), int(iters))
         tune = ctx.batch_tuning() + (None,)
@@ -243,7 +254,7 @@ def bench_batch(args):
 
     # frontier-admitted subset: plain vs batched (plain plans) vs batched (learned plans)
     sub = {}
-    if fidx:
+    if fidx and not args.only_timed:
         say("frontier subset: plain, batched, batched + learned")
         reps = max(3, args.steps)
         hip_sync()
@@ -268,11 +279,15 @@ def bench_batch(args):
                "batched_learned_vs_plain": round((b - a) / (d - c), 3),
                "embeddings_per_s": round(sum(want[i] for i in fidx) * reps / (d - c), 1)}
 
-    # the whole admitted batch: plain, then batched + learned plans (the value, profiled)
-    say(f"timed: plain path over {len(qs)} queries x {args.steps}")
+    # the whole admitted batch: plain, then batched + learned plans (the value, profiled).  The
+    # plain path is a side measurement: at most two passes, so a long run (--steps 20) spends its
+    # time on the value's K steps
+    plain_steps = 0 if args.only_timed else max(1, min(args.steps, 2))
+    say(f"timed: plain path over {len(qs)} queries x {plain_steps}")
     hip_sync()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    got = want
+    for _ in range(plain_steps):
         got = plain(every)
     hip_sync()
     t2 = time.perf_counter()
@@ -290,11 +305,28 @@ def bench_batch(args):
     kstats = ctx.kernel_stats()
     ctx.set_profiling(False)
     nq = len(qs) * args.steps
-    plain_qps, learned_qps = nq / (t2 - t1), nq / (t5 - t4)
+    plain_qps = len(qs) * plain_steps / (t2 - t1) if plain_steps else None
+    learned_qps = nq / (t5 - t4)
     lats.sort()
+    # The timed step counts the bag-engine queries on several streams at once, so its kernels
+    # share the device and their launch durations include that sharing.  One more step, untimed,
+    # with a single worker gives the same kernels' durations when each has the device alone.
+    iso = None
+    if not args.only_timed:
+        workers = ctx.bag_workers()
+        say("untimed: one batched step with one bag worker (isolated kernel durations)")
+        ctx.set_bag_workers(1)
+        ctx.set_profiling(True)
+        t6 = time.perf_counter()
+        batched(every, m)
+        hip_sync()
+        t7 = time.perf_counter()
+        iso = (ctx.kernel_stats(), t7 - t6)
+        ctx.set_profiling(False)
+        ctx.set_bag_workers(workers)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.only_timed:
         # the 2P admitted queries with the fewest embeddings (the reference enumerates every one)
         workers = min(16, os.cpu_count() or 1)
         order = sorted(fidx, key=lambda i: want[i])[: 2 * workers]
@@ -310,13 +342,13 @@ def bench_batch(args):
         "not_admitted_note": "slots whose draws are neither within the frontier row limit nor countable by the "
                              "bag-table engine (a quotient of treewidth > 2, more than 2^14 partition terms, or over "
                              "the 2 s guard); sizes 21-32 have none admitted",
-        "admission_row_limit": man["row_limit"], "learned_plans": m is not None,
+        "admission_row_limit": man["row_limit"], "learned_plans": m is not None, "bag_workers": ctx.bag_workers(),
         "batch_launch_shape": {"leaf_lanes": tune[0], "class_scale": tune[1], "tile_iters": tune[2],
                                "autotuned_on": "every fourth frontier query" if tune[3] is not None else None}})
     out.update({
-        "plain_gpu_queries_per_s": round(plain_qps, 4),
+        "plain_gpu_queries_per_s": round(plain_qps, 4) if plain_qps else None, "plain_gpu_steps": plain_steps,
         "batched_learned_plan_queries_per_s": round(learned_qps, 4),
-        "batched_learned_vs_plain": round(learned_qps / plain_qps, 3),
+        "batched_learned_vs_plain": round(learned_qps / plain_qps, 3) if plain_qps else None,
         "embeddings_per_s": None,
         "embeddings_note": "counts are exact mod 2^64 like the reference's size_t; bag-engine queries have far more "
                            "than 2^64 embeddings, so their counts are residues and no embedding rate is summed over "
@@ -324,8 +356,13 @@ def bench_batch(args):
         "frontier_subset": sub,
         "latency_ms_p50": round(lats[len(lats) // 2], 3), "latency_ms_p99": round(lats[int(len(lats) * 0.99)], 3),
         "latency_note": "per query, from the call start to the host synchronization after its last level",
-        "roofline": roofline_of(kstats, "c4"), "cpu_baseline": cpu, "kernels": kernel_table(kstats),
-        "setup_s": round(gen_s, 2)})
+        "roofline": roofline_of(kstats, "c4"),
+        "roofline_note": "timed step: the bag-engine queries run on several streams at once, so launch durations "
+                         "include the sharing of the device; roofline_isolated is the same from an untimed step with "
+                         "one bag worker",
+        "roofline_isolated": ({**roofline_of(iso[0], "c4_isolated"), "step_s": round(iso[1], 3)} if iso else None),
+        "cpu_baseline": cpu, "kernels": kernel_table(kstats),
+        "kernels_isolated": kernel_table(iso[0]) if iso else None, "setup_s": round(gen_s, 2)})
     print(json.dumps(out), flush=True)
 
 
@@ -471,6 +508,10 @@ def main():
     ap.add_argument("--batch-shape", default=None,
                     help="LEAF_LANES,CLASS_SCALE,TILE_ITERS: use this batch launch shape instead of autotuning")
     ap.add_argument("--max-queries", type=int, default=0, help="c4 only: the first N admitted queries (profiling)")
+    ap.add_argument("--only-timed", action="store_true",
+                    help="c4 only: warmup and the timed batched steps alone (no plain path, frontier subset, "
+                         "autotuning, isolated step or CPU baseline), for profilers")
+    ap.add_argument("--bag-workers", type=int, default=0, help="c4 only: bag-engine worker streams (default: library's)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
     args = ap.parse_args()

@@ -29,6 +29,7 @@
  *   asdm_batch_count_learned    (no counterpart: asdm_batch_count with learned plans)
  *   asdm_batch_autotune,        (no counterpart: launch shape of the batched engine)
  *   asdm_ctx_{set,get}_batch_tuning
+ *   asdm_ctx_{set,get}_bag_workers (no counterpart: concurrent bag-engine queries of a batch)
  *   asdm_plain_estimate         (no counterpart: this planner's own independence-model estimate,
  *                               the learned estimator's fallback)
  */
@@ -250,6 +251,11 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
 int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_scale, uint32_t tile_iters);
 int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale,
                               uint32_t *tile_iters);
+/* Worker contexts that count the bag-engine queries of a batched call concurrently, each with its
+ * own stream (1..16; default 4, the device's hardware queues per process; 1 counts them one after
+ * the other on ctx).  Counts do not depend on it. */
+int asdm_ctx_set_bag_workers(asdm_ctx *ctx, uint32_t workers);
+int asdm_ctx_get_bag_workers(const asdm_ctx *ctx, uint32_t *workers);
 /* Autotuner: runs the batch once untimed, then `reps` timed runs (at least one) for each shape of the grid leaf
  * lanes {4, 16, 64} x class factor {0.25, 0.5, 1, 2} x tile rows {2, 4, 8}; keeps the shape with the
  * smallest median time on the context and returns it (ms_best: that median).  Fails with

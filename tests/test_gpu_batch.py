@@ -136,3 +136,31 @@ def test_autotune_and_launch_shapes_keep_counts(ctx):
         assert [int(c) for c in counts] == want
     finally:
         ctx.set_batch_tuning(*default)
+
+
+def test_bag_workers_keep_counts(ctx):
+    """The bag-engine queries of a batch counted by 1, 2 and 4 worker contexts (one stream each):
+    the same exact counts, every query stamped with its own completion time, and the setting
+    bounded to 1..16."""
+    from asdmatch_amd import workloads as W
+
+    man = W.load_manifest()["c4"]
+    g = W.graph("c4")
+    sl = W.slots("c4")
+    bag = [(i, seed, c) for (i, seed, c), e in zip(man["queries"], man["engines"]) if e == "bags"][::6]
+    assert len(bag) >= 40
+    qs = [W.query(g, sl[i], seed) for i, seed, _ in bag]
+    want = [c for _, _, c in bag]
+    default = ctx.bag_workers()
+    try:
+        for n in (1, 2, 4):
+            ctx.set_bag_workers(n)
+            assert ctx.bag_workers() == n
+            counts, st, lat = ctx.batch_count_learned(g, qs)
+            assert [int(c) for c in counts] == want and not any(st), n
+            assert np.all(lat > 0) and len(set(np.round(lat, 3).tolist())) > len(qs) // 2, n
+        for bad in (0, 17):
+            with pytest.raises(A.AsdmError):
+                ctx.set_bag_workers(bad)
+    finally:
+        ctx.set_bag_workers(default)
