@@ -17,8 +17,6 @@ namespace asdm {
 namespace {
 
 constexpr int kThreads = 256;
-// Columns of one table row accumulated in LDS (80 KiB: two workgroups per CU).
-constexpr uint32_t kLdsCols = 10240;
 // Largest bytes of factor results kept per query for reuse (also at most a third of free HBM).
 constexpr uint64_t kBagCacheBytes = 64ull << 30;
 
@@ -227,46 +225,78 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
 }
 
 // G[x][y] = sum_{j in R(X->v)(x)} u[j] wx * wv [y in R(v->Y)(j)]: both factors are edge relations
-// (wx, wv their edge weights, 1 where the weight array is null).  One workgroup per row x; the
-// row is accumulated in LDS when it fits (ny <= kLdsCols), else in the zeroed output with global
-// atomics.  The waves take the row's entries j in turn and the lanes of a wave split j's list,
-// so a hub's long list is read coalesced and added by 64 lanes at once.
-constexpr int kEeThreads = 512;
+// (wx, wv their edge weights, 1 where the weight array is null).  The work items are (row x, chunk
+// of kEeCols columns) pairs, taken in turn by a resident grid.  A workgroup accumulates one item's
+// columns in LDS (20 KiB, so that eight workgroups share a CU and their waves hide each other's
+// list loads) with 64-bit LDS atomics: its waves take the row's entries j in turn and the lanes of
+// a wave split j's list (ascending, so a chunk's part of it is found by two binary searches).  It
+// then writes the chunk with 16-byte non-temporal stores, clearing the LDS for the next item in
+// the same pass.  (One workgroup per whole row with 80 KiB of LDS, two per CU, a separate
+// clearing pass and 8-byte stores wrote the table at about 1 TB/s on config 4.)
+constexpr int kEeThreads = 256;
+constexpr uint32_t kEeCols = 2560;
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
                                                  const uint32_t *__restrict__ vval, const uint64_t *__restrict__ vw,
                                                  const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny,
                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
-    __shared__ unsigned long long acc[kLdsCols];
-    const bool in_lds = ny <= kLdsCols;
+    __shared__ __align__(16) unsigned long long acc[kEeCols];
+    for (uint32_t y = threadIdx.x; y < kEeCols; y += kEeThreads) acc[y] = 0;
+    __syncthreads();
+    const uint32_t nchunk = (ny + kEeCols - 1) / kEeCols;
+    const uint64_t items = (uint64_t)nx * nchunk;
     const uint64_t xbase = xoff[0], vbase = voff[0];  // weights are indexed from each relation's first entry
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t bytes = 0;
-    for (uint32_t x = blockIdx.x; x < nx; x += gridDim.x) {
-        uint64_t *row = G + (uint64_t)x * ny;
-        if (in_lds) {
-            for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) acc[y] = 0;
-            __syncthreads();
-        }
+    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t x = (uint32_t)(it / nchunk), c = (uint32_t)(it % nchunk);
+        const uint32_t y0 = c * kEeCols, n = min(ny - y0, kEeCols);
         const uint64_t lb = xoff[x], le = xoff[x + 1];
-        const uint32_t lane = threadIdx.x & 63;
-        for (uint64_t e = lb + threadIdx.x / 64; e < le; e += blockDim.x / 64) {
+        for (uint64_t e = lb + wave; e < le; e += kEeThreads / 64) {
             const uint32_t j = xval[e];
             const uint64_t w = u[j] * (xw ? xw[e - xbase] : 1ull);
-            const uint64_t b2 = voff[j], e2 = voff[j + 1];
+            uint64_t b2 = voff[j], e2 = voff[j + 1];
             if (lane == 0) bytes += 28 + 4 * (e2 - b2);
             if (!w) continue;
+            if (nchunk > 1) {  // the entries of j's list in [y0, y0 + n)
+                const uint32_t len = (uint32_t)(e2 - b2);
+                const uint32_t lo = lower_bound_u32(vval + b2, len, y0);
+                const uint32_t hi = lo + lower_bound_u32(vval + b2 + lo, len - lo, y0 + n);
+                e2 = b2 + hi;
+                b2 += lo;
+            }
             for (uint64_t f = b2 + lane; f < e2; f += 64) {
                 const uint64_t wf = vw ? w * vw[f - vbase] : w;
-                if (in_lds) atomicAdd(&acc[vval[f]], (unsigned long long)wf);
-                else atomicAdd((unsigned long long *)&row[vval[f]], (unsigned long long)wf);
+                atomicAdd(&acc[vval[f] - y0], (unsigned long long)wf);
             }
         }
-        if (in_lds) {
-            __syncthreads();
-            for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) row[y] = acc[y];
-            __syncthreads();
+        __syncthreads();
+        // write [y0, y0 + n) of row x: one 8-byte store to reach a 16-byte boundary, pairs, and one
+        // 8-byte store for an odd tail; every read LDS word is cleared for the next item
+        const uint64_t base = (uint64_t)x * ny + y0;
+        const uint32_t head = (uint32_t)(base & 1) < n ? (uint32_t)(base & 1) : 0;
+        const uint32_t pairs = (n - head) / 2, tail = head + 2 * pairs;
+        if (threadIdx.x == 0 && head) {
+            __builtin_nontemporal_store((uint64_t)acc[0], G + base);
+            acc[0] = 0;
         }
-        if (threadIdx.x == 0) bytes += 16 + 8ull * ny;
+        if (threadIdx.x == kEeThreads - 1 && tail < n) {
+            __builtin_nontemporal_store((uint64_t)acc[tail], G + base + tail);
+            acc[tail] = 0;
+        }
+        u64x2 *dst = (u64x2 *)(G + base + head);
+        for (uint32_t q = threadIdx.x; q < pairs; q += kEeThreads) {
+            const uint32_t y = head + 2 * q;
+            u64x2 v;
+            v.x = acc[y];
+            v.y = acc[y + 1];
+            acc[y] = 0;
+            acc[y + 1] = 0;
+            __builtin_nontemporal_store(v, dst + q);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) bytes += 16 + 8ull * n;
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -288,6 +318,7 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
 // SIDE only names the instantiation after the step that launches it (0: edge (X, v) x dense
 // (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed side), so that
 // profilers report the two apart.
+constexpr int kEdInFlight = 8;
 template <int SIDE, int GL, int CPL>
 __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
@@ -325,24 +356,24 @@ __global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xo
                 wl = xw ? u[jl] * xw[ke - xbase] : u[jl];
             }
             const uint32_t cnt = (uint32_t)(le - k0 < GL ? le - k0 : GL);
-            uint32_t t = 0;
-            for (; t + 4 <= cnt; t += 4) {
-                uint64_t jr[4], wr[4];
+            // up to kEdInFlight dense-row loads issued before the first is used (a typical row
+            // has about ten entries); the table has at most 2^31 entries, so an element offset
+            // j * ny fits 32 bits
+            for (uint32_t t = 0; t < cnt; t += kEdInFlight) {
+                uint32_t jr[kEdInFlight];
+                uint64_t fv[kEdInFlight][CPL];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    jr[q] = (uint64_t)__shfl(jl, (int)t + q, GL) * ny;
-                    wr[q] = __shfl(wl, (int)t + q, GL);
+                for (int q = 0; q < kEdInFlight; ++q) jr[q] = (uint32_t)__shfl(jl, (int)(t + q) & (GL - 1), GL) * ny;
+#pragma unroll
+                for (int q = 0; q < kEdInFlight; ++q)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) fv[q][c] = t + q < cnt && ok[c] ? fc[c][jr[q]] : 0;
+#pragma unroll
+                for (int q = 0; q < kEdInFlight; ++q) {
+                    const uint64_t w0 = __shfl(wl, (int)(t + q) & (GL - 1), GL);
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c) acc[c] += w0 * fv[q][c];
                 }
-#pragma unroll
-                for (int c = 0; c < CPL; ++c)
-                    if (ok[c]) acc[c] += wr[0] * fc[c][jr[0]] + wr[1] * fc[c][jr[1]] + wr[2] * fc[c][jr[2]] + wr[3] * fc[c][jr[3]];
-            }
-            for (; t < cnt; ++t) {
-                const uint64_t j0 = (uint64_t)__shfl(jl, (int)t, GL) * ny;
-                const uint64_t w0 = __shfl(wl, (int)t, GL);
-#pragma unroll
-                for (int c = 0; c < CPL; ++c)
-                    if (ok[c]) acc[c] += w0 * fc[c][j0];
             }
         }
         uint64_t *gr = G + (uint64_t)x * ny + c0;
@@ -1207,9 +1238,11 @@ struct BagRun {
         if (fxv.kind == 1 && fvy.kind == 1) {
             const Rel rx = relv(fxv, X), rv = relfrom_v(fvy, Y);
             KTimer kt(ctx, kname);
-            if (ny > kLdsCols && (rc = fill_u64(ctx, G, (uint64_t)nx * ny, 0))) return rc;
-            hipLaunchKernelGGL(k_ee, dim3(std::min(grid_for(nx, 1), 1u << 16)), dim3(kEeThreads), 0, ctx->stream, rx.off,
-                               rx.val, rx.w, rv.off, rv.val, rv.w, uv, nx, ny, G, bytes_slot(8));
+            // a resident grid: eight workgroups per CU
+            const uint64_t items = (uint64_t)nx * ((ny + kEeCols - 1) / kEeCols);
+            const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, 8ull * ctx->num_cus));
+            hipLaunchKernelGGL(k_ee, dim3(grid), dim3(kEeThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off, rv.val,
+                               rv.w, uv, nx, ny, G, bytes_slot(8));
             kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
         } else if (fxv.kind == 1 || fvy.kind == 1) {
             // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;

@@ -233,10 +233,10 @@ int fill_u32(asdm_ctx *ctx, uint32_t *p, uint64_t n, uint32_t v);
 int filter_many(asdm_ctx *ctx, const asdm_graph *g, const asdm_graph *const *qs, uint32_t nb, asdm_cs **out,
                 int *status);
 
-// Sorts (key, value) pairs by key and sums equal keys: ukeys/usums[*nruns] (local_sort.hip).
-int sort_reduce_pairs(asdm_ctx *ctx, const uint64_t *keys, const uint64_t *vals, uint64_t n, int end_bit,
-                      uint64_t *keys_sorted, uint64_t *vals_sorted, uint64_t *ukeys, uint64_t *usums,
-                      uint32_t *nruns);
+// mat[key] += value for n (key, value) pairs, keys < cells, with one writer per cell
+// (owner-partitioned buckets, local_agg.hip).
+int aggregate_pairs(asdm_ctx *ctx, const uint64_t *keys, const uint64_t *vals, uint64_t n, uint64_t cells,
+                    uint64_t *mat);
 
 // Planner (plan.cpp).  card_of(mask, fallback) returns the cardinality estimate of the
 // sub-query induced by `mask` (the subsetToCard role), or `fallback` when it has none.

@@ -526,7 +526,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 // candidate x of leaf e gets (x * |Vq| + u_e, product of the other groups' counts).  The count pass
 // (FILL = false) writes the number of pairs of every row; after an exclusive scan the fill pass
 // (FILL = true) writes them at the row's offset (ordered ballot compaction within the lane group).
-// The pairs are then radix-sorted and reduced by key, and every cell receives one plain add.
+// The pairs are then aggregated by owner bucket (local_agg.hip): every cell receives one plain add.
 template <int G, bool FILL>
 __global__ void __launch_bounds__(256) k_leaf_local_pairs(const DPlan *__restrict__ P, const uint32_t *__restrict__ rows,
                                                           uint64_t nrows, const uint64_t *__restrict__ ce_off,
@@ -621,14 +621,6 @@ __global__ void __launch_bounds__(256) k_leaf_local_pairs(const DPlan *__restric
         acc = group_sum<64>(acc);
         if (lane == 0 && acc) atomicAdd(total, (unsigned long long)acc);
     }
-}
-
-// One plain add per reduced cell (keys are unique within a batch).
-__global__ void k_mat_add(const uint64_t *__restrict__ ukeys, const uint64_t *__restrict__ usums,
-                          const uint32_t *__restrict__ nruns, unsigned long long *__restrict__ mat) {
-    const uint32_t n = *nruns;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        mat[ukeys[i]] += (unsigned long long)usums[i];
 }
 
 // -------------------------------------------------------------------------- fused last level
@@ -1552,7 +1544,7 @@ namespace {
 constexpr int kG = 16;                      // lanes per row
 constexpr uint64_t kLeafChunk = 1ull << 22;  // rows per leaf-count launch
 constexpr uint64_t kFusedMinRows = 1ull << 19;  // rows per fused launch, at least
-constexpr uint64_t kPairCap = 1ull << 26;    // local-count (cell, value) pairs per sort batch
+constexpr uint64_t kPairCap = 1ull << 26;    // local-count (cell, value) pairs per aggregation batch
 
 unsigned groups_grid(uint64_t items, int G) {
     uint64_t threads = items * (uint64_t)G;
@@ -1981,10 +1973,10 @@ struct Engine {
     }
 
     // LOCAL_COUNT of leaf rows r[0, m) without atomics on the matrix: count pass, scan, then per
-    // batch of at most `cap` pairs a fill pass, a radix sort + reduce by key, and one plain add
-    // per distinct cell.
+    // batch of at most `cap` pairs a fill pass and the owner-partitioned aggregation (one writer
+    // per cell).
     int leaf_local(uint32_t *r, uint64_t m) {
-        uint32_t *np = nullptr, *flag = nullptr, *nruns = nullptr;
+        uint32_t *np = nullptr, *flag = nullptr;
         uint64_t *po = nullptr;
         std::vector<void *> tmp;
         auto alloc = [&](void **p, size_t bytes) {
@@ -1999,7 +1991,6 @@ struct Engine {
         int rc = alloc((void **)&np, 4 * m);
         if (!rc) rc = alloc((void **)&po, 8 * (m + 1));
         if (!rc) rc = alloc((void **)&flag, 4);
-        if (!rc) rc = alloc((void **)&nruns, 4);
         if (!rc) rc = fill_u32(ctx, flag, 1, 0);
         if (rc) return done(rc);
         {
@@ -2019,9 +2010,8 @@ struct Engine {
         if (rc) return done(rc);
         if (hflag) return done(ctx->fail(ASDM_E_UNSUPPORTED, "more than 2^32 local-count pairs in one row"));
         const uint64_t cells = (uint64_t)cs->nv * cs->nq;
-        const int end_bit = cells > 1 ? 64 - __builtin_clzll(cells - 1) : 1;
-        // six 8-byte buffers per pair (pairs, sorted pairs, reduced cells) within the frontier budget
-        const uint64_t cap = std::max<uint64_t>(1024, std::min<uint64_t>(kPairCap, budget / 48));
+        // the pairs (16 bytes) and the aggregation's bucketed copy (12 bytes) within the budget
+        const uint64_t cap = std::max<uint64_t>(1024, std::min<uint64_t>(kPairCap, budget / 28));
         for (uint64_t a = 0; a < m;) {
             // largest b > a with at most cap pairs in rows [a, b) (one row may exceed the cap)
             uint64_t lo = a + 1, hi = m;
@@ -2032,19 +2022,12 @@ struct Engine {
             }
             const uint64_t b = lo, n = hpo[b] - hpo[a];
             if (n) {
-                uint64_t *keys = nullptr, *vals = nullptr, *ks = nullptr, *vs = nullptr, *uk = nullptr, *us = nullptr;
-                std::vector<void *> bt;
-                auto balloc = [&](void **p, size_t bytes) {
-                    int r2 = dev_alloc(ctx, p, bytes);
-                    if (!r2) bt.push_back(*p);
-                    return r2;
-                };
-                rc = balloc((void **)&keys, 8 * n);
-                if (!rc) rc = balloc((void **)&vals, 8 * n);
-                if (!rc) rc = balloc((void **)&ks, 8 * n);
-                if (!rc) rc = balloc((void **)&vs, 8 * n);
-                if (!rc) rc = balloc((void **)&uk, 8 * n);
-                if (!rc) rc = balloc((void **)&us, 8 * n);
+                uint64_t *keys = nullptr, *vals = nullptr;
+                rc = dev_alloc(ctx, (void **)&keys, 8 * n);
+                if (!rc && (rc = dev_alloc(ctx, (void **)&vals, 8 * n))) {
+                    dev_free(ctx, keys);
+                    keys = nullptr;
+                }
                 if (!rc) {
                     KTimer kt(ctx, "leaf_local_fill");
                     hipLaunchKernelGGL((k_leaf_local_pairs<kG, true>), dim3(groups_grid(b - a, kG)), dim3(256), 0,
@@ -2053,19 +2036,9 @@ struct Engine {
                     kt.stop();
                     rc = check_hip(ctx, hipGetLastError(), "local pair fill");
                 }
-                if (!rc) {
-                    KTimer kt(ctx, "local_sort_reduce");
-                    rc = sort_reduce_pairs(ctx, keys, vals, n, end_bit, ks, vs, uk, us, nruns);
-                    kt.stop();
-                }
-                if (!rc) {
-                    KTimer kt(ctx, "local_mat_add");
-                    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 16);
-                    hipLaunchKernelGGL(k_mat_add, dim3(grid), dim3(256), 0, ctx->stream, uk, us, nruns, d_mat);
-                    kt.stop();
-                    rc = check_hip(ctx, hipGetLastError(), "matrix add");
-                }
-                for (void *p : bt) dev_free(ctx, p);
+                if (!rc) rc = aggregate_pairs(ctx, keys, vals, n, cells, (uint64_t *)d_mat);
+                dev_free(ctx, keys);
+                dev_free(ctx, vals);
                 if (rc) return done(rc);
             }
             a = b;
