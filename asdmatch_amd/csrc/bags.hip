@@ -143,28 +143,53 @@ __global__ void k_vec_mul(uint64_t *__restrict__ u, const uint64_t *__restrict__
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) u[i] *= t[i];
 }
 
+// Row r of a CSR with n rows that holds entry e (off[0] <= e < off[n]): the last r with off[r] <= e.
+__device__ inline uint32_t row_of(const uint64_t *__restrict__ off, uint32_t n, uint64_t e) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // t[i] = sum over row i of the relation of w x[col], w the entry's weight (1 when the weight array
-// is null): kRelLanes lanes per row.
+// is null), flat over the entries: each lane takes kSpmvPer consecutive entries, finds the row of
+// its first one by binary search, and adds each row's partial sum into t (zeroed before) with one
+// atomic per (lane, row) piece.  A hub row's entries are spread over many lanes, so no lane walks
+// a long row alone (16 lanes per row spent most of a launch on the longest row); the u64 atomics
+// wrap mod 2^64 like the sum itself.
+constexpr int kSpmvPer = 8;
 __global__ void k_spmv(const uint64_t *__restrict__ off, const uint32_t *__restrict__ val, const uint64_t *__restrict__ w,
                        uint32_t n, const uint64_t *__restrict__ x, uint64_t *__restrict__ t,
                        unsigned long long *__restrict__ nbytes) {
-    constexpr int G = kRelLanes;
-    const int lg = threadIdx.x & (G - 1);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
-    const uint64_t base = off[0];  // weights are indexed from the relation's first entry
+    const uint64_t base = off[0], nnz = off[n] - base;  // weights are indexed from the relation's first entry
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kSpmvPer;
     uint64_t bytes = 0;
-    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; i < n; i += stride) {
-        uint64_t s = 0;
-        const uint64_t lb = off[i], le = off[i + 1];
-        if (w)
-            for (uint64_t e = lb + lg; e < le; e += G) s += w[e - base] * x[val[e]];
-        else
-            for (uint64_t e = lb + lg; e < le; e += G) s += x[val[e]];
-        s = group_sum<G>(s);
-        if (lg == 0) {
-            t[i] = s;
-            bytes += 24 + (w ? 20 : 12) * (le - lb);
+    for (uint64_t s0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kSpmvPer; s0 < nnz; s0 += stride) {
+        const uint32_t m = (uint32_t)min<uint64_t>(kSpmvPer, nnz - s0);
+        uint64_t prod[kSpmvPer];
+        uint32_t col[kSpmvPer];
+#pragma unroll
+        for (int q = 0; q < kSpmvPer; ++q) col[q] = val[base + s0 + (q < (int)m ? q : 0)];
+#pragma unroll
+        for (int q = 0; q < kSpmvPer; ++q) prod[q] = q < (int)m ? x[col[q]] * (w ? w[s0 + q] : 1ull) : 0;
+        uint32_t r = row_of(off, n, base + s0);
+        uint64_t rend = off[r + 1], acc = 0;
+#pragma unroll
+        for (int q = 0; q < kSpmvPer; ++q) {
+            if (q < (int)m) {
+                while (base + s0 + q >= rend) {  // the entry starts a later row: flush this one
+                    if (acc) atomicAdd((unsigned long long *)&t[r], (unsigned long long)acc);
+                    acc = 0;
+                    rend = off[++r + 1];
+                }
+                acc += prod[q];
+            }
         }
+        if (acc) atomicAdd((unsigned long long *)&t[r], (unsigned long long)acc);
+        bytes += (w ? 20 : 12) * (uint64_t)m;
     }
     wave_add_bytes(bytes, nbytes);
 }
@@ -442,21 +467,10 @@ __global__ void __launch_bounds__(256) k_dd(const uint64_t *__restrict__ A, cons
 // R(X->Y) that already joins X and Y, times each edge's own weight (1 where the relation has
 // none), written to wout.  The work items are the relation's edges, kJoinLanes lanes per edge
 // (a power-law hub's row holds many edges, so rows would leave most lanes idle behind a few);
-// an edge finds its row by binary search in the offsets.
+// an edge finds its row by binary search in the offsets (row_of).
 // Weight arrays are indexed by entry - off[0]: a relation over single query vertices is a slice
 // of the candidate-edge store, whose offsets do not start at zero.
 constexpr int kJoinLanes = 8;
-
-// Row r of a CSR with n rows that holds entry e (off[0] <= e < off[n]): the last r with off[r] <= e.
-__device__ inline uint32_t row_of(const uint64_t *__restrict__ off, uint32_t n, uint64_t e) {
-    uint32_t lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Both factors edge relations: the product at (x, y) is the sum of u[j] wa wb over the common
 // entries j of the rows R(X->v)(x) and R(Y->v)(y) (ascending D(v) positions).  The lanes split
@@ -1042,7 +1056,9 @@ struct BagRun {
                     KTimer kt(ctx, "bag_vector");
                     if (f.kind == 1) {
                         const Rel r = rel(a, v);
-                        hipLaunchKernelGGL(k_spmv, dim3(grid_for((uint64_t)na * kRelLanes, kThreads)), dim3(kThreads), 0, ctx->stream,
+                        if ((rc = fill_u64(ctx, t, na, 0))) return rc;
+                        const uint64_t lanes = (r.nnz + kSpmvPer - 1) / kSpmvPer;
+                        hipLaunchKernelGGL(k_spmv, dim3(std::max(1u, grid_for(lanes, kThreads))), dim3(kThreads), 0, ctx->stream,
                                            r.off, r.val, r.w, na, u[v], t, bytes_slot(1));
                     } else if (f.t.rows == a) {
                         hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(na, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
@@ -1118,6 +1134,7 @@ struct BagRun {
         if (rc) return rc;
         const dim3 grid(grid_for(rxy.nnz * kJoinLanes, kThreads));
         auto relv = [&](const Fac &f, uint32_t a) { return a < v ? f.lo : f.hi; };  // rows D(a) of (a, v)
+        bool reversed = false;  // the join wrote wr (entries of R(Y->X)) instead of wf
         if (fxv.kind == 1 && fvy.kind == 1) {
             const Rel ra = relv(fxv, X), rb = relv(fvy, Y);
             KTimer kt(ctx, "bag_join_ee");
@@ -1131,24 +1148,42 @@ struct BagRun {
             const Table &td = (own_x ? fvy : fxv).t;
             // F(a, j): rows D(a) -> a * nv + j; rows D(v) -> j * |D(a)| + a
             const uint64_t sa = td.rows == a ? nv : 1, sj = td.rows == a ? 1 : dom[a].n;
+            // walk the edges grouped by a (the orientation of the joining relation whose rows are
+            // D(a)), so that consecutive edges gather from the same part of F; the list owner o is
+            // then the column of each entry
+            const Rel rw_ = a == X ? rxy : ryx;
+            const uint32_t nrows = a == X ? nx : ny;
+            reversed = a != X;
             KTimer kt(ctx, "bag_join_ed");
-            hipLaunchKernelGGL(k_join_ed, grid, dim3(kThreads), 0, ctx->stream, rxy.off, rxy.val, rxy.w, nx, rxy.nnz, rl.off, rl.val,
-                               rl.w, (int)own_x, td.p, sa, sj, uv, wf, bytes_slot(13));
+            hipLaunchKernelGGL(k_join_ed, dim3(grid_for(rw_.nnz * kJoinLanes, kThreads)), dim3(kThreads), 0, ctx->stream,
+                               rw_.off, rw_.val, rw_.w, nrows, rw_.nnz, rl.off, rl.val, rl.w, 0, td.p, sa, sj, uv,
+                               reversed ? wr : wf, bytes_slot(13));
             // a lower bound: the gathered table entries are not counted (at most the whole table)
-            kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * (dom[o].n + 1) + 12 * rl.nnz + 8ull * nv);
+            kt.stop(8ull * (nrows + 1) + 12 * rw_.nnz + 8ull * (dom[o].n + 1) + 12 * rl.nnz + 8ull * nv);
         } else {
-            // A(x, j): rows D(X) -> x * nv + j, rows D(v) -> j * nx + x; B(j, y): rows D(v) -> j * ny + y,
-            // rows D(Y) -> y * nv + j
-            const bool ax = fxv.t.rows == X, bv = fvy.t.rows == v;
+            // A(x, j) and B(j, y) read with j contiguous: A with rows D(X) (x * nv + j) and B with rows
+            // D(Y) (y * nv + j), transposed first when stored the other way (a wave reads one row of
+            // each per edge; a strided row would touch one cache line per element)
+            uint64_t *Ar = nullptr, *Br = nullptr;
+            bool ta = false, tb = false;
+            if ((rc = oriented(fxv.t, fxv.t.rows == X ? v : X, X, dom, &Ar, ta))) return rc;
+            if ((rc = oriented(fvy.t, fvy.t.rows == Y ? v : Y, Y, dom, &Br, tb))) return rc;
             KTimer kt(ctx, "bag_join_dd");
             hipLaunchKernelGGL(k_join_dd, dim3(grid_for(rxy.nnz, kThreads / 64)), dim3(kThreads), 0, ctx->stream, rxy.off,
-                               rxy.val, rxy.w, nx, rxy.nnz, fxv.t.p, ax ? (uint64_t)nv : 1, ax ? 1 : (uint64_t)nx, fvy.t.p,
-                               bv ? (uint64_t)ny : 1, bv ? 1 : (uint64_t)nv, nv, uv, wf, bytes_slot(13));
+                               rxy.val, rxy.w, nx, rxy.nnz, Ar, (uint64_t)nv, 1ull, Br, 1ull, (uint64_t)nv, nv, uv, wf,
+                               bytes_slot(13));
             kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * nv * (nx + ny + 1));
+            if (ta) release(Ar);
+            if (tb) release(Br);
         }
         if ((rc = launched("bag sparse join"))) return rc;
-        hipLaunchKernelGGL(k_join_reverse, dim3(grid_for(ryx.nnz, kThreads)), dim3(kThreads), 0, ctx->stream, rxy.off,
-                           rxy.val, wf, ryx.off, ryx.val, ny, ryx.nnz, wr);
+        // the other orientation's weights: entry (y, x) of R(Y->X) takes the weight of (x, y)
+        if (reversed)
+            hipLaunchKernelGGL(k_join_reverse, dim3(grid_for(rxy.nnz, kThreads)), dim3(kThreads), 0, ctx->stream, ryx.off,
+                               ryx.val, wr, rxy.off, rxy.val, nx, rxy.nnz, wf);
+        else
+            hipLaunchKernelGGL(k_join_reverse, dim3(grid_for(ryx.nnz, kThreads)), dim3(kThreads), 0, ctx->stream, rxy.off,
+                               rxy.val, wf, ryx.off, ryx.val, ny, ryx.nnz, wr);
         if ((rc = launched("bag sparse join reverse"))) return rc;
         install(wf, wr, !store(k, CacheEnt{wf, wr, mask[X]}, bytes));
         return ASDM_OK;
