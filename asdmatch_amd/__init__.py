@@ -97,6 +97,9 @@ _sig("asdm_plan_uses_bags", C.c_int, _P)
 _sig("asdm_plan_bag_info", C.c_int, _P, _u32p, C.POINTER(C.c_double), C.POINTER(C.c_double))
 _sig("asdm_cs_candidates", C.c_int, _P, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_cs_arc_weight", C.c_int, _P, C.c_uint32, _u64p)
+_sig("asdm_cs_query_paths", C.c_int, _P, _u64p, _u32p)
+_sig("asdm_cs_query_path", C.c_int, _P, C.c_uint32, C.c_uint32, _u32p, C.c_uint32, _u32p)
+_sig("asdm_cartesian_candidates", C.c_int, _P, _P, C.c_uint32, _u32p, C.c_uint32, _u32p, C.c_uint32, _u32p)
 _sig("asdm_plan_create", C.c_int, _P, _P, _u32p, C.c_uint32, C.POINTER(_P))
 _sig("asdm_plan_create_with_cards", C.c_int, _P, _P, _u64p, C.POINTER(C.c_double), C.c_uint32, C.c_uint32,
      C.POINTER(_P))
@@ -466,6 +469,37 @@ class CandSpace:
         w = C.c_uint64()
         _lib.asdm_cs_arc_weight(self._h, a, C.byref(w))
         return w.value
+
+    def query_paths(self):
+        """(dist uint64[nq, nq], next uint32[nq, nq]) as the reference's setQueryGraphWeights leaves
+        them (products of 32-bit arc weights, wrapping mod 2^64)."""
+        dist = np.zeros((self.nq, self.nq), np.uint64)
+        nxt = np.zeros((self.nq, self.nq), np.uint32)
+        rc = _lib.asdm_cs_query_paths(self._h, _ptr(dist, C.c_uint64), _ptr(nxt))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "query_paths")
+        return dist, nxt
+
+    def query_path(self, i: int, j: int):
+        """reconstructPath(i, j): the list of query vertices ([] when there is no path).  Raises
+        AsdmError (ASDM_E_UNSUPPORTED) where the reference's reconstruction would not end."""
+        out = np.zeros(self.nq + 1, np.uint32)
+        n = C.c_uint32()
+        rc = _lib.asdm_cs_query_path(self._h, i, j, _ptr(out), len(out), C.byref(n))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, "query_path")
+        return out[: n.value].tolist()
+
+    def cartesian_candidates(self, v: int, path) -> np.ndarray:
+        """optimizedCartesianProduct: ascending candidates of path[-1] reached from data vertex v."""
+        p = _u32(path)
+        cap = int(self.sizes()[int(p[-1])]) if len(p) else 0
+        out = np.zeros(max(cap, 1), np.uint32)
+        n = C.c_uint32()
+        rc = _lib.asdm_cartesian_candidates(self._ctx._h, self._h, int(v), _ptr(p), len(p), _ptr(out), cap, C.byref(n))
+        if rc != ASDM_OK:
+            raise AsdmError(rc, f"cartesian_candidates: {_lib.asdm_last_error(self._ctx._h).decode()}")
+        return out[: n.value]
 
     def plan(self, q: Graph, order=None, local: bool = False, cards=None, factor: bool = True,
              num_enumerated: Optional[int] = None, cost: bool = False, bags: Optional[bool] = None) -> "Plan":

@@ -14,6 +14,9 @@
  *                               (+ buildCandidateEdge(false)        relation/candidate_space.cpp:458-487)
  *   asdm_cs_candidates          CandidateSpace::candidateSet        relation/candidate_space.h:62
  *   asdm_cs_arc_weight          setQueryGraphWeights' edge weight   relation/candidate_space.cpp:507-524
+ *   asdm_cs_query_paths         allSourcesShortestPaths             relation/graph.cpp:341-372
+ *   asdm_cs_query_path          CandidateSpace::reconstructPath     relation/candidate_space.cpp:529-537
+ *   asdm_cartesian_candidates   optimizedCartesianProduct           plan/estimator.cpp:41-102
  *   asdm_plan_create            simplePlan / optCostPlan            plan/optimizer.cpp:7-200, :380-548
  *   asdm_plan_create_with_cards subsetToCard consumed by the order  plan/estimator.cpp:7
  *   asdm_plan_create_ordered    plan from a given attribute order   plan/optimizer.cpp:7-200
@@ -142,6 +145,23 @@ int asdm_cs_sizes(const asdm_cs *cs, uint32_t *sizes);
 int asdm_cs_candidates(const asdm_cs *cs, uint32_t u, uint32_t *out, uint32_t cap, uint32_t *n);
 /* Sum over v in C(u) of the candidate-edge list length for query arc a (query CSR order). */
 int asdm_cs_arc_weight(const asdm_cs *cs, uint32_t a, uint64_t *w);
+/* The query's all-pairs paths as setQueryGraphWeights leaves them (relation/candidate_space.cpp:507-528,
+ * relation/graph.cpp:341-372): each arc weighs its candidate-edge total reduced to 32 bits (the
+ * reference's ui), and Floyd-Warshall minimises the product of weights in 64 bits, wrapping, with
+ * dist[i][i] starting at UINT64_MAX.  dist[nq * nq] and next[nq * nq] (either may be NULL) receive the
+ * distances and the successor matrix (UINT32_MAX: no path), row-major by (i, j). */
+int asdm_cs_query_paths(const asdm_cs *cs, uint64_t *dist, uint32_t *next);
+/* CandidateSpace::reconstructPath (relation/candidate_space.cpp:529-537): path[0] = i, ..., path[*len - 1]
+ * = j; *len = 0 when there is no path.  When wrapped products leave a cycle in the successor matrix the
+ * reference's loop never ends; here the call returns ASDM_E_UNSUPPORTED.  ASDM_E_BUDGET if cap is too small. */
+int asdm_cs_query_path(const asdm_cs *cs, uint32_t i, uint32_t j, uint32_t *path, uint32_t cap, uint32_t *len);
+/* optimizedCartesianProduct (plan/estimator.cpp:41-102), the candidates of a cartesian step: the candidates
+ * of path[len-1] reached from data vertex v (the image of path[0]) through the candidate-edge lists of the
+ * query arcs path[0] -> path[1] -> ... -> path[len-1], as ascending data ids, without v.  A v outside
+ * C(path[0]) or a consecutive pair that is not a query arc reaches nothing.  *n receives the number of
+ * candidates; ASDM_E_BUDGET when it exceeds cap.  Runs on the device (a bitmap per path layer). */
+int asdm_cartesian_candidates(asdm_ctx *ctx, const asdm_cs *cs, uint32_t v, const uint32_t *path, uint32_t len,
+                              uint32_t *out, uint32_t cap, uint32_t *n);
 
 /* ---- plan ---- */
 /* order: NULL to let the planner choose, else a connected permutation of the query vertices.

@@ -23,8 +23,15 @@
  *   orc_local_count                         matching/join.cpp:1016-1041 (LOCAL_COUNT) + main.cpp:135-146
  *   orc_unique_combine / orc_bf_unique_combine  utility/utils.cpp:145-201 / :203-233
  *   orc_bruteforce_count                    definition of the count (no filter), for pinning
+ *   orc_paths_from_weights / orc_query_paths relation/candidate_space.cpp:507-528 +
+ *                                           relation/graph.cpp:341-372 (query arc weights in 32
+ *                                           bits, Floyd-Warshall over products in size_t)
+ *   orc_reconstruct_path                    relation/candidate_space.cpp:529-537
+ *   orc_cartesian                           plan/estimator.cpp:41-102 (optimizedCartesianProduct)
  *
- * Pinning (tests/): counts against the reference binary's golden counts (tests/golden), against
+ * Pinning (tests/): distances, successor matrices, paths and cartesian candidate sets against the
+ * reference's own CandidateSpace (oracle/_ref/ref_cs_dump --paths, tests/golden/paths.json),
+ * including config-2 queries whose products wrap mod 2^64; counts against the reference binary's golden counts (tests/golden), against
  * orc_bruteforce_count and closed forms; candidate sets against the reference's own
  * CandidateSpace output (tests/golden, produced by oracle/_ref/ref_cs_dump); unique-combine
  * against its brute-force twin; local counts against the reference's LOCAL_COUNT build.
@@ -793,4 +800,102 @@ uint64_t orc_bf_unique_combine(const uint32_t *sets, const uint32_t *sizes, int 
         else { ++d; idx[d] = 0; }
     }
     return num;
+}
+
+/* -------------------------------------------------------- query paths, cartesian candidates */
+
+/* relation/candidate_space.cpp:507-528 (setQueryGraphWeights) + relation/graph.cpp:341-372
+ * (allSourcesShortestPaths): each arc (u -> u2) of the query, in query CSR order, weighs the total
+ * length of its candidate-edge lists, held in a 32-bit `ui` (so it wraps mod 2^32); Floyd-Warshall
+ * then minimises the PRODUCT of weights in size_t (wrapping mod 2^64), with dist[i][i] starting at
+ * the maximum and next[i][i] = i.  w32[arc] are the arc weights already reduced to 32 bits. */
+void orc_paths_from_weights(uint32_t nq, const uint32_t *qoff, const uint32_t *qnbr, const uint32_t *w32,
+                            uint64_t *dist, uint32_t *next) {
+    const uint64_t INF = UINT64_MAX;
+    for (uint32_t i = 0; i < nq * nq; ++i) {
+        dist[i] = INF;
+        next[i] = UINT32_MAX;
+    }
+    for (uint32_t i = 0; i < nq; ++i) next[i * nq + i] = i;
+    for (uint32_t v = 0; v < nq; ++v)
+        for (uint32_t e = qoff[v]; e < qoff[v + 1]; ++e) {
+            dist[v * nq + qnbr[e]] = w32[e];
+            next[v * nq + qnbr[e]] = qnbr[e];
+        }
+    for (uint32_t k = 0; k < nq; ++k)
+        for (uint32_t i = 0; i < nq; ++i)
+            for (uint32_t j = 0; j < nq; ++j) {
+                const uint64_t a = dist[i * nq + k], b = dist[k * nq + j];
+                if (a == INF || b == INF) continue;
+                if (a * b < dist[i * nq + j]) {
+                    dist[i * nq + j] = a * b;
+                    next[i * nq + j] = next[i * nq + k];
+                }
+            }
+}
+
+void orc_query_paths(const orc_graph *q, const orc_cs *cs, uint64_t *dist, uint32_t *next) {
+    uint32_t *w = (uint32_t *)calloc(q->m ? q->m : 1, sizeof(uint32_t));
+    /* the oracle's arcs are the query CSR's arcs in the same order */
+    for (uint32_t a = 0; a < q->m && a < cs->narcs; ++a) w[a] = (uint32_t)orc_cs_arc_weight(cs, a);
+    orc_paths_from_weights(q->n, q->off, q->nbr, w, dist, next);
+    free(w);
+}
+
+/* relation/candidate_space.cpp:529-537 (reconstructPath): i, next[i][j], ... up to j.  Returns
+ * the number of vertices, 0 when next[i][j] is unset, and -1 when following next does not reach
+ * j within nq steps (the reference's loop would not end). */
+int orc_reconstruct_path(uint32_t nq, const uint32_t *next, uint32_t i, uint32_t j, uint32_t *path) {
+    if (next[i * nq + j] == UINT32_MAX) return 0;
+    int len = 0;
+    path[len++] = i;
+    while (i != j) {
+        if (len > (int)nq) return -1;
+        i = next[i * nq + j];
+        if (i == UINT32_MAX) return -1;
+        path[len++] = i;
+    }
+    return len;
+}
+
+/* plan/estimator.cpp:41-102 (optimizedCartesianProduct): the candidates of path[len-1] reachable
+ * from data vertex v (a candidate of path[0]) along candidate-edge lists of the arcs path[t] ->
+ * path[t+1], ascending, without v.  The reference walks every path of candidate edges depth first
+ * and merges the last lists; the union of their ends is the set reached layer by layer, which is
+ * how it is restated here.  Returns the number written to out (at most |C(path[len-1])|). */
+uint32_t orc_cartesian(const orc_graph *q, const orc_cs *cs, uint32_t v, const uint32_t *path, uint32_t len,
+                       uint32_t *out) {
+    if (cs->empty || len < 2) return 0;
+    /* layer: a membership flag over the positions of C(path[t]) */
+    uint32_t u0 = path[0];
+    char *cur = (char *)calloc(cs->csize[u0] + 1, 1);
+    uint32_t p = 0, lo = 0, hi = cs->csize[u0];
+    while (lo < hi) { uint32_t mid = (lo + hi) / 2; if (cs->cand[u0][mid] < v) lo = mid + 1; else hi = mid; }
+    p = lo;
+    if (p < cs->csize[u0] && cs->cand[u0][p] == v) cur[p] = 1;
+    for (uint32_t t = 0; t + 1 < len; ++t) {
+        const uint32_t a = path[t], b = path[t + 1];
+        char *nxt = (char *)calloc(cs->csize[b] + 1, 1);
+        uint32_t arc = UINT32_MAX;
+        for (uint32_t e = q->off[a]; e < q->off[a + 1]; ++e)
+            if (q->nbr[e] == b) arc = e;
+        if (arc != UINT32_MAX)
+            for (uint32_t x = 0; x < cs->csize[a]; ++x) {
+                if (!cur[x]) continue;
+                for (uint64_t e = cs->ce_off[arc][x]; e < cs->ce_off[arc][x + 1]; ++e) {
+                    const uint32_t id = cs->ce_val[arc][e];
+                    uint32_t l2 = 0, h2 = cs->csize[b];
+                    while (l2 < h2) { uint32_t mid = (l2 + h2) / 2; if (cs->cand[b][mid] < id) l2 = mid + 1; else h2 = mid; }
+                    nxt[l2] = 1;
+                }
+            }
+        free(cur);
+        cur = nxt;
+    }
+    const uint32_t ul = path[len - 1];
+    uint32_t n = 0;
+    for (uint32_t x = 0; x < cs->csize[ul]; ++x)
+        if (cur[x] && cs->cand[ul][x] != v) out[n++] = cs->cand[ul][x];
+    free(cur);
+    return n;
 }

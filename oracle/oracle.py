@@ -51,6 +51,10 @@ _sig("orc_local_count", C.c_int, _P, _P, _P, _u32p, _u64p, _u64p)
 _sig("orc_bruteforce_count", C.c_int, _P, _P, _u64p)
 _sig("orc_unique_combine", C.c_uint64, _u32p, _u32p, C.c_int)
 _sig("orc_bf_unique_combine", C.c_uint64, _u32p, _u32p, C.c_int)
+_sig("orc_paths_from_weights", None, C.c_uint32, _u32p, _u32p, _u32p, _u64p, _u32p)
+_sig("orc_query_paths", None, _P, _P, _u64p, _u32p)
+_sig("orc_reconstruct_path", C.c_int, C.c_uint32, _u32p, C.c_uint32, C.c_uint32, _u32p)
+_sig("orc_cartesian", C.c_uint32, _P, _P, C.c_uint32, _u32p, C.c_uint32, _u32p)
 
 
 def _u32(a):
@@ -132,6 +136,20 @@ class OCandSpace:
             raise RuntimeError(f"orc_count failed: {rc}")
         return c.value
 
+    def query_paths(self):
+        """(dist uint64[nq, nq], next uint32[nq, nq]) as setQueryGraphWeights leaves them."""
+        dist = np.zeros((self.nq, self.nq), np.uint64)
+        nxt = np.zeros((self.nq, self.nq), np.uint32)
+        _lib.orc_query_paths(self._q._h, self._h, _ptr(dist, C.c_uint64), _ptr(nxt))
+        return dist, nxt
+
+    def cartesian(self, v, path):
+        """optimizedCartesianProduct: ascending candidates of path[-1] reachable from v."""
+        p = _u32(path)
+        out = np.zeros(max(1, len(self.candidates(int(p[-1])))), np.uint32)
+        n = _lib.orc_cartesian(self._q._h, self._h, int(v), _ptr(p), len(p), _ptr(out))
+        return out[:n]
+
     def local_count(self, order=None):
         nv = self._g.info()[0]
         mat = np.zeros((nv, self.nq), np.uint64)
@@ -166,3 +184,21 @@ def bf_unique_combine(sets):
     flat = _u32(np.concatenate([np.asarray(s, np.uint32) for s in sets]) if sets else [])
     sizes = _u32([len(s) for s in sets])
     return int(_lib.orc_bf_unique_combine(_ptr(flat) if len(flat) else None, _ptr(sizes), len(sets)))
+
+
+def paths_from_weights(nq, qoff, qnbr, w32):
+    """allSourcesShortestPaths over 32-bit arc weights (query CSR order): (dist, next)."""
+    dist = np.zeros((nq, nq), np.uint64)
+    nxt = np.zeros((nq, nq), np.uint32)
+    _lib.orc_paths_from_weights(nq, _ptr(_u32(qoff)), _ptr(_u32(qnbr)), _ptr(_u32(w32)), _ptr(dist, C.c_uint64),
+                                _ptr(nxt))
+    return dist, nxt
+
+
+def reconstruct_path(nxt, i, j):
+    """reconstructPath: the list of vertices, [] when there is none, None when following next
+    never reaches j (the reference's loop would not end)."""
+    nq = nxt.shape[0]
+    out = np.zeros(nq + 1, np.uint32)
+    n = _lib.orc_reconstruct_path(nq, _ptr(np.ascontiguousarray(nxt, np.uint32)), i, j, _ptr(out))
+    return None if n < 0 else out[:n].tolist()

@@ -5,6 +5,7 @@ sources (oracle/ref.mk -> oracle/_ref/).  Run in the build container, where /roo
     make -f oracle/ref.mk -j8 && python3 tests/golden/make_golden.py
 
     python3 tests/golden/make_golden.py c3_small12 c4_sample c2_more   (selected steps)
+    python3 tests/golden/make_golden.py paths
 
 Fixtures (data only: inputs and the reference's outputs):
   yeast/yeast.graph            the data graph shipped in the reference's dataset/yeast.tar.gz
@@ -18,6 +19,10 @@ Fixtures (data only: inputs and the reference's outputs):
                                reference's LOCAL_COUNT matrices
   c4_reference.json,           reference counts of the admitted benchmark queries that finish
   c2_reference_all.json        within the time limit (and the count build's output beside them)
+  paths.json                   the reference's query distances and reconstructed paths
+                               (setQueryGraphWeights + allSourcesShortestPaths + reconstructPath)
+                               and cartesian candidate sets (optimizedCartesianProduct), from
+                               oracle/_ref/ref_cs_dump --paths, for yeast and synthetic queries
   synth.json                   seeded synthetic cases (generator parameters + sha256 of the graph
                                text the generator writes) with the reference's counts and candidate
                                set sizes; local-count matrices as .npz
@@ -327,6 +332,104 @@ def workload_sample(cfg, timeout, procs=8):
                    "graph": f"workloads.graph('{cfg}')", "data_sha256": sha256_file(dpath), "queries": recs}, f,
                   indent=1)
     os.remove(dpath)
+
+
+def paths_dump(q, d, max_x_path=None):
+    """The reference's query distances, reconstructed paths and cartesian candidate sets
+    (ref_cs_dump --paths) of one query: D, P and X lines."""
+    extra = [str(max_x_path)] if max_x_path is not None else []
+    r = subprocess.run([os.path.join(REFBIN, "ref_cs_dump"), q, d, "--paths", *extra], capture_output=True, text=True,
+                       timeout=300, check=True)
+    lines = r.stdout.splitlines()
+    if lines and lines[0] == "EMPTY":
+        return None
+    nq = sum(1 for ln in lines if ln.startswith("C "))
+    weights = [[int(x) for x in ln.split()[1:4]] for ln in lines if ln.startswith("W ")]
+    qg = A.Graph.load(q)
+    _, _, qlab = qg.csr()
+    dist = [[0] * nq for _ in range(nq)]
+    nxt = [[0] * nq for _ in range(nq)]
+    paths = [[None] * nq for _ in range(nq)]
+    cart = []
+    for ln in lines:
+        t = ln.split()
+        if t[0] == "D":
+            dist[int(t[1])][int(t[2])] = int(t[3])
+        elif t[0] == "N":
+            nxt[int(t[1])][int(t[2])] = int(t[3])
+        elif t[0] == "P":  # None: the reference's reconstruction would not end
+            paths[int(t[1])][int(t[2])] = None if t[3] == "-1" else [int(x) for x in t[4:]]
+        elif t[0] == "X":
+            ids = [int(x) for x in t[5:]]
+            rec = {"i": int(t[1]), "j": int(t[2]), "v": int(t[3]), "n": len(ids),
+                   "sha256": hashlib.sha256(" ".join(map(str, ids)).encode()).hexdigest()}
+            if len(ids) <= 64:
+                rec["ids"] = ids
+            cart.append(rec)
+    return {"nq": nq, "labels": [int(x) for x in qlab], "edges": [[int(a), int(b)] for a, b in qg.edges()],
+            "arc_weight": weights, "dist": dist, "next": nxt, "paths": paths, "cart": cart}
+
+
+def paths():
+    """tests/golden/paths.json: for yeast queries of 8-16 vertices and the first queries of the
+    synthetic cases, the reference's distances, paths and cartesian candidate sets."""
+    out = []
+    ydir = os.path.join(OUT, "yeast")
+    for nm in ["query_8_1", "query_8_2", "query_10_1", "query_10_7", "query_12_2", "query_12_26", "query_14_1",
+               "query_16_1", "query_16_52"]:
+        qp = os.path.join(ydir, "queries", nm + ".graph")
+        if not os.path.exists(qp):
+            continue
+        rec = paths_dump(qp, os.path.join(ydir, "yeast.graph"))
+        if rec is None:
+            continue
+        rec["cart"] = rec["cart"][::max(1, len(rec["cart"]) // 40)][:40]
+        out.append({"source": "yeast", "query": nm, **rec})
+        print("paths", nm, rec["nq"], len(rec["cart"]))
+    for s in SYNTH:
+        g = make_graph(s)
+        dpath = f"/tmp/golden_{s['name']}.graph"
+        g.write(dpath)
+        for qi, (k, dens, mx, qseed) in enumerate(s["queries"][:2]):
+            q = g.random_walk_query(k, dens, mx, qseed)
+            qpath = f"/tmp/golden_{s['name']}_q{qi}.graph"
+            q.write(qpath)
+            rec = paths_dump(qpath, dpath)
+            os.remove(qpath)
+            if rec is None:
+                continue
+            rec["cart"] = rec["cart"][::max(1, len(rec["cart"]) // 40)][:40]
+            out.append({"source": s["name"], "query": qi, **rec})
+            print("paths", s["name"], qi, rec["nq"], len(rec["cart"]))
+        os.remove(dpath)
+    # config-2 queries, whose arc weights (10^5-10^7) make the products along longer paths wrap
+    # mod 2^64 in the reference's size_t distances
+    from asdmatch_amd import workloads as W
+
+    g = W.graph("c2")
+    dpath = "/tmp/golden_c2.graph"
+    g.write(dpath)
+    sl = W.slots("c2")
+    adm = {i: seed for i, seed, _ in W.load_manifest()["c2"]["queries"]}
+    picks = [(i, adm[i]) for i in sorted(adm)[:1]] + [(i, W.draw_seed("c2", i, 0)) for i, s in enumerate(sl)
+                                                       if s.k == 16][:1]
+    for i, seed in picks:
+        q = W.query(g, sl[i], seed)
+        qpath = f"/tmp/golden_c2_{i}.graph"
+        q.write(qpath)
+        # distances and paths only: on this graph the reference's cartesian product ends with
+        # std::bad_alloc (seen for a 3-vertex path from its largest-id candidate), so no
+        # cartesian set is sampled here
+        rec = paths_dump(qpath, dpath, max_x_path=1)
+        os.remove(qpath)
+        if rec is None:
+            continue
+        rec["cart"] = rec["cart"][::max(1, len(rec["cart"]) // 40)][:40]
+        out.append({"source": "c2", "query": [i, seed], **rec})
+        print("paths c2", i, rec["nq"], len(rec["cart"]))
+    os.remove(dpath)
+    with open(os.path.join(OUT, "paths.json"), "w") as f:
+        json.dump(out, f)
 
 
 if __name__ == "__main__":

@@ -2,6 +2,7 @@
 against the reference's own outputs (golden fixtures made by tests/golden/make_golden.py from the
 reference built from its sources), against brute force, closed forms and invariants."""
 import itertools
+import os
 import math
 import random
 
@@ -158,3 +159,105 @@ def test_unique_combine_matches_bruteforce_twin():
         assert O.unique_combine(sets) == O.bf_unique_combine(sets)
     a, b = [1, 2, 3, 5], [2, 5, 7]
     assert O.unique_combine([a, b]) == len(a) * len(b) - 2
+
+
+def _paths_cases():
+    import json
+
+    with open(os.path.join(T.GOLDEN, "paths.json")) as f:
+        return json.load(f)
+
+
+def _weights32(case, off, nbr):
+    wmap = {(u, u2): w for u, u2, w in case["arc_weight"]}
+    return [wmap[(u, int(nbr[e]))] & 0xFFFFFFFF for u in range(case["nq"]) for e in range(off[u], off[u + 1])]
+
+
+def test_query_paths_match_reference():
+    """setQueryGraphWeights + allSourcesShortestPaths + reconstructPath, restated, against the
+    reference's own distances, successor matrices and paths (ref_cs_dump --paths) on yeast,
+    synthetic and config-2 queries; the config-2 ones have products that wrap mod 2^64 and pairs
+    whose reference reconstruction would not end (None in the fixture)."""
+    cases = _paths_cases()
+    assert len(cases) >= 16
+    wrapped = loops = 0
+    for c in cases:
+        q = O.OGraph.from_edges(c["labels"], c["edges"])
+        off, nbr, _ = q.csr()
+        dist, nxt = O.paths_from_weights(c["nq"], off, nbr, _weights32(c, off, nbr))
+        assert dist.tolist() == c["dist"], (c["source"], c["query"])
+        assert nxt.tolist() == c["next"], (c["source"], c["query"])
+        for i in range(c["nq"]):
+            for j in range(c["nq"]):
+                assert O.reconstruct_path(nxt, i, j) == c["paths"][i][j], (c["source"], c["query"], i, j)
+        wrapped += sum(1 for row in c["dist"] for x in row if x >= 1 << 63 and x != (1 << 64) - 1)
+        loops += sum(1 for row in c["paths"] for p in row if p is None)
+    assert wrapped > 0 and loops > 0  # the fixtures reach the wrapping behaviour
+
+
+def test_query_paths_invariants():
+    """What the definition fixes, independently of the fixtures: every reconstructed path walks
+    query edges from i to j, and without wrap-around its product of weights is dist[i][j] and no
+    single edge beats it."""
+    for c in _paths_cases()[:16]:
+        q = O.OGraph.from_edges(c["labels"], c["edges"])
+        off, nbr, _ = q.csr()
+        w = _weights32(c, off, nbr)
+        dist, nxt = O.paths_from_weights(c["nq"], off, nbr, w)
+        wmap = {(u, int(nbr[e])): w[e] for u in range(c["nq"]) for e in range(off[u], off[u + 1])}
+        for i in range(c["nq"]):
+            for j in range(c["nq"]):
+                p = O.reconstruct_path(nxt, i, j)
+                if i == j or not p:
+                    continue
+                assert p[0] == i and p[-1] == j
+                prod = 1
+                for a, b in zip(p, p[1:]):
+                    assert (a, b) in wmap
+                    prod *= wmap[(a, b)]
+                if prod < 1 << 64:
+                    assert int(dist[i, j]) == prod
+                    if (i, j) in wmap:
+                        assert prod <= wmap[(i, j)]
+
+
+def test_cartesian_candidates_match_reference():
+    """optimizedCartesianProduct, restated layer by layer, against the reference's own sets on the
+    sampled (i, j, v) of the yeast and synthetic queries."""
+    import hashlib
+
+    synth = {c["name"]: c for c in T.synth_golden()}
+    gy = O.OGraph.load(T.yeast_data_path())
+    n = 0
+    for c in _paths_cases():
+        if c["source"] == "c2":
+            continue
+        q = O.OGraph.from_edges(c["labels"], c["edges"])
+        if c["source"] == "yeast":
+            g = gy
+        else:
+            gg = T.make_graph(synth[c["source"]])
+            _, _, lab = gg.csr()
+            g = O.OGraph.from_edges(lab, gg.edges())
+        cs = O.OCandSpace(q, g)
+        for r in c["cart"]:
+            got = cs.cartesian(r["v"], c["paths"][r["i"]][r["j"]]).tolist()
+            assert len(got) == r["n"]
+            assert hashlib.sha256(" ".join(map(str, got)).encode()).hexdigest() == r["sha256"]
+            if "ids" in r:
+                assert got == r["ids"]
+            n += 1
+    assert n >= 500
+
+
+def test_cartesian_candidates_small_cases():
+    """Closed cases: a path whose first vertex is not matched to a candidate reaches nothing; on a
+    path graph the two-hop set is the data vertices two steps away with the right labels."""
+    # data: 0-1-2-3 path, labels 0,1,0,1; query: a-b-c path, labels 0,1,0
+    g = O.OGraph.from_edges([0, 1, 0, 1], [[0, 1], [1, 2], [2, 3]])
+    q = O.OGraph.from_edges([0, 1, 0], [[0, 1], [1, 2]])
+    cs = O.OCandSpace(q, g)
+    assert cs.candidates(0).tolist() == [0, 2] and cs.candidates(2).tolist() == [0, 2]
+    assert cs.cartesian(0, [0, 1, 2]).tolist() == [2]   # 0 -> 1 -> {0, 2}, without 0 itself
+    assert cs.cartesian(2, [0, 1, 2]).tolist() == [0]   # 2 -> {1, 3}: 1 -> {0, 2}, 3 -> {2}; without 2
+    assert cs.cartesian(1, [0, 1, 2]).tolist() == []    # 1 is not a candidate of query vertex 0
