@@ -179,6 +179,10 @@ struct asdm_cs {
         uint32_t *d_ce_val = nullptr;
     };
     Store *store = nullptr;
+    // A candidate space extended with cartesian arcs (cartesian.hip) owns new candidate-edge
+    // arrays in `store` and keeps the store of the space it extends, whose candidate sets and
+    // bitmaps it shares, in base_store.
+    Store *base_store = nullptr;
     asdm_ctx *ctx = nullptr;
     uint32_t nq = 0, nv = 0, root = 0;
     bool loop_free = false;                  // the data graph has no self-loops
@@ -216,6 +220,10 @@ struct asdm_plan {
     // Set when the query is counted by the bag-table engine (bags.h) instead of the frontier
     // engine: partition terms and their elimination orders.
     std::shared_ptr<asdm::BagPlan> bags;
+    // Set for an explicit order with cartesian steps: the candidate space the plan was made on,
+    // the caller's extended with one arc per cartesian step (cartesian.hip); the engine counts on
+    // it in place of the caller's.
+    std::shared_ptr<asdm_cs> cart_cs;
 };
 
 namespace asdm {
@@ -268,6 +276,12 @@ int learned_plans(asdm_mpnn *m, const asdm_graph *g, const std::vector<const asd
 uint32_t cfl_root(const asdm_graph *q, const asdm_graph *g, const std::vector<uint32_t> &base_count);
 std::vector<uint32_t> bfs_order(const asdm_graph *q, uint32_t root);
 
+// For an order whose depth d >= 1 has no earlier neighbour (a cartesian step), a copy of cs with one
+// more arc per such depth, from its cartesian parent (the earlier vertex nearest under the query
+// paths' products, as the reference chooses it) to it, whose candidate-edge lists are the
+// cartesian candidates of the path between them (cartesian.hip).  *out stays null when the order
+// has no cartesian step.
+int cartesian_cs(const asdm_cs *cs, const std::vector<uint32_t> &order, asdm_cs **out, std::string &err);
 // Attaches a bag-table plan to p when flags ask for one, or by the automatic rule (plan.cpp).
 int attach_bags(const asdm_cs *cs, uint32_t flags, bool explicit_order, asdm_plan *p, std::string &err);
 // Bag-table engine (bags.hip): count (mat_host == NULL) or local counts of a plan with bags.

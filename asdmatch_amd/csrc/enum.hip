@@ -2187,6 +2187,9 @@ int run_engine(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t
                uint64_t *count, bool enumerate = false, uint32_t *enum_out = nullptr, uint64_t enum_cap = 0) {
     if (!ctx || !plan || !cs || !count) return ASDM_E_INVALID;
     if (plan->h.nq != cs->nq) return ctx->fail(ASDM_E_INVALID, "plan and candidate space disagree");
+    // a plan with cartesian steps counts on the candidate space it was made on: the caller's,
+    // extended with the cartesian arcs
+    if (plan->cart_cs) cs = plan->cart_cs.get();
     if (plan->bags) {
         if (enumerate) return ctx->fail(ASDM_E_INVALID, "enumeration needs a plan made with ASDM_PLAN_NO_FACTOR");
         return run_bags(ctx, plan, cs, mat_host, count);

@@ -257,6 +257,14 @@ void asdm_cs_free(asdm_cs *cs) {
             delete cs->store;
         }
     }
+    if (cs->base_store && --cs->base_store->refs == 0) {
+        dev_free(ctx, cs->base_store->d_cand);
+        dev_free(ctx, cs->base_store->d_bits);
+        dev_free(ctx, cs->base_store->d_rank);
+        dev_free(ctx, cs->base_store->d_ce_off);
+        dev_free(ctx, cs->base_store->d_ce_val);
+        delete cs->base_store;
+    }
     (void)hipStreamSynchronize(ctx->stream);
     delete cs;
 }

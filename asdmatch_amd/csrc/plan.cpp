@@ -460,8 +460,23 @@ int asdm_plan_create(const asdm_graph *q, const asdm_cs *cs, const uint32_t *ord
     auto t0 = std::chrono::steady_clock::now();
     auto p = std::make_unique<asdm_plan>();
     std::string err;
-    int rc = build_plan(q, cs, order, flags, [](uint64_t, double fallback) { return fallback; }, p.get(), err);
-    if (!rc) rc = attach_bags(cs, flags, order != nullptr, p.get(), err);
+    // an explicit order with cartesian steps is planned on cs extended with cartesian arcs
+    const asdm_cs *pcs = cs;
+    if (order) {
+        for (uint32_t i = 0; i < cs->nq; ++i)
+            if (order[i] >= cs->nq) return cs->ctx->fail(ASDM_E_INVALID, "order is not a permutation of the query vertices");
+        asdm_cs *ext = nullptr;
+        if (int rc = cartesian_cs(cs, std::vector<uint32_t>(order, order + cs->nq), &ext, err)) {
+            cs->ctx->err = err;
+            return rc;
+        }
+        if (ext) {
+            p->cart_cs.reset(ext, asdm_cs_free);
+            pcs = ext;
+        }
+    }
+    int rc = build_plan(q, pcs, order, flags, [](uint64_t, double fallback) { return fallback; }, p.get(), err);
+    if (!rc) rc = attach_bags(pcs, flags, order != nullptr, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;
@@ -498,7 +513,18 @@ int asdm_plan_create_ordered(const asdm_graph *q, const asdm_cs *cs, const uint3
     auto t0 = std::chrono::steady_clock::now();
     auto p = std::make_unique<asdm_plan>();
     std::string err;
-    int rc = finish_plan(cs, o, D, flags, [](uint64_t, double fb) { return fb; }, p.get(), err);
+    // cartesian steps (depths with no earlier neighbour) get cartesian arcs in an extended space
+    const asdm_cs *pcs = cs;
+    asdm_cs *ext = nullptr;
+    if (int rc0 = cartesian_cs(cs, o, &ext, err)) {
+        cs->ctx->err = err;
+        return rc0;
+    }
+    if (ext) {
+        p->cart_cs.reset(ext, asdm_cs_free);
+        pcs = ext;
+    }
+    int rc = finish_plan(pcs, o, D, flags, [](uint64_t, double fb) { return fb; }, p.get(), err);
     if (rc) {
         cs->ctx->err = err;
         return rc;

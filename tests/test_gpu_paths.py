@@ -100,3 +100,76 @@ def test_cartesian_candidates_edge_cases(ctx):
     assert cs.cartesian_candidates(1, [0, 1, 2]).tolist() == []
     assert cs.cartesian_candidates(0, [0, 2]).tolist() == []  # 0 and 2 are not adjacent in the query
     assert cs.query_path(0, 2) == [0, 1, 2]
+
+
+def _cartesian_order(q, rng):
+    """An order of q's vertices whose second vertex is not adjacent to the first (a cartesian
+    step); later vertices follow adjacency where they can, so there may be more such steps."""
+    off, nbr, _ = q.csr()
+    n = q.n
+    adj = [set(nbr[off[u]:off[u + 1]].tolist()) for u in range(n)]
+    for _ in range(50):
+        a = int(rng.integers(n))
+        far = [w for w in range(n) if w != a and w not in adj[a]]
+        if far:
+            break
+    else:
+        return None
+    order = [a, far[int(rng.integers(len(far)))]]
+    seen = set(order)
+    while len(order) < n:
+        nxt = [w for w in range(n) if w not in seen and adj[w] & seen]
+        w = min(nxt) if nxt else min(w for w in range(n) if w not in seen)
+        order.append(w)
+        seen.add(w)
+    return order
+
+
+def test_cartesian_orders_keep_counts(ctx):
+    """Explicit orders with cartesian steps (asdm_plan_create and asdm_plan_create_ordered, which
+    refused them before) count exactly the reference's numbers: the step's candidates come from a
+    cartesian arc built from the query path to the cartesian parent."""
+    gold = T.yeast_golden()["queries"]
+    g = A.Graph.load(T.yeast_data_path())
+    rng = np.random.default_rng(5)
+    names = [n for n in sorted(gold) if not gold[n]["empty"]][::6][:14]
+    assert len(names) >= 10
+    n = 0
+    for name in names:
+        q = A.Graph.load(T.yeast_query_path(name))
+        order = _cartesian_order(q, rng)
+        if order is None:
+            continue
+        cs = ctx.filter(q, g)
+        want = gold[name]["count"]
+        assert ctx.count(cs.plan(q, order=order), cs) == want, name
+        assert ctx.count(cs.plan(q, order=order, num_enumerated=q.n - 1), cs) == want, name
+        if want < 10 ** 7:
+            assert ctx.count(cs.plan(q, order=order, factor=False), cs) == want, name
+        n += 1
+    assert n >= 8
+
+
+def test_cartesian_orders_keep_local_counts(ctx):
+    """The reference's LOCAL_COUNT matrices of the synthetic golden cases, through plans whose
+    orders have cartesian steps."""
+    rng = np.random.default_rng(9)
+    n = 0
+    for case in T.synth_golden():
+        if not case.get("local"):
+            continue
+        g = T.make_graph(case)
+        for r in case["queries"]:
+            if r.get("local_npz") is None:
+                continue
+            q = g.random_walk_query(r["k"], r["density"], r["max_extra"], r["seed"])
+            order = _cartesian_order(q, rng)
+            if order is None:
+                continue
+            cs = ctx.filter(q, g)
+            mat, c = ctx.local_count(cs.plan(q, order=order, local=True), cs)
+            want = np.load(os.path.join(T.GOLDEN, r["local_npz"]))["local"]
+            assert c == r["count"]
+            assert np.array_equal(mat, want), (case["name"], r["seed"])
+            n += 1
+    assert n >= 5
