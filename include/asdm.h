@@ -218,7 +218,9 @@ int asdm_plan_create_with_cards(const asdm_graph *q, const asdm_cs *cs, const ui
 int asdm_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes,
                uint64_t *count);
 /* LOCAL_COUNT: mat[v * nq + u] = number of embeddings f with f(u) = v, row-major |Vg| x |Vq|,
- * caller-owned host memory; *count = total.  The plan must be created with ASDM_PLAN_LOCAL. */
+ * caller-owned host memory; *count = total.  The plan must be created with ASDM_PLAN_LOCAL.
+ * mat must hold nv * nq values, nv and nq as asdm_cs_info reports them for cs (the vertex count of
+ * the data graph cs was filtered against): the whole matrix is written. */
 int asdm_local_count(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t budget_bytes,
                      uint64_t *mat, uint64_t *count);
 /* COLLECT_RESULT: every embedding, out[i * nq + u] = data vertex of query vertex u in the i-th one,
