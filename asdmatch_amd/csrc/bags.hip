@@ -65,13 +65,8 @@ __global__ void k_dom_fill(const uint64_t *__restrict__ bits, uint32_t W, const 
 
 // Relation CSRs of quotient edges A -> B from the candidate-edge lists of query arcs a -> b
 // (a in A, b in B): row i of a relation (data vertex ids_a[i]) holds the D(B) positions of its
-// neighbours in D(B), ascending.  One launch builds the relations of many jobs: their rows are
-// numbered one after the other (job j: rows row0[j] .. row0[j + 1]), and one scan over all rows'
-// counts gives every job's offsets into one shared value array.  Count pass (FILL = false), then
-// fill pass at the scanned offsets.  kRelLanes lanes per row stride over the row's candidate-edge
-// list; the fill keeps list order with a ballot prefix inside the lane group.  (A wave per row
-// was twice as slow on config 4.)
-constexpr int kRelLanes = 16;
+// neighbours in D(B), ascending.  One build covers the relations of many jobs: their rows are
+// numbered one after the other (job j: rows row0[j] .. row0[j + 1]) and share one value array.
 struct RelJob {
     const uint32_t *ids_a;   // D(A), ascending data ids
     const uint64_t *bits_a;  // membership bitmap and rank prefix of C(a)
@@ -81,18 +76,17 @@ struct RelJob {
     const uint64_t *bits_B;  // membership bitmap and rank prefix of D(B)
     const uint32_t *rank_B;
 };
-template <bool FILL>
-__global__ void k_rel(const RelJob *__restrict__ jobs, const uint64_t *__restrict__ row0, uint32_t nj,
-                      const uint32_t *__restrict__ ce_val, uint32_t *__restrict__ cnt, const uint64_t *__restrict__ off,
-                      uint32_t *__restrict__ val, unsigned long long *__restrict__ nbytes) {
-    constexpr int G = kRelLanes;
-    const int lg = threadIdx.x & (G - 1);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
+
+// Flat relation build.  A hub's candidate-edge list can hold thousands of entries, and a lane
+// group that walks one row alone sets the time of the whole launch; here the work is spread over
+// entries instead.  k_rel_len: each row's list (source offset and length); a scan of the lengths
+// numbers the entries of all rows; k_rel_keep: per entry, whether its data vertex is in D(B) and
+// its position there; a scan of the keep flags gives every kept entry its slot, in list order
+// within each row; k_rel_place writes the positions and k_rel_off the rows' offsets.
+__global__ void k_rel_len(const RelJob *__restrict__ jobs, const uint64_t *__restrict__ row0, uint32_t nj,
+                          uint64_t *__restrict__ src, uint32_t *__restrict__ len) {
     const uint64_t rows = row0[nj];
-    uint64_t bytes = 0;
-    // one row per aligned group of G lanes: the group's lanes share the row and its trip count, so
-    // the ballots see the whole group (other groups of the wave may have left their loops)
-    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; r < rows; r += stride) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t lo = 0, hi = nj - 1;  // the job holding row r: the last j with row0[j] <= r
         while (lo < hi) {
             const uint32_t mid = (lo + hi + 1) >> 1;
@@ -102,41 +96,74 @@ __global__ void k_rel(const RelJob *__restrict__ jobs, const uint64_t *__restric
         const RelJob J = jobs[lo];
         const uint32_t p = bit_rank(J.bits_a, J.rank_a, J.ids_a[r - row0[lo]]);
         const uint64_t lb = J.ce_off[p], le = J.ce_off[p + 1];
-        uint64_t o = FILL ? off[r] : 0;
-        uint32_t c = 0;
-        // four entries per lane per pass, their lookup chains in flight together (a hub's list
-        // holds thousands of entries); the passes and the ballots keep list order
-        constexpr int Q = 4;
-        for (uint64_t e0 = lb; e0 < le; e0 += Q * G) {
-            uint32_t w[Q];
-            bool ok[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const uint64_t e = e0 + q * G + lg;
-                ok[q] = e < le;
-                w[q] = J.cand_b[ce_val[ok[q] ? e : lb]];
-            }
-#pragma unroll
-            for (int q = 0; q < Q; ++q) ok[q] = ok[q] && bit_test(J.bits_B, w[q]);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                if constexpr (FILL) {
-                    const uint32_t pos = ok[q] ? bit_rank(J.bits_B, J.rank_B, w[q]) : 0;
-                    const uint64_t gm = group_bits<G>(__ballot(ok[q]));
-                    if (ok[q]) val[o + __popcll(gm & ((1ull << lg) - 1))] = pos;
-                    o += __popcll(gm);
-                } else {
-                    c += ok[q];
-                }
-            }
+        src[r] = lb;
+        len[r] = (uint32_t)(le - lb);
+    }
+}
+
+constexpr int kRelPer = 8;  // consecutive entries per lane in k_rel_keep
+__global__ void k_rel_keep(const RelJob *__restrict__ jobs, const uint64_t *__restrict__ row0, uint32_t nj,
+                           const uint64_t *__restrict__ src, const uint64_t *__restrict__ ebase,
+                           const uint32_t *__restrict__ ce_val, uint32_t *__restrict__ keep, uint32_t *__restrict__ pos,
+                           unsigned long long *__restrict__ nbytes) {
+    const uint64_t rows = row0[nj], E = ebase[rows];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * kRelPer;
+    uint64_t bytes = 0;
+    for (uint64_t e0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kRelPer; e0 < E; e0 += stride) {
+        // the row of entry e0 (last r with ebase[r] <= e0), then its job
+        uint32_t lo = 0, hi = (uint32_t)rows - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (ebase[mid] <= e0) lo = mid;
+            else hi = mid - 1;
         }
-        if constexpr (!FILL) {
-            c = (uint32_t)group_sum<G>(c);
-            if (lg == 0) cnt[r] = c;
+        uint64_t r = lo;
+        uint32_t jl = 0, jh = nj - 1;
+        while (jl < jh) {
+            const uint32_t mid = (jl + jh + 1) >> 1;
+            if (row0[mid] <= r) jl = mid;
+            else jh = mid - 1;
         }
-        if (lg == 0) bytes += 28 + 16 * (le - lb) + (FILL ? 4 * (o - off[r]) : 0);
+        const uint64_t e1 = min(E, e0 + kRelPer);
+        uint32_t w[kRelPer];
+        uint64_t rr[kRelPer];
+        uint32_t jj[kRelPer];
+#pragma unroll
+        for (int q = 0; q < kRelPer; ++q) {  // the entries' rows and jobs, walking forward
+            const uint64_t e = e0 + q;
+            if (e < e1) {
+                while (ebase[r + 1] <= e) ++r;
+                while (jl + 1 < nj && row0[jl + 1] <= r) ++jl;
+            }
+            rr[q] = r;
+            jj[q] = jl;
+        }
+#pragma unroll
+        for (int q = 0; q < kRelPer; ++q)  // the loads of all eight entries in flight together
+            w[q] = e0 + q < e1 ? jobs[jj[q]].cand_b[ce_val[src[rr[q]] + (e0 + q - ebase[rr[q]])]] : 0;
+#pragma unroll
+        for (int q = 0; q < kRelPer; ++q) {
+            if (e0 + q >= e1) break;
+            const RelJob &J = jobs[jj[q]];
+            const bool ok = bit_test(J.bits_B, w[q]);
+            keep[e0 + q] = ok;
+            pos[e0 + q] = ok ? bit_rank(J.bits_B, J.rank_B, w[q]) : 0;
+        }
+        bytes += 16ull * (e1 - e0);
     }
     wave_add_bytes(bytes, nbytes);
+}
+
+__global__ void k_rel_place(const uint32_t *__restrict__ keep, const uint32_t *__restrict__ pos,
+                            const uint64_t *__restrict__ kpos, uint64_t E, uint32_t *__restrict__ val) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x)
+        if (keep[e]) val[kpos[e]] = pos[e];
+}
+
+__global__ void k_rel_off(const uint64_t *__restrict__ ebase, const uint64_t *__restrict__ kpos, uint64_t rows,
+                          uint64_t *__restrict__ off) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= rows; r += (uint64_t)gridDim.x * blockDim.x)
+        off[r] = kpos[ebase[r]];
 }
 
 __global__ void k_vec_mul(uint64_t *__restrict__ u, const uint64_t *__restrict__ t, uint32_t n) {
@@ -842,11 +869,17 @@ struct BagRun {
         if (!rc) rc = alloc((void **)&val, 4ull * std::max<uint64_t>(ub, 1));
         keep_allocs = false;
         RelJob *d_jobs = nullptr;
-        uint64_t *d_row0 = nullptr;
-        uint32_t *cnt = nullptr;
+        uint64_t *d_row0 = nullptr, *src = nullptr, *ebase = nullptr, *kpos = nullptr;
+        uint32_t *len = nullptr, *keep = nullptr, *pos = nullptr;
+        const uint64_t R1 = std::max<uint64_t>(rows, 1), U1 = std::max<uint64_t>(ub, 1);
         if (!rc) rc = alloc((void **)&d_jobs, sizeof(RelJob) * nj);
         if (!rc) rc = alloc((void **)&d_row0, 8ull * (nj + 1));
-        if (!rc) rc = alloc((void **)&cnt, 4ull * std::max<uint64_t>(rows, 1));
+        if (!rc) rc = alloc((void **)&src, 8ull * R1);
+        if (!rc) rc = alloc((void **)&len, 4ull * R1);
+        if (!rc) rc = alloc((void **)&ebase, 8ull * (R1 + 1));
+        if (!rc) rc = alloc((void **)&keep, 4ull * U1);
+        if (!rc) rc = alloc((void **)&pos, 4ull * U1);
+        if (!rc) rc = alloc((void **)&kpos, 8ull * (U1 + 1));
         if (rc) return rc;
         // the host copies stay alive until the queued copies have run (the run's end)
         host_jobs.push_back(std::move(jobs));
@@ -856,26 +889,36 @@ struct BagRun {
         if (!rc)
             rc = check_hip(ctx, hipMemcpyAsync(d_row0, host_row0.back().data(), 8ull * (nj + 1), hipMemcpyHostToDevice, ctx->stream),
                            "H2D relation rows");
+        // the entries of all rows are numbered up to ub (every relation is a subset of its arc's
+        // candidate edges), so no size has to come back to the host: flags past the last entry
+        // stay zero
+        if (!rc) rc = fill_u32(ctx, keep, U1, 0);
         if (rc) return rc;
-        const dim3 grid(grid_for(std::max<uint64_t>(rows, 1) * kRelLanes, kThreads));
         {
             KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel<false>, grid, dim3(kThreads), 0, ctx->stream, d_jobs, d_row0, nj, cs->d_ce_val, cnt, nullptr,
-                               nullptr, bytes_slot(0));
+            hipLaunchKernelGGL(k_rel_len, dim3(grid_for(R1, kThreads)), dim3(kThreads), 0, ctx->stream, d_jobs, d_row0, nj, src,
+                               len);
+            rc = launched("bag relation lengths");
+            if (!rc) rc = scan_u32_to_u64(ctx, len, ebase, rows);
+            if (!rc) {
+                hipLaunchKernelGGL(k_rel_keep, dim3(grid_for((U1 + kRelPer - 1) / kRelPer, kThreads)), dim3(kThreads), 0,
+                                   ctx->stream, d_jobs, d_row0, nj, src, ebase, cs->d_ce_val, keep, pos, bytes_slot(0));
+                rc = launched("bag relation keep");
+            }
+            if (!rc) rc = scan_u32_to_u64(ctx, keep, kpos, ub);
+            if (!rc) {
+                hipLaunchKernelGGL(k_rel_place, dim3(grid_for(U1, kThreads)), dim3(kThreads), 0, ctx->stream, keep, pos, kpos, ub,
+                                   val);
+                hipLaunchKernelGGL(k_rel_off, dim3(grid_for(R1 + 1, kThreads)), dim3(kThreads), 0, ctx->stream, ebase, kpos, rows,
+                                   off);
+                rc = launched("bag relation place");
+            }
             kt.stop();
         }
-        if ((rc = launched("bag relation count"))) return rc;
-        if ((rc = scan_u32_to_u64(ctx, cnt, off, rows))) return rc;
-        {
-            KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel<true>, grid, dim3(kThreads), 0, ctx->stream, d_jobs, d_row0, nj, cs->d_ce_val, nullptr, off,
-                               val, bytes_slot(0));
-            kt.stop();
-        }
-        if ((rc = launched("bag relation fill"))) return rc;
-        release(cnt);
-        release(d_jobs);
-        release(d_row0);
+        if (rc) return rc;
+        for (void *p : {(void *)d_jobs, (void *)d_row0, (void *)src, (void *)len, (void *)ebase, (void *)keep, (void *)pos,
+                        (void *)kpos})
+            release(p);
         const std::vector<uint64_t> &r0 = host_row0.back();
         for (uint32_t j = 0; j < nj; ++j) {
             Rel r;

@@ -312,7 +312,16 @@ def bench_batch(args):
     # share the device and their launch durations include that sharing.  One more step, untimed,
     # with a single worker gives the same kernels' durations when each has the device alone.
     iso = None
+    plain_plan_qps = None
     if not args.only_timed:
+        # the same batched call with the plain planner's plans: what the learned plans themselves
+        # change, apart from batching
+        say("untimed: one batched step with plain plans")
+        hip_sync()
+        t8 = time.perf_counter()
+        batched(every, None)
+        hip_sync()
+        plain_plan_qps = len(qs) / (time.perf_counter() - t8)
         workers = ctx.bag_workers()
         say("untimed: one batched step with one bag worker (isolated kernel durations)")
         ctx.set_bag_workers(1)
@@ -349,6 +358,8 @@ def bench_batch(args):
         "plain_gpu_queries_per_s": round(plain_qps, 4) if plain_qps else None, "plain_gpu_steps": plain_steps,
         "batched_learned_plan_queries_per_s": round(learned_qps, 4),
         "batched_learned_vs_plain": round(learned_qps / plain_qps, 3) if plain_qps else None,
+        "batched_plain_plan_queries_per_s": round(plain_plan_qps, 4) if plain_plan_qps else None,
+        "learned_vs_plain_plans": round(learned_qps / plain_plan_qps, 3) if plain_plan_qps else None,
         "embeddings_per_s": None,
         "embeddings_note": "counts are exact mod 2^64 like the reference's size_t; bag-engine queries have far more "
                            "than 2^64 embeddings, so their counts are residues and no embedding rate is summed over "
