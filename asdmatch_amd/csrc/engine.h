@@ -119,7 +119,7 @@ struct asdm_ctx {
     // bag-engine queries of a batched call concurrently, created on first use; bag_workers of
     // them (1: the calling context counts them one after the other).  mem_share is the share of
     // the device memory this context's caches may hold (1 / workers for a worker).
-    uint32_t bag_workers = 4;
+    uint32_t bag_workers = 6;
     std::vector<asdm_ctx *> workers;
     double mem_share = 1.0;
     uint64_t *d_counters = nullptr;  // [kNumCounters] device counters: 0 count, then algorithmic

@@ -2815,6 +2815,9 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
             else
                 bplans[k]->bags = plain.bags;
         }
+        uint64_t nmoved = 0;
+        for (auto &f : fallback) nmoved += f != nullptr;
+        add_kernel_count(ctx, "learned: bag queries moved to the frontier", nmoved, 0);
         if (rc) ctx->err = err;
         host_phase("host: engine choice");
     } else {
@@ -2898,6 +2901,7 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
                 const size_t k = moved[j];
                 if (st[j] == ASDM_E_TIMEOUT) {  // over the cap: count it with its bag plan
                     bplans[k]->bags = fallback[k];
+                    add_kernel_count(ctx, "learned: moved queries back to bags (row cap)", 1, 0);
                     pool.push({k, bplans[k], bcs[k], fallback[k]->cost});
                 } else {
                     bc[k] = cnt[j];

@@ -274,8 +274,9 @@ int asdm_ctx_set_batch_tuning(asdm_ctx *ctx, uint32_t leaf_lanes, double class_s
 int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double *class_scale,
                               uint32_t *tile_iters);
 /* Worker contexts that count the bag-engine queries of a batched call concurrently, each with its
- * own stream (1..16; default 4, the device's hardware queues per process; 1 counts them one after
- * the other on ctx).  Counts do not depend on it. */
+ * own stream (1..16; default 6: more streams than the device's 4 hardware queues per process still
+ * hide the workers' host synchronizations; 1 counts them one after the other on ctx).  Counts do not
+ * depend on it. */
 int asdm_ctx_set_bag_workers(asdm_ctx *ctx, uint32_t workers);
 int asdm_ctx_get_bag_workers(const asdm_ctx *ctx, uint32_t *workers);
 /* Autotuner: runs the batch once untimed, then `reps` timed runs (at least one) for each shape of the grid leaf
