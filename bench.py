@@ -139,14 +139,18 @@ def read_traffic(kernel, workload, alg_bytes_per_launch):
     path = TRAFFIC_FILES.get(workload)
     if not path or not os.path.exists(path):
         return None, "no PMC summary for this workload"
-    vals = {}
+    blocks = []  # one block of "key: value" lines per kernel, each opened by its "kernel:" line
     with open(path) as f:
         for line in f:
             if ":" in line:
                 k, v = line.split(":", 1)
-                vals[k.strip()] = v.strip()
-    if vals.get("kernel") != kernel or "traffic_bytes_per_launch" not in vals:
-        return None, f"the PMC summary covers {vals.get('kernel')}, not {kernel}"
+                if k.strip() == "kernel":
+                    blocks.append({})
+                if blocks:
+                    blocks[-1][k.strip()] = v.strip()
+    vals = next((b for b in blocks if b.get("kernel") == kernel), None)
+    if vals is None or "traffic_bytes_per_launch" not in vals:
+        return None, f"the PMC summary covers {', '.join(b.get('kernel', '?') for b in blocks)}, not {kernel}"
     ref = float(vals.get("alg_bytes_per_launch", 0))
     if not ref or abs(alg_bytes_per_launch - ref) > 0.05 * ref:
         return None, (f"the PMC summary was measured on launches of {ref:.4g} algorithmic bytes, these average "
