@@ -135,6 +135,7 @@ _sig("asdm_ctx_set_batch_tuning", C.c_int, _P, C.c_uint32, C.c_double, C.c_uint3
 _sig("asdm_ctx_get_batch_tuning", C.c_int, _P, _u32p, C.POINTER(C.c_double), _u32p)
 _sig("asdm_ctx_set_bag_workers", C.c_int, _P, C.c_uint32)
 _sig("asdm_ctx_get_bag_workers", C.c_int, _P, _u32p)
+_sig("asdm_ctx_set_table_form", C.c_int, _P, C.c_int)
 _sig("asdm_batch_autotune", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _P, C.c_uint32, _u32p,
      C.POINTER(C.c_double), _u32p, _f32p)
 
@@ -352,6 +353,11 @@ class Context:
 
     def set_bag_workers(self, n: int):
         self._check(_lib.asdm_ctx_set_bag_workers(self._h, n), "set_bag_workers")
+
+    def set_table_form(self, form: int):
+        """Form of the bag engine's relation products: 0 per product from its term count, 1 LDS row
+        accumulation, 2 zero-fill + atomic scatter.  Counts do not depend on it."""
+        self._check(_lib.asdm_ctx_set_table_form(self._h, form), "set_table_form")
 
     def set_batch_tuning(self, leaf_lanes: int, class_scale: float, tile_iters: int):
         self._check(_lib.asdm_ctx_set_batch_tuning(self._h, leaf_lanes, float(class_scale), tile_iters),

@@ -125,6 +125,7 @@ int BagPool::start() {
         asdm_ctx *w = ctx->workers[i];
         w->time_limit_ms = ctx->time_limit_ms;
         w->row_limit = ctx->row_limit;
+        w->table_form = ctx->table_form;
         w->mem_share = kWorkerShare / nw;
         s_->trims0.push_back(w->trims);
         if (w->profiling != ctx->profiling) {

@@ -276,6 +276,97 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
     if (threadIdx.x == 0 && blockIdx.x == 0) total[0] += (unsigned long long)((uint64_t)mu * scal[0]);
 }
 
+// The product of two edge relations has two forms.  Its terms are the pairs (entry x -> j of
+// R(X->v), entry j -> y of R(v->Y)) with a nonzero weight u[j] wx; when they are few against the
+// table's cells (the tables of sparse relations are mostly zeros), the table is cleared with
+// streaming stores and the terms are added into it with global atomics (k_ee_zero, k_ee_scatter);
+// otherwise the rows are accumulated in LDS (k_ee).  k_ee_terms counts the terms on the device and
+// every kernel of the product reads the count, so the form is chosen without a host round trip.
+constexpr uint64_t kEeSparse = 8;  // scatter form while terms * kEeSparse < cells
+__device__ inline bool ee_scatter_form(int form, const unsigned long long *terms, uint64_t cells) {
+    return form ? form == 2 : terms[0] * kEeSparse < cells;  // form: 0 by the count, 1 LDS, 2 scatter
+}
+
+// terms[0] += the number of the product's terms (zeroed before)
+__global__ void k_ee_terms(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
+                           const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
+                           const uint64_t *__restrict__ u, uint32_t nx, unsigned long long *__restrict__ terms) {
+    const uint64_t base = xoff[0], nnz = xoff[nx] - base;
+    uint64_t s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = xval[base + i];
+        if (u[j] * (xw ? xw[i] : 1ull)) s += voff[j + 1] - voff[j];
+    }
+    s = group_sum<64>(s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(terms, (unsigned long long)s);
+}
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// Scatter form, first step: G = 0 (G is 256-byte aligned: 16-byte streaming stores, then the odd tail).
+__global__ void k_ee_zero(uint64_t *__restrict__ G, uint64_t cells, const unsigned long long *__restrict__ terms,
+                          int form) {
+    if (!ee_scatter_form(form, terms, cells)) return;
+    const uint64_t pairs = cells / 2, tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64x2 z = {0ull, 0ull};
+    for (uint64_t i = tid; i < pairs; i += (uint64_t)gridDim.x * blockDim.x) __builtin_nontemporal_store(z, (u64x2 *)G + i);
+    if (tid == 0 && (cells & 1)) G[cells - 1] = 0;
+}
+
+// Scatter form, second step: G[x][y] += u[j] wx wv for every term.  A wave takes 64 consecutive
+// entries of R(X->v), one per lane, and then the terms of their lists together, 64 at a time: the
+// lane of term t finds the entry that owns it by a binary search over the lanes' running list
+// lengths (cross-lane reads), so a long list is spread over the wave instead of held by one lane.
+__global__ void k_ee_scatter(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
+                             const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
+                             const uint32_t *__restrict__ vval, const uint64_t *__restrict__ vw,
+                             const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny, uint64_t *__restrict__ G,
+                             const unsigned long long *__restrict__ terms, int form,
+                             unsigned long long *__restrict__ nbytes) {
+    if (!ee_scatter_form(form, terms, (uint64_t)nx * ny)) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t xbase = xoff[0], vbase = voff[0], nnz = xoff[nx] - xbase;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64);
+    uint64_t bytes = 0;
+    for (uint64_t c0 = ((uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * 64; c0 < nnz; c0 += nwaves * 64) {
+        const uint64_t i = c0 + lane;
+        uint64_t len = 0, b2 = 0, w = 0;
+        uint32_t row = 0;
+        if (i < nnz) {
+            const uint32_t j = xval[xbase + i];
+            w = u[j] * (xw ? xw[i] : 1ull);
+            if (w) {
+                b2 = voff[j];
+                len = voff[j + 1] - b2;
+                row = row_of(xoff, nx, xbase + i);
+            }
+        }
+        const uint64_t inc = wave_inclusive_scan(len);
+        const uint64_t total = __shfl(inc, 63, 64);
+        for (uint64_t t0 = 0; t0 < total; t0 += 64) {
+            const uint64_t t = t0 + lane;
+            const uint64_t tq = t < total ? t : total - 1;
+            uint32_t lo = 0, hi = 63;  // the first lane whose running length exceeds tq
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (__shfl(inc, (int)mid, 64) > tq) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint64_t o_inc = __shfl(inc, (int)lo, 64), o_len = __shfl(len, (int)lo, 64);
+            const uint64_t o_b2 = __shfl(b2, (int)lo, 64), o_w = __shfl(w, (int)lo, 64);
+            const uint32_t o_row = __shfl(row, (int)lo, 64);
+            if (t < total) {
+                const uint64_t f = o_b2 + (t - (o_inc - o_len));
+                const uint64_t wf = vw ? o_w * vw[f - vbase] : o_w;
+                if (wf) atomicAdd((unsigned long long *)&G[(uint64_t)o_row * ny + vval[f]], (unsigned long long)wf);
+                bytes += vw ? 28 : 20;
+            }
+        }
+        if (i < nnz) bytes += 28;
+    }
+    wave_add_bytes(bytes, nbytes);
+}
+
 // G[x][y] = sum_{j in R(X->v)(x)} u[j] wx * wv [y in R(v->Y)(j)]: both factors are edge relations
 // (wx, wv their edge weights, 1 where the weight array is null).  The work items are (row x, chunk
 // of kEeCols columns) pairs, taken in turn by a resident grid.  A workgroup accumulates one item's
@@ -287,12 +378,13 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
 // clearing pass and 8-byte stores wrote the table at about 1 TB/s on config 4.)
 constexpr int kEeThreads = 256;
 constexpr uint32_t kEeCols = 2560;
-typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
                                                  const uint32_t *__restrict__ vval, const uint64_t *__restrict__ vw,
                                                  const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny,
-                                                 uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
+                                                 uint64_t *__restrict__ G, const unsigned long long *__restrict__ terms,
+                                                 int form, unsigned long long *__restrict__ nbytes) {
+    if (ee_scatter_form(form, terms, (uint64_t)nx * ny)) return;  // the scatter form makes this table
     __shared__ __align__(16) unsigned long long acc[kEeCols];
     for (uint32_t y = threadIdx.x; y < kEeCols; y += kEeThreads) acc[y] = 0;
     __syncthreads();
@@ -717,7 +809,8 @@ struct BagRun {
     unsigned long long *d_acc = nullptr;    // ctx counters: [0] total, [4] scratch sum, [5] term scalar,
                                             // [6] relation and [7] vector algorithmic bytes, [14..17]
                                             // bytes the table kernels' loops touch (ee, de, ed, dd),
-                                            // [18..19] those of the sparse joins (ee; ed and dd)
+                                            // [18..19] those of the sparse joins (ee; ed and dd),
+                                            // [20] the term count of a relation product
     uint64_t *d_mat = nullptr;
     std::vector<void *> owned;              // per-term device buffers
     // Domains of merged blocks and relations between blocks depend only on the blocks' members,
@@ -1316,11 +1409,25 @@ struct BagRun {
         if (fxv.kind == 1 && fvy.kind == 1) {
             const Rel rx = relv(fxv, X), rv = relfrom_v(fvy, Y);
             KTimer kt(ctx, kname);
-            // a resident grid: eight workgroups per CU
+            // count the terms, then both forms: each kernel returns at once unless the count picks
+            // its form (the LDS form for many terms per cell, the scatter form for few)
+            unsigned long long *terms = d_acc + 20;
+            if ((rc = fill_u64(ctx, (uint64_t *)terms, 1, 0))) return rc;
+            const uint64_t cells = (uint64_t)nx * ny, resident = 8ull * ctx->num_cus;
+            auto capped = [&](uint64_t items, uint64_t cap) {  // grid-stride kernels: at most cap workgroups
+                return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + kThreads - 1) / kThreads, cap)));
+            };
+            hipLaunchKernelGGL(k_ee_terms, capped(rx.nnz, resident), dim3(kThreads), 0, ctx->stream, rx.off, rx.val,
+                               rx.w, rv.off, uv, nx, terms);
+            hipLaunchKernelGGL(k_ee_zero, capped(cells / 2, 2 * resident), dim3(kThreads), 0, ctx->stream, G, cells,
+                               terms, ctx->table_form);
+            hipLaunchKernelGGL(k_ee_scatter, capped(rx.nnz, resident), dim3(kThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off,
+                               rv.val, rv.w, uv, nx, ny, G, terms, ctx->table_form, bytes_slot(8));
+            // the LDS form: a resident grid, eight workgroups per CU
             const uint64_t items = (uint64_t)nx * ((ny + kEeCols - 1) / kEeCols);
-            const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, 8ull * ctx->num_cus));
+            const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, resident));
             hipLaunchKernelGGL(k_ee, dim3(grid), dim3(kEeThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off, rv.val,
-                               rv.w, uv, nx, ny, G, bytes_slot(8));
+                               rv.w, uv, nx, ny, G, terms, ctx->table_form, bytes_slot(8));
             kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
         } else if (fxv.kind == 1 || fvy.kind == 1) {
             // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;

@@ -435,6 +435,12 @@ int asdm_ctx_set_bag_workers(asdm_ctx *ctx, uint32_t workers) try {
     return ASDM_OK;
 } ASDM_CTX_CATCH(ctx)
 
+int asdm_ctx_set_table_form(asdm_ctx *ctx, int form) try {
+    if (!ctx || form < 0 || form > 2) return ASDM_E_INVALID;
+    ctx->table_form = form;
+    return ASDM_OK;
+} ASDM_CTX_CATCH(ctx)
+
 int asdm_ctx_get_bag_workers(const asdm_ctx *ctx, uint32_t *workers) try {
     if (!ctx || !workers) return ASDM_E_INVALID;
     *workers = ctx->bag_workers;

@@ -120,6 +120,9 @@ struct asdm_ctx {
     // them (1: the calling context counts them one after the other).  mem_share is the share of
     // the device memory this context's caches may hold (1 / workers for a worker).
     uint32_t bag_workers = 6;
+    // Form of the bag engine's relation products: 0 chosen per product from its term count, 1 the
+    // LDS row accumulation, 2 the zero-fill + atomic scatter (for tests and tuning).
+    int table_form = 0;
     std::vector<asdm_ctx *> workers;
     double mem_share = 1.0;
     uint64_t *d_counters = nullptr;  // [kNumCounters] device counters: 0 count, then algorithmic

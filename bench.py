@@ -219,6 +219,8 @@ def bench_batch(args):
     ctx.upload(g)
     if args.bag_workers:
         ctx.set_bag_workers(args.bag_workers)
+    if args.table_form:
+        ctx.set_table_form(args.table_form)
     m = None
     if os.path.exists(ESTIMATOR):
         m = A.Mpnn(ctx)
@@ -355,7 +357,7 @@ def bench_batch(args):
         "not_admitted_note": "slots whose draws are neither within the frontier row limit nor countable by the "
                              "bag-table engine (a quotient of treewidth > 2, more than 2^14 partition terms, or over "
                              "the 2 s guard); sizes 21-32 have none admitted",
-        "admission_row_limit": man["row_limit"], "learned_plans": m is not None, "bag_workers": ctx.bag_workers(),
+        "admission_row_limit": man["row_limit"], "learned_plans": m is not None, "bag_workers": ctx.bag_workers(), "table_form": args.table_form,
         "batch_launch_shape": {"leaf_lanes": tune[0], "class_scale": tune[1], "tile_iters": tune[2],
                                "autotuned_on": "every fourth frontier query" if tune[3] is not None else None}})
     out.update({
@@ -527,6 +529,8 @@ def main():
                     help="c4 only: warmup and the timed batched steps alone (no plain path, frontier subset, "
                          "autotuning, isolated step or CPU baseline), for profilers")
     ap.add_argument("--bag-workers", type=int, default=0, help="c4 only: bag-engine worker streams (default: library's)")
+    ap.add_argument("--table-form", type=int, default=0, choices=(0, 1, 2),
+                    help="c4 only: relation-product form (0 per product, 1 LDS rows, 2 zero-fill + scatter)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
     args = ap.parse_args()

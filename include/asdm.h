@@ -33,6 +33,7 @@
  *   asdm_batch_autotune,        (no counterpart: launch shape of the batched engine)
  *   asdm_ctx_{set,get}_batch_tuning
  *   asdm_ctx_{set,get}_bag_workers (no counterpart: concurrent bag-engine queries of a batch)
+ *   asdm_ctx_set_table_form     (no counterpart: form of the bag engine's relation products)
  *   asdm_plain_estimate         (no counterpart: this planner's own independence-model estimate,
  *                               the learned estimator's fallback)
  */
@@ -279,6 +280,11 @@ int asdm_ctx_get_batch_tuning(const asdm_ctx *ctx, uint32_t *leaf_lanes, double 
  * depend on it. */
 int asdm_ctx_set_bag_workers(asdm_ctx *ctx, uint32_t workers);
 int asdm_ctx_get_bag_workers(const asdm_ctx *ctx, uint32_t *workers);
+/* Form of the bag engine's products of two edge relations into a dense table: 0 (default) picks per
+ * product from its term count (few terms per table cell: zero-fill + atomic scatter; many: row
+ * accumulation in LDS), 1 always the LDS form, 2 always the scatter form.  For tests and tuning;
+ * counts do not depend on it.  ASDM_E_INVALID outside 0..2. */
+int asdm_ctx_set_table_form(asdm_ctx *ctx, int form);
 /* Autotuner: runs the batch once untimed, then `reps` timed runs (at least one) for each shape of the grid leaf
  * lanes {4, 16, 64} x class factor {0.25, 0.5, 1, 2} x tile rows {2, 4, 8}; keeps the shape with the
  * smallest median time on the context and returns it (ms_best: that median).  Fails with

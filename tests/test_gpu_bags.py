@@ -220,3 +220,82 @@ def test_config3_query_at_full_size(ctx):
     mat2, c2 = ctx.local_count(cs2.plan(q2, local=True), cs2)
     assert c2 == c
     assert np.array_equal(mat2[:, perm], mat)
+
+
+def test_relation_product_forms_agree(ctx):
+    """The two forms of the product of two edge relations into a dense table (row accumulation in
+    LDS; zero-fill + atomic scatter of the terms), each forced for every product: the same counts
+    and local-count matrices as brute force and the reference's golden matrices, and the same
+    config-3 matrix and config-4 batch counts at full size.  Out-of-range forms are rejected."""
+    from asdmatch_amd import workloads as W
+
+    try:
+        rng = random.Random(21)
+        cases = []
+        for trial in range(120):
+            glab, ge, qlab, qe = random_case(rng, 6, 14, 2, 7, rng.randint(1, 3))
+            cases.append((glab, ge, qlab, qe))
+        for form in (1, 2):
+            ctx.set_table_form(form)
+            done = 0
+            for trial, (glab, ge, qlab, qe) in enumerate(cases):
+                g, og = T.graph_from_lists(glab, ge)
+                q, oq = T.graph_from_lists(qlab, qe)
+                cs = ctx.filter(q, g)
+                if cs is None:
+                    continue
+                p = bag_plan(cs, q, local=True)
+                if p is None:
+                    continue
+                mat, c = ctx.local_count(p, cs)
+                assert c == O.bruteforce_count(oq, og), (form, trial)
+                assert np.array_equal(mat, O.OCandSpace(oq, og).local_count()[0]), (form, trial)
+                done += 1
+            assert done >= 40, form
+            nloc = 0
+            for case in T.synth_golden():
+                g = T.make_graph(case)
+                for rec in case["queries"]:
+                    if rec["count"] is None or not rec.get("local_npz"):
+                        continue
+                    q = g.random_walk_query(rec["k"], rec["density"], rec["max_extra"], rec["seed"])
+                    cs = ctx.filter(q, g)
+                    p = bag_plan(cs, q, local=True)
+                    if p is None:
+                        continue
+                    mat, c = ctx.local_count(p, cs)
+                    ref = np.load(f"{T.GOLDEN}/{rec['local_npz']}")["local"]
+                    assert c == rec["count"] and np.array_equal(mat, ref), (form, case["name"], rec["seed"])
+                    nloc += 1
+            assert nloc >= 12, form
+            say(f"table form {form}: {done} brute-force cases, {nloc} reference matrices")
+        # config 3 at full size: both forms give the same matrix, with the manifest's count
+        (slot, seed, count), = [tuple(x) for x in W.load_manifest()["c3"]["queries"]]
+        g = W.graph("c3")
+        ctx.upload(g)
+        q = W.query(g, W.slots("c3")[slot], seed)
+        cs = ctx.filter(q, g)
+        mats = []
+        for form in (1, 2):
+            ctx.set_table_form(form)
+            mat, c = ctx.local_count(cs.plan(q, local=True), cs)
+            assert c == count, form
+            mats.append(mat)
+        assert np.array_equal(mats[0], mats[1])
+        # a share of config 4's bag-engine queries through the batched call
+        man = W.load_manifest()["c4"]
+        g4 = W.graph("c4")
+        sl = W.slots("c4")
+        bag = [(i, s, c) for (i, s, c), e in zip(man["queries"], man["engines"]) if e == "bags"][1::8]
+        qs = [W.query(g4, sl[i], s) for i, s, _ in bag]
+        want = [c for _, _, c in bag]
+        for form in (1, 2):
+            ctx.set_table_form(form)
+            counts, st, _ = ctx.batch_count_learned(g4, qs)
+            assert [int(c) for c in counts] == want and not any(st), form
+            say(f"table form {form}: {len(qs)} config-4 bag queries")
+        for bad in (-1, 3):
+            with pytest.raises(A.AsdmError):
+                ctx.set_table_form(bad)
+    finally:
+        ctx.set_table_form(0)
