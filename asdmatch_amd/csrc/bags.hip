@@ -282,7 +282,7 @@ __global__ void k_total_add(unsigned long long *__restrict__ total, const uint64
 // streaming stores and the terms are added into it with global atomics (k_ee_zero, k_ee_scatter);
 // otherwise the rows are accumulated in LDS (k_ee).  k_ee_terms counts the terms on the device and
 // every kernel of the product reads the count, so the form is chosen without a host round trip.
-constexpr uint64_t kEeSparse = 8;  // scatter form while terms * kEeSparse < cells
+constexpr uint64_t kEeSparse = 64;  // scatter form while terms * kEeSparse < cells
 __device__ inline bool ee_scatter_form(int form, const unsigned long long *terms, uint64_t cells) {
     return form ? form == 2 : terms[0] * kEeSparse < cells;  // form: 0 by the count, 1 LDS, 2 scatter
 }
@@ -370,14 +370,17 @@ __global__ void k_ee_scatter(const uint64_t *__restrict__ xoff, const uint32_t *
 // G[x][y] = sum_{j in R(X->v)(x)} u[j] wx * wv [y in R(v->Y)(j)]: both factors are edge relations
 // (wx, wv their edge weights, 1 where the weight array is null).  The work items are (row x, chunk
 // of kEeCols columns) pairs, taken in turn by a resident grid.  A workgroup accumulates one item's
-// columns in LDS (20 KiB, so that eight workgroups share a CU and their waves hide each other's
-// list loads) with 64-bit LDS atomics: its waves take the row's entries j in turn and the lanes of
-// a wave split j's list (ascending, so a chunk's part of it is found by two binary searches).  It
-// then writes the chunk with 16-byte non-temporal stores, clearing the LDS for the next item in
-// the same pass.  (One workgroup per whole row with 80 KiB of LDS, two per CU, a separate
-// clearing pass and 8-byte stores wrote the table at about 1 TB/s on config 4.)
+// columns in LDS (64 KiB, two workgroups per CU) with 64-bit LDS atomics: its waves take the row's
+// entries j in turn and the lanes of a wave split j's list (ascending, so when a row has several
+// chunks, a chunk's part of the list is found by two binary searches).  It then writes the chunk
+// with 16-byte non-temporal stores, clearing the LDS for the next item in the same pass.  On the
+// config-4 batch (tables of ~7.5k x 7k) 8,192 columns per chunk took 464 ms per batch against 790
+// with 2,560 (eight workgroups per CU), 766 with 4,096 and 564 with 12,288 or 16,384 (1,024
+// threads): most rows fit one chunk, so the searches and the per-item latency go.  (One
+// workgroup per whole row with 80 KiB of LDS, a separate clearing pass and 8-byte stores wrote the
+// table at about 1 TB/s.)
 constexpr int kEeThreads = 256;
-constexpr uint32_t kEeCols = 2560;
+constexpr uint32_t kEeCols = 8192;
 __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
                                                  const uint32_t *__restrict__ vval, const uint64_t *__restrict__ vw,
@@ -447,90 +450,96 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
 
 // G[x][y] = sum_{j in R(x)} u[j] w[e] F[j][y]: an edge relation (rows x, columns j; w its edge
 // weights, 1 where null) times a dense table whose rows are the relation's columns.
-// The columns are cut into strips of GL * CPL columns and the work items are (strip, row x)
-// pairs, one per group of GL lanes; lane l of a group owns the CPL columns l, l + GL, ... of the
-// strip, so every load of a group reads GL consecutive u64 of one row of F.  The strips are dealt
-// to the eight XCDs (workgroup b runs on XCD b % 8; the grid is a multiple of 8 and fits the GPU
-// at once), and each XCD walks its strips in turn, so the rows of an F strip, re-read once per
-// relation entry that names them, are re-read within one XCD's L2.  A group loads GL list
-// entries (column j and weight) into its lanes and broadcasts them by shuffles; the table is
-// written with non-temporal stores, which keep it out of L2.  Measured on the config-4 batch
-// (tables of ~7.5k x 7k): 64 lanes x 1 column beat 32 x 1 by 23 %, 16 x 4, 32 x 2, 32 x 4 and
-// 64 x 2 were slower, and a wave-per-item form with the list in scalar registers was 7 % slower.
-// The push form (each dense row read once, its entries added into an LDS row over the relation's
-// lists with 64-bit LDS atomics) was four times slower: ~450M random LDS atomics per table.
-// SIDE only names the instantiation after the step that launches it (0: edge (X, v) x dense
-// (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed side), so that
-// profilers report the two apart.
+// A work unit is (strip of 64 columns, block of kEdsRows consecutive rows), one wave, lane l
+// owning column l of the strip, so every dense-row load of the wave reads 64 consecutive u64.
+// The wave reads the block's entries 64 at a time (one per lane: column j, weight) and walks them
+// in order, broadcasting each one's j and weight through scalar registers and gathering
+// kEdInFlight dense rows before it uses the first; a row's sum is stored when the walk passes the
+// row's last entry, and rows without entries get zeros.  The strips are dealt to the eight XCDs
+// (workgroup b runs on XCD b % 8; the grid is a multiple of 8 and fits the GPU at once), and each
+// XCD walks its strips in turn, so the rows of an F strip, re-read once per relation entry that
+// names them, are re-read within one XCD's L2.  The table is written with non-temporal stores,
+// which keep it out of L2.  Measured on the config-4 batch (tables of ~7.5k x 7k), one item per
+// (strip, row) instead: 64 lanes x 1 column beat 32 x 1 by 23 %, and 16 x 4, 32 x 2, 32 x 4, 64 x 2
+// and 16 loads in flight were slower; this streamed form is 2 % faster than that one, with 32 rows
+// per unit (8, 16 and 64 slower).  The push form (each dense row read once, its entries added into
+// an LDS row over the relation's lists with 64-bit LDS atomics) was four times slower: ~450M random
+// LDS atomics per table.  SIDE only names the instantiation after the step that launches it (0:
+// edge (X, v) x dense (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed
+// side), so that profilers report the two apart.
 constexpr int kEdInFlight = 8;
-template <int SIDE, int GL, int CPL>
-__global__ void __launch_bounds__(kThreads) k_ed(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
-                                                 const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
-                                                 const uint64_t *__restrict__ F, uint32_t nx, uint32_t ny,
-                                                 uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
-    constexpr uint32_t W = GL * CPL;  // columns per strip
-    const uint32_t lg = threadIdx.x & (GL - 1);
-    const uint32_t nstrips = (ny + W - 1) / W;
+constexpr uint32_t kEdsRows = 32;
+template <int SIDE>
+__global__ void __launch_bounds__(kThreads) k_eds(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
+                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
+                                                  const uint64_t *__restrict__ F, uint32_t nx, uint32_t ny,
+                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nstrips = (ny + 63) / 64, nrb = (nx + kEdsRows - 1) / kEdsRows;
     const uint32_t nxcd = nstrips >= 8 && gridDim.x % 8 == 0 ? 8 : 1;
     const uint32_t xcd = blockIdx.x % nxcd, lblk = blockIdx.x / nxcd, nblk = gridDim.x / nxcd;
     const uint64_t my_strips = xcd < nstrips ? (nstrips - xcd + nxcd - 1) / nxcd : 0;
-    const uint64_t items = my_strips * nx;
-    constexpr uint32_t gpb = kThreads / GL;  // lane groups per workgroup
-    const uint64_t xbase = xoff[0];          // weights are indexed from the relation's first entry
+    const uint64_t units = my_strips * nrb;
+    constexpr uint32_t wpb = kThreads / 64;
+    const uint64_t xbase = xoff[0];  // weights are indexed from the relation's first entry
     uint64_t bytes = 0;
-    for (uint64_t it = (uint64_t)lblk * gpb + threadIdx.x / GL; it < items; it += (uint64_t)nblk * gpb) {
-        const uint32_t strip = xcd + nxcd * (uint32_t)(it / nx), x = (uint32_t)(it % nx);
-        const uint32_t c0 = strip * W + lg;
-        bool ok[CPL];
-        const uint64_t *fc[CPL];
-        uint64_t acc[CPL];
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            ok[c] = c0 + c * GL < ny;
-            fc[c] = F + (ok[c] ? c0 + c * GL : 0);
-            acc[c] = 0;
-        }
-        const uint64_t lb = xoff[x], le = xoff[x + 1];
-        for (uint64_t k0 = lb; k0 < le; k0 += GL) {
-            const uint64_t ke = k0 + lg;
+    for (uint64_t it = (uint64_t)lblk * wpb + wave; it < units; it += (uint64_t)nblk * wpb) {
+        const uint32_t strip = xcd + nxcd * (uint32_t)(it / nrb), rb = (uint32_t)(it % nrb);
+        const uint32_t c = strip * 64 + lane;
+        const bool ok = c < ny;
+        const uint64_t *fc = F + (ok ? c : 0);
+        const uint32_t r0 = rb * kEdsRows, nr = min(nx - r0, kEdsRows);
+        // lane l < nr holds the end of row r0 + l
+        const uint64_t rend_l = lane < nr ? xoff[r0 + lane + 1] : 0;
+        const uint64_t e0 = xoff[r0], e1 = xoff[r0 + nr];
+        uint64_t *gr = G + (uint64_t)r0 * ny + c;
+        uint32_t cur = 0;  // the row being summed, relative to r0
+        uint64_t rend = __builtin_amdgcn_readfirstlane((uint32_t)rend_l) |
+                        ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(rend_l >> 32)) << 32);
+        uint64_t acc = 0;
+        for (uint64_t k0 = e0; k0 < e1; k0 += 64) {
+            const uint64_t ke = k0 + lane;
             uint32_t jl = 0;
             uint64_t wl = 0;
-            if (ke < le) {
+            if (ke < e1) {
                 jl = xval[ke];
                 wl = xw ? u[jl] * xw[ke - xbase] : u[jl];
             }
-            const uint32_t cnt = (uint32_t)(le - k0 < GL ? le - k0 : GL);
-            // up to kEdInFlight dense-row loads issued before the first is used (a typical row
-            // has about ten entries); the table has at most 2^31 entries, so an element offset
-            // j * ny fits 32 bits
+            const uint32_t cnt = (uint32_t)min<uint64_t>(64, e1 - k0);
             for (uint32_t t = 0; t < cnt; t += kEdInFlight) {
-                uint32_t jr[kEdInFlight];
-                uint64_t fv[kEdInFlight][CPL];
-#pragma unroll
-                for (int q = 0; q < kEdInFlight; ++q) jr[q] = (uint32_t)__shfl(jl, (int)(t + q) & (GL - 1), GL) * ny;
+                uint64_t jr[kEdInFlight];  // element offsets of the dense rows (tables may pass 2^32 entries)
+                uint64_t fv[kEdInFlight];
 #pragma unroll
                 for (int q = 0; q < kEdInFlight; ++q)
+                    jr[q] = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)jl, (int)((t + q) & 63)) * ny;
 #pragma unroll
-                    for (int c = 0; c < CPL; ++c) fv[q][c] = t + q < cnt && ok[c] ? fc[c][jr[q]] : 0;
+                for (int q = 0; q < kEdInFlight; ++q) fv[q] = t + q < cnt && ok ? fc[jr[q]] : 0;
 #pragma unroll
                 for (int q = 0; q < kEdInFlight; ++q) {
-                    const uint64_t w0 = __shfl(wl, (int)(t + q) & (GL - 1), GL);
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c) acc[c] += w0 * fv[q][c];
+                    if (t + q >= cnt) break;
+                    const uint64_t e = k0 + t + q;
+                    while (e >= rend) {  // entry e is past the row: store the row, move on
+                        if (ok) __builtin_nontemporal_store(acc, gr + (uint64_t)cur * ny);
+                        acc = 0;
+                        ++cur;
+                        rend = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rend_l, (int)cur) |
+                               ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(rend_l >> 32), (int)cur) << 32);
+                    }
+                    const uint64_t w = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wl, (int)(t + q)) |
+                                       ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(wl >> 32), (int)(t + q)) << 32);
+                    acc += w * fv[q];
                 }
             }
         }
-        uint64_t *gr = G + (uint64_t)x * ny + c0;
-#pragma unroll
-        for (int c = 0; c < CPL; ++c)
-            if (ok[c]) __builtin_nontemporal_store(acc[c], gr + c * GL);
-        if (lg == 0) bytes += 16 + (le - lb) * (12 + 8ull * W) + 8ull * W;
+        for (; cur < nr; ++cur) {  // the last summed row, then rows with no entries left
+            if (ok) __builtin_nontemporal_store(acc, gr + (uint64_t)cur * ny);
+            acc = 0;
+        }
+        if (lane == 0) bytes += 8ull * (nr + 1) + (e1 - e0) * (12 + 8ull * 64) + 8ull * 64 * nr;
     }
     wave_add_bytes(bytes, nbytes);
 }
-
-// Strip shape of k_ed: lanes per group and columns per lane.
-constexpr int kEdLanes = 64, kEdCols = 1;
 
 // G = A diag(u) B with A (nx x nv) and B (nv x ny) dense: 64 x 64 output tiles, 16 x 16 threads
 // with 4 x 4 outputs each, K-chunks of 16 staged in LDS.
@@ -1442,15 +1451,14 @@ struct BagRun {
             const uint32_t ne = dom[e].n, nd = dom[d].n;
             KTimer kt(ctx, kname);
             // a resident grid, a multiple of 8 (one share per XCD): at most 8 workgroups per CU
-            constexpr int gl = kEdLanes, cpl = kEdCols;
-            const uint64_t items = (uint64_t)ne * ((nd + gl * cpl - 1) / (gl * cpl));
-            const uint64_t want = (items + kThreads / gl - 1) / (kThreads / gl);
+            const uint64_t units = (uint64_t)((nd + 63) / 64) * ((ne + kEdsRows - 1) / kEdsRows);
+            const uint64_t want = (units + kThreads / 64 - 1) / (kThreads / 64);
             const dim3 grid((unsigned)(8 * std::max<uint64_t>(1, std::min<uint64_t>((want + 7) / 8, ctx->num_cus))));
             if (dense_x)
-                hipLaunchKernelGGL((k_ed<1, gl, cpl>), grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
+                hipLaunchKernelGGL(k_eds<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
                                    bytes_slot(9));
             else
-                hipLaunchKernelGGL((k_ed<0, gl, cpl>), grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
+                hipLaunchKernelGGL(k_eds<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
                                    bytes_slot(10));
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);

@@ -396,6 +396,8 @@ def bench_local(args):
     q = W.query(g, W.slots("c3")[slot], seed)
     gen_s = time.time() - t0
     ctx = A.Context(0)
+    if args.table_form:
+        ctx.set_table_form(args.table_form)
     ctx.upload(g)
 
     def step():
@@ -469,6 +471,8 @@ def bench_c2(args):
     queries = [(W.query(g, s, seed), seed, cnt) for s, seed, cnt in adm]
     gen_s = time.time() - t0
     ctx = A.Context(0)
+    if args.table_form:
+        ctx.set_table_form(args.table_form)
     ctx.upload(g)
 
     def step():
@@ -530,7 +534,7 @@ def main():
                          "autotuning, isolated step or CPU baseline), for profilers")
     ap.add_argument("--bag-workers", type=int, default=0, help="c4 only: bag-engine worker streams (default: library's)")
     ap.add_argument("--table-form", type=int, default=0, choices=(0, 1, 2),
-                    help="c4 only: relation-product form (0 per product, 1 LDS rows, 2 zero-fill + scatter)")
+                    help="relation-product form of the bag engine (0 per product, 1 LDS rows, 2 zero-fill + scatter)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration) and exit")
     args = ap.parse_args()
