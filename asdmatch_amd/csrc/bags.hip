@@ -386,9 +386,11 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
                                                  const uint32_t *__restrict__ vval, const uint64_t *__restrict__ vw,
                                                  const uint64_t *__restrict__ u, uint32_t nx, uint32_t ny,
                                                  uint64_t *__restrict__ G, const unsigned long long *__restrict__ terms,
-                                                 int form, unsigned long long *__restrict__ nbytes) {
+                                                 int form, unsigned long long *__restrict__ next,
+                                                 unsigned long long *__restrict__ nbytes) {
     if (ee_scatter_form(form, terms, (uint64_t)nx * ny)) return;  // the scatter form makes this table
     __shared__ __align__(16) unsigned long long acc[kEeCols];
+    __shared__ uint64_t s_it;
     for (uint32_t y = threadIdx.x; y < kEeCols; y += kEeThreads) acc[y] = 0;
     __syncthreads();
     const uint32_t nchunk = (ny + kEeCols - 1) / kEeCols;
@@ -396,7 +398,13 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
     const uint64_t xbase = xoff[0], vbase = voff[0];  // weights are indexed from each relation's first entry
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t bytes = 0;
-    for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    for (;;) {
+        // the next item from the launch's queue (zeroed before the launch): workgroups that start
+        // late, behind other streams' kernels, take what is left instead of a fixed share
+        if (threadIdx.x == 0) s_it = atomicAdd(next, 1ull);
+        __syncthreads();
+        const uint64_t it = s_it;
+        if (it >= items) break;
         const uint32_t x = (uint32_t)(it / nchunk), c = (uint32_t)(it % nchunk);
         const uint32_t y0 = c * kEeCols, n = min(ny - y0, kEeCols);
         const uint64_t lb = xoff[x], le = xoff[x + 1];
@@ -473,18 +481,23 @@ template <int SIDE>
 __global__ void __launch_bounds__(kThreads) k_eds(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                   const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
                                                   const uint64_t *__restrict__ F, uint32_t nx, uint32_t ny,
-                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
+                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ next,
+                                                  unsigned long long *__restrict__ nbytes) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nstrips = (ny + 63) / 64, nrb = (nx + kEdsRows - 1) / kEdsRows;
     const uint32_t nxcd = nstrips >= 8 && gridDim.x % 8 == 0 ? 8 : 1;
-    const uint32_t xcd = blockIdx.x % nxcd, lblk = blockIdx.x / nxcd, nblk = gridDim.x / nxcd;
+    const uint32_t xcd = blockIdx.x % nxcd;
     const uint64_t my_strips = xcd < nstrips ? (nstrips - xcd + nxcd - 1) / nxcd : 0;
-    const uint64_t units = my_strips * nrb;
-    constexpr uint32_t wpb = kThreads / 64;
+    const uint32_t units = (uint32_t)(my_strips * nrb);
     const uint64_t xbase = xoff[0];  // weights are indexed from the relation's first entry
     uint64_t bytes = 0;
-    for (uint64_t it = (uint64_t)lblk * wpb + wave; it < units; it += (uint64_t)nblk * wpb) {
+    for (;;) {
+        // the XCD's next unit from its queue (next[xcd], zeroed before the launch): waves that
+        // start late, behind other streams' kernels, take what is left instead of a fixed share
+        uint32_t got = 0;
+        if (lane == 0) got = (uint32_t)atomicAdd(next + xcd, 1ull);
+        const uint32_t it = __builtin_amdgcn_readfirstlane(got);
+        if (it >= units) break;
         const uint32_t strip = xcd + nxcd * (uint32_t)(it / nrb), rb = (uint32_t)(it % nrb);
         const uint32_t c = strip * 64 + lane;
         const bool ok = c < ny;
@@ -819,7 +832,8 @@ struct BagRun {
                                             // [6] relation and [7] vector algorithmic bytes, [14..17]
                                             // bytes the table kernels' loops touch (ee, de, ed, dd),
                                             // [18..19] those of the sparse joins (ee; ed and dd),
-                                            // [20] the term count of a relation product
+                                            // [20..21] a relation product's term count and item
+                                            // queue, [22..29] an edge x dense product's unit queues
     uint64_t *d_mat = nullptr;
     std::vector<void *> owned;              // per-term device buffers
     // Domains of merged blocks and relations between blocks depend only on the blocks' members,
@@ -1420,8 +1434,8 @@ struct BagRun {
             KTimer kt(ctx, kname);
             // count the terms, then both forms: each kernel returns at once unless the count picks
             // its form (the LDS form for many terms per cell, the scatter form for few)
-            unsigned long long *terms = d_acc + 20;
-            if ((rc = fill_u64(ctx, (uint64_t *)terms, 1, 0))) return rc;
+            unsigned long long *terms = d_acc + 20;  // [0] the term count, [1] the LDS form's item queue
+            if ((rc = fill_u64(ctx, (uint64_t *)terms, 2, 0))) return rc;
             const uint64_t cells = (uint64_t)nx * ny, resident = 8ull * ctx->num_cus;
             auto capped = [&](uint64_t items, uint64_t cap) {  // grid-stride kernels: at most cap workgroups
                 return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + kThreads - 1) / kThreads, cap)));
@@ -1432,11 +1446,11 @@ struct BagRun {
                                terms, ctx->table_form);
             hipLaunchKernelGGL(k_ee_scatter, capped(rx.nnz, resident), dim3(kThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off,
                                rv.val, rv.w, uv, nx, ny, G, terms, ctx->table_form, bytes_slot(8));
-            // the LDS form: a resident grid, eight workgroups per CU
+            // the LDS form: as many workgroups as fit the GPU at once (two per CU), fed by a queue
             const uint64_t items = (uint64_t)nx * ((ny + kEeCols - 1) / kEeCols);
-            const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, resident));
+            const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, 2ull * ctx->num_cus));
             hipLaunchKernelGGL(k_ee, dim3(grid), dim3(kEeThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off, rv.val,
-                               rv.w, uv, nx, ny, G, terms, ctx->table_form, bytes_slot(8));
+                               rv.w, uv, nx, ny, G, terms, ctx->table_form, terms + 1, bytes_slot(8));
             kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
         } else if (fxv.kind == 1 || fvy.kind == 1) {
             // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;
@@ -1454,12 +1468,14 @@ struct BagRun {
             const uint64_t units = (uint64_t)((nd + 63) / 64) * ((ne + kEdsRows - 1) / kEdsRows);
             const uint64_t want = (units + kThreads / 64 - 1) / (kThreads / 64);
             const dim3 grid((unsigned)(8 * std::max<uint64_t>(1, std::min<uint64_t>((want + 7) / 8, ctx->num_cus))));
+            unsigned long long *next = d_acc + 22;  // the eight XCDs' unit queues
+            if ((rc = fill_u64(ctx, (uint64_t *)next, 8, 0))) return rc;
             if (dense_x)
                 hipLaunchKernelGGL(k_eds<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
-                                   bytes_slot(9));
+                                   next, bytes_slot(9));
             else
                 hipLaunchKernelGGL(k_eds<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
-                                   bytes_slot(10));
+                                   next, bytes_slot(10));
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);
             out.rows = e;

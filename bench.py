@@ -174,6 +174,16 @@ def roofline_of(kstats, workload):
             "avg_launch_ms": round(avg_s * 1e3, 4), "launches": dom["launches"]}
 
 
+def device_aggregate(kstats, steps, wall_s):
+    """Algorithmic bytes of every timed kernel per step over the step's wall time: the whole device's
+    rate while several streams share it (the table's host entries and derived counters excluded)."""
+    b = sum(k["bytes"] for k in kstats
+            if k["ms"] > 0 and not k["name"].startswith("host") and not k["name"].endswith("_touched"))
+    gbs = b / steps / wall_s / 1e9
+    return {"alg_bytes_per_step": round(b / steps), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
 def kernel_table(kstats):
     return [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in ks.items()} for ks in kstats]
 
@@ -378,6 +388,7 @@ def bench_batch(args):
                          "include the sharing of the device; roofline_isolated is the same from an untimed step with "
                          "one bag worker",
         "roofline_isolated": ({**roofline_of(iso[0], "c4_isolated"), "step_s": round(iso[1], 3)} if iso else None),
+        "device_aggregate": device_aggregate(kstats, args.steps, (t5 - t4) / args.steps),
         "cpu_baseline": cpu, "kernels": kernel_table(kstats),
         "kernels_isolated": kernel_table(iso[0]) if iso else None, "setup_s": round(gen_s, 2)})
     print(json.dumps(out), flush=True)
