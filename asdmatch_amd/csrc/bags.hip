@@ -666,13 +666,14 @@ __global__ void k_join_ed(const uint64_t *__restrict__ roff, const uint32_t *__r
                           const uint64_t *__restrict__ rw, uint32_t nx, uint64_t nnz, const uint64_t *__restrict__ loff,
                           const uint32_t *__restrict__ lval, const uint64_t *__restrict__ lw, int own_x,
                           const uint64_t *__restrict__ F, uint64_t sa, uint64_t sj, const uint64_t *__restrict__ u,
-                          uint64_t *__restrict__ wout, unsigned long long *__restrict__ nbytes) {
+                          uint64_t *__restrict__ wout, unsigned long long *__restrict__ nbytes,
+                          unsigned long long *__restrict__ ngather) {
     constexpr int G = kJoinLanes;
     const int lg = threadIdx.x & (G - 1);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x / G;
     const uint64_t rbase = roff[0], lbase = loff[0];
     nnz = min(nnz, roff[nx] - rbase);  // nnz: the host's bound on the entries
-    uint64_t bytes = 0;
+    uint64_t bytes = 0, gathered = 0;
     for (uint64_t k = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; k < nnz; k += stride) {
         const uint64_t e = rbase + k;
         const uint32_t x = row_of(roff, nx, e), y = rval[e];
@@ -702,9 +703,11 @@ __global__ void k_join_ed(const uint64_t *__restrict__ roff, const uint32_t *__r
         if (lg == 0) {
             wout[k] = rw ? acc * rw[k] : acc;
             bytes += 32 + 28 * (l1 - l0);
+            gathered += l1 - l0;
         }
     }
     wave_add_bytes(bytes, nbytes);
+    wave_add_bytes(gathered, ngather);
 }
 
 // Both factors dense, A(x, j) = A[x * sax + j * saj] and B(j, y) = B[j * sbj + y * sby]: the
@@ -833,7 +836,8 @@ struct BagRun {
                                             // bytes the table kernels' loops touch (ee, de, ed, dd),
                                             // [18..19] those of the sparse joins (ee; ed and dd),
                                             // [20..21] a relation product's term count and item
-                                            // queue, [22..29] an edge x dense product's unit queues
+                                            // queue, [22..29] an edge x dense product's unit queues,
+                                            // [30] the table entries the ed joins gathered
     uint64_t *d_mat = nullptr;
     std::vector<void *> owned;              // per-term device buffers
     // Domains of merged blocks and relations between blocks depend only on the blocks' members,
@@ -1316,8 +1320,9 @@ struct BagRun {
             KTimer kt(ctx, "bag_join_ed");
             hipLaunchKernelGGL(k_join_ed, dim3(grid_for(rw_.nnz * kJoinLanes, kThreads)), dim3(kThreads), 0, ctx->stream,
                                rw_.off, rw_.val, rw_.w, nrows, rw_.nnz, rl.off, rl.val, rl.w, 0, td.p, sa, sj, uv,
-                               reversed ? wr : wf, bytes_slot(13));
-            // a lower bound: the gathered table entries are not counted (at most the whole table)
+                               reversed ? wr : wf, bytes_slot(13), d_acc + 30);
+            // the relations and lists once; the table entries it gathers (8 B each, one per list
+            // entry of each joined edge) are counted on the device and added when the query ends
             kt.stop(8ull * (nrows + 1) + 12 * rw_.nnz + 8ull * (dom[o].n + 1) + 12 * rl.nnz + 8ull * nv);
         } else {
             // A(x, j) and B(j, y) read with j contiguous: A with rows D(X) (x * nv + j) and B with rows
@@ -1594,6 +1599,7 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
                               "bag_table_dd_touched", "bag_join_ee_touched", "bag_join_ed_touched"};
     for (int k = 0; k < 6; ++k)
         if (res[14 + k]) add_kernel_bytes(ctx, touched[k], res[14 + k]);
+    if (res[30]) add_kernel_bytes(ctx, "bag_join_ed", 8 * res[30]);  // the table entries the ed joins gathered
     // factor results taken from the per-query cache instead of computed: entry "bag_cache_reuse",
     // launches = results reused, bytes = their sizes
     if (R.reuse_n) add_kernel_count(ctx, "bag_cache_reuse", R.reuse_n, R.reuse_bytes);
