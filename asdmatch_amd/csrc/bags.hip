@@ -470,19 +470,18 @@ __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ 
 // which keep it out of L2.  Measured on the config-4 batch (tables of ~7.5k x 7k), one item per
 // (strip, row) instead: 64 lanes x 1 column beat 32 x 1 by 23 %, and 16 x 4, 32 x 2, 32 x 4, 64 x 2
 // and 16 loads in flight were slower; this streamed form is 2 % faster than that one, with 32 rows
-// per unit (8, 16 and 64 slower).  The push form (each dense row read once, its entries added into
+// per unit (8, 16 and 64 slower) and 16 loads in flight (4, 8 and 12 slower).  The push form (each dense row read once, its entries added into
 // an LDS row over the relation's lists with 64-bit LDS atomics) was four times slower: ~450M random
 // LDS atomics per table.  SIDE only names the instantiation after the step that launches it (0:
 // edge (X, v) x dense (v, Y); 1: the dense (X, v) x edge (v, Y) step computed on its transposed
 // side), so that profilers report the two apart.
-constexpr int kEdInFlight = 8;
+constexpr int kEdInFlight = 16;
 constexpr uint32_t kEdsRows = 32;
 template <int SIDE>
 __global__ void __launch_bounds__(kThreads) k_eds(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                   const uint64_t *__restrict__ xw, const uint64_t *__restrict__ u,
                                                   const uint64_t *__restrict__ F, uint32_t nx, uint32_t ny,
-                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ next,
-                                                  unsigned long long *__restrict__ nbytes) {
+                                                  uint64_t *__restrict__ G, unsigned long long *__restrict__ nbytes) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t nstrips = (ny + 63) / 64, nrb = (nx + kEdsRows - 1) / kEdsRows;
     const uint32_t nxcd = nstrips >= 8 && gridDim.x % 8 == 0 ? 8 : 1;
@@ -491,13 +490,12 @@ __global__ void __launch_bounds__(kThreads) k_eds(const uint64_t *__restrict__ x
     const uint32_t units = (uint32_t)(my_strips * nrb);
     const uint64_t xbase = xoff[0];  // weights are indexed from the relation's first entry
     uint64_t bytes = 0;
-    for (;;) {
-        // the XCD's next unit from its queue (next[xcd], zeroed before the launch): waves that
-        // start late, behind other streams' kernels, take what is left instead of a fixed share
-        uint32_t got = 0;
-        if (lane == 0) got = (uint32_t)atomicAdd(next + xcd, 1ull);
-        const uint32_t it = __builtin_amdgcn_readfirstlane(got);
-        if (it >= units) break;
+    // each XCD's units are dealt to its waves in turn (a queue per XCD, one atomic per unit, was 3-7 %
+    // slower alone and no faster beside other streams)
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr uint32_t wpb = kThreads / 64;
+    const uint32_t lblk = blockIdx.x / nxcd, nblk = gridDim.x / nxcd;
+    for (uint32_t it = lblk * wpb + wave; it < units; it += nblk * wpb) {
         const uint32_t strip = xcd + nxcd * (uint32_t)(it / nrb), rb = (uint32_t)(it % nrb);
         const uint32_t c = strip * 64 + lane;
         const bool ok = c < ny;
@@ -836,8 +834,7 @@ struct BagRun {
                                             // bytes the table kernels' loops touch (ee, de, ed, dd),
                                             // [18..19] those of the sparse joins (ee; ed and dd),
                                             // [20..21] a relation product's term count and item
-                                            // queue, [22..29] an edge x dense product's unit queues,
-                                            // [30] the table entries the ed joins gathered
+                                            // queue, [30] the table entries the ed joins gathered
     uint64_t *d_mat = nullptr;
     std::vector<void *> owned;              // per-term device buffers
     // Domains of merged blocks and relations between blocks depend only on the blocks' members,
@@ -1473,14 +1470,12 @@ struct BagRun {
             const uint64_t units = (uint64_t)((nd + 63) / 64) * ((ne + kEdsRows - 1) / kEdsRows);
             const uint64_t want = (units + kThreads / 64 - 1) / (kThreads / 64);
             const dim3 grid((unsigned)(8 * std::max<uint64_t>(1, std::min<uint64_t>((want + 7) / 8, ctx->num_cus))));
-            unsigned long long *next = d_acc + 22;  // the eight XCDs' unit queues
-            if ((rc = fill_u64(ctx, (uint64_t *)next, 8, 0))) return rc;
             if (dense_x)
                 hipLaunchKernelGGL(k_eds<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
-                                   next, bytes_slot(9));
+                                   bytes_slot(9));
             else
                 hipLaunchKernelGGL(k_eds<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
-                                   next, bytes_slot(10));
+                                   bytes_slot(10));
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);
             out.rows = e;
