@@ -136,6 +136,8 @@ _sig("asdm_ctx_get_batch_tuning", C.c_int, _P, _u32p, C.POINTER(C.c_double), _u3
 _sig("asdm_ctx_set_bag_workers", C.c_int, _P, C.c_uint32)
 _sig("asdm_ctx_get_bag_workers", C.c_int, _P, _u32p)
 _sig("asdm_ctx_set_table_form", C.c_int, _P, C.c_int)
+_sig("asdm_dense_product", C.c_int, _P, _u64p, C.c_int, _u64p, _u64p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
+     _u64p, C.c_int)
 _sig("asdm_batch_autotune", C.c_int, _P, _P, C.POINTER(_P), C.c_uint32, C.c_uint64, _P, C.c_uint32, _u32p,
      C.POINTER(C.c_double), _u32p, _f32p)
 
@@ -414,6 +416,26 @@ class Context:
         n = C.c_uint32()
         self._check(_lib.asdm_intersect_k(self._h, ptrs, _ptr(sizes), k, _ptr(out), cap, C.byref(n)), "intersect")
         return out[: n.value]
+
+    def dense_product(self, A: np.ndarray, u, B: np.ndarray, method: int = 2) -> np.ndarray:
+        """G = A diag(u) B mod 2^64 (uint64; u None for ones) by the bag engine's dense x dense
+        kernels: method 1 the LDS-tiled 64-bit kernel, 2 the int8-digit matrix-core form.  A may be
+        given as a transposed view (A.T of an nv x nx array) and B likewise: the kernel reads them
+        in the layout they are stored in."""
+        nx, nv = A.shape
+        nv2, ny = B.shape
+        if nv2 != nv:
+            raise ValueError("inner dimensions differ")
+        a_kmajor = not A.flags.c_contiguous and A.T.flags.c_contiguous
+        b_kmajor = B.flags.c_contiguous or not B.T.flags.c_contiguous
+        Ah = np.ascontiguousarray(A.T if a_kmajor else A, dtype=np.uint64)
+        Bh = np.ascontiguousarray(B if b_kmajor else B.T, dtype=np.uint64)
+        uh = None if u is None else np.ascontiguousarray(u, dtype=np.uint64)
+        G = np.zeros((nx, ny), np.uint64)
+        self._check(_lib.asdm_dense_product(self._h, _ptr(Ah, C.c_uint64), int(a_kmajor),
+                                            None if uh is None else _ptr(uh, C.c_uint64), _ptr(Bh, C.c_uint64),
+                                            int(b_kmajor), nx, nv, ny, _ptr(G, C.c_uint64), method), "dense_product")
+        return G
 
     def intersect_noalloc(self, lists) -> np.ndarray:
         """intersect() through the allocation-free entry point (fixed per-GPU scratch; at most 64

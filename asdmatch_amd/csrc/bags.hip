@@ -602,6 +602,191 @@ __global__ void __launch_bounds__(256) k_dd(const uint64_t *__restrict__ A, cons
     wave_add_bytes(bytes, nbytes);
 }
 
+// ---- dense x dense on the matrix cores.  G = A diag(u) B mod 2^64 is computed from balanced base-256
+// digits: every u64 value v is written v = sum_i d_i 256^i (mod 2^64) with d_i in [-128, 127] (byte i of
+// v plus the carry from below, less 256 when that reaches 128), so
+//   G = sum_{i + j <= 7} 256^(i + j) (A_i B_j)  (mod 2^64)
+// with A_i, B_j int8 matrices of digits; pairs with i + j >= 8 vanish mod 2^64.  Each A_i B_j is an
+// exact int8 product on v_mfma_i32_32x32x32_i8: the partial sums of one shift s = i + j over
+// kDdFold values of k stay below 8 * 2^14 * 8192 = 2^30 in magnitude, so they are folded into the
+// u64 result (sign-extended, shifted by 8 s) every kDdFold values of k.
+// Digit layout, for both operands (rows of G's rows for A, of G's columns for B, K = nv padded to
+// kDdK): per row and per block of 16 values of k, 128 contiguous bytes, the 16 digits of plane 0, then
+// of plane 1, ... plane 7.  A lane's operands for all eight planes of one MFMA step are then one
+// 128-byte line.
+constexpr uint32_t kDdK = 64, kDdFold = 8192;
+
+// The balanced digits of v, digit i in byte i (two's complement).
+__device__ inline uint64_t balanced_digits(uint64_t v) {
+    uint64_t out = 0;
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t t = (uint32_t)((v >> (8 * i)) & 255) + carry;  // 0..256
+        carry = t >= 128;
+        out |= (uint64_t)(t & 255) << (8 * i);  // t, or t - 256 as an int8
+    }
+    return out;
+}
+
+// Writes the 128-byte digit block of 16 packed values p[0..15] to dst.
+__device__ inline void store_digit_block(const uint64_t (&p)[16], int8_t *__restrict__ dst) {
+    u64x2 *o = (u64x2 *)dst;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            lo |= ((p[q] >> (8 * d)) & 255) << (8 * q);
+            hi |= ((p[q + 8] >> (8 * d)) & 255) << (8 * q);
+        }
+        u64x2 v;
+        v.x = lo;
+        v.y = hi;
+        o[d] = v;
+    }
+}
+
+// Digits of M (rows x K, row-major, k contiguous) times u[k] (null: 1), for rows_p x Kp (zeros in
+// the padding): one thread per (row, block of 16 k).
+__global__ void k_dd_digits_rows(const uint64_t *__restrict__ M, const uint64_t *__restrict__ u, uint32_t rows,
+                                 uint32_t K, uint32_t rows_p, uint32_t Kp, int8_t *__restrict__ dst) {
+    const uint32_t kbs = Kp / 16;
+    const uint64_t n = (uint64_t)rows_p * kbs;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(t / kbs), kb = (uint32_t)(t % kbs);
+        uint64_t p[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t k = kb * 16 + q;
+            const uint64_t v = r < rows && k < K ? M[(uint64_t)r * K + k] * (u ? u[k] : 1ull) : 0;
+            p[q] = balanced_digits(v);
+        }
+        store_digit_block(p, dst + t * 128);
+    }
+}
+
+// The same for M stored k-major (K x rows, row-major: M[k * rows + r]): 64 x 64 tiles transposed in LDS.
+__global__ void __launch_bounds__(256) k_dd_digits_cols(const uint64_t *__restrict__ M, const uint64_t *__restrict__ u,
+                                                        uint32_t rows, uint32_t K, uint32_t rows_p, uint32_t Kp,
+                                                        int8_t *__restrict__ dst) {
+    __shared__ uint64_t tile[64][65];
+    const uint32_t r0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    {
+        const uint32_t rr = threadIdx.x & 63;
+        for (uint32_t kk = threadIdx.x >> 6; kk < 64; kk += 4) {
+            const uint32_t r = r0 + rr, k = k0 + kk;
+            const uint64_t v = r < rows && k < K ? M[(uint64_t)k * rows + r] * (u ? u[k] : 1ull) : 0;
+            tile[kk][rr] = balanced_digits(v);
+        }
+    }
+    __syncthreads();
+    const uint32_t rr = threadIdx.x >> 2, part = threadIdx.x & 3;
+    uint64_t p[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p[q] = tile[part * 16 + q][rr];
+    const uint32_t kbs = Kp / 16;
+    store_digit_block(p, dst + ((uint64_t)(r0 + rr) * kbs + k0 / 16 + part) * 128);
+}
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// G (nx x ny) from the digit blocks of A (rows of G) and B (columns of G): a workgroup of four
+// waves per 64 x 64 tile, one 32 x 32 block per wave.  Lane l holds row l % 32 of its A block and
+// column l % 32 of its B block, and k block 2 s + l / 32 of each 32-wide step s; the result D[i][j]
+// of the 32 x 32 block is in lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8).
+__global__ void __launch_bounds__(256) k_dd_mfma(const int8_t *__restrict__ Ad, const int8_t *__restrict__ Bd,
+                                                 uint32_t nx, uint32_t ny, uint32_t Kp, uint64_t *__restrict__ G,
+                                                 unsigned long long *__restrict__ nbytes) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t m0 = blockIdx.y * 64 + (w >> 1) * 32, n0 = blockIdx.x * 64 + (w & 1) * 32;
+    const uint32_t kbs = Kp / 16;
+    const int8_t *arow = Ad + ((uint64_t)(m0 + (lane & 31)) * kbs + (lane >> 5)) * 128;
+    const int8_t *brow = Bd + ((uint64_t)(n0 + (lane & 31)) * kbs + (lane >> 5)) * 128;
+    uint64_t res[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) res[r] = 0;
+    for (uint32_t kc = 0; kc < Kp; kc += kDdFold) {
+        v16i acc[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[s][r] = 0;
+        const uint32_t ke = min(Kp, kc + kDdFold);
+        // the next step's operands are loaded while this step's products run
+        v4i a[8], b[8];
+        {
+            const v4i *ap = (const v4i *)(arow + (uint64_t)(kc / 16) * 128);
+            const v4i *bp = (const v4i *)(brow + (uint64_t)(kc / 16) * 128);
+#pragma unroll
+            for (int d = 0; d < 8; ++d) a[d] = ap[d];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) b[d] = bp[d];
+        }
+        for (uint32_t k = kc; k < ke; k += 32) {
+            v4i an[8], bn[8];
+            const uint32_t kn = k + 32 < ke ? k + 32 : k;  // the last step reloads its own operands
+            const v4i *ap = (const v4i *)(arow + (uint64_t)(kn / 16) * 128);
+            const v4i *bp = (const v4i *)(brow + (uint64_t)(kn / 16) * 128);
+#pragma unroll
+            for (int d = 0; d < 8; ++d) an[d] = ap[d];
+#pragma unroll
+            for (int d = 0; d < 8; ++d) bn[d] = bp[d];
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+#pragma unroll
+                for (int i = 0; i <= s; ++i) acc[s] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[s - i], acc[s], 0, 0, 0);
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                a[d] = an[d];
+                b[d] = bn[d];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) res[r] += (uint64_t)(int64_t)acc[s][r] << (8 * s);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const uint32_t m = m0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4), n = n0 + (lane & 31);
+        if (m < nx && n < ny) G[(uint64_t)m * ny + n] = res[r];
+    }
+    if (threadIdx.x == 0) atomicAdd(nbytes, (unsigned long long)(16ull * 64 * Kp + 8ull * 64 * 64));
+}
+
+// G = A diag(u) B (mod 2^64) on the matrix cores.  A (nx x nv) is stored row-major with rows D(X)
+// (a_kmajor false) or as its transpose (nv x nx, a_kmajor true); B (nv x ny) row-major (b_kmajor true)
+// or as its transpose (ny x nv); u may be null (all ones).  The digit blocks are temporaries.
+int dense_product_mfma(asdm_ctx *ctx, const uint64_t *A, bool a_kmajor, const uint64_t *u, const uint64_t *B,
+                       bool b_kmajor, uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *G, unsigned long long *nbytes) {
+    auto up = [](uint64_t x, uint64_t m) { return (std::max<uint64_t>(x, 1) + m - 1) / m * m; };
+    const uint64_t Kp = up(nv, kDdK), nxp = up(nx, 64), nyp = up(ny, 64);
+    if (Kp > 0xffffffffull || nxp / 64 > 65535) return ctx->fail(ASDM_E_UNSUPPORTED, "dense product beyond the tile grid");
+    int8_t *Ad = nullptr, *Bd = nullptr;
+    int rc = dev_alloc(ctx, (void **)&Ad, nxp * Kp * 8);
+    if (!rc) rc = dev_alloc(ctx, (void **)&Bd, nyp * Kp * 8);
+    if (!rc) {
+        auto digits = [&](const uint64_t *M, bool kmajor, const uint64_t *scale, uint32_t rows, uint64_t rows_p, int8_t *dst) {
+            if (kmajor)
+                hipLaunchKernelGGL(k_dd_digits_cols, dim3((unsigned)(rows_p / 64), (unsigned)(Kp / 64)), dim3(256), 0,
+                                   ctx->stream, M, scale, rows, nv, (uint32_t)rows_p, (uint32_t)Kp, dst);
+            else
+                hipLaunchKernelGGL(k_dd_digits_rows, dim3(grid_for(rows_p * (Kp / 16), 256)), dim3(256), 0, ctx->stream, M,
+                                   scale, rows, nv, (uint32_t)rows_p, (uint32_t)Kp, dst);
+        };
+        digits(A, a_kmajor, u, nx, nxp, Ad);
+        digits(B, b_kmajor, nullptr, ny, nyp, Bd);
+        hipLaunchKernelGGL(k_dd_mfma, dim3((unsigned)(nyp / 64), (unsigned)(nxp / 64)), dim3(256), 0, ctx->stream, Ad, Bd, nx,
+                           ny, (uint32_t)Kp, G, nbytes);
+        rc = check_hip(ctx, hipGetLastError(), "dense product");
+    }
+    if (Ad) dev_free(ctx, Ad);
+    if (Bd) dev_free(ctx, Bd);
+    return rc;
+}
+
 // ---- sparse joins: the product of v's two factors evaluated only on the edges of the relation
 // R(X->Y) that already joins X and Y, times each edge's own weight (1 where the relation has
 // none), written to wout.  The work items are the relation's edges, kJoinLanes lanes per edge
@@ -1479,6 +1664,13 @@ struct BagRun {
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);
             out.rows = e;
+        } else if (nx >= 32 && ny >= 32 && nv >= 64) {
+            // on the matrix cores, each operand read in the orientation it is stored in
+            KTimer kt(ctx, kname);
+            if ((rc = dense_product_mfma(ctx, fxv.t.p, fxv.t.rows != X, uv, fvy.t.p, fvy.t.rows == v, nx, nv, ny, G,
+                                         bytes_slot(11))))
+                return rc;
+            kt.stop(8ull * nx * nv + 8ull * nv * ny + 8ull * nv + out_bytes);
         } else {
             uint64_t *Fa = nullptr, *Fb = nullptr;
             bool ta = false, tb = false;
@@ -1609,3 +1801,44 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
 }
 
 }  // namespace asdm
+
+using namespace asdm;
+
+extern "C" {
+
+int asdm_dense_product(asdm_ctx *ctx, const uint64_t *A, int a_kmajor, const uint64_t *u, const uint64_t *B, int b_kmajor,
+                       uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *G, int method) try {
+    if (!ctx || !A || !B || !G || (method != 1 && method != 2)) return ASDM_E_INVALID;
+    if (method == 1 && (a_kmajor || !b_kmajor)) return ctx->fail(ASDM_E_INVALID, "the LDS-tile kernel takes A by rows, B k-major");
+    (void)hipSetDevice(ctx->device);
+    const uint64_t na = (uint64_t)nx * nv, nb = (uint64_t)nv * ny, ng = (uint64_t)nx * ny;
+    uint64_t *dA = nullptr, *dB = nullptr, *du = nullptr, *dG = nullptr;
+    int rc = dev_alloc(ctx, (void **)&dA, 8 * std::max<uint64_t>(na, 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&dB, 8 * std::max<uint64_t>(nb, 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&du, 8 * std::max<uint64_t>(nv, 1));
+    if (!rc) rc = dev_alloc(ctx, (void **)&dG, 8 * std::max<uint64_t>(ng, 1));
+    if (!rc && na) rc = check_hip(ctx, hipMemcpyAsync(dA, A, 8 * na, hipMemcpyHostToDevice, ctx->stream), "copy");
+    if (!rc && nb) rc = check_hip(ctx, hipMemcpyAsync(dB, B, 8 * nb, hipMemcpyHostToDevice, ctx->stream), "copy");
+    if (!rc && nv) {
+        if (u) rc = check_hip(ctx, hipMemcpyAsync(du, u, 8 * nv, hipMemcpyHostToDevice, ctx->stream), "copy");
+        else rc = fill_u64(ctx, du, nv, 1);
+    }
+    unsigned long long *nbytes = (unsigned long long *)ctx->d_counters + 1;
+    if (!rc && ng) {
+        if (method == 2) {
+            rc = dense_product_mfma(ctx, dA, a_kmajor != 0, du, dB, b_kmajor != 0, nx, nv, ny, dG, nbytes);
+        } else {
+            const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
+            hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, dA, du, dB, nx, nv, ny, dG, nbytes);
+            rc = check_hip(ctx, hipGetLastError(), "dense product");
+        }
+    }
+    if (!rc && ng) rc = check_hip(ctx, hipMemcpyAsync(G, dG, 8 * ng, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    (void)hipStreamSynchronize(ctx->stream);
+    for (uint64_t *p : {dA, dB, du, dG})
+        if (p) dev_free(ctx, p);
+    return rc;
+} ASDM_CTX_CATCH(ctx)
+
+}  // extern "C"

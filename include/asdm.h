@@ -34,6 +34,7 @@
  *   asdm_ctx_{set,get}_batch_tuning
  *   asdm_ctx_{set,get}_bag_workers (no counterpart: concurrent bag-engine queries of a batch)
  *   asdm_ctx_set_table_form     (no counterpart: form of the bag engine's relation products)
+ *   asdm_dense_product          (no counterpart: the bag engine's dense x dense product, for tests)
  *   asdm_plain_estimate         (no counterpart: this planner's own independence-model estimate,
  *                               the learned estimator's fallback)
  */
@@ -361,6 +362,13 @@ int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t
  * scratch, so calls that overlap on one GPU (from different host threads or contexts) race. */
 int asdm_intersect_k_noalloc(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t *sizes, uint32_t k,
                              uint32_t *out, uint32_t cap, uint32_t *n);
+/* The bag engine's dense x dense table product G = A diag(u) B (mod 2^64) on host arrays (no
+ * counterpart in the reference; for tests): A is nx x nv row-major, or its transpose (nv x nx) when
+ * a_kmajor; B is nv x ny row-major when b_kmajor, else its transpose (ny x nv); u (nv) may be NULL
+ * (all ones); G is nx x ny row-major.  method 1: the LDS-tiled 64-bit kernel (a_kmajor 0, b_kmajor 1
+ * only); 2: the int8-digit matrix-core form. */
+int asdm_dense_product(asdm_ctx *ctx, const uint64_t *A, int a_kmajor, const uint64_t *u, const uint64_t *B,
+                       int b_kmajor, uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *G, int method);
 
 #ifdef __cplusplus
 }

@@ -299,3 +299,45 @@ def test_relation_product_forms_agree(ctx):
                 ctx.set_table_form(bad)
     finally:
         ctx.set_table_form(0)
+
+
+def _dense_ref(A, u, B):
+    """A diag(u) B mod 2^64 in numpy (uint64 products and sums wrap like the kernels')."""
+    Au = A * (np.ones(A.shape[1], np.uint64) if u is None else u)[None, :]
+    G = np.zeros((A.shape[0], B.shape[1]), np.uint64)
+    for k in range(A.shape[1]):
+        G += np.outer(Au[:, k], B[k])
+    return G
+
+
+def test_dense_product_forms_match_numpy(ctx):
+    """The dense x dense table product, G = A diag(u) B mod 2^64: the LDS-tiled 64-bit kernel and the
+    int8-digit matrix-core form against numpy's wrapping uint64 arithmetic, on shapes that are not
+    multiples of the tiles, full-range values (every balanced digit, carries through all eight
+    bytes), all four storage orientations of the matrix-core form, and an inner size past one
+    int32 accumulation span (8,192)."""
+    rng = np.random.default_rng(17)
+    full = lambda *s: rng.integers(0, 2**64, size=s, dtype=np.uint64)
+    cases = [(1, 1, 1), (31, 63, 33), (64, 64, 64), (70, 200, 97), (129, 65, 130), (40, 300, 50)]
+    for nx, nv, ny in cases:
+        A, B, u = full(nx, nv), full(nv, ny), full(nv)
+        want = _dense_ref(A, u, B)
+        assert np.array_equal(ctx.dense_product(A, u, B, method=1), want), (nx, nv, ny)
+        assert np.array_equal(ctx.dense_product(A, u, B, method=2), want), (nx, nv, ny)
+        At = np.ascontiguousarray(A.T).T  # stored nv x nx
+        Bt = np.ascontiguousarray(B.T).T  # stored ny x nv
+        for a, b in ((At, B), (A, Bt), (At, Bt)):
+            assert np.array_equal(ctx.dense_product(a, u, b, method=2), want), (nx, nv, ny)
+        say(f"dense product {nx}x{nv}x{ny}: both forms equal numpy")
+    # digits at the extremes: all-ones bytes (0xff.. -> -1 digits with carries), 0x80 bytes (the
+    # carry edge), and u = None
+    A = np.full((33, 100), 0xFFFFFFFFFFFFFFFF, np.uint64)
+    A[::2] = 0x8080808080808080
+    B = np.full((100, 35), 0x7F7F7F7F7F7F7F7F, np.uint64)
+    B[:, ::3] = 0x8000000000000001
+    assert np.array_equal(ctx.dense_product(A, None, B, method=2), _dense_ref(A, None, B))
+    # an inner size beyond one int32 accumulation span
+    A, B, u = full(40, 9000), full(9000, 36), full(9000)
+    want = _dense_ref(A, u, B)
+    assert np.array_equal(ctx.dense_product(A, u, B, method=2), want)
+    assert np.array_equal(ctx.dense_product(A, u, B, method=1), want)
