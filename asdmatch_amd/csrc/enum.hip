@@ -17,6 +17,9 @@
 // data ids of earlier depths with the same label (other labels cannot collide).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <thread>
+
 #include <algorithm>
 #include <chrono>
 #include <functional>
@@ -2803,22 +2806,49 @@ int asdm_batch_count_learned(asdm_ctx *ctx, const asdm_graph *g, const asdm_grap
         // The learned estimates are not: a query whose learned row estimate makes the frontier
         // cheaper than its bag plan runs on the frontier instead, under a row cap, and falls back
         // to the bag plan if it reaches the cap.
-        std::string err;
-        for (size_t k = 0; k < bplans.size() && !rc; ++k) {
-            asdm_plan plain;
-            rc = build_plan(qs[bidx[k]], bcs[k], nullptr, 0, [](uint64_t, double fb) { return fb; }, &plain, err);
-            if (!rc) rc = attach_bags(bcs[k], 0, false, &plain, err);
-            if (rc || !plain.bags) continue;
-            const double rows = bplans[k]->est_total;
-            if (rows <= kLearnedFrontierRows && rows * kFrontierSecondsPerRow < plain.bags->cost * kBagSecondsPerCost)
-                fallback[k] = plain.bags;
-            else
-                bplans[k]->bags = plain.bags;
-        }
+        // host-only planning, one query per task over a few host threads; the first failing query
+        // in batch order fails the call, as a serial loop would
+        const size_t nb = bplans.size();
+        std::vector<int> prc(nb, ASDM_OK);
+        std::vector<std::string> perr(nb);
+        std::atomic<size_t> next_k{0};
+        auto choose = [&]() {
+            for (size_t k; (k = next_k.fetch_add(1)) < nb;) {
+                try {  // an exception must not leave a helper thread
+                    asdm_plan plain;
+                    int r = build_plan(qs[bidx[k]], bcs[k], nullptr, 0, [](uint64_t, double fb) { return fb; }, &plain,
+                                       perr[k]);
+                    if (!r) r = attach_bags(bcs[k], 0, false, &plain, perr[k]);
+                    prc[k] = r;
+                    if (r || !plain.bags) continue;
+                    const double rows = bplans[k]->est_total;
+                    if (rows <= kLearnedFrontierRows && rows * kFrontierSecondsPerRow < plain.bags->cost * kBagSecondsPerCost)
+                        fallback[k] = plain.bags;
+                    else
+                        bplans[k]->bags = plain.bags;
+                } catch (const std::bad_alloc &) {
+                    prc[k] = ASDM_E_OOM;
+                    perr[k] = "host allocation failed";
+                } catch (...) {
+                    prc[k] = ASDM_E_INVALID;
+                    perr[k] = "unexpected host exception in planning";
+                }
+            }
+        };
+        const size_t nthreads = std::min<size_t>({(size_t)8, std::max(1u, std::thread::hardware_concurrency()), (nb + 15) / 16});
+        std::vector<std::thread> helpers;
+        for (size_t t = 1; t < nthreads; ++t) helpers.emplace_back(choose);
+        choose();
+        for (auto &h : helpers) h.join();
+        for (size_t k = 0; k < nb; ++k)
+            if (prc[k]) {
+                rc = prc[k];
+                ctx->err = perr[k];
+                break;
+            }
         uint64_t nmoved = 0;
         for (auto &f : fallback) nmoved += f != nullptr;
         add_kernel_count(ctx, "learned: bag queries moved to the frontier", nmoved, 0);
-        if (rc) ctx->err = err;
         host_phase("host: engine choice");
     } else {
         host_phase("host: plans");
