@@ -208,7 +208,8 @@ static hipEvent_t take_event(asdm_ctx *ctx) {
         e = ctx->event_pool.back();
         ctx->event_pool.pop_back();
     } else {
-        (void)hipEventCreate(&e);
+        // timing only: no system-scope fence (cache writeback and invalidation) when it is recorded
+        (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     }
     return e;
 }
