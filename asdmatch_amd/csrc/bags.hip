@@ -1751,9 +1751,9 @@ int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *
         rc = dev_alloc(ctx, (void **)&R.d_mat, 8ull * std::max<uint64_t>(matn, 1));
         if (!rc) rc = fill_u64(ctx, R.d_mat, matn, 0);
     }
-    hipEvent_t ea, eb;
-    (void)hipEventCreate(&ea);
-    (void)hipEventCreate(&eb);
+    hipEvent_t ea, eb;  // the call's device time: timing only, no system-scope fence on record
+    (void)hipEventCreateWithFlags(&ea, hipEventDisableSystemFence);
+    (void)hipEventCreateWithFlags(&eb, hipEventDisableSystemFence);
     (void)hipEventRecord(ea, ctx->stream);
     if (!rc) rc = R.prebuild();
     for (size_t t = 0; t < B.terms.size() && !rc; ++t) {

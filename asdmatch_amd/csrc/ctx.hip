@@ -295,8 +295,9 @@ void release_device_graph(asdm_graph &g) {
 }
 
 // ---------------------------------------------------------------- intersection primitive
-// One wave per call: the shortest list is the probe; each lane tests probe elements by binary
-// search in the other lists; survivors are compacted in order with a ballot prefix.
+// The allocation-free entry point's kernel, one wave per call: the shortest list is the probe;
+// each lane tests probe elements by binary search in the other lists; survivors are compacted in
+// order with a ballot prefix.
 __global__ void k_intersect_k(const uint32_t *__restrict__ flat, const uint64_t *__restrict__ offs, uint32_t k,
                               uint32_t *__restrict__ out, uint32_t *__restrict__ nout) {
     const int lane = threadIdx.x & 63;
@@ -324,6 +325,73 @@ __global__ void k_intersect_k(const uint32_t *__restrict__ flat, const uint64_t 
         run += __popcll(bal);
     }
     if (lane == 0) *nout = run;
+}
+
+// Pairwise intersection R n L of two ascending unique lists, as the k-way call's steps (the
+// shortest list first, then each other list from the shortest up), in two passes over tiles of
+// kIsectTile: pass 0 counts each tile's matches into cnt[tile], pass 1 writes them, in order, from
+// off[tile] (the exclusive scan of the counts).
+//  - Balanced pairs (|L| <= kIsectBalanced |R|): merge path.  Tile t holds merge positions
+//    [t T, (t + 1) T) of R and L merged (R first on ties); both ends are found by a binary search
+//    along their diagonals, the tile's part of L (plus the next element, where R's last element can
+//    meet its equal) is staged in LDS, and each element of the tile's part of R is searched there.
+//  - Otherwise: tile t holds R's elements [t T, (t + 1) T), each searched in L in global memory.
+constexpr uint32_t kIsectTile = 2048, kIsectBalanced = 16;
+
+// Elements of R among the first d of the merge of R (nr) and L (nl), R first on ties.
+__device__ inline uint32_t merge_split(const uint32_t *__restrict__ R, uint32_t nr, const uint32_t *__restrict__ L,
+                                       uint32_t nl, uint64_t d) {
+    uint32_t lo = d > nl ? (uint32_t)(d - nl) : 0, hi = (uint32_t)min<uint64_t>(d, nr);
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (R[mid] <= L[d - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool MERGE>
+__global__ void __launch_bounds__(256) k_isect_pair(const uint32_t *__restrict__ R, uint32_t nr,
+                                                    const uint32_t *__restrict__ L, uint32_t nl, int pass,
+                                                    uint32_t *__restrict__ cnt, const uint64_t *__restrict__ off,
+                                                    uint32_t *__restrict__ out) {
+    __shared__ uint32_t seg[kIsectTile + 1];
+    __shared__ uint64_t scan_lds[256 / 64 + 1];
+    uint32_t r0, r1;
+    const uint32_t *lseg;
+    uint32_t ln;
+    if (MERGE) {
+        const uint64_t d0 = (uint64_t)blockIdx.x * kIsectTile, d1 = min<uint64_t>((uint64_t)nr + nl, d0 + kIsectTile);
+        r0 = merge_split(R, nr, L, nl, d0);
+        r1 = merge_split(R, nr, L, nl, d1);
+        const uint32_t l0 = (uint32_t)(d0 - r0), l1 = min(nl, (uint32_t)(d1 - r1) + 1);
+        ln = l1 - l0;  // at most kIsectTile + 1
+        for (uint32_t i = threadIdx.x; i < ln; i += 256) seg[i] = L[l0 + i];
+        __syncthreads();
+        lseg = seg;
+    } else {
+        r0 = blockIdx.x * kIsectTile;
+        r1 = min(nr, r0 + kIsectTile);
+        lseg = L;
+        ln = nl;
+    }
+    uint64_t base = pass ? off[blockIdx.x] : 0, total = 0;
+    for (uint32_t i0 = r0; i0 < r1; i0 += 256) {  // rounds of 256 consecutive elements keep the order
+        const uint32_t i = i0 + threadIdx.x;
+        bool hit = false;
+        uint32_t x = 0;
+        if (i < r1) {
+            x = R[i];
+            const uint32_t q = lower_bound_u32(lseg, ln, x);
+            hit = q < ln && lseg[q] == x;
+        }
+        uint64_t round = 0;
+        const uint64_t ex = block_exclusive_scan<256>(hit ? 1 : 0, scan_lds, round);
+        if (pass && hit) out[base + ex] = x;
+        base += round;
+        total += round;
+    }
+    if (!pass && threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)total;
 }
 
 // ---------------------------------------------------------------- allocation-free scratch
@@ -527,34 +595,65 @@ int asdm_intersect_k(asdm_ctx *ctx, const uint32_t *const *lists, const uint32_t
     for (uint32_t j = 0; j < k; ++j) offs[j + 1] = offs[j] + sizes[j];
     uint64_t minsz = UINT64_MAX;
     for (uint32_t j = 0; j < k; ++j) minsz = std::min<uint64_t>(minsz, sizes[j]);
-    uint32_t *dflat = nullptr, *dout = nullptr, *dn = nullptr;
-    uint64_t *doffs = nullptr;
+    uint32_t *dflat = nullptr, *dout = nullptr;
     int rc = dev_alloc(ctx, (void **)&dflat, sizeof(uint32_t) * std::max<uint64_t>(offs[k], 1));
-    if (!rc) rc = dev_alloc(ctx, (void **)&doffs, sizeof(uint64_t) * (k + 1ull));
     if (!rc) rc = dev_alloc(ctx, (void **)&dout, sizeof(uint32_t) * std::max<uint64_t>(minsz, 1));
-    if (!rc) rc = dev_alloc(ctx, (void **)&dn, sizeof(uint32_t));
     for (uint32_t j = 0; j < k && !rc; ++j)
         if (sizes[j])
             rc = check_hip(ctx, hipMemcpyAsync(dflat + offs[j], lists[j], sizeof(uint32_t) * sizes[j], hipMemcpyHostToDevice,
                                                ctx->stream), "copy");
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(doffs, offs.data(), sizeof(uint64_t) * (k + 1ull), hipMemcpyHostToDevice, ctx->stream), "copy");
-    if (!rc) {
-        hipLaunchKernelGGL(k_intersect_k, dim3(1), dim3(64), 0, ctx->stream, dflat, doffs, k, dout, dn);
-        rc = check_hip(ctx, hipGetLastError(), "intersect launch");
+    // the shortest list first, then the others from the shortest up, one pairwise step each; the
+    // running result alternates between two buffers of the shortest list's size
+    std::vector<uint32_t> order(k);
+    for (uint32_t j = 0; j < k; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return sizes[a] < sizes[b]; });
+    uint32_t *buf[2] = {dout, nullptr};
+    uint32_t *dcnt = nullptr;
+    uint64_t *doff = nullptr;
+    const uint64_t max_tiles = (minsz + offs[k]) / kIsectTile + 2;
+    if (!rc && k > 1) rc = dev_alloc(ctx, (void **)&buf[1], sizeof(uint32_t) * std::max<uint64_t>(minsz, 1));
+    if (!rc && k > 1) rc = dev_alloc(ctx, (void **)&dcnt, sizeof(uint32_t) * max_tiles);
+    if (!rc && k > 1) rc = dev_alloc(ctx, (void **)&doff, sizeof(uint64_t) * (max_tiles + 1));
+    uint64_t hn = minsz;
+    const uint32_t *cur = dflat + offs[order[0]];
+    int which = 0;
+    for (uint32_t s = 1; s < k && !rc && hn; ++s) {
+        const uint32_t j = order[s];
+        const uint32_t *L = dflat + offs[j];
+        const uint32_t nl = sizes[j], nr = (uint32_t)hn;
+        const bool merge = (uint64_t)nl <= (uint64_t)kIsectBalanced * nr;
+        const uint64_t tiles = merge ? ((uint64_t)nr + nl + kIsectTile - 1) / kIsectTile : (nr + kIsectTile - 1) / kIsectTile;
+        uint32_t *dst = buf[which];
+        for (int pass = 0; pass < 2 && !rc; ++pass) {
+            if (merge)
+                hipLaunchKernelGGL(k_isect_pair<true>, dim3((unsigned)std::max<uint64_t>(tiles, 1)), dim3(256), 0, ctx->stream,
+                                   cur, nr, L, nl, pass, dcnt, doff, dst);
+            else
+                hipLaunchKernelGGL(k_isect_pair<false>, dim3((unsigned)std::max<uint64_t>(tiles, 1)), dim3(256), 0, ctx->stream,
+                                   cur, nr, L, nl, pass, dcnt, doff, dst);
+            rc = check_hip(ctx, hipGetLastError(), "intersect launch");
+            if (!rc && pass == 0) rc = scan_u32_to_u64(ctx, dcnt, doff, std::max<uint64_t>(tiles, 1));
+        }
+        if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&hn, doff + std::max<uint64_t>(tiles, 1), sizeof(uint64_t),
+                                                    hipMemcpyDeviceToHost, ctx->stream), "copy");
+        if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        cur = dst;
+        which ^= 1;
     }
-    uint32_t hn = 0;
-    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(&hn, dn, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    if (!rc && k == 1 && hn)  // one list: itself
+        rc = check_hip(ctx, hipMemcpyAsync(dout, cur, sizeof(uint32_t) * hn, hipMemcpyDeviceToDevice, ctx->stream), "copy");
     if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (!rc) {
-        *n = hn;
+        *n = (uint32_t)hn;
         if (out && hn > cap) rc = ctx->fail(ASDM_E_INVALID, "output capacity too small");
         else if (out && hn)
-            rc = check_hip(ctx, hipMemcpy(out, dout, sizeof(uint32_t) * hn, hipMemcpyDeviceToHost), "copy");
+            rc = check_hip(ctx, hipMemcpy(out, k == 1 ? dout : cur, sizeof(uint32_t) * hn, hipMemcpyDeviceToHost), "copy");
     }
     dev_free(ctx, dflat);
-    dev_free(ctx, doffs);
     dev_free(ctx, dout);
-    dev_free(ctx, dn);
+    dev_free(ctx, buf[1]);
+    dev_free(ctx, dcnt);
+    dev_free(ctx, doff);
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
 } ASDM_CTX_CATCH(ctx)

@@ -38,6 +38,30 @@ def test_intersect_k_matches_numpy(ctx):
             assert np.array_equal(got, want), (ratio, k)
 
 
+def test_intersect_k_merge_path_tiles(ctx):
+    """Long balanced lists, intersected by merge path over many 2,048-position tiles: dense overlaps
+    (so equal elements straddle tile boundaries), size ratios on both sides of the merge-path
+    cutoff (16), every list order, an empty list, a single list, and identical lists."""
+    rng = np.random.default_rng(7)
+    def rand_list(n, hi):
+        return np.unique(rng.integers(0, hi, size=n, dtype=np.uint64)).astype(np.uint32)
+    a, b = rand_list(200_000, 400_000), rand_list(200_000, 400_000)
+    cases = [
+        [a, b], [b, a], [a, a], [a, b, rand_list(150_000, 400_000)],
+        [rand_list(10_000, 1_000_000), rand_list(160_000, 1_000_000)],  # ratio just under 16
+        [rand_list(10_000, 1_000_000), rand_list(170_000, 1_000_000)],  # just over
+        [np.arange(0, 300_000, 2, dtype=np.uint32), np.arange(0, 300_000, 3, dtype=np.uint32)],
+        [a, np.zeros(0, np.uint32)], [a],
+    ]
+    for lists in cases:
+        want = lists[0]
+        for l in lists[1:]:
+            want = np.intersect1d(want, l)
+        got = ctx.intersect(lists)
+        assert np.array_equal(got, want), [len(l) for l in lists]
+        assert np.array_equal(ctx.intersect_noalloc(lists), want) if sum(len(l) for l in lists) <= 1 << 21 else True
+
+
 def test_filter_matches_reference_on_yeast(ctx, yeast):
     gold = T.yeast_golden()["queries"]
     n = 0
