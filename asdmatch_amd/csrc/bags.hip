@@ -763,7 +763,7 @@ int dense_product_mfma(asdm_ctx *ctx, const uint64_t *A, bool a_kmajor, const ui
                        bool b_kmajor, uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *G, unsigned long long *nbytes) {
     auto up = [](uint64_t x, uint64_t m) { return (std::max<uint64_t>(x, 1) + m - 1) / m * m; };
     const uint64_t Kp = up(nv, kDdK), nxp = up(nx, 64), nyp = up(ny, 64);
-    if (Kp > 0xffffffffull || nxp / 64 > 65535) return ctx->fail(ASDM_E_UNSUPPORTED, "dense product beyond the tile grid");
+    if (nxp / 64 > 65535 || Kp / 64 > 65535) return ctx->fail(ASDM_E_UNSUPPORTED, "dense product beyond the tile grid");
     int8_t *Ad = nullptr, *Bd = nullptr;
     int rc = dev_alloc(ctx, (void **)&Ad, nxp * Kp * 8);
     if (!rc) rc = dev_alloc(ctx, (void **)&Bd, nyp * Kp * 8);
@@ -1664,8 +1664,10 @@ struct BagRun {
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);
             out.rows = e;
-        } else if (nx >= 32 && ny >= 32 && nv >= 64) {
-            // on the matrix cores, each operand read in the orientation it is stored in
+        } else if (nx >= 32 && ny >= 32 && nv >= 64 && (uint64_t)nx + 63 <= 64ull * 65535 &&
+                   (uint64_t)nv + 63 <= 64ull * 65535) {
+            // on the matrix cores (within the tile grid's 65,535 rows of workgroups), each operand
+            // read in the orientation it is stored in
             KTimer kt(ctx, kname);
             if ((rc = dense_product_mfma(ctx, fxv.t.p, fxv.t.rows != X, uv, fvy.t.p, fvy.t.rows == v, nx, nv, ny, G,
                                          bytes_slot(11))))
