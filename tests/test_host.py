@@ -264,3 +264,43 @@ def test_bench_refuses_more_than_one_gpu():
                        cwd=root)
     assert r.returncode == 2
     assert "measures one GPU" in r.stderr
+
+
+def test_bench_traffic_summaries_and_device_aggregate(tmp_path, monkeypatch):
+    """The bench's roofline plumbing: each committed traffic summary parses into one block per kernel,
+    a block is used only for its own kernel and for launches of its algorithmic size (within 5 %),
+    and the device aggregate sums the timed kernels' algorithmic bytes (host phases and the loops'
+    touched-byte counters excluded) over the step's wall time."""
+    import sys
+
+    root = os.path.join(os.path.dirname(__file__), "..")
+    sys.path.insert(0, root)
+    import bench
+
+    for w, path in bench.TRAFFIC_FILES.items():
+        if not os.path.exists(path):
+            continue
+        kernels = [line.split(":", 1)[1].strip() for line in open(path) if line.startswith("kernel:")]
+        assert kernels, w
+        for line in open(path):
+            if line.startswith("alg_bytes_per_launch:"):
+                assert float(line.split(":", 1)[1]) > 0, w
+    f = tmp_path / "t.txt"
+    f.write_text("# header\n\nkernel: a\ntraffic_bytes_per_launch: 200\nalg_bytes_per_launch: 100\nsource: x\n\n"
+                 "kernel: b\ntraffic_bytes_per_launch: 900\nalg_bytes_per_launch: 1000\nsource: y\n")
+    monkeypatch.setitem(bench.TRAFFIC_FILES, "t", str(f))
+    assert bench.read_traffic("a", "t", 103) == (200.0, "x")
+    assert bench.read_traffic("b", "t", 960) == (900.0, "y")
+    assert bench.read_traffic("b", "t", 1100)[0] is None  # launches of another size
+    assert bench.read_traffic("c", "t", 100)[0] is None   # no block for this kernel
+    ks = [{"name": "k1", "ms": 2.0, "launches": 4, "bytes": 8e9},
+          {"name": "k2", "ms": 1.0, "launches": 1, "bytes": 2e9},
+          {"name": "host: wait", "ms": 5.0, "launches": 1, "bytes": 7e9},
+          {"name": "k1_touched", "ms": 0.0, "launches": 0, "bytes": 9e9},
+          {"name": "counter only", "ms": 0.0, "launches": 3, "bytes": 4e9}]
+    agg = bench.device_aggregate(ks, steps=2, wall_s=0.5)
+    assert agg["alg_bytes_per_step"] == 5e9
+    assert agg["achieved"] == 10.0 and agg["frac"] == round(10.0 / bench.HBM_PEAK_GBS, 5)
+    r = bench.roofline_of(ks[:2], "t")
+    assert r["kernel"] == "k1" and r["launches"] == 4 and r["avg_launch_ms"] == 0.5
+    assert r["achieved"] == 4000.0  # GB/s: 2e9 bytes per launch in 0.5 ms
