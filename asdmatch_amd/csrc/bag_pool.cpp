@@ -132,6 +132,7 @@ int BagPool::start() {
             resolve_timers(w);
             w->kstats.clear();
             w->profiling = ctx->profiling;
+            if (w->profiling) reserve_timer_events(w);
         }
     }
     for (uint32_t i = 0; i < nw; ++i) s_->threads.emplace_back([this, w = ctx->workers[i]] { s_->worker(w); });

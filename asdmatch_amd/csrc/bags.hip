@@ -760,7 +760,8 @@ __global__ void __launch_bounds__(256) k_dd_mfma(const int8_t *__restrict__ Ad, 
 // (a_kmajor false) or as its transpose (nv x nx, a_kmajor true); B (nv x ny) row-major (b_kmajor true)
 // or as its transpose (ny x nv); u may be null (all ones).  The digit blocks are temporaries.
 int dense_product_mfma(asdm_ctx *ctx, const uint64_t *A, bool a_kmajor, const uint64_t *u, const uint64_t *B,
-                       bool b_kmajor, uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *G, unsigned long long *nbytes) {
+                       bool b_kmajor, uint32_t nx, uint32_t nv, uint32_t ny, uint64_t *G, unsigned long long *nbytes,
+                       KTimer &kt) {
     auto up = [](uint64_t x, uint64_t m) { return (std::max<uint64_t>(x, 1) + m - 1) / m * m; };
     const uint64_t Kp = up(nv, kDdK), nxp = up(nx, 64), nyp = up(ny, 64);
     if (nxp / 64 > 65535 || Kp / 64 > 65535) return ctx->fail(ASDM_E_UNSUPPORTED, "dense product beyond the tile grid");
@@ -770,16 +771,16 @@ int dense_product_mfma(asdm_ctx *ctx, const uint64_t *A, bool a_kmajor, const ui
     if (!rc) {
         auto digits = [&](const uint64_t *M, bool kmajor, const uint64_t *scale, uint32_t rows, uint64_t rows_p, int8_t *dst) {
             if (kmajor)
-                hipLaunchKernelGGL(k_dd_digits_cols, dim3((unsigned)(rows_p / 64), (unsigned)(Kp / 64)), dim3(256), 0,
-                                   ctx->stream, M, scale, rows, nv, (uint32_t)rows_p, (uint32_t)Kp, dst);
+                kt.launch(k_dd_digits_cols, dim3((unsigned)(rows_p / 64), (unsigned)(Kp / 64)), dim3(256), 0, M, scale, rows,
+                          nv, (uint32_t)rows_p, (uint32_t)Kp, dst);
             else
-                hipLaunchKernelGGL(k_dd_digits_rows, dim3(grid_for(rows_p * (Kp / 16), 256)), dim3(256), 0, ctx->stream, M,
-                                   scale, rows, nv, (uint32_t)rows_p, (uint32_t)Kp, dst);
+                kt.launch(k_dd_digits_rows, dim3(grid_for(rows_p * (Kp / 16), 256)), dim3(256), 0, M, scale, rows, nv,
+                          (uint32_t)rows_p, (uint32_t)Kp, dst);
         };
         digits(A, a_kmajor, u, nx, nxp, Ad);
         digits(B, b_kmajor, nullptr, ny, nyp, Bd);
-        hipLaunchKernelGGL(k_dd_mfma, dim3((unsigned)(nyp / 64), (unsigned)(nxp / 64)), dim3(256), 0, ctx->stream, Ad, Bd, nx,
-                           ny, (uint32_t)Kp, G, nbytes);
+        kt.launch(k_dd_mfma, dim3((unsigned)(nyp / 64), (unsigned)(nxp / 64)), dim3(256), 0, (const int8_t *)Ad,
+                  (const int8_t *)Bd, nx, ny, (uint32_t)Kp, G, nbytes);
         rc = check_hip(ctx, hipGetLastError(), "dense product");
     }
     if (Ad) dev_free(ctx, Ad);
@@ -1197,22 +1198,19 @@ struct BagRun {
         if (!rc) rc = fill_u32(ctx, keep, U1, 0);
         if (rc) return rc;
         {
-            KTimer kt(ctx, "bag_relation");
-            hipLaunchKernelGGL(k_rel_len, dim3(grid_for(R1, kThreads)), dim3(kThreads), 0, ctx->stream, d_jobs, d_row0, nj, src,
-                               len);
+            KTimer kt(ctx, "bag_relation", KTimer::kDispatch);
+            kt.launch(k_rel_len, dim3(grid_for(R1, kThreads)), dim3(kThreads), 0, d_jobs, d_row0, nj, src, len);
             rc = launched("bag relation lengths");
             if (!rc) rc = scan_u32_to_u64(ctx, len, ebase, rows);
             if (!rc) {
-                hipLaunchKernelGGL(k_rel_keep, dim3(grid_for((U1 + kRelPer - 1) / kRelPer, kThreads)), dim3(kThreads), 0,
-                                   ctx->stream, d_jobs, d_row0, nj, src, ebase, cs->d_ce_val, keep, pos, bytes_slot(0));
+                kt.launch(k_rel_keep, dim3(grid_for((U1 + kRelPer - 1) / kRelPer, kThreads)), dim3(kThreads), 0, d_jobs,
+                          d_row0, nj, src, ebase, cs->d_ce_val, keep, pos, bytes_slot(0));
                 rc = launched("bag relation keep");
             }
             if (!rc) rc = scan_u32_to_u64(ctx, keep, kpos, ub);
             if (!rc) {
-                hipLaunchKernelGGL(k_rel_place, dim3(grid_for(U1, kThreads)), dim3(kThreads), 0, ctx->stream, keep, pos, kpos, ub,
-                                   val);
-                hipLaunchKernelGGL(k_rel_off, dim3(grid_for(R1 + 1, kThreads)), dim3(kThreads), 0, ctx->stream, ebase, kpos, rows,
-                                   off);
+                kt.launch(k_rel_place, dim3(grid_for(U1, kThreads)), dim3(kThreads), 0, keep, pos, kpos, ub, val);
+                kt.launch(k_rel_off, dim3(grid_for(R1 + 1, kThreads)), dim3(kThreads), 0, ebase, kpos, rows, off);
                 rc = launched("bag relation place");
             }
             kt.stop();
@@ -1292,8 +1290,8 @@ struct BagRun {
         if (rc) return rc;
         // two timed spans, so that the host synchronization between them is not counted as
         // kernel time
-        KTimer kt(ctx, "bag_domain");
-        hipLaunchKernelGGL(k_dom_words, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, ctx->stream, cs->d_bits, W, m, bits, cnt);
+        KTimer kt(ctx, "bag_domain", KTimer::kDispatch);
+        kt.launch(k_dom_words, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, cs->d_bits, W, m, bits, cnt);
         if ((rc = launched("bag domain words"))) return rc;
         if ((rc = scan_u32_to_u64(ctx, cnt, off, W))) return rc;
         kt.stop(8ull * W * m.n + 12ull * W + 8ull * (W + 1));
@@ -1302,8 +1300,8 @@ struct BagRun {
         if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (!rc) rc = alloc((void **)&ids, 4ull * std::max<uint64_t>(n, 1));
         if (rc) return rc;
-        KTimer kf(ctx, "bag_domain");
-        hipLaunchKernelGGL(k_dom_fill, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, ctx->stream, bits, W, off, rank, ids);
+        KTimer kf(ctx, "bag_domain", KTimer::kDispatch);
+        kf.launch(k_dom_fill, dim3(grid_for(W, kThreads)), dim3(kThreads), 0, bits, W, off, rank, ids);
         kf.stop(8ull * W + 8ull * W + 4ull * W + 4 * n);
         if ((rc = launched("bag domain fill"))) return rc;
         release(cnt);
@@ -1328,8 +1326,8 @@ struct BagRun {
         if (rc) return rc;
         temp = true;
         const uint64_t tiles = ((uint64_t)(r + 31) / 32) * ((c + 31) / 32);
-        KTimer kt(ctx, "bag_transpose");
-        hipLaunchKernelGGL(k_transpose, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, t.p, r, c, *out);
+        KTimer kt(ctx, "bag_transpose", KTimer::kDispatch);
+        kt.launch(k_transpose, dim3(grid_for(tiles, 1)), dim3(256), 0, t.p, r, c, *out);
         kt.stop(16ull * r * c);
         return launched("bag transpose");
     }
@@ -1398,21 +1396,20 @@ struct BagRun {
                 uint64_t *t = ce ? ce->p : nullptr;
                 if (!ce) {
                     if ((rc = alloc((void **)&t, 8ull * std::max<uint32_t>(na, 1)))) return rc;
-                    KTimer kt(ctx, "bag_vector");
+                    KTimer kt(ctx, "bag_vector", KTimer::kDispatch);
                     if (f.kind == 1) {
                         const Rel r = rel(a, v);
                         if ((rc = fill_u64(ctx, t, na, 0))) return rc;
                         const uint64_t lanes = (r.nnz + kSpmvPer - 1) / kSpmvPer;
-                        hipLaunchKernelGGL(k_spmv, dim3(std::max(1u, grid_for(lanes, kThreads))), dim3(kThreads), 0, ctx->stream,
-                                           r.off, r.val, r.w, na, u[v], t, bytes_slot(1));
+                        kt.launch(k_spmv, dim3(std::max(1u, grid_for(lanes, kThreads))), dim3(kThreads), 0, r.off,
+                                  r.val, r.w, na, u[v], t, bytes_slot(1));
                     } else if (f.t.rows == a) {
-                        hipLaunchKernelGGL(k_dense_mv, dim3(grid_for(na, kThreads / 64)), dim3(kThreads), 0, ctx->stream,
-                                           f.t.p, na, dv.n, u[v], t, bytes_slot(1));
+                        kt.launch(k_dense_mv, dim3(grid_for(na, kThreads / 64)), dim3(kThreads), 0, f.t.p, na, dv.n,
+                                  u[v], t, bytes_slot(1));
                     } else {  // table rows D(v): the transposed product, accumulated with atomics
                         if ((rc = fill_u64(ctx, t, na, 0))) return rc;
                         const dim3 grid((na + kThreads - 1) / kThreads, std::max<uint32_t>(1, (dv.n + kMvtRows - 1) / kMvtRows));
-                        hipLaunchKernelGGL(k_dense_mv_t, grid, dim3(kThreads), 0, ctx->stream, f.t.p, dv.n, na, u[v], t,
-                                           bytes_slot(1));
+                        kt.launch(k_dense_mv_t, grid, dim3(kThreads), 0, f.t.p, dv.n, na, u[v], t, bytes_slot(1));
                     }
                     kt.stop();
                     if ((rc = launched("bag vector step"))) return rc;
@@ -1482,9 +1479,9 @@ struct BagRun {
         bool reversed = false;  // the join wrote wr (entries of R(Y->X)) instead of wf
         if (fxv.kind == 1 && fvy.kind == 1) {
             const Rel ra = relv(fxv, X), rb = relv(fvy, Y);
-            KTimer kt(ctx, "bag_join_ee");
-            hipLaunchKernelGGL(k_join_ee, grid, dim3(kThreads), 0, ctx->stream, rxy.off, rxy.val, rxy.w, nx, rxy.nnz, ra.off, ra.val,
-                               ra.w, rb.off, rb.val, rb.w, uv, wf, bytes_slot(12));
+            KTimer kt(ctx, "bag_join_ee", KTimer::kDispatch);
+            kt.launch(k_join_ee, grid, dim3(kThreads), 0, rxy.off, rxy.val, rxy.w, nx, rxy.nnz, ra.off, ra.val, ra.w,
+                      rb.off, rb.val, rb.w, uv, wf, bytes_slot(12));
             kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * (nx + ny + 2) + 12 * (ra.nnz + rb.nnz) + 8ull * nv);
         } else if (fxv.kind == 1 || fvy.kind == 1) {
             const bool own_x = fxv.kind == 1;  // the list side is X: F is the (v, Y) table
@@ -1499,10 +1496,10 @@ struct BagRun {
             const Rel rw_ = a == X ? rxy : ryx;
             const uint32_t nrows = a == X ? nx : ny;
             reversed = a != X;
-            KTimer kt(ctx, "bag_join_ed");
-            hipLaunchKernelGGL(k_join_ed, dim3(grid_for(rw_.nnz * kJoinLanes, kThreads)), dim3(kThreads), 0, ctx->stream,
-                               rw_.off, rw_.val, rw_.w, nrows, rw_.nnz, rl.off, rl.val, rl.w, 0, td.p, sa, sj, uv,
-                               reversed ? wr : wf, bytes_slot(13), d_acc + 30);
+            KTimer kt(ctx, "bag_join_ed", KTimer::kDispatch);
+            kt.launch(k_join_ed, dim3(grid_for(rw_.nnz * kJoinLanes, kThreads)), dim3(kThreads), 0, rw_.off, rw_.val,
+                      rw_.w, nrows, rw_.nnz, rl.off, rl.val, rl.w, 0, td.p, sa, sj, uv, reversed ? wr : wf,
+                      bytes_slot(13), d_acc + 30);
             // the relations and lists once; the table entries it gathers (8 B each, one per list
             // entry of each joined edge) are counted on the device and added when the query ends
             kt.stop(8ull * (nrows + 1) + 12 * rw_.nnz + 8ull * (dom[o].n + 1) + 12 * rl.nnz + 8ull * nv);
@@ -1514,10 +1511,9 @@ struct BagRun {
             bool ta = false, tb = false;
             if ((rc = oriented(fxv.t, fxv.t.rows == X ? v : X, X, dom, &Ar, ta))) return rc;
             if ((rc = oriented(fvy.t, fvy.t.rows == Y ? v : Y, Y, dom, &Br, tb))) return rc;
-            KTimer kt(ctx, "bag_join_dd");
-            hipLaunchKernelGGL(k_join_dd, dim3(grid_for(rxy.nnz, kThreads / 64)), dim3(kThreads), 0, ctx->stream, rxy.off,
-                               rxy.val, rxy.w, nx, rxy.nnz, Ar, (uint64_t)nv, 1ull, Br, 1ull, (uint64_t)nv, nv, uv, wf,
-                               bytes_slot(13));
+            KTimer kt(ctx, "bag_join_dd", KTimer::kDispatch);
+            kt.launch(k_join_dd, dim3(grid_for(rxy.nnz, kThreads / 64)), dim3(kThreads), 0, rxy.off, rxy.val, rxy.w, nx,
+                      rxy.nnz, Ar, (uint64_t)nv, 1ull, Br, 1ull, (uint64_t)nv, nv, uv, wf, bytes_slot(13));
             kt.stop(8ull * (nx + 1) + 12 * rxy.nnz + 8ull * nv * (nx + ny + 1));
             if (ta) release(Ar);
             if (tb) release(Br);
@@ -1618,7 +1614,7 @@ struct BagRun {
         auto relfrom_v = [&](const Fac &f, uint32_t a) { return a < v ? f.hi : f.lo; };  // rows D(v) of (a, v)
         if (fxv.kind == 1 && fvy.kind == 1) {
             const Rel rx = relv(fxv, X), rv = relfrom_v(fvy, Y);
-            KTimer kt(ctx, kname);
+            KTimer kt(ctx, kname, KTimer::kDispatch);
             // count the terms, then both forms: each kernel returns at once unless the count picks
             // its form (the LDS form for many terms per cell, the scatter form for few)
             unsigned long long *terms = d_acc + 20;  // [0] the term count, [1] the LDS form's item queue
@@ -1627,17 +1623,16 @@ struct BagRun {
             auto capped = [&](uint64_t items, uint64_t cap) {  // grid-stride kernels: at most cap workgroups
                 return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((items + kThreads - 1) / kThreads, cap)));
             };
-            hipLaunchKernelGGL(k_ee_terms, capped(rx.nnz, resident), dim3(kThreads), 0, ctx->stream, rx.off, rx.val,
-                               rx.w, rv.off, uv, nx, terms);
-            hipLaunchKernelGGL(k_ee_zero, capped(cells / 2, 2 * resident), dim3(kThreads), 0, ctx->stream, G, cells,
-                               terms, ctx->table_form);
-            hipLaunchKernelGGL(k_ee_scatter, capped(rx.nnz, resident), dim3(kThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off,
-                               rv.val, rv.w, uv, nx, ny, G, terms, ctx->table_form, bytes_slot(8));
+            kt.launch(k_ee_terms, capped(rx.nnz, resident), dim3(kThreads), 0, rx.off, rx.val, rx.w, rv.off, uv, nx,
+                      terms);
+            kt.launch(k_ee_zero, capped(cells / 2, 2 * resident), dim3(kThreads), 0, G, cells, terms, ctx->table_form);
+            kt.launch(k_ee_scatter, capped(rx.nnz, resident), dim3(kThreads), 0, rx.off, rx.val, rx.w, rv.off, rv.val,
+                      rv.w, uv, nx, ny, G, terms, ctx->table_form, bytes_slot(8));
             // the LDS form: as many workgroups as fit the GPU at once (two per CU), fed by a queue
             const uint64_t items = (uint64_t)nx * ((ny + kEeCols - 1) / kEeCols);
             const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(items, 2ull * ctx->num_cus));
-            hipLaunchKernelGGL(k_ee, dim3(grid), dim3(kEeThreads), 0, ctx->stream, rx.off, rx.val, rx.w, rv.off, rv.val,
-                               rv.w, uv, nx, ny, G, terms, ctx->table_form, terms + 1, bytes_slot(8));
+            kt.launch(k_ee, dim3(grid), dim3(kEeThreads), 0, rx.off, rx.val, rx.w, rv.off, rv.val, rv.w, uv, nx, ny, G,
+                      terms, ctx->table_form, terms + 1, bytes_slot(8));
             kt.stop(8ull * (nx + 1) + 4 * rx.nnz + 8ull * (nv + 1) + 4 * rv.nnz + 8ull * nv + out_bytes);
         } else if (fxv.kind == 1 || fvy.kind == 1) {
             // one dense side: G = R(e -> v) diag(u) F(v, d) with e the edge side and d the dense side;
@@ -1650,17 +1645,15 @@ struct BagRun {
             if ((rc = oriented(td, td.rows == v ? d : v, v, dom, &F, tmp))) return rc;
             const Rel re = relv(dense_x ? fvy : fxv, e);
             const uint32_t ne = dom[e].n, nd = dom[d].n;
-            KTimer kt(ctx, kname);
+            KTimer kt(ctx, kname, KTimer::kDispatch);
             // a resident grid, a multiple of 8 (one share per XCD): at most 8 workgroups per CU
             const uint64_t units = (uint64_t)((nd + 63) / 64) * ((ne + kEdsRows - 1) / kEdsRows);
             const uint64_t want = (units + kThreads / 64 - 1) / (kThreads / 64);
             const dim3 grid((unsigned)(8 * std::max<uint64_t>(1, std::min<uint64_t>((want + 7) / 8, ctx->num_cus))));
             if (dense_x)
-                hipLaunchKernelGGL(k_eds<1>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
-                                   bytes_slot(9));
+                kt.launch(k_eds<1>, grid, dim3(kThreads), 0, re.off, re.val, re.w, uv, F, ne, nd, G, bytes_slot(9));
             else
-                hipLaunchKernelGGL(k_eds<0>, grid, dim3(kThreads), 0, ctx->stream, re.off, re.val, re.w, uv, F, ne, nd, G,
-                                   bytes_slot(10));
+                kt.launch(k_eds<0>, grid, dim3(kThreads), 0, re.off, re.val, re.w, uv, F, ne, nd, G, bytes_slot(10));
             kt.stop(8ull * (ne + 1) + 4 * re.nnz + 8ull * nv + 8ull * nv * nd + out_bytes);
             if (tmp) release(F);
             out.rows = e;
@@ -1668,9 +1661,9 @@ struct BagRun {
                    (uint64_t)nv + 63 <= 64ull * 65535) {
             // on the matrix cores (within the tile grid's 65,535 rows of workgroups), each operand
             // read in the orientation it is stored in
-            KTimer kt(ctx, kname);
+            KTimer kt(ctx, kname, KTimer::kDispatch);
             if ((rc = dense_product_mfma(ctx, fxv.t.p, fxv.t.rows != X, uv, fvy.t.p, fvy.t.rows == v, nx, nv, ny, G,
-                                         bytes_slot(11))))
+                                         bytes_slot(11), kt)))
                 return rc;
             kt.stop(8ull * nx * nv + 8ull * nv * ny + 8ull * nv + out_bytes);
         } else {
@@ -1679,9 +1672,8 @@ struct BagRun {
             if ((rc = oriented(fxv.t, fxv.t.rows == X ? v : X, X, dom, &Fa, ta))) return rc;
             if ((rc = oriented(fvy.t, fvy.t.rows == v ? Y : v, v, dom, &Fb, tb))) return rc;
             const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
-            KTimer kt(ctx, kname);
-            hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, Fa, uv, Fb, nx, nv, ny, G,
-                               bytes_slot(11));
+            KTimer kt(ctx, kname, KTimer::kDispatch);
+            kt.launch(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, Fa, uv, Fb, nx, nv, ny, G, bytes_slot(11));
             kt.stop(8ull * nx * nv + 8ull * nv * ny + 8ull * nv + out_bytes);
             if (ta) release(Fa);
             if (tb) release(Fb);
@@ -1720,9 +1712,9 @@ struct BagRun {
                 Members m{};
                 m.n = (uint32_t)T.blocks[t].size();
                 for (uint32_t k = 0; k < m.n; ++k) m.u[k] = T.blocks[t][k];
-                KTimer kt(ctx, "bag_local_scatter");
-                hipLaunchKernelGGL(k_scatter_local, dim3(grid_for(dom[t].n, kThreads)), dim3(kThreads), 0, ctx->stream,
-                                   dom[t].ids, dom[t].n, u[t], scal, T.mu, m, cs->nq, d_mat);
+                KTimer kt(ctx, "bag_local_scatter", KTimer::kDispatch);
+                kt.launch(k_scatter_local, dim3(grid_for(dom[t].n, kThreads)), dim3(kThreads), 0, dom[t].ids, dom[t].n,
+                          u[t], scal, T.mu, m, cs->nq, d_mat);
                 kt.stop(12ull * dom[t].n + 16ull * dom[t].n * m.n);
                 if ((rc = launched("bag local scatter"))) return rc;
                 release(u[0]);
@@ -1828,7 +1820,9 @@ int asdm_dense_product(asdm_ctx *ctx, const uint64_t *A, int a_kmajor, const uin
     unsigned long long *nbytes = (unsigned long long *)ctx->d_counters + 1;
     if (!rc && ng) {
         if (method == 2) {
-            rc = dense_product_mfma(ctx, dA, a_kmajor != 0, du, dB, b_kmajor != 0, nx, nv, ny, dG, nbytes);
+            KTimer kt(ctx, "dense_product", KTimer::kDispatch);
+            rc = dense_product_mfma(ctx, dA, a_kmajor != 0, du, dB, b_kmajor != 0, nx, nv, ny, dG, nbytes, kt);
+            kt.stop();
         } else {
             const uint64_t tiles = ((uint64_t)(nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
             hipLaunchKernelGGL(k_dd, dim3(grid_for(tiles, 1)), dim3(256), 0, ctx->stream, dA, du, dB, nx, nv, ny, dG, nbytes);

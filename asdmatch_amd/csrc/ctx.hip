@@ -202,7 +202,19 @@ static int kstat_slot(asdm_ctx *ctx, const char *name) {
     return (int)ctx->kstats.size() - 1;
 }
 
-static hipEvent_t take_event(asdm_ctx *ctx) {
+// Creates the timing events a profiled call typically needs up front, so that their creation does
+// not fall inside the first timed calls.
+void reserve_timer_events(asdm_ctx *ctx) {
+    constexpr size_t kReserve = 1024;
+    (void)hipSetDevice(ctx->device);
+    while (ctx->event_pool.size() < kReserve) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) break;
+        ctx->event_pool.push_back(e);
+    }
+}
+
+hipEvent_t take_timer_event(asdm_ctx *ctx) {
     hipEvent_t e = nullptr;
     if (!ctx->event_pool.empty()) {
         e = ctx->event_pool.back();
@@ -214,11 +226,12 @@ static hipEvent_t take_event(asdm_ctx *ctx) {
     return e;
 }
 
-KTimer::KTimer(asdm_ctx *c, const char *name) : ctx(c), slot(-1) {
+KTimer::KTimer(asdm_ctx *c, const char *name, Mode m) : ctx(c), slot(-1), mode(m) {
     if (!ctx->profiling) return;
     slot = kstat_slot(ctx, name);
-    a = take_event(ctx);
-    b = take_event(ctx);
+    if (mode == kDispatch) return;
+    a = take_timer_event(ctx);
+    b = take_timer_event(ctx);
     (void)hipEventRecord(a, ctx->stream);
 }
 
@@ -226,9 +239,13 @@ KTimer::KTimer(asdm_ctx *c, const char *name) : ctx(c), slot(-1) {
 // not serialize the launch sequence.
 void KTimer::stop(uint64_t bytes) {
     if (!ctx->profiling || slot < 0) return;
-    (void)hipEventRecord(b, ctx->stream);
     ctx->kstats[slot].bytes += bytes;
-    ctx->pending.push_back({slot, a, b});
+    if (mode == kDispatch) {
+        ctx->kstats[slot].launches += 1;
+    } else {
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->pending.push_back({slot, a, b, 1});
+    }
     slot = -1;
 }
 
@@ -238,7 +255,7 @@ void resolve_timers(asdm_ctx *ctx) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, t.a, t.b);
         ctx->kstats[t.slot].ms += ms;
-        ctx->kstats[t.slot].launches += 1;
+        ctx->kstats[t.slot].launches += t.launches;
         ctx->event_pool.push_back(t.a);
         ctx->event_pool.push_back(t.b);
     }
@@ -522,6 +539,13 @@ int asdm_ctx_set_profiling(asdm_ctx *ctx, int on) try {
     resolve_timers(ctx);
     ctx->profiling = on != 0;
     ctx->kstats.clear();
+    if (ctx->profiling) reserve_timer_events(ctx);
+    for (asdm_ctx *w : ctx->workers) {  // the bag workers follow, their events made now too
+        resolve_timers(w);
+        w->kstats.clear();
+        w->profiling = ctx->profiling;
+        if (w->profiling) reserve_timer_events(w);
+    }
     return ASDM_OK;
 } ASDM_CTX_CATCH(ctx)
 

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <functional>
@@ -109,6 +110,7 @@ struct asdm_ctx {
     struct PendingTimer {
         int slot;
         hipEvent_t a, b;
+        uint32_t launches;  // sections the pair completes (0 for all but a marker pair)
     };
     std::vector<PendingTimer> pending;  // recorded, resolved after the stream has drained
     std::vector<hipEvent_t> event_pool;  // timer events reused across launches
@@ -291,11 +293,31 @@ int attach_bags(const asdm_cs *cs, uint32_t flags, bool explicit_order, asdm_pla
 int run_bags(asdm_ctx *ctx, const asdm_plan *plan, const asdm_cs *cs, uint64_t *mat_host, uint64_t *count);
 
 // Kernel timing helpers (ctx.hip)
+// A timing event from the context's pool (created without a system-scope fence).
+hipEvent_t take_timer_event(asdm_ctx *ctx);
+// Fills the context's pool of timing events (profiling is about to start).
+void reserve_timer_events(asdm_ctx *ctx);
+// A timed section of launches, while the context is profiling.  By default the section is bracketed
+// by two event markers on the stream.  With kDispatch, the kernels launched through launch() carry
+// their own start and end events in their dispatches (no marker packets, so no pause between
+// kernels), and the section's time is the sum of those kernels' durations.
 struct KTimer {
+    enum Mode { kMarkers, kDispatch };
     asdm_ctx *ctx;
     int slot;
+    Mode mode;
     hipEvent_t a = nullptr, b = nullptr;
-    KTimer(asdm_ctx *c, const char *name);
+    KTimer(asdm_ctx *c, const char *name, Mode m = kMarkers);
+    template <typename F, typename... Args>
+    void launch(F kernel, const dim3 &grid, const dim3 &block, uint32_t shmem, Args... args) {
+        if (mode != kDispatch || slot < 0) {
+            hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+            return;
+        }
+        hipEvent_t e0 = take_timer_event(ctx), e1 = take_timer_event(ctx);
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, e0, e1, 0, args...);
+        ctx->pending.push_back({slot, e0, e1, 0});
+    }
     void stop(uint64_t bytes = 0);
 };
 // Resolves the recorded kernel timers (call after the stream has been synchronized).

@@ -132,22 +132,22 @@ int aggregate_pairs(asdm_ctx *ctx, const uint64_t *keys, const uint64_t *vals, u
         const uint64_t ntiles = (n + kAggTile - 1) / kAggTile;
         const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 8ull * ctx->num_cus);
         {
-            KTimer kt(ctx, "local_agg_bucket");
-            hipLaunchKernelGGL(k_agg_count, dim3(grid), dim3(kAggThreads), 0, ctx->stream, keys, n, nb, hist);
+            KTimer kt(ctx, "local_agg_bucket", KTimer::kDispatch);
+            kt.launch(k_agg_count, dim3(grid), dim3(kAggThreads), 0, keys, n, nb, hist);
             rc = check_hip(ctx, hipGetLastError(), "aggregation count");
             if (!rc) rc = scan_u32_to_u64(ctx, hist, boff, nb);
             if (!rc) {
-                hipLaunchKernelGGL(k_agg_scatter, dim3(grid), dim3(kAggThreads), 0, ctx->stream, keys, vals, n, nb, boff,
-                                   cursor, ocell, oval);
+                kt.launch(k_agg_scatter, dim3(grid), dim3(kAggThreads), 0, keys, vals, n, nb, boff, cursor, ocell,
+                          oval);
                 rc = check_hip(ctx, hipGetLastError(), "aggregation scatter");
             }
             kt.stop(8 * n + 16 * n + 12 * n + 16ull * nb);
         }
         if (!rc) {
-            KTimer kt(ctx, "local_agg_sum");
+            KTimer kt(ctx, "local_agg_sum", KTimer::kDispatch);
             const unsigned g2 = (unsigned)std::min<uint64_t>(nb, 8ull * ctx->num_cus);
-            hipLaunchKernelGGL(k_agg_sum, dim3(g2), dim3(kAggThreads), 0, ctx->stream, ocell, oval, boff, nb, cells,
-                               (unsigned long long *)mat);
+            kt.launch(k_agg_sum, dim3(g2), dim3(kAggThreads), 0, ocell, oval, boff, nb, cells,
+                      (unsigned long long *)mat);
             rc = check_hip(ctx, hipGetLastError(), "aggregation sum");
             kt.stop(12 * n + 8ull * nb + 16ull * std::min<uint64_t>(n, cells));
         }
