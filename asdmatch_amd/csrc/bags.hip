@@ -378,8 +378,9 @@ __global__ void k_ee_scatter(const uint64_t *__restrict__ xoff, const uint32_t *
 // with 2,560 (eight workgroups per CU), 766 with 4,096 and 564 with 12,288 or 16,384 (1,024
 // threads): most rows fit one chunk, so the searches and the per-item latency go.  (One
 // workgroup per whole row with 80 KiB of LDS, a separate clearing pass and 8-byte stores wrote the
-// table at about 1 TB/s.)
-constexpr int kEeThreads = 256;
+// table at about 1 TB/s.)  Two workgroups of 512 threads per CU (the LDS allows two): 344 ms per
+// batch against 395 with 256 threads and 368 with 1,024 (whose registers allow one per CU).
+constexpr int kEeThreads = 512;
 constexpr uint32_t kEeCols = 8192;
 __global__ void __launch_bounds__(kEeThreads) k_ee(const uint64_t *__restrict__ xoff, const uint32_t *__restrict__ xval,
                                                  const uint64_t *__restrict__ xw, const uint64_t *__restrict__ voff,
