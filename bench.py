@@ -531,7 +531,9 @@ def bench_c2(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed steps (c4: one batched call over the whole batch each, ~3 s; its rate varies by about 5 %% "
+                         "between single steps)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c4",
